@@ -1,0 +1,119 @@
+/*
+ * Thin C-ABI over the hand-written CDNA4 (gfx950) HIP kernels.
+ *
+ * The C host code of the trainers (libirwls_amd/csrc/host, C99) drives the
+ * GPU only through these functions: plain pointers and sizes, no HIP or
+ * torch types.  Pointers named d_* are device pointers obtained from
+ * irwls_dev_alloc.  All work is issued on one library-owned HIP stream, in
+ * order; functions that return host data synchronise that stream.
+ * HIP errors print a message to stderr and exit(2), the reference's
+ * convention for fatal errors (src/IOStructures.c:98-107).
+ *
+ * This layer has no counterpart in the reference: it replaces the scalar
+ * kernelFunction loops (src/kernels.c:63-118) and the BLAS/LAPACK calls of
+ * src/ParallelAlgorithms.c and src/budgeted-train.c.
+ */
+#ifndef IRWLS_DEVICE_H
+#define IRWLS_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- context, memory, copies ------------------------------------------ */
+int   irwls_dev_init(void);                 /* idempotent; returns 0 */
+void  irwls_dev_shutdown(void);
+void *irwls_dev_alloc(size_t bytes);        /* zero-filled */
+void  irwls_dev_free(void *d_ptr);
+void  irwls_dev_h2d(void *d_dst, const void *src, size_t bytes);   /* synchronous */
+void  irwls_dev_d2h(void *dst, const void *d_src, size_t bytes);   /* synchronous */
+void  irwls_dev_d2d(void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
+void  irwls_dev_memset0(void *d_dst, size_t bytes);                /* fill kernel */
+void  irwls_dev_sync(void);
+void  irwls_dev_trim(void);                 /* release scratch and solver workspace */
+
+/* ---- per-kernel timing (HIP events on the library stream) -------------- */
+enum irwls_timer_kind {
+    IRWLS_T_KMV = 0,      /* fused kernel-matrix x vector (E-update, GIN, predict) */
+    IRWLS_T_KMAT,         /* kernel-matrix materialisation (working-set block) */
+    IRWLS_T_POTF2,        /* diagonal-block factorisation */
+    IRWLS_T_TRSM,         /* panel solve */
+    IRWLS_T_SYRK,         /* trailing update */
+    IRWLS_T_POTRS,        /* triangular solves */
+    IRWLS_T_KKT,          /* KKT matrix assembly */
+    IRWLS_T_GEMV,         /* working-set gathers (inner error update, G13) */
+    IRWLS_T_SGMA,         /* SGMA residual / gain */
+    IRWLS_T_CSR,          /* CSR kernel columns */
+    IRWLS_T_SYRKD,        /* IRWLSpar weighted Gram update */
+    IRWLS_T_MATVEC,       /* IRWLSpar e = y - Phi beta */
+    IRWLS_T_COUNT
+};
+void irwls_dev_timing(int enable);          /* enable/disable and reset */
+/* Accumulated launches, milliseconds, algorithmic flops and bytes. */
+void irwls_dev_timer_get(int kind, long *launches, double *ms, double *flops, double *bytes);
+
+/* ---- kernel tiles -------------------------------------------------------
+ * Rows/columns of a kernel block are rows of a dense row-major FP64 matrix
+ * X (leading dimension ldx) selected by an optional int32 index list
+ * (NULL = identity), or rows of a CSR matrix.  k(x,z) is
+ *   linear:  x.z
+ *   RBF:     exp(-gamma * (|x|^2 + |z|^2 - 2 x.z))   (no clamping, as the
+ *            reference's norm form, src/kernels.c:101-116)
+ * with norms taken from the nrm vectors (indexed like the rows).          */
+typedef struct irwls_rows {
+    const double  *X;        /* dense rows (row-major), or NULL for CSR */
+    int            ldx;
+    const int64_t *rowptr;   /* CSR (0-based columns), used when X == NULL */
+    const int32_t *col;
+    const double  *val;
+    const double  *nrm;      /* |x|^2 per row of X / CSR (before index mapping) */
+    const int32_t *idx;      /* optional row selection */
+    int            n;        /* number of selected rows */
+} irwls_rows;
+
+/* |x|^2 of n dense rows, summed sequentially in feature order. */
+void irwls_dev_row_norms(const double *d_X, int ldx, int n, int d, double *d_nrm);
+
+/* out[i] = a*out[i] + rs[i]*(b * sum_j w[j] k(r_i, c_j)) + c
+   (rs NULL means 1).  Deterministic: fixed summation order for a given
+   (rows, cols) shape. */
+void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d,
+                             int kernel, double gamma, const double *d_w,
+                             double a, double b, double cst, const double *d_rs,
+                             double *d_out);
+
+/* K[i*ldk + j] = k(r_i, c_j) for all i < r->n, j < c->n. */
+void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d,
+                             int kernel, double gamma, double *d_K, int ldk);
+
+/* ---- full-training working-set helpers ---------------------------------- */
+/* H (column-major N x N, N = nS1+1): H[i][j] = Kw[s_i][s_j] y_i y_j + (i==j)/a_i,
+   border y, corner 0.  s = d_S1 (indices into Kw), y/a gathered by s. */
+void irwls_dev_build_kkt(const double *d_Kw, int ldk, const int32_t *d_S1, int nS1,
+                         const double *d_y, const double *d_a, double *d_H);
+/* out[i] = a*out[i] + rs[i] * (b * sum_j Kw[ri][cj] w[j]) + cst,
+   ri = rows ? rows[i] : i, cj = cols ? cols[j] : j */
+void irwls_dev_gemv_gather(const double *d_Kw, int ldk, const int32_t *d_rows, int nr,
+                           const int32_t *d_cols, int nc, const double *d_w,
+                           double a, double b, double cst, const double *d_rs,
+                           double *d_out);
+
+/* y = A x (trans 0) or A^T x (trans 1), A column-major m x n. */
+void irwls_dev_gemv(const double *d_A, int m, int n, int lda, int trans, const double *d_x, double *d_y);
+
+/* ---- dense Cholesky (column-major lower, LAPACK failure semantics) ------ */
+/* Factor A (n x n, lda) in place.  Returns info: 0, or j+1 when pivot j is
+   not positive (the Schur value is left at A(j,j) and the rest of the matrix
+   is left as LAPACK's blocked potrf leaves it). */
+int  irwls_dev_potrf(double *d_A, int n, int lda);
+/* Solve L L^T x = b in place (b on device). */
+void irwls_dev_potrs(const double *d_L, int n, int lda, double *d_b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,69 @@
+/*
+ * Additive C API of libirwls (no counterpart in the reference).
+ *
+ * The reference entry points keep their prototypes and struct layouts; the
+ * knobs of the GPU build live here instead, so sizeof(properties) does not
+ * change.  Every function is optional: a caller that only uses the reference
+ * interface gets the plain FP64 path with its defaults.
+ */
+#ifndef IRWLS_GPU_H
+#define IRWLS_GPU_H
+
+#include "IOStructures.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct irwls_gpu_options {
+    int precision;        /* 0 = FP64 (default), 1 = mixed (BF16x3 kernel GEMM, FP32 Cholesky + refinement) */
+    int sgma_candidates;  /* SGMA candidates per step (default 64, as the reference) */
+    int proposal;         /* 0 = plain SGMA draws (default), 1 = learned shortlist */
+    int proposal_keep;    /* learned proposal: candidates kept for exact scoring */
+    double refine_tol;    /* mixed precision: relative residual target (default 1e-10) */
+} irwls_gpu_options;
+
+/* Defaults, overridable by IRWLS_PRECISION=fp64|mixed, IRWLS_SGMA_CAND=<n>,
+   IRWLS_PROPOSAL=sgma|learned. */
+void irwls_gpu_get_options(irwls_gpu_options *out);
+void irwls_gpu_set_options(const irwls_gpu_options *opts);
+
+/* Keep a dataset resident in device memory so later trainFULL / SGMA /
+   IRWLSpar / test calls on it skip the host-to-device upload.  The dataset is
+   identified by its `features` pointer; it must not change while pinned. */
+void irwls_gpu_pin_dataset(svm_dataset dataset, int with_means);
+void irwls_gpu_unpin_dataset(svm_dataset dataset);
+
+/* Per-kernel timers (HIP events on the library stream). */
+void irwls_gpu_timing(int enable);
+void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double *bytes);
+
+/* Counters of the last training call. */
+typedef struct irwls_gpu_stats {
+    int outer_iters;
+    int inner_iters;
+    int solves;
+    int sgma_steps;
+    int irwlspar_iters;
+    double t_total;        /* seconds, host wall clock */
+    double t_solve;        /* seconds inside the dense solver */
+} irwls_gpu_stats;
+void irwls_gpu_last_stats(irwls_gpu_stats *out);
+
+/* In-memory dataset builders with the reference readers' semantics
+   (src/IOStructures.c:88-275): the two class-mean rows are appended and
+   |x|^2 is stored.  Dense rows list every index 1..d; CSR columns are
+   0-based (stored as index col+1).  Free with freeDataset. */
+svm_dataset irwls_dataset_from_dense(const double *X, const double *y, int n, int d);
+svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, const double *val,
+                                   const double *y, int n);
+
+/* Release the device context (initMemory/freeMemory do this for callers of
+   the reference interface). */
+void irwls_gpu_shutdown(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

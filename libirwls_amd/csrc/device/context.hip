@@ -1,0 +1,186 @@
+// Device context: one HIP stream, memory helpers and per-kind launch timers.
+#include "common.h"
+
+#include <string.h>
+
+namespace irwls {
+
+static bool g_init = false;
+static hipStream_t g_stream = nullptr;
+
+hipStream_t stream() {
+    if (!g_init) irwls_dev_init();
+    return g_stream;
+}
+
+// ---- timers --------------------------------------------------------------
+static bool g_timing = false;
+static long g_launch[IRWLS_T_COUNT];
+static double g_ms[IRWLS_T_COUNT], g_flops[IRWLS_T_COUNT], g_bytes[IRWLS_T_COUNT];
+static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+
+TimerScope::TimerScope(int k, double f, double b) : kind(k), flops(f), bytes(b), on(g_timing) {
+    if (on) IRWLS_HIP(hipEventRecord(g_ev0, stream()));
+}
+
+TimerScope::~TimerScope() {
+    if (!on) return;
+    IRWLS_HIP(hipEventRecord(g_ev1, stream()));
+    IRWLS_HIP(hipEventSynchronize(g_ev1));
+    float ms = 0.f;
+    IRWLS_HIP(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+    g_launch[kind] += 1;
+    g_ms[kind] += ms;
+    g_flops[kind] += flops;
+    g_bytes[kind] += bytes;
+}
+
+// ---- scratch slots -------------------------------------------------------
+static const int kSlots = 16;
+static void *g_scr[kSlots];
+static size_t g_scr_sz[kSlots];
+
+void *scratch(int slot, size_t bytes) {
+    if (slot < 0 || slot >= kSlots) {
+        fprintf(stderr, "libirwls: bad scratch slot %d\n", slot);
+        exit(2);
+    }
+    if (g_scr_sz[slot] < bytes) {
+        if (g_scr[slot]) {
+            IRWLS_HIP(hipStreamSynchronize(stream()));
+            IRWLS_HIP(hipFree(g_scr[slot]));
+        }
+        size_t sz = bytes + (bytes >> 2) + 256;
+        IRWLS_HIP(hipMalloc(&g_scr[slot], sz));
+        g_scr_sz[slot] = sz;
+    }
+    return g_scr[slot];
+}
+
+}  // namespace irwls
+
+using namespace irwls;
+
+__global__ void fill_zero_kernel(unsigned char *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    // 8-byte body when aligned, bytes otherwise
+    if (((uintptr_t)p & 7) == 0) {
+        size_t n8 = n / 8;
+        uint64_t *q = (uint64_t *)p;
+        for (size_t k = i; k < n8; k += stride) q[k] = 0;
+        for (size_t k = n8 * 8 + i; k < n; k += stride) p[k] = 0;
+    } else {
+        for (size_t k = i; k < n; k += stride) p[k] = 0;
+    }
+}
+
+extern "C" {
+
+int irwls_dev_init(void) {
+    if (g_init) return 0;
+    int ndev = 0;
+    IRWLS_HIP(hipGetDeviceCount(&ndev));
+    if (ndev < 1) {
+        fprintf(stderr, "libirwls: no GPU device visible\n");
+        exit(2);
+    }
+    IRWLS_HIP(hipSetDevice(0));
+    IRWLS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    IRWLS_HIP(hipEventCreate(&g_ev0));
+    IRWLS_HIP(hipEventCreate(&g_ev1));
+    g_init = true;
+    return 0;
+}
+
+void irwls_dev_shutdown(void) {
+    if (!g_init) return;
+    IRWLS_HIP(hipStreamSynchronize(g_stream));
+    for (int s = 0; s < kSlots; s++) {
+        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
+        g_scr[s] = nullptr;
+        g_scr_sz[s] = 0;
+    }
+    IRWLS_HIP(hipEventDestroy(g_ev0));
+    IRWLS_HIP(hipEventDestroy(g_ev1));
+    IRWLS_HIP(hipStreamDestroy(g_stream));
+    g_stream = nullptr;
+    g_init = false;
+}
+
+void *irwls_dev_alloc(size_t bytes) {
+    irwls_dev_init();
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    IRWLS_HIP(hipMalloc(&p, bytes));
+    irwls_dev_memset0(p, bytes);
+    return p;
+}
+
+void irwls_dev_free(void *p) {
+    if (!p) return;
+    IRWLS_HIP(hipStreamSynchronize(stream()));
+    IRWLS_HIP(hipFree(p));
+}
+
+void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    IRWLS_HIP(hipStreamSynchronize(stream()));
+}
+
+void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    IRWLS_HIP(hipStreamSynchronize(stream()));
+}
+
+void irwls_dev_d2d(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+}
+
+void irwls_dev_memset0(void *dst, size_t bytes) {
+    if (bytes == 0) return;
+    size_t work = (bytes + 7) / 8;
+    int blocks = (int)((work + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream(),
+                       (unsigned char *)dst, bytes);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sync(void) { IRWLS_HIP(hipStreamSynchronize(stream())); }
+
+void irwls_dev_trim(void) {
+    if (!g_init) return;
+    IRWLS_HIP(hipStreamSynchronize(g_stream));
+    for (int s = 0; s < kSlots; s++) {
+        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
+        g_scr[s] = nullptr;
+        g_scr_sz[s] = 0;
+    }
+}
+
+void irwls_dev_timing(int enable) {
+    irwls_dev_init();
+    g_timing = enable != 0;
+    memset(g_launch, 0, sizeof(g_launch));
+    memset(g_ms, 0, sizeof(g_ms));
+    memset(g_flops, 0, sizeof(g_flops));
+    memset(g_bytes, 0, sizeof(g_bytes));
+}
+
+void irwls_dev_timer_get(int kind, long *launches, double *ms, double *flops, double *bytes) {
+    if (kind < 0 || kind >= IRWLS_T_COUNT) {
+        *launches = 0; *ms = 0; *flops = 0; *bytes = 0;
+        return;
+    }
+    *launches = g_launch[kind];
+    *ms = g_ms[kind];
+    *flops = g_flops[kind];
+    *bytes = g_bytes[kind];
+}
+
+}  // extern "C"

@@ -1,0 +1,449 @@
+// Kernel-matrix tiles for gfx950.
+//
+// Dense rows: k(x_i, z_j) from the norm expansion |x|^2 + |z|^2 - 2 x.z with
+// the x.z block computed by FP64 MFMA (v_mfma_f64_16x16x4f64) on 64x64 tiles
+// staged through LDS, and the exp / weight / row-reduction epilogue fused.
+// CSR rows: lanes run over columns of a transposed dense column block, each
+// row's (col, val) pairs are broadcast; same epilogues.
+//
+// Replaces the scalar kernelFunction loops of the reference
+// (src/kernels.c:63-118; call sites src/full-train.c:178,243,336,447,504,
+// src/budgeted-train.c:180,244,340,354, src/LIBIRWLS-predict.c:69,112).
+#include "common.h"
+
+using namespace irwls;
+
+namespace {
+
+constexpr int TM = 64;       // tile rows
+constexpr int TN = 64;       // tile cols
+constexpr int BK = 16;       // k chunk staged in LDS
+constexpr int LDP = BK + 1;  // padded LDS row (doubles)
+
+enum { MODE_MATVEC = 0, MODE_MATRIX = 1 };
+
+struct DenseSide {
+    const double *X;
+    int ldx;
+    const int32_t *idx;  // may be null
+    const double *nrm;   // indexed by original row
+    int n;
+};
+
+__device__ __forceinline__ double kfun(int kernel, double gamma, double dot, double nr, double nc) {
+    if (kernel == 0) return dot;
+    return exp(-gamma * ((nr + nc) - 2.0 * dot));
+}
+
+// Loads a TM x BK chunk of rows [r0, r0+TM) x [k0, k0+BK) into LDS (zero padded).
+__device__ __forceinline__ void load_chunk(double (*S)[LDP], const DenseSide &s, int r0, int k0, int d) {
+    const int t = threadIdx.x;          // 256 threads: 64 rows x 4 quads
+    const int row = t >> 2;
+    const int kq = (t & 3) * 4;
+    const int gr = r0 + row;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (gr < s.n) {
+        const int src = s.idx ? s.idx[gr] : gr;
+        const double *p = s.X + (size_t)src * s.ldx + k0 + kq;
+        const int kk = k0 + kq;
+        if (kk + 3 < d) {
+            v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+        } else {
+            if (kk < d) v0 = p[0];
+            if (kk + 1 < d) v1 = p[1];
+            if (kk + 2 < d) v2 = p[2];
+        }
+    }
+    S[row][kq] = v0;
+    S[row][kq + 1] = v1;
+    S[row][kq + 2] = v2;
+    S[row][kq + 3] = v3;
+}
+
+// One 64x64 tile of dot products, accumulated in acc[mb][nb] (v4d each):
+// lane l of wave (wr, wc) owns rows wr*32 + mb*16 + 4*(l/16) + r and column
+// wc*32 + nb*16 + l%16 of the tile.
+__device__ __forceinline__ void tile_dots(v4d acc[2][2], double (*As)[TM][LDP], double (*Bs)[TN][LDP],
+                                          const DenseSide &rs, const DenseSide &cs,
+                                          int r0, int c0, int d) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int nchunk = (d + BK - 1) / BK;
+    load_chunk(As[0], rs, r0, 0, d);
+    load_chunk(Bs[0], cs, c0, 0, d);
+    __syncthreads();
+    for (int ch = 0; ch < nchunk; ch++) {
+        const int cur = ch & 1;
+        if (ch + 1 < nchunk) {
+            load_chunk(As[cur ^ 1], rs, r0, (ch + 1) * BK, d);
+            load_chunk(Bs[cur ^ 1], cs, c0, (ch + 1) * BK, d);
+        }
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ks++) {
+            const int kk = ks * 4 + (lane >> 4);
+            double a0 = As[cur][wr * 32 + (lane & 15)][kk];
+            double a1 = As[cur][wr * 32 + 16 + (lane & 15)][kk];
+            double b0 = Bs[cur][wc * 32 + (lane & 15)][kk];
+            double b1 = Bs[cur][wc * 32 + 16 + (lane & 15)][kk];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double row_nrm(const DenseSide &s, int i) {
+    return s.nrm ? s.nrm[s.idx ? s.idx[i] : i] : 0.0;
+}
+
+// Fused kernel-block x vector.  grid = (row tiles, column splits).  Split s
+// covers column tiles [s*tps, (s+1)*tps).  Writes partial sums part[s*n + i]
+// when nsplit > 1, else applies the output epilogue directly.
+__global__ __launch_bounds__(256) void kmv_dense_kernel(
+    DenseSide rs, DenseSide cs, int d, int kernel, double gamma, const double *__restrict__ w,
+    int tps, int nsplit, double *__restrict__ part,
+    double a, double b, double cst, const double *__restrict__ rsc, double *__restrict__ out) {
+    __shared__ double As[2][TM][LDP];
+    __shared__ double Bs[2][TN][LDP];
+    __shared__ double red[2][TM];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int r0 = blockIdx.x * TM;
+    const int ct_begin = blockIdx.y * tps;
+    int ct_end = ct_begin + tps;
+    const int ntiles = (cs.n + TN - 1) / TN;
+    if (ct_end > ntiles) ct_end = ntiles;
+
+    // per-lane row partial sums: rows wr*32 + mb*16 + 4*(lane/16) + r
+    double rsum[2][4];
+    double nr[2][4];
+    for (int mb = 0; mb < 2; mb++)
+        for (int r = 0; r < 4; r++) {
+            rsum[mb][r] = 0.0;
+            const int gi = r0 + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+            nr[mb][r] = (gi < rs.n) ? row_nrm(rs, gi) : 0.0;
+        }
+    for (int ct = ct_begin; ct < ct_end; ct++) {
+        const int c0 = ct * TN;
+        v4d acc[2][2];
+        tile_dots(acc, As, Bs, rs, cs, r0, c0, d);
+        for (int nb = 0; nb < 2; nb++) {
+            const int gj = c0 + wc * 32 + nb * 16 + (lane & 15);
+            if (gj < cs.n) {
+                const double wj = w[gj];
+                const double ncj = row_nrm(cs, gj);
+                for (int mb = 0; mb < 2; mb++)
+                    for (int r = 0; r < 4; r++)
+                        rsum[mb][r] += wj * kfun(kernel, gamma, acc[mb][nb][r], nr[mb][r], ncj);
+            }
+        }
+    }
+    // reduce over the 16 lanes that share rows (lane % 16 differs)
+    for (int mb = 0; mb < 2; mb++)
+        for (int r = 0; r < 4; r++) {
+            double v = rsum[mb][r];
+            v += __shfl_xor(v, 1);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 4);
+            v += __shfl_xor(v, 8);
+            rsum[mb][r] = v;
+        }
+    if ((lane & 15) == 0) {
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) red[wc][wr * 32 + mb * 16 + 4 * (lane >> 4) + r] = rsum[mb][r];
+    }
+    __syncthreads();
+    if (threadIdx.x < TM) {
+        const int i = threadIdx.x;
+        const int gi = r0 + i;
+        if (gi < rs.n) {
+            const double s = red[0][i] + red[1][i];
+            if (nsplit > 1) {
+                part[(size_t)blockIdx.y * rs.n + gi] = s;
+            } else {
+                const double sc = rsc ? rsc[gi] : 1.0;
+                out[gi] = (a == 0.0 ? 0.0 : a * out[gi]) + sc * (b * s) + cst;
+            }
+        }
+    }
+}
+
+// Materialised kernel block K[i*ldk + j].
+__global__ __launch_bounds__(256) void kmat_dense_kernel(DenseSide rs, DenseSide cs, int d, int kernel,
+                                                         double gamma, double *__restrict__ K, int ldk) {
+    __shared__ double As[2][TM][LDP];
+    __shared__ double Bs[2][TN][LDP];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int r0 = blockIdx.x * TM;
+    const int c0 = blockIdx.y * TN;
+    v4d acc[2][2];
+    tile_dots(acc, As, Bs, rs, cs, r0, c0, d);
+    for (int nb = 0; nb < 2; nb++) {
+        const int gj = c0 + wc * 32 + nb * 16 + (lane & 15);
+        if (gj >= cs.n) continue;
+        const double ncj = row_nrm(cs, gj);
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) {
+                const int gi = r0 + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+                if (gi < rs.n)
+                    K[(size_t)gi * ldk + gj] = kfun(kernel, gamma, acc[mb][nb][r], row_nrm(rs, gi), ncj);
+            }
+    }
+}
+
+// Deterministic split reduction + output epilogue.
+__global__ void kmv_finish_kernel(const double *__restrict__ part, int n, int nsplit, double a, double b,
+                                  double cst, const double *__restrict__ rsc, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; k++) s += part[(size_t)k * n + i];
+    const double sc = rsc ? rsc[i] : 1.0;
+    out[i] = (a == 0.0 ? 0.0 : a * out[i]) + sc * (b * s) + cst;
+}
+
+// ---- CSR rows x transposed dense columns ---------------------------------
+// One wave per row; lanes over 64 columns per chunk.  XcT is d x ldc.
+__global__ __launch_bounds__(256) void kmv_csr_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
+    const double *__restrict__ XcT, int ldc, int d, const double *__restrict__ cnrm, int ncols,
+    int kernel, double gamma, const double *__restrict__ w, int rows_per_block,
+    double a, double b, double cst, const double *__restrict__ rsc, double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int rbeg = blockIdx.x * rows_per_block;
+    const int rend = min(nrows, rbeg + rows_per_block);
+    for (int i = rbeg + wave; i < rend; i += 4) {
+        const int src = ridx ? ridx[i] : i;
+        const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+        const double nr = rnrm ? rnrm[src] : 0.0;
+        double acc = 0.0;
+        for (int c0 = 0; c0 < ncols; c0 += 64) {
+            const int j = c0 + lane;
+            if (j < ncols) {
+                double dot = 0.0;
+                for (int64_t p = p0; p < p1; p++) {
+                    const int f = col[p];
+                    if (f < d) dot += val[p] * XcT[(size_t)f * ldc + j];
+                }
+                acc += w[j] * kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) {
+            const double sc = rsc ? rsc[i] : 1.0;
+            out[i] = (a == 0.0 ? 0.0 : a * out[i]) + sc * (b * acc) + cst;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kmat_csr_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
+    const double *__restrict__ XcT, int ldc, int d, const double *__restrict__ cnrm, int ncols,
+    int kernel, double gamma, int rows_per_block, double *__restrict__ K, int ldk) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int rbeg = blockIdx.x * rows_per_block;
+    const int rend = min(nrows, rbeg + rows_per_block);
+    for (int i = rbeg + wave; i < rend; i += 4) {
+        const int src = ridx ? ridx[i] : i;
+        const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+        const double nr = rnrm ? rnrm[src] : 0.0;
+        for (int c0 = 0; c0 < ncols; c0 += 64) {
+            const int j = c0 + lane;
+            if (j < ncols) {
+                double dot = 0.0;
+                for (int64_t p = p0; p < p1; p++) {
+                    const int f = col[p];
+                    if (f < d) dot += val[p] * XcT[(size_t)f * ldc + j];
+                }
+                K[(size_t)i * ldk + j] = kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
+            }
+        }
+    }
+}
+
+// Column block as a transposed dense matrix XcT (d x nc) + compact norms.
+__global__ void cols_T_from_dense(const double *__restrict__ X, int ldx, const int32_t *__restrict__ idx,
+                                  const double *__restrict__ nrm, int nc, int d, double *__restrict__ XcT,
+                                  double *__restrict__ cn) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (j >= nc) return;
+    const int src = idx ? idx[j] : j;
+    XcT[(size_t)f * nc + j] = X[(size_t)src * ldx + f];
+    if (f == 0) cn[j] = nrm ? nrm[src] : 0.0;
+}
+
+// Scatter CSR rows into a zero-filled dense block: transposed (d x nc) when
+// trans != 0, else row-major (nc x d).
+__global__ void cols_from_csr(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                              const double *__restrict__ val, const int32_t *__restrict__ idx,
+                              const double *__restrict__ nrm, int nc, int d, int trans,
+                              double *__restrict__ out, double *__restrict__ cn) {
+    const int j = blockIdx.x;
+    if (j >= nc) return;
+    const int src = idx ? idx[j] : j;
+    const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int f = col[p];
+        if (f < d) {
+            if (trans) out[(size_t)f * nc + j] = val[p];
+            else out[(size_t)j * d + f] = val[p];
+        }
+    }
+    if (threadIdx.x == 0) cn[j] = nrm ? nrm[src] : 0.0;
+}
+
+__global__ void row_norms_kernel(const double *__restrict__ X, int ldx, int n, int d, double *__restrict__ nrm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *p = X + (size_t)i * ldx;
+    // separately rounded multiply and add (no FMA contraction), the order and
+    // rounding of the readers' running sum of squares
+    double s = 0.0;
+    for (int k = 0; k < d; k++) s = __dadd_rn(s, __dmul_rn(p[k], p[k]));
+    nrm[i] = s;
+}
+
+__global__ void axpb_kernel(int n, double a, double cst, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (a == 0.0 ? 0.0 : a * out[i]) + cst;
+}
+
+DenseSide dense_side(const irwls_rows *r) {
+    DenseSide s;
+    s.X = r->X;
+    s.ldx = r->ldx;
+    s.idx = r->idx;
+    s.nrm = r->nrm;
+    s.n = r->n;
+    return s;
+}
+
+// Columns given as CSR rows are scattered into a dense row-major block so the
+// MFMA path can use them.  Returns a DenseSide over scratch memory.
+DenseSide dense_cols(const irwls_rows *c, int d, int slot_vals, int slot_nrm) {
+    if (c->X) return dense_side(c);
+    double *buf = (double *)scratch(slot_vals, (size_t)c->n * d * sizeof(double) + 8);
+    double *cn = (double *)scratch(slot_nrm, (size_t)c->n * sizeof(double) + 8);
+    irwls_dev_memset0(buf, (size_t)c->n * d * sizeof(double));
+    if (c->n > 0) {
+        hipLaunchKernelGGL(cols_from_csr, dim3(c->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val,
+                           c->idx, c->nrm, c->n, d, 0, buf, cn);
+        IRWLS_LAUNCH_CHECK();
+    }
+    DenseSide s;
+    s.X = buf;
+    s.ldx = d;
+    s.idx = nullptr;
+    s.nrm = cn;
+    s.n = c->n;
+    return s;
+}
+
+// Transposed dense column block for the CSR row kernels.
+void cols_T(const irwls_rows *c, int d, double **XcT, double **cn) {
+    *XcT = (double *)scratch(4, (size_t)c->n * d * sizeof(double) + 8);
+    *cn = (double *)scratch(5, (size_t)c->n * sizeof(double) + 8);
+    if (c->n == 0) return;
+    if (c->X) {
+        hipLaunchKernelGGL(cols_T_from_dense, dim3((c->n + 255) / 256, d), dim3(256), 0, stream(), c->X,
+                           c->ldx, c->idx, c->nrm, c->n, d, *XcT, *cn);
+    } else {
+        irwls_dev_memset0(*XcT, (size_t)c->n * d * sizeof(double));
+        hipLaunchKernelGGL(cols_from_csr, dim3(c->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val,
+                           c->idx, c->nrm, c->n, d, 1, *XcT, *cn);
+    }
+    IRWLS_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+extern "C" {
+
+void irwls_dev_row_norms(const double *X, int ldx, int n, int d, double *nrm) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(row_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), X, ldx, n, d, nrm);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, int kernel, double gamma,
+                             const double *w, double a, double b, double cst, const double *rs, double *out) {
+    if (r->n <= 0) return;
+    if (c->n <= 0) {
+        hipLaunchKernelGGL(axpb_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->n, a, cst, out);
+        IRWLS_LAUNCH_CHECK();
+        return;
+    }
+    const double flops = 2.0 * r->n * (double)c->n * d;
+    if (r->X) {
+        DenseSide rsd = dense_side(r);
+        DenseSide csd = dense_cols(c, d, 6, 7);
+        const int rt = (r->n + TM - 1) / TM;
+        const int ct = (c->n + TN - 1) / TN;
+        // enough work groups to fill 256 CUs a few times over
+        int nsplit = 1;
+        while (rt * nsplit < 1024 && nsplit * 2 <= ct) nsplit *= 2;
+        const int tps = (ct + nsplit - 1) / nsplit;
+        nsplit = (ct + tps - 1) / tps;
+        double *part = nullptr;
+        if (nsplit > 1) part = (double *)scratch(8, (size_t)nsplit * r->n * sizeof(double));
+        const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + 2.0 * r->n);
+        TimerScope ts(IRWLS_T_KMV, flops, bytes);
+        hipLaunchKernelGGL(kmv_dense_kernel, dim3(rt, nsplit), dim3(256), 0, stream(), rsd, csd, d, kernel,
+                           gamma, w, tps, nsplit, part, a, b, cst, rs, out);
+        IRWLS_LAUNCH_CHECK();
+        if (nsplit > 1) {
+            hipLaunchKernelGGL(kmv_finish_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), part, r->n,
+                               nsplit, a, b, cst, rs, out);
+            IRWLS_LAUNCH_CHECK();
+        }
+    } else {
+        double *XcT, *cn;
+        cols_T(c, d, &XcT, &cn);
+        const int rpb = 16;
+        const double bytes = 8.0 * ((double)c->n * d + 2.0 * r->n);
+        TimerScope ts(IRWLS_T_CSR, flops, bytes);
+        hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + rpb - 1) / rpb), dim3(256), 0, stream(), r->rowptr,
+                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cn, c->n, kernel, gamma, w, rpb, a, b,
+                           cst, rs, out);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, int kernel, double gamma,
+                             double *K, int ldk) {
+    if (r->n <= 0 || c->n <= 0) return;
+    const double flops = 2.0 * r->n * (double)c->n * d;
+    if (r->X) {
+        DenseSide rsd = dense_side(r);
+        DenseSide csd = dense_cols(c, d, 6, 7);
+        const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + (double)r->n * c->n);
+        TimerScope ts(IRWLS_T_KMAT, flops, bytes);
+        hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TM - 1) / TM, (c->n + TN - 1) / TN), dim3(256), 0,
+                           stream(), rsd, csd, d, kernel, gamma, K, ldk);
+        IRWLS_LAUNCH_CHECK();
+    } else {
+        double *XcT, *cn;
+        cols_T(c, d, &XcT, &cn);
+        const int rpb = 16;
+        const double bytes = 8.0 * ((double)c->n * d + (double)r->n * c->n);
+        TimerScope ts(IRWLS_T_CSR, flops, bytes);
+        hipLaunchKernelGGL(kmat_csr_kernel, dim3((r->n + rpb - 1) / rpb), dim3(256), 0, stream(), r->rowptr,
+                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cn, c->n, kernel, gamma, rpb, K, ldk);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Working-set helpers of full IRWLS training on gfx950.
+//
+// The working-set kernel block Kw = K(X_SW, X_SW) is materialised once per
+// outer iteration; every kernel value the inner IRWLS needs (the KKT matrix,
+// the inner error update, G13) is a gather from it, so the inner loop is
+// memory-bound gathers instead of kernel re-evaluation.
+// Reference loops replaced: src/full-train.c:169-186 (H), :236-249 (inner e),
+// :313-344 (G13).
+#include "common.h"
+
+using namespace irwls;
+
+namespace {
+
+// H column-major N x N (symmetric), N = nS1 + 1.
+__global__ void build_kkt_kernel(const double *__restrict__ Kw, int ldk, const int32_t *__restrict__ S1, int nS1,
+                                 const double *__restrict__ ys, const double *__restrict__ as,
+                                 double *__restrict__ H) {
+    const int N = nS1 + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // row (contiguous)
+    const int j = blockIdx.y;                               // column
+    if (i >= N) return;
+    double v;
+    if (i < nS1 && j < nS1) {
+        v = Kw[(size_t)S1[i] * ldk + S1[j]] * ys[i] * ys[j];
+        if (i == j) v += 1.0 / as[i];
+    } else if (i < nS1) {
+        v = ys[i];          // last column: y border
+    } else if (j < nS1) {
+        v = ys[j];          // last row: y border
+    } else {
+        v = 0.0;
+    }
+    H[(size_t)j * N + i] = v;
+}
+
+// One wave per output row; lanes stride the gathered columns.
+__global__ __launch_bounds__(256) void gemv_gather_kernel(
+    const double *__restrict__ Kw, int ldk, const int32_t *__restrict__ rows, int nr,
+    const int32_t *__restrict__ cols, int nc, const double *__restrict__ w, double a, double b, double cst,
+    const double *__restrict__ rsc, double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nr) return;
+    const int ri = rows ? rows[i] : i;
+    const double *krow = Kw + (size_t)ri * ldk;
+    double acc = 0.0;
+    for (int j = lane; j < nc; j += 64) acc += krow[cols ? cols[j] : j] * w[j];
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        const double sc = rsc ? rsc[i] : 1.0;
+        out[i] = (a == 0.0 ? 0.0 : a * out[i]) + sc * (b * acc) + cst;
+    }
+}
+
+// y = A x (trans 0) or A^T x (trans 1) for a column-major m x n matrix A.
+__global__ __launch_bounds__(256) void gemv_cm_kernel(const double *__restrict__ A, int m, int n, int lda, int trans,
+                                                      const double *__restrict__ x, double *__restrict__ y) {
+    if (!trans) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= m) return;
+        double acc = 0.0;
+        for (int j = 0; j < n; j++) acc += A[(size_t)j * lda + i] * x[j];
+        y[i] = acc;
+    } else {
+        const int lane = threadIdx.x & 63;
+        const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+        if (j >= n) return;
+        const double *col = A + (size_t)j * lda;
+        double acc = 0.0;
+        for (int i = lane; i < m; i += 64) acc += col[i] * x[i];
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) y[j] = acc;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void irwls_dev_gemv(const double *A, int m, int n, int lda, int trans, const double *x, double *y) {
+    if (m <= 0 || n <= 0) return;
+    TimerScope ts(IRWLS_T_GEMV, 2.0 * m * (double)n, 8.0 * m * (double)n);
+    if (!trans)
+        hipLaunchKernelGGL(gemv_cm_kernel, dim3((m + 255) / 256), dim3(256), 0, stream(), A, m, n, lda, 0, x, y);
+    else
+        hipLaunchKernelGGL(gemv_cm_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), A, m, n, lda, 1, x, y);
+    IRWLS_LAUNCH_CHECK();
+}
+
+
+void irwls_dev_build_kkt(const double *Kw, int ldk, const int32_t *S1, int nS1, const double *ys, const double *as,
+                         double *H) {
+    const int N = nS1 + 1;
+    TimerScope ts(IRWLS_T_KKT, 0.0, 8.0 * (double)N * N * 2.0);
+    hipLaunchKernelGGL(build_kkt_kernel, dim3((N + 255) / 256, N), dim3(256), 0, stream(), Kw, ldk, S1, nS1, ys, as,
+                       H);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_gemv_gather(const double *Kw, int ldk, const int32_t *rows, int nr, const int32_t *cols, int nc,
+                           const double *w, double a, double b, double cst, const double *rs, double *out) {
+    if (nr <= 0) return;
+    TimerScope ts(IRWLS_T_GEMV, 2.0 * nr * (double)nc, 8.0 * nr * (double)nc);
+    hipLaunchKernelGGL(gemv_gather_kernel, dim3((nr + 3) / 4), dim3(256), 0, stream(), Kw, ldk, rows, nr, cols, nc,
+                       w, a, b, cst, rs, out);
+    IRWLS_LAUNCH_CHECK();
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+/*
+ * Device residency of datasets, options, timers and small host utilities.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdio.h>
+#include <string.h>
+#include <strings.h>
+#include <time.h>
+#include "internal.h"
+
+double irwls_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+void *irwls_xmalloc(size_t bytes) {
+    void *p = malloc(bytes ? bytes : 1);
+    if (!p) {
+        fprintf(stderr, "libirwls: out of host memory (%zu bytes)\n", bytes);
+        exit(2);
+    }
+    return p;
+}
+
+void *irwls_xcalloc(size_t n, size_t sz) {
+    void *p = calloc(n ? n : 1, sz ? sz : 1);
+    if (!p) {
+        fprintf(stderr, "libirwls: out of host memory (%zu x %zu bytes)\n", n, sz);
+        exit(2);
+    }
+    return p;
+}
+
+/* ---- options ----------------------------------------------------------- */
+static irwls_gpu_options g_opts;
+static int g_opts_init = 0;
+
+static void opts_defaults(void) {
+    if (g_opts_init) return;
+    g_opts.precision = 0;
+    g_opts.sgma_candidates = 64;
+    g_opts.proposal = 0;
+    g_opts.proposal_keep = 8;
+    g_opts.refine_tol = 1e-10;
+    const char *p = getenv("IRWLS_PRECISION");
+    if (p && strcasecmp(p, "mixed") == 0) g_opts.precision = 1;
+    const char *c = getenv("IRWLS_SGMA_CAND");
+    if (c && atoi(c) > 0) g_opts.sgma_candidates = atoi(c);
+    const char *q = getenv("IRWLS_PROPOSAL");
+    if (q && strcasecmp(q, "learned") == 0) g_opts.proposal = 1;
+    g_opts_init = 1;
+}
+
+const irwls_gpu_options *irwls_opts(void) {
+    opts_defaults();
+    return &g_opts;
+}
+
+void irwls_gpu_get_options(irwls_gpu_options *out) {
+    opts_defaults();
+    *out = g_opts;
+}
+
+void irwls_gpu_set_options(const irwls_gpu_options *o) {
+    opts_defaults();
+    g_opts = *o;
+    if (g_opts.sgma_candidates < 1) g_opts.sgma_candidates = 1;
+    if (g_opts.sgma_candidates > 1024) g_opts.sgma_candidates = 1024;
+    if (g_opts.proposal_keep < 1) g_opts.proposal_keep = 1;
+}
+
+void irwls_gpu_timing(int enable) { irwls_dev_timing(enable); }
+
+void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double *bytes) {
+    irwls_dev_timer_get(kind, launches, ms, flops, bytes);
+}
+
+/* ---- residency --------------------------------------------------------- */
+#define MAX_PINNED 16
+static irwls_devdata *g_pinned[MAX_PINNED];
+
+int irwls_dataset_base(const svm_dataset *ds, int rows) {
+    for (int i = 0; i < rows; i++)
+        for (const svm_sample *s = ds->x[i]; s->index != -1; s++)
+            if (s->index < 1) return 0;
+    return 1;
+}
+
+int irwls_dataset_maxindex(const svm_dataset *ds, int rows) {
+    int m = 0;
+    for (int i = 0; i < rows; i++)
+        for (const svm_sample *s = ds->x[i]; s->index != -1; s++)
+            if (s->index > m) m = s->index;
+    return m;
+}
+
+static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, int min_cols) {
+    irwls_devdata *dd = (irwls_devdata *)irwls_xcalloc(1, sizeof(irwls_devdata));
+    dd->key = ds->features;
+    dd->l = ds->l;
+    dd->rows = ds->l + (with_means ? 2 : 0);
+    dd->dense = (ds->sparse == 0);
+    const int rows = dd->rows;
+    if (base < 0) base = irwls_dataset_base(ds, rows);
+    dd->base = base;
+    int d = irwls_dataset_maxindex(ds, rows) + 1 - base;
+    if (d < min_cols) d = min_cols;
+    if (d < 1) d = 1;
+    dd->d = d;
+    const size_t rr = (size_t)(rows > 0 ? rows : 1);
+    double *hy = (double *)irwls_xcalloc(rr, sizeof(double));
+    for (int i = 0; i < rows; i++) hy[i] = ds->y ? ds->y[i] : 0.0;
+    dd->y = (double *)irwls_dev_alloc(rr * sizeof(double));
+    irwls_dev_h2d(dd->y, hy, (size_t)rows * sizeof(double));
+    free(hy);
+    dd->qv = (double *)irwls_dev_alloc(rr * sizeof(double));
+    if (ds->quadratic_value) irwls_dev_h2d(dd->qv, ds->quadratic_value, (size_t)rows * sizeof(double));
+    if (dd->dense) {
+        double *hx = (double *)irwls_xcalloc((size_t)rows * d, sizeof(double));
+        for (int i = 0; i < rows; i++)
+            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
+                const int c = s->index - base;
+                if (c >= 0 && c < d) hx[(size_t)i * d + c] = s->value;
+            }
+        dd->X = (double *)irwls_dev_alloc((size_t)rows * d * sizeof(double));
+        irwls_dev_h2d(dd->X, hx, (size_t)rows * d * sizeof(double));
+        free(hx);
+        dd->nrm = (double *)irwls_dev_alloc(rr * sizeof(double));
+        irwls_dev_row_norms(dd->X, d, rows, d, dd->nrm);
+    } else {
+        int64_t nnz = 0;
+        for (int i = 0; i < rows; i++)
+            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) nnz++;
+        int64_t *rp = (int64_t *)irwls_xmalloc((size_t)(rows + 1) * sizeof(int64_t));
+        int32_t *cl = (int32_t *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+        double *vl = (double *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+        int64_t k = 0;
+        rp[0] = 0;
+        for (int i = 0; i < rows; i++) {
+            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
+                cl[k] = s->index - base;
+                vl[k] = s->value;
+                k++;
+            }
+            rp[i + 1] = k;
+        }
+        dd->rowptr = (int64_t *)irwls_dev_alloc((size_t)(rows + 1) * sizeof(int64_t));
+        dd->col = (int32_t *)irwls_dev_alloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+        dd->val = (double *)irwls_dev_alloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+        irwls_dev_h2d(dd->rowptr, rp, (size_t)(rows + 1) * sizeof(int64_t));
+        irwls_dev_h2d(dd->col, cl, (size_t)nnz * sizeof(int32_t));
+        irwls_dev_h2d(dd->val, vl, (size_t)nnz * sizeof(double));
+        free(rp);
+        free(cl);
+        free(vl);
+        dd->nrm = dd->qv;
+    }
+    return dd;
+}
+
+static void destroy(irwls_devdata *dd) {
+    irwls_dev_free(dd->X);
+    irwls_dev_free(dd->rowptr);
+    irwls_dev_free(dd->col);
+    irwls_dev_free(dd->val);
+    if (dd->nrm != dd->qv) irwls_dev_free(dd->nrm);
+    irwls_dev_free(dd->qv);
+    irwls_dev_free(dd->y);
+    free(dd);
+}
+
+irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols) {
+    irwls_dev_init();
+    for (int i = 0; i < MAX_PINNED; i++) {
+        irwls_devdata *p = g_pinned[i];
+        if (p && p->key == ds->features && p->l == ds->l && p->dense == (ds->sparse == 0) &&
+            (p->rows >= ds->l + (with_means ? 2 : 0)) && (base < 0 || p->base == base) && p->d >= min_cols)
+            return p;
+    }
+    return upload(ds, with_means, base, min_cols);
+}
+
+void irwls_devdata_release(irwls_devdata *dd) {
+    if (!dd || dd->pinned) return;
+    destroy(dd);
+}
+
+irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int n) {
+    irwls_rows r;
+    memset(&r, 0, sizeof(r));
+    if (dd->dense) {
+        r.X = dd->X;
+        r.ldx = dd->d;
+    } else {
+        r.rowptr = dd->rowptr;
+        r.col = dd->col;
+        r.val = dd->val;
+    }
+    r.nrm = dd->nrm;
+    r.idx = d_idx;
+    r.n = n;
+    return r;
+}
+
+void irwls_gpu_pin_dataset(svm_dataset ds, int with_means) {
+    irwls_dev_init();
+    for (int i = 0; i < MAX_PINNED; i++)
+        if (g_pinned[i] && g_pinned[i]->key == ds.features) return;
+    for (int i = 0; i < MAX_PINNED; i++) {
+        if (!g_pinned[i]) {
+            g_pinned[i] = upload(&ds, with_means, -1, 0);
+            g_pinned[i]->pinned = 1;
+            return;
+        }
+    }
+    fprintf(stderr, "libirwls: more than %d pinned datasets\n", MAX_PINNED);
+    exit(2);
+}
+
+void irwls_gpu_unpin_dataset(svm_dataset ds) {
+    for (int i = 0; i < MAX_PINNED; i++) {
+        if (g_pinned[i] && g_pinned[i]->key == ds.features) {
+            destroy(g_pinned[i]);
+            g_pinned[i] = NULL;
+        }
+    }
+}
+
+void irwls_gpu_shutdown(void) {
+    for (int i = 0; i < MAX_PINNED; i++) {
+        if (g_pinned[i]) {
+            destroy(g_pinned[i]);
+            g_pinned[i] = NULL;
+        }
+    }
+    irwls_dev_shutdown();
+}
+
+/* ---- working-set buffers ------------------------------------------------ */
+void irwls_ws_reserve(irwls_ws *ws, int cap) {
+    if (ws->cap >= cap) return;
+    irwls_ws_free(ws);
+    const size_t c = (size_t)(cap > 0 ? cap : 1);
+    ws->cap = (int)c;
+    ws->Kw = (double *)irwls_dev_alloc(c * c * sizeof(double));
+    ws->H = (double *)irwls_dev_alloc((c + 1) * (c + 1) * sizeof(double));
+    ws->rhs = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->esub = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->w = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->G13 = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->ys = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->as = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->rs = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->S1 = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
+    ws->S3 = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
+    ws->cols = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
+}
+
+void irwls_ws_free(irwls_ws *ws) {
+    if (ws->cap == 0) return;
+    irwls_dev_free(ws->Kw);
+    irwls_dev_free(ws->H);
+    irwls_dev_free(ws->rhs);
+    irwls_dev_free(ws->esub);
+    irwls_dev_free(ws->w);
+    irwls_dev_free(ws->G13);
+    irwls_dev_free(ws->ys);
+    irwls_dev_free(ws->as);
+    irwls_dev_free(ws->rs);
+    irwls_dev_free(ws->S1);
+    irwls_dev_free(ws->S3);
+    irwls_dev_free(ws->cols);
+    memset(ws, 0, sizeof(*ws));
+}

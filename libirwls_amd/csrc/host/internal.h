@@ -1,0 +1,81 @@
+/*
+ * Internal declarations shared by the C host files of libirwls.
+ */
+#ifndef IRWLS_HOST_INTERNAL_H
+#define IRWLS_HOST_INTERNAL_H
+
+#include <stdint.h>
+#include <stdlib.h>
+#include "IOStructures.h"
+#include "irwls_device.h"
+#include "irwls_gpu.h"
+
+/* A dataset resident in device memory.  Feature index f maps to column
+   f - base (base 1 when every index is >= 1, the libsvm convention, else 0).
+   Dense (svm_dataset.sparse == 0): X is rows x d row-major.  For well-formed
+   dense rows (every row lists the same indices 1..maxdim) this equals the
+   reference's positional walk (src/kernels.c:90-97); nrm = |x|^2 computed on
+   the device in feature order.
+   CSR (sparse == 1): rowptr/col/val; nrm = the dataset's stored
+   quadratic_value, which the reference's sparse kernel uses
+   (src/kernels.c:101), including the class-mean rows' stored norms.
+   qv always holds the stored quadratic_value (prediction uses it,
+   src/kernels.c:163). */
+typedef struct irwls_devdata {
+    const svm_sample *key;     /* dataset.features, identifies the dataset */
+    int l;                     /* labelled rows */
+    int rows;                  /* uploaded rows (l, or l+2 with the mean rows) */
+    int d;                     /* column count of the device layout */
+    int dense;
+    int base;                  /* feature index of column 0 */
+    double *X;
+    int64_t *rowptr;
+    int32_t *col;
+    double *val;
+    double *nrm;
+    double *qv;
+    double *y;
+    int pinned;                /* kept resident between calls */
+} irwls_devdata;
+
+/* 1 when every feature index of rows [0, rows) is >= 1, else 0. */
+int irwls_dataset_base(const svm_dataset *ds, int rows);
+/* Largest feature index of rows [0, rows). */
+int irwls_dataset_maxindex(const svm_dataset *ds, int rows);
+
+/* Upload (or find the pinned copy of) a dataset.  with_means asks for the two
+   class-mean rows l and l+1 as well; base as above (-1 = derive it);
+   min_cols widens the column space (prediction aligns data and model). */
+irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols);
+void irwls_devdata_release(irwls_devdata *dd);
+/* Row view of the whole device dataset (idx may select rows). */
+irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int n);
+
+/* Current options (environment defaults merged with irwls_gpu_set_options). */
+const irwls_gpu_options *irwls_opts(void);
+
+/* Host wall clock in seconds. */
+double irwls_now(void);
+
+/* Device work buffers of the full trainer's working-set IRWLS. */
+typedef struct irwls_ws {
+    int cap;                   /* capacity in working-set rows */
+    double *Kw, *H, *rhs, *esub, *w, *G13, *ys, *as, *rs;
+    int32_t *S1, *S3, *cols;
+} irwls_ws;
+
+void irwls_ws_reserve(irwls_ws *ws, int cap);
+void irwls_ws_free(irwls_ws *ws);
+
+/* Inner IRWLS on a working set whose rows are `wrows` (device view).  Same
+   contract as subIRWLS (src/full-train.c:96-362): y = labels of the working
+   set (host), GIN has l+1 entries, e and beta (l and l+1 entries) are updated
+   in place; returns betaBest (malloc, l+1). */
+double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l, const double *y,
+                            properties props, const double *GIN, double *e, double *beta);
+
+/* libc-compatible allocation with exit(2) on failure. */
+void *irwls_xmalloc(size_t bytes);
+void *irwls_xcalloc(size_t n, size_t sz);
+
+#endif

@@ -1,0 +1,388 @@
+/*
+ * Data model and I/O: dataset readers, in-memory builders, the binary model
+ * file and the prediction text output.
+ *
+ * Semantics follow the reference readers and writers
+ * (src/IOStructures.c:53-848): rows end with index -1, |x|^2 is a running sum
+ * of squares in file order, `sparse` is set when the k-th index of a row
+ * differs from the k-th entry of the feature array, and the labelled readers
+ * append the positive-class and negative-class mean rows l and l+1 (the
+ * negative mean's norm uses the positive count, src/IOStructures.c:259,477,
+ * kept because budgeted models store that norm).  The model file is the
+ * reference's raw host-endian dump, byte for byte.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <ctype.h>
+#include <errno.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <string.h>
+#include "internal.h"
+
+/* struct layouts every existing caller and model file relies on */
+_Static_assert(sizeof(svm_sample) == 16, "svm_sample layout");
+_Static_assert(offsetof(svm_sample, value) == 8, "svm_sample layout");
+_Static_assert(sizeof(svm_dataset) == 48, "svm_dataset layout");
+_Static_assert(sizeof(properties) == 72, "properties layout");
+_Static_assert(sizeof(predictProperties) == 32, "predictProperties layout");
+_Static_assert(sizeof(model) == 72, "model layout");
+
+void freeDataset(svm_dataset data) {
+    free(data.y);
+    free(data.quadratic_value);
+    free(data.x);
+    free(data.features);
+}
+
+void freeModel(model modelo) {
+    free(modelo.weights);
+    free(modelo.quadratic_value);
+    free(modelo.x);
+    free(modelo.features);
+}
+
+static void bad_format(void) {
+    fprintf(stderr, "Wrong file format\n");
+    exit(2);
+}
+
+/* Row store used while parsing: CSR with the reference's 1-based indices. */
+typedef struct rowbuf {
+    int64_t *rowptr;
+    int32_t *idx;
+    double *val;
+    double *y;
+    int n, ncap;
+    int64_t nnz, zcap;
+} rowbuf;
+
+static void rb_init(rowbuf *b) {
+    memset(b, 0, sizeof(*b));
+    b->ncap = 1024;
+    b->zcap = 16384;
+    b->rowptr = (int64_t *)irwls_xmalloc((size_t)(b->ncap + 1) * sizeof(int64_t));
+    b->y = (double *)irwls_xmalloc((size_t)b->ncap * sizeof(double));
+    b->idx = (int32_t *)irwls_xmalloc((size_t)b->zcap * sizeof(int32_t));
+    b->val = (double *)irwls_xmalloc((size_t)b->zcap * sizeof(double));
+    b->rowptr[0] = 0;
+}
+
+static void rb_entry(rowbuf *b, int index, double v) {
+    if (b->nnz == b->zcap) {
+        b->zcap *= 2;
+        b->idx = (int32_t *)realloc(b->idx, (size_t)b->zcap * sizeof(int32_t));
+        b->val = (double *)realloc(b->val, (size_t)b->zcap * sizeof(double));
+        if (!b->idx || !b->val) {
+            fprintf(stderr, "libirwls: out of host memory\n");
+            exit(2);
+        }
+    }
+    b->idx[b->nnz] = index;
+    b->val[b->nnz] = v;
+    b->nnz++;
+}
+
+static void rb_endrow(rowbuf *b, double label) {
+    if (b->n == b->ncap) {
+        b->ncap *= 2;
+        b->rowptr = (int64_t *)realloc(b->rowptr, (size_t)(b->ncap + 1) * sizeof(int64_t));
+        b->y = (double *)realloc(b->y, (size_t)b->ncap * sizeof(double));
+        if (!b->rowptr || !b->y) {
+            fprintf(stderr, "libirwls: out of host memory\n");
+            exit(2);
+        }
+    }
+    b->y[b->n] = label;
+    b->n++;
+    b->rowptr[b->n] = b->nnz;
+}
+
+static void rb_free(rowbuf *b) {
+    free(b->rowptr);
+    free(b->idx);
+    free(b->val);
+    free(b->y);
+}
+
+/* Dataset from parsed rows with the reader semantics; with_means appends the
+   two class-mean rows (labelled readers). */
+static svm_dataset assemble(const int64_t *rowptr, const int32_t *idx, const double *val, const double *y, int n,
+                            int with_means) {
+    svm_dataset ds;
+    memset(&ds, 0, sizeof(ds));
+    int maxidx = 0;
+    const int64_t nnz = rowptr[n];
+    for (int64_t k = 0; k < nnz; k++)
+        if (idx[k] > maxidx) maxidx = idx[k];
+    const int extra = with_means ? 2 : 0;
+    const int64_t elements = nnz + n + (with_means ? 2 * (int64_t)(maxidx + 2) : 0);
+    ds.l = n;
+    ds.y = (double *)irwls_xcalloc((size_t)n + extra, sizeof(double));
+    ds.quadratic_value = (double *)irwls_xcalloc((size_t)n + extra, sizeof(double));
+    ds.x = (svm_sample **)irwls_xcalloc((size_t)n + extra, sizeof(svm_sample *));
+    ds.features = (svm_sample *)irwls_xcalloc((size_t)elements, sizeof(svm_sample));
+    double *mp = with_means ? (double *)irwls_xcalloc((size_t)maxidx + 1, sizeof(double)) : NULL;
+    double *mn = with_means ? (double *)irwls_xcalloc((size_t)maxidx + 1, sizeof(double)) : NULL;
+    double cpos = 0.0, cneg = 0.0;
+    int64_t j = 0;
+    for (int i = 0; i < n; i++) {
+        ds.x[i] = &ds.features[j];
+        ds.y[i] = y ? y[i] : 0.0;
+        if (ds.y[i] == 1.0) cpos += 1;
+        else cneg += 1;
+        int64_t pos = 0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++, pos++, j++) {
+            svm_sample *f = &ds.features[j];
+            f->index = idx[k];
+            f->value = val[k];
+            if (ds.features[pos].index != f->index) ds.sparse = 1;
+            if (with_means) {
+                if (ds.y[i] == 1.0) mp[f->index] += f->value;
+                else mn[f->index] += f->value;
+            }
+            ds.quadratic_value[i] += f->value * f->value;
+        }
+        ds.features[j++].index = -1;
+    }
+    if (with_means) {
+        ds.y[n] = 1.0;
+        ds.x[n] = &ds.features[j];
+        for (int f = 0; f <= maxidx; f++) {
+            if (mp[f] == 0.0) continue;
+            const double v = mp[f] / cpos;
+            ds.features[j].index = f;
+            ds.features[j].value = v;
+            ds.quadratic_value[n] += v * v;
+            j++;
+        }
+        ds.features[j++].index = -1;
+        ds.y[n + 1] = -1.0;
+        ds.x[n + 1] = &ds.features[j];
+        for (int f = 0; f <= maxidx; f++) {
+            if (mn[f] == 0.0) continue;
+            const double skew = mn[f] / cpos;   /* norm uses the positive count */
+            ds.features[j].index = f;
+            ds.features[j].value = mn[f] / cneg;
+            ds.quadratic_value[n + 1] += skew * skew;
+            j++;
+        }
+        ds.features[j++].index = -1;
+        free(mp);
+        free(mn);
+    }
+    ds.maxdim = maxidx;
+    return ds;
+}
+
+svm_dataset irwls_dataset_from_dense(const double *X, const double *y, int n, int d) {
+    int64_t *rp = (int64_t *)irwls_xmalloc((size_t)(n + 1) * sizeof(int64_t));
+    int32_t *ix = (int32_t *)irwls_xmalloc((size_t)n * d * sizeof(int32_t) + 4);
+    for (int i = 0; i <= n; i++) rp[i] = (int64_t)i * d;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < d; k++) ix[(size_t)i * d + k] = k + 1;
+    svm_dataset ds = assemble(rp, ix, X, y, n, 1);
+    free(rp);
+    free(ix);
+    return ds;
+}
+
+svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, const double *val, const double *y,
+                                   int n) {
+    const int64_t nnz = (int64_t)rowptr[n];
+    int64_t *rp = (int64_t *)irwls_xmalloc((size_t)(n + 1) * sizeof(int64_t));
+    int32_t *ix = (int32_t *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+    for (int i = 0; i <= n; i++) rp[i] = (int64_t)rowptr[i];
+    for (int64_t k = 0; k < nnz; k++) ix[k] = col[k] + 1;
+    svm_dataset ds = assemble(rp, ix, val, y, n, 1);
+    free(rp);
+    free(ix);
+    return ds;
+}
+
+/* ---- text readers --------------------------------------------------------- */
+
+static FILE *open_or_die(const char *filename) {
+    if (filename == NULL) {
+        fprintf(stderr, "File not specified");
+        exit(2);
+    }
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+        fprintf(stderr, "File not found: %s\n", filename);
+        exit(2);
+    }
+    return f;
+}
+
+/* Reads one line of any length; returns NULL at end of file. */
+static char *read_line(FILE *f, char **buf, size_t *cap) {
+    size_t len = 0;
+    if (!*buf) {
+        *cap = 1 << 16;
+        *buf = (char *)irwls_xmalloc(*cap);
+    }
+    for (;;) {
+        if (!fgets(*buf + len, (int)(*cap - len), f)) return len ? *buf : NULL;
+        len += strlen(*buf + len);
+        if (len > 0 && (*buf)[len - 1] == '\n') return *buf;
+        if (len + 1 < *cap) return *buf;   /* last line without newline */
+        *cap *= 2;
+        *buf = (char *)realloc(*buf, *cap);
+        if (!*buf) {
+            fprintf(stderr, "libirwls: out of host memory\n");
+            exit(2);
+        }
+    }
+}
+
+static int blank_line(const char *s) {
+    for (; *s; s++)
+        if (!isspace((unsigned char)*s)) return 0;
+    return 1;
+}
+
+/* libsvm rows "[label] idx:val idx:val ..." */
+static svm_dataset read_libsvm(const char *filename, int labelled) {
+    FILE *f = open_or_die(filename);
+    rowbuf b;
+    rb_init(&b);
+    char *line = NULL;
+    size_t cap = 0;
+    while (read_line(f, &line, &cap)) {
+        if (blank_line(line)) continue;
+        char *p = line;
+        double label = 0.0;
+        if (labelled) {
+            while (*p == ' ' || *p == '\t') p++;
+            char *end;
+            label = strtod(p, &end);
+            if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+            p = end;
+        }
+        int last = -1;
+        for (;;) {
+            while (*p == ' ' || *p == '\t') p++;
+            if (*p == '\0' || *p == '\n' || *p == '\r') break;
+            char *end;
+            errno = 0;
+            long ix = strtol(p, &end, 10);
+            if (end == p || *end != ':' || errno != 0 || ix <= last) bad_format();
+            p = end + 1;
+            double v = strtod(p, &end);
+            if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+            p = end;
+            rb_entry(&b, (int)ix, v);
+            last = (int)ix;
+        }
+        rb_endrow(&b, label);
+    }
+    free(line);
+    fclose(f);
+    svm_dataset ds = assemble(b.rowptr, b.idx, b.val, labelled ? b.y : NULL, b.n, labelled);
+    rb_free(&b);
+    return ds;
+}
+
+/* CSV rows "[label]<sep>v1<sep>v2..."; nonzero values are stored with their
+   1-based column position as index. */
+static svm_dataset read_csv(const char *filename, const char *sep, int labelled) {
+    FILE *f = open_or_die(filename);
+    rowbuf b;
+    rb_init(&b);
+    char *line = NULL;
+    size_t cap = 0;
+    const char *delim = (sep && *sep) ? sep : ",";
+    while (read_line(f, &line, &cap)) {
+        if (blank_line(line)) continue;
+        char *save = NULL;
+        char *tok = strtok_r(line, delim, &save);
+        double label = 0.0;
+        int column = 0;
+        if (labelled) {
+            if (!tok) bad_format();
+            char *end;
+            label = strtod(tok, &end);
+            if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+            tok = strtok_r(NULL, delim, &save);
+        }
+        for (; tok; tok = strtok_r(NULL, delim, &save)) {
+            column++;
+            if (blank_line(tok)) break;
+            char *end;
+            double v = strtod(tok, &end);
+            if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+            if (v != 0.0) rb_entry(&b, column, v);
+        }
+        rb_endrow(&b, label);
+    }
+    free(line);
+    fclose(f);
+    svm_dataset ds = assemble(b.rowptr, b.idx, b.val, labelled ? b.y : NULL, b.n, labelled);
+    rb_free(&b);
+    return ds;
+}
+
+svm_dataset readTrainFile(char filename[]) { return read_libsvm(filename, 1); }
+svm_dataset readTrainFileCSV(char filename[], char *separator) { return read_csv(filename, separator, 1); }
+svm_dataset readUnlabeledFile(char filename[]) { return read_libsvm(filename, 0); }
+svm_dataset readUnlabeledFileCSV(char filename[], char *separator) { return read_csv(filename, separator, 0); }
+
+/* ---- model file ------------------------------------------------------------ */
+
+void storeModel(model *mod, FILE *Output) {
+    size_t w = 0;
+    w += fwrite(&mod->Kgamma, sizeof(double), 1, Output);
+    w += fwrite(&mod->bias, sizeof(double), 1, Output);
+    w += fwrite(&mod->maxdim, sizeof(int), 1, Output);
+    w += fwrite(&mod->kernelType, sizeof(int), 1, Output);
+    w += fwrite(&mod->sparse, sizeof(int), 1, Output);
+    w += fwrite(&mod->nSVs, sizeof(int), 1, Output);
+    w += fwrite(&mod->nElem, sizeof(int), 1, Output);
+    if (mod->nSVs > 0) {
+        w += fwrite(mod->weights, sizeof(double), (size_t)mod->nSVs, Output);
+        w += fwrite(mod->quadratic_value, sizeof(double), (size_t)mod->nSVs, Output);
+    }
+    if (mod->nElem > 0) w += fwrite(mod->features, sizeof(svm_sample), (size_t)mod->nElem, Output);
+    (void)w;
+    fflush(Output);
+}
+
+void readModel(model *mod, FILE *Input) {
+    size_t r = 0;
+    r += fread(&mod->Kgamma, sizeof(double), 1, Input);
+    r += fread(&mod->bias, sizeof(double), 1, Input);
+    r += fread(&mod->maxdim, sizeof(int), 1, Input);
+    r += fread(&mod->kernelType, sizeof(int), 1, Input);
+    r += fread(&mod->sparse, sizeof(int), 1, Input);
+    r += fread(&mod->nSVs, sizeof(int), 1, Input);
+    r += fread(&mod->nElem, sizeof(int), 1, Input);
+    const int ns = mod->nSVs > 0 ? mod->nSVs : 0;
+    const int ne = mod->nElem > 0 ? mod->nElem : 0;
+    mod->weights = (double *)irwls_xmalloc((size_t)(ns ? ns : 1) * sizeof(double));
+    mod->quadratic_value = (double *)irwls_xmalloc((size_t)(ns ? ns : 1) * sizeof(double));
+    r += fread(mod->weights, sizeof(double), (size_t)ns, Input);
+    r += fread(mod->quadratic_value, sizeof(double), (size_t)ns, Input);
+    mod->x = (svm_sample **)irwls_xmalloc((size_t)(ns ? ns : 1) * sizeof(svm_sample *));
+    mod->features = (svm_sample *)irwls_xcalloc((size_t)(ne ? ne : 1), sizeof(svm_sample));
+    r += fread(mod->features, sizeof(svm_sample), (size_t)ne, Input);
+    (void)r;
+    /* row k starts after the k-th terminator (src/IOStructures.c:817-824) */
+    if (ns > 0) mod->x[0] = &mod->features[0];
+    int row = 1;
+    for (int k = 0; k < ne; k++) {
+        if (mod->features[k].index == -1) {
+            if (row < ns) mod->x[row] = &mod->features[k + 1];
+            row++;
+        }
+    }
+}
+
+void writeOutput(char fileoutput[], double *predictions, int size) {
+    FILE *f = fopen(fileoutput, "w+");
+    if (!f) {
+        fprintf(stderr, "Cannot open output file: %s\n", fileoutput);
+        exit(2);
+    }
+    for (int i = 0; i < size; i++) fprintf(f, "%lf\n", predictions[i]);
+    fclose(f);
+}

@@ -1,0 +1,253 @@
+"""GPU parity of the full-training path: kernel tiles, the device Cholesky and
+the end-to-end trainFULL / predict entry points against the oracle and
+against closed forms computed with numpy."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from libirwls_amd import abi, lib, synth
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+# ---- device helpers over the library's C-ABI --------------------------------
+class Dev:
+    def __init__(self, arr):
+        self.L = lib.load()
+        arr = np.ascontiguousarray(arr)
+        self.nbytes = arr.nbytes
+        self.dtype, self.shape = arr.dtype, arr.shape
+        self.p = self.L.irwls_dev_alloc(max(arr.nbytes, 8))
+        if arr.nbytes:
+            self.L.irwls_dev_h2d(self.p, arr.ctypes.data, arr.nbytes)
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            self.L.irwls_dev_d2h(out.ctypes.data, self.p, self.nbytes)
+        return out
+
+    def __del__(self):
+        try:
+            self.L.irwls_dev_free(self.p)
+        except Exception:
+            pass
+
+
+def dense_rows(Xd, nrm, n, ldx, idx=None):
+    r = lib.irwls_rows()
+    r.X, r.ldx, r.nrm, r.n = Xd.p, ldx, nrm.p, n
+    r.idx = idx.p if idx is not None else None
+    return r
+
+
+def rbf(X, Z, g):
+    d2 = ((X[:, None, :] - Z[None, :, :]) ** 2).sum(-1)
+    return np.exp(-g * d2)
+
+
+# ---- kernel tiles -------------------------------------------------------------
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_kernel_matrix_dense(kernel):
+    rng = np.random.default_rng(11)
+    n, d, m = 150, 37, 70
+    X = rng.standard_normal((n, d))
+    idx = np.sort(rng.choice(n, m, replace=False)).astype(np.int32)
+    g = 0.05
+    L = lib.load()
+    Xd, nd, idd = Dev(X), Dev((X * X).sum(1)), Dev(idx)
+    K = Dev(np.zeros((n, m)))
+    r = dense_rows(Xd, nd, n, d)
+    c = dense_rows(Xd, nd, m, d, idd)
+    L.irwls_dev_kernel_matrix(C.byref(r), C.byref(c), d, kernel, g, K.p, m)
+    got = K.get()
+    want = rbf(X, X[idx], g) if kernel == 1 else X @ X[idx].T
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_kernel_matvec_dense_epilogue():
+    rng = np.random.default_rng(12)
+    n, d, m = 777, 130, 300
+    X = rng.standard_normal((n, d))
+    Z = rng.standard_normal((m, d))
+    w = rng.standard_normal(m)
+    out0 = rng.standard_normal(n)
+    rs = rng.choice([-1.0, 1.0], n)
+    g = 1.0 / d
+    L = lib.load()
+    Xd, Zd = Dev(X), Dev(Z)
+    xn, zn = Dev((X * X).sum(1)), Dev((Z * Z).sum(1))
+    wd, od, rsd = Dev(w), Dev(out0), Dev(rs)
+    r = dense_rows(Xd, xn, n, d)
+    c = dense_rows(Zd, zn, m, d)
+    L.irwls_dev_kernel_matvec(C.byref(r), C.byref(c), d, 1, g, wd.p, 1.0, -1.0, 0.25, rsd.p, od.p)
+    want = out0 + rs * (-(rbf(X, Z, g) @ w)) + 0.25
+    np.testing.assert_allclose(od.get(), want, rtol=1e-11, atol=1e-12)
+
+
+def test_kernel_matvec_many_columns():
+    """Column split path (partials + deterministic reduction)."""
+    rng = np.random.default_rng(13)
+    n, d, m = 96, 16, 5000
+    X = rng.standard_normal((n, d))
+    Z = rng.standard_normal((m, d))
+    w = rng.standard_normal(m)
+    g = 1.0 / 16
+    L = lib.load()
+    Xd, Zd = Dev(X), Dev(Z)
+    xn, zn = Dev((X * X).sum(1)), Dev((Z * Z).sum(1))
+    wd, od = Dev(w), Dev(np.zeros(n))
+    r = dense_rows(Xd, xn, n, d)
+    c = dense_rows(Zd, zn, m, d)
+    L.irwls_dev_kernel_matvec(C.byref(r), C.byref(c), d, 1, g, wd.p, 0.0, 1.0, 0.0, None, od.p)
+    first = od.get()
+    np.testing.assert_allclose(first, rbf(X, Z, g) @ w, rtol=1e-11, atol=1e-12)
+    L.irwls_dev_kernel_matvec(C.byref(r), C.byref(c), d, 1, g, wd.p, 0.0, 1.0, 0.0, None, od.p)
+    assert np.array_equal(first, od.get())   # bitwise reproducible
+
+
+def test_kernel_matvec_csr_rows():
+    rng = np.random.default_rng(14)
+    n, d, m = 500, 300, 90
+    rp, col, val = synth.sparse_uniform(n, d, 0.05, rng)
+    Xs = np.zeros((n, d))
+    for i in range(n):
+        Xs[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    Z = rng.random((m, d)) * (rng.random((m, d)) < 0.05)
+    w = rng.standard_normal(m)
+    g = 0.5
+    L = lib.load()
+    rpd, cd, vd = Dev(rp), Dev(col), Dev(val)
+    xn = Dev((Xs * Xs).sum(1))
+    Zd, zn = Dev(Z), Dev((Z * Z).sum(1))
+    wd, od = Dev(w), Dev(np.zeros(n))
+    r = lib.irwls_rows()
+    r.rowptr, r.col, r.val, r.nrm, r.n = rpd.p, cd.p, vd.p, xn.p, n
+    c = dense_rows(Zd, zn, m, d)
+    L.irwls_dev_kernel_matvec(C.byref(r), C.byref(c), d, 1, g, wd.p, 0.0, 1.0, 0.0, None, od.p)
+    np.testing.assert_allclose(od.get(), rbf(Xs, Z, g) @ w, rtol=1e-11, atol=1e-12)
+    K = Dev(np.zeros((n, m)))
+    L.irwls_dev_kernel_matrix(C.byref(r), C.byref(c), d, 1, g, K.p, m)
+    np.testing.assert_allclose(K.get(), rbf(Xs, Z, g), rtol=1e-12, atol=1e-13)
+
+
+# ---- device Cholesky ------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 333])
+def test_potrf_potrs_spd(n):
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    b = rng.standard_normal(n)
+    L = lib.load()
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    info = L.irwls_dev_potrf(Ad.p, n, n)
+    assert info == 0
+    F = Ad.get().reshape((n, n), order="F")
+    np.testing.assert_allclose(np.tril(F), np.linalg.cholesky(A), rtol=1e-10, atol=1e-12)
+    bd = Dev(b)
+    L.irwls_dev_potrs(Ad.p, n, n, bd.p)
+    np.testing.assert_allclose(bd.get(), np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 50, 129, 257])
+def test_kkt_failed_last_pivot(n):
+    """Indefinite KKT [A y; y^T 0]: the factorization stops at the last pivot
+    with the Schur value -s there; the solve then matches the closed form
+    b = (r_n - y^T A^-1 r_1)/s^2, beta = A^-1 (r_1 - y b) (s = y^T A^-1 y)."""
+    rng = np.random.default_rng(100 + n)
+    m = n - 1
+    X = rng.standard_normal((m, 4))
+    y = rng.choice([-1.0, 1.0], m)
+    A = rbf(X, X, 0.3) * np.outer(y, y) + np.diag(rng.uniform(0.1, 1.0, m))
+    H = np.zeros((n, n))
+    H[:m, :m] = A
+    H[:m, m] = y
+    H[m, :m] = y
+    r = rng.standard_normal(n)
+    L = lib.load()
+    Hd = Dev(np.asfortranarray(H).ravel(order="F"))
+    info = L.irwls_dev_potrf(Hd.p, n, n)
+    assert info == n
+    F = Hd.get().reshape((n, n), order="F")
+    s = y @ np.linalg.solve(A, y)
+    np.testing.assert_allclose(F[m, m], -s, rtol=1e-10)
+    rd = Dev(r)
+    L.irwls_dev_potrs(Hd.p, n, n, rd.p)
+    x = rd.get()
+    u = np.linalg.solve(A, r[:m])
+    b = (r[m] - y @ u) / s ** 2
+    beta = u - np.linalg.solve(A, y) * b
+    np.testing.assert_allclose(x[m], b, rtol=1e-8)
+    np.testing.assert_allclose(x[:m], beta, rtol=1e-8, atol=1e-10)
+    # and the oracle's restatement agrees
+    np.testing.assert_allclose(x, po.linear_system(H, r), rtol=1e-8, atol=1e-10)
+
+
+def test_parallel_linear_system_entry_point():
+    rng = np.random.default_rng(5)
+    n = 150
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + np.eye(n)
+    b = rng.standard_normal(n)
+    x, F = lib.linear_system(A, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9)
+    assert np.all(np.triu(F, 1) == 0.0)
+
+
+# ---- end-to-end full training ---------------------------------------------------
+def _parity(n, d, W, gamma, C_, seed, n_test=500, eta=0.001):
+    X, y = synth.two_gaussians(n, d, seed=seed)
+    Xt, yt = synth.two_gaussians(n_test, d, seed=seed + 100)
+    props = abi.make_properties(gamma=gamma, C_=C_, W=W, eta=eta)
+    ods = po.Dataset.dense(X, y)
+    want = po.train_full(ods, props, seed=0)
+    ostats = po.stats()
+    gds = lib.Dataset.dense(X, y)
+    got = lib.train_full(gds, props, seed=0)
+    gstats = lib.last_stats()
+    print("oracle", ostats, "gpu", gstats)
+    assert gstats["outer_iters"] == ostats["outer_iters"]
+    assert gstats["inner_iters"] == ostats["inner_iters"]
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("relative weight error", rel, "max abs", np.abs(got - want).max())
+    assert rel < 1e-6
+    assert np.array_equal(got[:-1] != 0, want[:-1] != 0)
+    # predictions on a test set: oracle model/oracle predictor vs GPU model/GPU predictor
+    om = po.full_model(props, ods, want)
+    gm = lib.full_model(props, gds, got)
+    assert om.bytes()[:16] != b"" and gm.m.nSVs == om.m.nSVs
+    otd = po.Dataset.dense(Xt, yt)
+    gtd = lib.Dataset.dense(Xt, yt)
+    pp = abi.make_predict_properties(labels=0)
+    osoft = po.soft_test(otd, om)
+    gsoft = lib.soft_test(gtd, gm, pp)
+    np.testing.assert_allclose(gsoft, osoft, rtol=1e-6, atol=1e-8)
+    assert np.array_equal(lib.hard_test(gtd, gm, pp), po.hard_test(otd, om))
+    return got, want
+
+
+def test_train_full_small():
+    _parity(n=1000, d=16, W=150, gamma=1 / 16, C_=10.0, seed=3)
+
+
+def test_train_full_two_gaussians_n4000():
+    """The CPU-reference workload: n=4000, d=16, gamma=1/16, C=10, W=500."""
+    _parity(n=4000, d=16, W=500, gamma=1 / 16, C_=10.0, seed=0, n_test=1000)
+
+
+def test_sub_irwls_entry_point():
+    rng = np.random.default_rng(21)
+    n, d = 300, 8
+    X, y = synth.two_gaussians(n, d, seed=21)
+    props = abi.make_properties(gamma=0.2, C_=5.0, W=n)
+    GIN = rng.standard_normal(n + 1) * 0.1
+    e = y.copy()
+    beta = np.zeros(n + 1)
+    ods = po.Dataset.dense(X, y)
+    gds = lib.Dataset.dense(X, y)
+    bw, ew, _ = po.sub_irwls(ods, props, GIN, e, beta)
+    bg, eg, _ = lib.sub_irwls(gds, props, GIN, e, beta)
+    np.testing.assert_allclose(bg, bw, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(eg, ew, rtol=1e-7, atol=1e-9)
