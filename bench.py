@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the IRWLS training hot path on one MI355X.
+
+A step is one full-training run (trainFULL through the C entry point) over
+the four-Gaussian workload n=50000, d=128 (FP64, RBF gamma=1/128, C=1,
+working set 2000, eta=0.001), with the dataset resident in HBM before the
+timed region.  `value` is seconds per training run (lower is better).
+
+The roofline object covers the kernel with the largest share of device time
+in the timed region; its duration comes from HIP events recorded around every
+launch on the library's stream.  The CPU baseline times the oracle (the FP64
+C restatement of the reference, OpenMP over the host cores) on a bounded
+sample: the first outer iterations of the same run, extrapolated to the GPU
+run's outer-iteration count.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# timer kinds whose bound is the matrix cores vs HBM
+MFMA_KINDS = {"kmv", "kmat", "trsm", "syrk", "potf2", "sgma", "syrkd", "csr"}
+
+WORKLOADS = {
+    "four_gaussians_n50000_d128": dict(n=50000, n_test=10000, d=128, gamma=1 / 128, C=1.0, W=2000, eta=0.001,
+                                       seed=1),
+    "four_gaussians_n10000_d128": dict(n=10000, n_test=2000, d=128, gamma=1 / 128, C=1.0, W=2000, eta=0.001,
+                                       seed=1),
+    "two_gaussians_n4000_d16": dict(n=4000, n_test=1000, d=16, gamma=1 / 16, C=10.0, W=500, eta=0.001, seed=0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_data(wl):
+    from libirwls_amd import synth
+    if wl["d"] == 16:
+        X, y = synth.two_gaussians(wl["n"], wl["d"], seed=wl["seed"])
+        Xt, yt = synth.two_gaussians(wl["n_test"], wl["d"], seed=wl["seed"] + 100)
+    else:
+        X, y = synth.four_gaussians(wl["n"], wl["d"], seed=wl["seed"], means_seed=wl["seed"])
+        Xt, yt = synth.four_gaussians(wl["n_test"], wl["d"], seed=wl["seed"] + 100, means_seed=wl["seed"])
+    return X, y, Xt, yt
+
+
+def cpu_baseline(wl, X, y, gpu_outer, max_outer):
+    """Oracle timed on the host cores for `max_outer` outer iterations."""
+    from libirwls_amd import abi
+    from oracle import pyoracle as po
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
+    ds = po.Dataset.dense(X, y)
+    po.set_limits(max_outer, 0, 0)
+    try:
+        t0 = time.time()
+        po.train_full(ds, props, seed=0)
+        dt = time.time() - t0
+        st = po.stats()
+    finally:
+        po.set_limits(0, 0, 0)
+    per_outer = dt / max(st["outer_iters"], 1)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": per_outer * gpu_outer,
+        "unit": "s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "oracle trainFULL, first %d outer iterations (%d inner) in %.1f s; %.2f s/outer x %d GPU "
+                  "outer iterations (extrapolated)" % (st["outer_iters"], st["inner_iters"], dt, per_outer,
+                                                       gpu_outer),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="four_gaussians_n50000_d128", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from libirwls_amd import abi, lib
+    wl = WORKLOADS[args.workload]
+    log("bench: generating %s" % args.workload)
+    X, y, Xt, yt = make_data(wl)
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
+    L = lib.load()
+    L.irwls_dev_init()
+    ds = lib.Dataset.dense(X, y)
+    ds.pin(with_means=False)            # inputs resident in HBM before timing
+    for i in range(args.warmup):
+        t0 = time.time()
+        lib.train_full(ds, props, seed=0)
+        log("bench: warmup %d done in %.2f s" % (i, time.time() - t0))
+    lib.timing(True)
+    times = []
+    beta = None
+    for i in range(args.steps):
+        L.irwls_dev_sync()
+        t0 = time.perf_counter()
+        beta = lib.train_full(ds, props, seed=0)
+        L.irwls_dev_sync()
+        times.append(time.perf_counter() - t0)
+        log("bench: step %d: %.3f s %s" % (i, times[-1], lib.last_stats()))
+    stats = lib.last_stats()
+    tm = lib.timers()
+    lib.timing(False)
+    # accuracy of the trained model on the held-out set
+    m = lib.full_model(props, ds, beta)
+    td = lib.Dataset.dense(Xt, yt)
+    pred = lib.hard_test(td, m, abi.make_predict_properties(labels=0))
+    acc = float(np.mean(pred == yt))
+    nsv = int(np.count_nonzero(beta[:-1]))
+
+    total_ms = sum(v["ms"] for v in tm.values())
+    dom = max(tm, key=lambda k: tm[k]["ms"])
+    t = tm[dom]
+    avg_ms = t["ms"] / max(t["launches"], 1)
+    if dom in MFMA_KINDS:
+        ach = (t["flops"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        ach = (t["bytes"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None}
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches_per_step"] = t["launches"] / args.steps
+    roof["device_share"] = t["ms"] / total_ms if total_ms else None
+
+    cpu = None
+    if args.cpu_outer > 0:
+        log("bench: cpu baseline (oracle, %d outer iterations)" % args.cpu_outer)
+        cpu = cpu_baseline(wl, X, y, stats["outer_iters"], args.cpu_outer)
+
+    value = float(np.mean(times))
+    out = {
+        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": value * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "mode": "plain_fp64", "n": wl["n"], "d": wl["d"], "W": wl["W"],
+                   "gamma": wl["gamma"], "C": wl["C"], "eta": wl["eta"], "n_test": wl["n_test"]},
+        "test_accuracy": acc,
+        "n_sv": nsv,
+        "outer_iters": stats["outer_iters"],
+        "inner_iters": stats["inner_iters"],
+        "solve_s": stats["t_solve"],
+        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in tm.items() if v["launches"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

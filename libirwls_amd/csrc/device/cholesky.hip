@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n
         if (c >= kb) continue;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int i = r0 + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+                const int i = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
                 if (i < n) A[(size_t)(k0 + c) * lda + i] = acc[mb][nb][r];
             }
     }
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n
         if (j >= n) continue;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int i = rI + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+                const int i = rI + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
                 if (i < n && i >= j) A[(size_t)j * lda + i] -= acc[mb][nb][r];
             }
     }

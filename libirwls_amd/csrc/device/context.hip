@@ -14,23 +14,41 @@ hipStream_t stream() {
 }
 
 // ---- timers --------------------------------------------------------------
+// Each timed launch records an event pair from a pool without synchronising;
+// pairs are resolved (elapsed time read) when the pool fills or the host asks
+// for the totals, so timing does not serialise the stream.
 static bool g_timing = false;
 static long g_launch[IRWLS_T_COUNT];
 static double g_ms[IRWLS_T_COUNT], g_flops[IRWLS_T_COUNT], g_bytes[IRWLS_T_COUNT];
-static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+static const int kPool = 4096;
+static hipEvent_t g_evs[kPool][2];
+static int g_evkind[kPool];
+static int g_evn = 0;
+static bool g_pool_ready = false;
+
+static void resolve_pool() {
+    if (g_evn == 0) return;
+    IRWLS_HIP(hipEventSynchronize(g_evs[g_evn - 1][1]));
+    for (int i = 0; i < g_evn; i++) {
+        float ms = 0.f;
+        IRWLS_HIP(hipEventElapsedTime(&ms, g_evs[i][0], g_evs[i][1]));
+        g_ms[g_evkind[i]] += ms;
+    }
+    g_evn = 0;
+}
 
 TimerScope::TimerScope(int k, double f, double b) : kind(k), flops(f), bytes(b), on(g_timing) {
-    if (on) IRWLS_HIP(hipEventRecord(g_ev0, stream()));
+    if (!on) return;
+    if (g_evn == kPool) resolve_pool();
+    IRWLS_HIP(hipEventRecord(g_evs[g_evn][0], stream()));
 }
 
 TimerScope::~TimerScope() {
     if (!on) return;
-    IRWLS_HIP(hipEventRecord(g_ev1, stream()));
-    IRWLS_HIP(hipEventSynchronize(g_ev1));
-    float ms = 0.f;
-    IRWLS_HIP(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+    IRWLS_HIP(hipEventRecord(g_evs[g_evn][1], stream()));
+    g_evkind[g_evn] = kind;
+    g_evn++;
     g_launch[kind] += 1;
-    g_ms[kind] += ms;
     g_flops[kind] += flops;
     g_bytes[kind] += bytes;
 }
@@ -87,8 +105,13 @@ int irwls_dev_init(void) {
     }
     IRWLS_HIP(hipSetDevice(0));
     IRWLS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    IRWLS_HIP(hipEventCreate(&g_ev0));
-    IRWLS_HIP(hipEventCreate(&g_ev1));
+    if (!g_pool_ready) {
+        for (int i = 0; i < kPool; i++) {
+            IRWLS_HIP(hipEventCreate(&g_evs[i][0]));
+            IRWLS_HIP(hipEventCreate(&g_evs[i][1]));
+        }
+        g_pool_ready = true;
+    }
     g_init = true;
     return 0;
 }
@@ -101,8 +124,7 @@ void irwls_dev_shutdown(void) {
         g_scr[s] = nullptr;
         g_scr_sz[s] = 0;
     }
-    IRWLS_HIP(hipEventDestroy(g_ev0));
-    IRWLS_HIP(hipEventDestroy(g_ev1));
+    resolve_pool();
     IRWLS_HIP(hipStreamDestroy(g_stream));
     g_stream = nullptr;
     g_init = false;
@@ -165,6 +187,7 @@ void irwls_dev_trim(void) {
 
 void irwls_dev_timing(int enable) {
     irwls_dev_init();
+    resolve_pool();
     g_timing = enable != 0;
     memset(g_launch, 0, sizeof(g_launch));
     memset(g_ms, 0, sizeof(g_ms));
@@ -173,6 +196,7 @@ void irwls_dev_timing(int enable) {
 }
 
 void irwls_dev_timer_get(int kind, long *launches, double *ms, double *flops, double *bytes) {
+    if (g_init) resolve_pool();
     if (kind < 0 || kind >= IRWLS_T_COUNT) {
         *launches = 0; *ms = 0; *flops = 0; *bytes = 0;
         return;

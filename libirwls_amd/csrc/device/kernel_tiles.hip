@@ -61,7 +61,7 @@ __device__ __forceinline__ void load_chunk(double (*S)[LDP], const DenseSide &s,
 }
 
 // One 64x64 tile of dot products, accumulated in acc[mb][nb] (v4d each):
-// lane l of wave (wr, wc) owns rows wr*32 + mb*16 + 4*(l/16) + r and column
+// lane l of wave (wr, wc) owns rows wr*32 + mb*16 + l/16 + 4*r and column
 // wc*32 + nb*16 + l%16 of the tile.
 __device__ __forceinline__ void tile_dots(v4d acc[2][2], double (*As)[TM][LDP], double (*Bs)[TN][LDP],
                                           const DenseSide &rs, const DenseSide &cs,
@@ -120,13 +120,13 @@ __global__ __launch_bounds__(256) void kmv_dense_kernel(
     const int ntiles = (cs.n + TN - 1) / TN;
     if (ct_end > ntiles) ct_end = ntiles;
 
-    // per-lane row partial sums: rows wr*32 + mb*16 + 4*(lane/16) + r
+    // per-lane row partial sums: rows wr*32 + mb*16 + lane/16 + 4*r
     double rsum[2][4];
     double nr[2][4];
     for (int mb = 0; mb < 2; mb++)
         for (int r = 0; r < 4; r++) {
             rsum[mb][r] = 0.0;
-            const int gi = r0 + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+            const int gi = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
             nr[mb][r] = (gi < rs.n) ? row_nrm(rs, gi) : 0.0;
         }
     for (int ct = ct_begin; ct < ct_end; ct++) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void kmv_dense_kernel(
         }
     if ((lane & 15) == 0) {
         for (int mb = 0; mb < 2; mb++)
-            for (int r = 0; r < 4; r++) red[wc][wr * 32 + mb * 16 + 4 * (lane >> 4) + r] = rsum[mb][r];
+            for (int r = 0; r < 4; r++) red[wc][wr * 32 + mb * 16 + (lane >> 4) + 4 * r] = rsum[mb][r];
     }
     __syncthreads();
     if (threadIdx.x < TM) {
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void kmat_dense_kernel(DenseSide rs, DenseSide
         const double ncj = row_nrm(cs, gj);
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int gi = r0 + wr * 32 + mb * 16 + 4 * (lane >> 4) + r;
+                const int gi = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
                 if (gi < rs.n)
                     K[(size_t)gi * ldk + gj] = kfun(kernel, gamma, acc[mb][nb][r], row_nrm(rs, gi), ncj);
             }

@@ -205,7 +205,8 @@ double orc_kernel_test(svm_dataset ds, int i, model m, int j) {
 /* Linear algebra (LAPACK potrf/potrs semantics)                             */
 /* ------------------------------------------------------------------------ */
 
-int orc_potrf(double *A, int n, int lda) {
+/* Unblocked left-looking factorization of an n x n block (LAPACK dpotf2). */
+static int potf2(double *A, int n, int lda) {
     for (int j = 0; j < n; j++) {
         double *cj = A + (size_t)j * lda;
         double ajj = cj[j];
@@ -224,6 +225,43 @@ int orc_potrf(double *A, int n, int lda) {
             for (int k = 0; k < j; k++)
                 s -= A[(size_t)k * lda + i] * A[(size_t)k * lda + j];
             cj[i] = s / ajj;
+        }
+    }
+    return 0;
+}
+
+/* Blocked right-looking factorization (LAPACK dpotrf, lower): the diagonal
+   block by potf2, the panel by a triangular solve, the trailing matrix by a
+   rank-nb update.  On a non-positive pivot it returns at once, as dpotrf
+   does, leaving the Schur value at the pivot. */
+int orc_potrf(double *A, int n, int lda) {
+    const int nb = 64;
+    for (int k0 = 0; k0 < n; k0 += nb) {
+        const int kb = (n - k0 < nb) ? n - k0 : nb;
+        double *D = A + (size_t)k0 * lda + k0;
+        int info = potf2(D, kb, lda);
+        if (info) return k0 + info;
+        const int r0 = k0 + kb;
+        if (r0 >= n) break;
+        /* panel rows r0..n-1: solve x L11^T = a */
+        #pragma omp parallel for schedule(static)
+        for (int i = r0; i < n; i++) {
+            for (int j = 0; j < kb; j++) {
+                double s = A[(size_t)(k0 + j) * lda + i];
+                for (int k = 0; k < j; k++)
+                    s -= A[(size_t)(k0 + k) * lda + i] * D[(size_t)k * lda + j];
+                A[(size_t)(k0 + j) * lda + i] = s / D[(size_t)j * lda + j];
+            }
+        }
+        /* trailing lower update A22 -= P P^T */
+        #pragma omp parallel for schedule(dynamic, 8)
+        for (int j = r0; j < n; j++) {
+            double *cj = A + (size_t)j * lda;
+            for (int k = 0; k < kb; k++) {
+                const double *pk = A + (size_t)(k0 + k) * lda;
+                const double ajk = pk[j];
+                for (int i = j; i < n; i++) cj[i] -= pk[i] * ajk;
+            }
         }
     }
     return 0;

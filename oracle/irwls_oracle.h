@@ -57,9 +57,9 @@ double orc_kernel(svm_dataset ds, int i, int j, properties props);
 /* src/kernels.c:137-183 */
 double orc_kernel_test(svm_dataset ds, int i, model m, int j);
 
-/* Column-major lower Cholesky with LAPACK potrf semantics on failure: at the
-   first pivot a_jj <= 0 (or NaN) the Schur value is stored in A(j,j) and
-   j+1 is returned; 0 on success. */
+/* Column-major lower Cholesky, blocked as LAPACK dpotrf (block 64), with its
+   failure semantics: at the first pivot a_jj <= 0 (or NaN) the Schur value is
+   stored in A(j,j) and j+1 is returned; 0 on success. */
 int orc_potrf(double *A, int n, int lda);
 /* Solve L L^T x = b in place with the lower factor (LAPACK potrs 'L'). */
 void orc_potrs(const double *L, int n, int lda, double *b);
