@@ -90,10 +90,11 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d,
                              int kernel, double gamma, double *d_K, int ldk);
 
 /* ---- full-training working-set helpers ---------------------------------- */
-/* H (column-major N x N, N = nS1+1): H[i][j] = Kw[s_i][s_j] y_i y_j + (i==j)/a_i,
-   border y, corner 0.  s = d_S1 (indices into Kw), y/a gathered by s. */
+/* H (column-major N x N, leading dimension ldh, N = nS1+1):
+   H[i][j] = Kw[s_i][s_j] y_i y_j + (i==j)/a_i, border y, corner 0.
+   s = d_S1 (indices into Kw); d_y, d_a hold y and a already gathered by s. */
 void irwls_dev_build_kkt(const double *d_Kw, int ldk, const int32_t *d_S1, int nS1,
-                         const double *d_y, const double *d_a, double *d_H);
+                         const double *d_y, const double *d_a, double *d_H, int ldh);
 /* out[i] = a*out[i] + rs[i] * (b * sum_j Kw[ri][cj] w[j]) + cst,
    ri = rows ? rows[i] : i, cj = cols ? cols[j] : j */
 void irwls_dev_gemv_gather(const double *d_Kw, int ldk, const int32_t *d_rows, int nr,
@@ -111,6 +112,11 @@ void irwls_dev_gemv(const double *d_A, int m, int n, int lda, int trans, const d
 int  irwls_dev_potrf(double *d_A, int n, int lda);
 /* Solve L L^T x = b in place (b on device). */
 void irwls_dev_potrs(const double *d_L, int n, int lda, double *d_b);
+/* Factor A (n x n, lda >= n+1) and solve A x = rhs in one pass: rhs is
+   placed in row n of A (the forward sweep rides along with the
+   factorization), then one backward sweep gives x.  Failure semantics as
+   irwls_dev_potrf (the solve uses the factor as it stands).  Returns info. */
+int  irwls_dev_posv_aug(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
 
 #ifdef __cplusplus
 }

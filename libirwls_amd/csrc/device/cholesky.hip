@@ -1,21 +1,27 @@
 // Blocked right-looking Cholesky and triangular solves for gfx950, FP64.
 //
-// Column-major lower factor in place, block size 64:
-//   potf2: one work-group factors the 64x64 diagonal block in LDS (wavefront
-//          level, one column step per barrier pair) and forms its inverse;
-//   trsm:  the panel below is multiplied by that inverse (FP64 MFMA);
-//   syrk:  the trailing lower tiles take the rank-64 update (FP64 MFMA).
-// A non-positive pivot ends the factorization as LAPACK potrf does: the Schur
-// value stays on the diagonal, the failing index is reported, and later steps
-// are skipped.  This is what the reference's solver relies on for the
-// indefinite KKT matrix of full training (src/ParallelAlgorithms.c:277-279,
-// 383-385, 441-445: LAPACK info is ignored and dpotrs uses the factor).
+// Column-major lower factor in place, block size 64.  One step per block
+// column k is two launches:
+//   chol_step: wave 0 of every work-group factors the 64x64 diagonal block in
+//              registers (lane i holds row i; column entries are broadcast
+//              with v_readlane), then every thread solves one panel row
+//              below it by forward substitution against the factor in LDS;
+//   syrk:      the trailing lower tiles take the rank-64 update on the FP64
+//              matrix cores (v_mfma_f64_16x16x4f64).
+// Right-hand sides can ride along as extra rows below the matrix
+// (rows n..nrows-1): the same steps then perform the forward sweep L z = r,
+// so a solve only adds the backward sweep.
 //
-// potrs: forward and backward sweeps as one persistent launch each, one
-// work-group per 64-row block, chained through per-block ready flags in
-// global memory (agent-scope release/acquire).  Every wait is bounded: a
-// wait that runs out sets an error word the host checks, so the kernel
-// always terminates.
+// A non-positive pivot ends the factorization as LAPACK potrf does: the Schur
+// value stays on the diagonal, the failing index is reported and later steps
+// are skipped; the right-hand-side rows are still solved against the factor
+// as it stands, as LAPACK potrs does with it.  The reference relies on
+// exactly this for the indefinite KKT matrix of full training
+// (src/ParallelAlgorithms.c:277-279, 383-385, 441-445: LAPACK info is
+// ignored and dpotrs uses the factor).
+//
+// No kernel here waits on another work-group: every dependency is a launch
+// boundary on the library stream.
 #include "common.h"
 
 using namespace irwls;
@@ -26,89 +32,90 @@ constexpr int NB = 64;
 constexpr int BK = 16;
 constexpr int LDP = BK + 1;
 
-struct CholWork {
-    double *Dinv = nullptr;     // per block 64x64 column-major inverse of L_kk
-    int *info = nullptr;        // [0] = info, [1] = potrs wait error
-    int *flags = nullptr;       // 2 * max blocks
-    int max_blocks = 0;
-    int epoch = 0;
-};
-CholWork g_cw;
+int *g_info = nullptr;     // [0] = potrf info
 
-void ensure_work(int nblk) {
-    if (g_cw.max_blocks >= nblk && g_cw.info) return;
-    if (g_cw.Dinv) {
-        irwls_dev_free(g_cw.Dinv);
-        irwls_dev_free(g_cw.flags);
-    }
-    int cap = nblk < 16 ? 16 : nblk;
-    g_cw.Dinv = (double *)irwls_dev_alloc((size_t)cap * NB * NB * sizeof(double));
-    g_cw.flags = (int *)irwls_dev_alloc((size_t)2 * cap * sizeof(int));
-    if (!g_cw.info) g_cw.info = (int *)irwls_dev_alloc(4 * sizeof(int));
-    g_cw.max_blocks = cap;
-    g_cw.epoch = 0;
+int *info_buf() {
+    if (!g_info) g_info = (int *)irwls_dev_alloc(4 * sizeof(int));
+    return g_info;
 }
 
-// ---------------------------------------------------------------- potf2 ---
-__global__ __launch_bounds__(256) void potf2_kernel(double *__restrict__ A, int lda, int k0, int kb,
-                                                    int *__restrict__ info, double *__restrict__ Dinv) {
-    __shared__ double L[NB][NB];
-    __shared__ double Inv[NB][NB];
-    // After an earlier failure the factorization has stopped; the block is
-    // only inverted as it stands, so the triangular solves use whatever the
-    // lower triangle holds (as LAPACK potrs does).
-    const bool stopped = info[0] != 0;
+__device__ __forceinline__ double rdl(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ----------------------------------------------------------- chol_step ---
+// Diagonal block [k0, k0+kb) and panel rows [k0+kb, nrows).  n = factored
+// order (rows >= n are right-hand sides).
+__global__ __launch_bounds__(256) void chol_step_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
+                                                        int kb, int *__restrict__ info) {
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ int s_fail;
+    const int prev = info[0];
+    if (prev != 0 && prev <= k0) return;   // stopped at an earlier block
     const int t = threadIdx.x;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e % NB, c = e / NB;   // column-major walk: coalesced
-        double v = 0.0;
-        if (i < kb && c < kb && i >= c) v = A[(size_t)(k0 + c) * lda + k0 + i];
-        L[i][c] = v;
-    }
-    __syncthreads();
-    int fail = -1;
-    if (!stopped) {
-        for (int j = 0; j < kb; j++) {
-            const double ajj = L[j][j];
-            if (!(ajj > 0.0)) {   // uniform: every thread read the same value
-                fail = j;
-                break;
+    const int lane = t & 63;
+    if (t < 64) {
+        // lane i holds row i of the lower diagonal block
+        double a[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            a[c] = (lane < kb && c < kb && c <= lane) ? A[(size_t)(k0 + c) * lda + k0 + lane] : 0.0;
+        int fail = -1;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j < kb && fail < 0) {
+                const double ajj = rdl(a[j], j);
+                if (!(ajj > 0.0)) {
+                    fail = j;           // uniform; lane j keeps the Schur value
+                } else {
+                    const double ljj = sqrt(ajj);
+                    a[j] = (lane == j) ? ljj : a[j] / ljj;
+#pragma unroll
+                    for (int c = j + 1; c < NB; c++) {
+                        if (c < kb) {
+                            const double lcj = rdl(a[j], c);
+                            a[c] -= a[j] * lcj;   // entries above the diagonal are never read
+                        }
+                    }
+                }
             }
-            const double ljj = sqrt(ajj);
-            __syncthreads();
-            if (t == 0) L[j][j] = ljj;
-            for (int i = j + 1 + t; i < kb; i += 256) L[i][j] = L[i][j] / ljj;
-            __syncthreads();
-            const int m = kb - j - 1;
-            for (int e = t; e < m * m; e += 256) {
-                const int i = j + 1 + e / m, c = j + 1 + e % m;
-                if (c <= i) L[i][c] -= L[i][j] * L[c][j];
-            }
-            __syncthreads();
         }
-        for (int e = t; e < NB * NB; e += 256) {
-            const int i = e % NB, c = e / NB;
-            if (i < kb && c < kb && i >= c) A[(size_t)(k0 + c) * lda + k0 + i] = L[i][c];
+#pragma unroll
+        for (int c = 0; c < NB; c++) Ls[lane][c] = (c <= lane) ? a[c] : 0.0;
+        if (blockIdx.x == 0 && lane < kb) {
+#pragma unroll
+            for (int c = 0; c < NB; c++)
+                if (c < kb && c <= lane) A[(size_t)(k0 + c) * lda + k0 + lane] = a[c];
         }
-        if (fail >= 0 && t == 0) info[0] = k0 + fail + 1;
-    }
-    // inverse of the lower triangular block, one column per thread
-    if (t < NB) {
-        for (int i = 0; i < NB; i++) {
-            double x = 0.0;
-            if (t < kb && i >= t && i < kb) {
-                double s = (i == t) ? 1.0 : 0.0;
-                for (int k = t; k < i; k++) s -= L[i][k] * Inv[k][t];
-                x = s / L[i][i];
-            }
-            Inv[i][t] = x;
+        if (lane == 0) {
+            s_fail = fail;
+            if (blockIdx.x == 0 && fail >= 0) info[0] = k0 + fail + 1;
         }
     }
     __syncthreads();
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e % NB, c = e / NB;
-        Dinv[(size_t)c * NB + i] = Inv[i][c];
+    const int fail = s_fail;
+    // panel rows; after a failure only the right-hand-side rows continue
+    const int row0 = (fail >= 0) ? ((n > k0 + kb) ? n : k0 + kb) : k0 + kb;
+    const int r = k0 + kb + blockIdx.x * 256 + t;
+    if (r < row0 || r >= nrows) return;
+    double x[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) x[j] = (j < kb) ? A[(size_t)(k0 + j) * lda + r] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j < kb) {
+            double s = x[j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= x[k] * Ls[j][k];
+            x[j] = s / Ls[j][j];
+        }
     }
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+        if (j < kb) A[(size_t)(k0 + j) * lda + r] = x[j];
 }
 
 // Loads a 64 (rows) x 16 (k) chunk of a column-major matrix:
@@ -128,8 +135,8 @@ __device__ __forceinline__ void load_cm(double (*S)[LDP], const double *M, int l
     }
 }
 
-// acc[mb][nb] += sum_k P(i, k) Q(j, k) over k < K, P/Q as column-major
-// matrices with row offsets pr0/qr0 and column offset kc0.
+// acc[mb][nb] += sum_k P(i, k) Q(j, k) over k in [kc0, kc0+K), P and Q
+// column-major with row offsets pr0 / qr0.
 __device__ __forceinline__ void mfma_cm(v4d acc[2][2], double (*As)[NB][LDP], double (*Bs)[NB][LDP],
                                         const double *P, int ldp, int pr0, int prmax, const double *Q, int ldq,
                                         int qr0, int qrmax, int kc0, int K) {
@@ -157,47 +164,16 @@ __device__ __forceinline__ void mfma_cm(v4d acc[2][2], double (*As)[NB][LDP], do
     }
 }
 
-// ----------------------------------------------------------------- trsm ---
-// Panel rows [k0+kb, n): P = A(:, k0:k0+kb) * inv(L_kk)^T, in place.
-__global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n, int lda, int k0, int kb,
-                                                   const int *__restrict__ info, const double *__restrict__ Dinv) {
-    __shared__ double As[1][NB][LDP];
-    __shared__ double Bs[1][NB][LDP];
-    if (info[0] != 0) return;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int r0 = k0 + kb + blockIdx.x * NB;
-    v4d acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    // Q(j, k) = Linv(j, k) = Dinv[k*64 + j]  (column-major, ld 64)
-    mfma_cm(acc, As, Bs, A + (size_t)k0 * lda, lda, r0, n, Dinv, NB, 0, kb, 0, kb);
-    // all reads of the panel are done (barrier at the end of mfma_cm)
-    for (int nb = 0; nb < 2; nb++) {
-        const int c = wc * 32 + nb * 16 + (lane & 15);
-        if (c >= kb) continue;
-        for (int mb = 0; mb < 2; mb++)
-            for (int r = 0; r < 4; r++) {
-                const int i = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
-                if (i < n) A[(size_t)(k0 + c) * lda + i] = acc[mb][nb][r];
-            }
-    }
-}
-
 // ----------------------------------------------------------------- syrk ---
-// Trailing lower tiles: A(I, J) -= P_I P_J^T, P = A(:, k0:k0+kb).
-__global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n, int lda, int k0, int kb,
-                                                   const int *__restrict__ info) {
+// Trailing tiles: A(I, J) -= P_I P_J^T, rows [t0, nrows) x columns [t0, n),
+// P = A(:, k0:k0+kb), t0 = k0 + kb; grid (column tiles, row tiles), tiles
+// strictly above the diagonal return at once.
+__global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
+                                                   int kb, const int *__restrict__ info) {
     __shared__ double As[1][NB][LDP];
     __shared__ double Bs[1][NB][LDP];
-    if (info[0] != 0) return;
-    // decode lower-triangular tile index: ti >= tj
-    const int b = blockIdx.x;
-    int ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
-    while (ti * (ti + 1) / 2 > b) ti--;
-    const int tj = b - ti * (ti + 1) / 2;
+    const int tj = blockIdx.x, ti = blockIdx.y;
+    if (ti < tj || info[0] != 0) return;
     const int t0 = k0 + kb;
     const int rI = t0 + ti * NB, rJ = t0 + tj * NB;
     const int lane = threadIdx.x & 63;
@@ -206,126 +182,155 @@ __global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n
     v4d acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int c = 0; c < 2; c++) acc[a][c] = (v4d){0.0, 0.0, 0.0, 0.0};
-    mfma_cm(acc, As, Bs, A, lda, rI, n, A, lda, rJ, n, k0, kb);
+    mfma_cm(acc, As, Bs, A, lda, rI, nrows, A, lda, rJ, n, k0, kb);
     for (int nb = 0; nb < 2; nb++) {
         const int j = rJ + wc * 32 + nb * 16 + (lane & 15);
         if (j >= n) continue;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
                 const int i = rI + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
-                if (i < n && i >= j) A[(size_t)j * lda + i] -= acc[mb][nb][r];
+                if (i < nrows && i >= j) A[(size_t)j * lda + i] -= acc[mb][nb][r];
             }
     }
 }
 
-// ---------------------------------------------------------------- potrs ---
-constexpr long kSpinLimit = 1L << 22;   // ~0.1 s of s_sleep(1) polls
+// -------------------------------------------------------- triangular sweeps ---
+// One launch per 64-row block w.  Every work-group first solves the diagonal
+// block for block w of the solution from the (final) accumulator with wave 0
+// and work-group 0 stores it in `out`; then the work-groups update the
+// accumulators of the blocks still to come.  Block w of `acc` is only read in
+// launch w, so no work-group reads what another one writes.
 
-__device__ __forceinline__ void wait_flag(const int *flag, int epoch, int *err) {
-    if (threadIdx.x == 0) {
-        long spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            if (++spins > kSpinLimit) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
-__device__ __forceinline__ void post_flag(int *flag, int epoch) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Forward sweep L z = b, block w = blockIdx.x, in place on b.
-__global__ __launch_bounds__(256) void trsv_fwd_kernel(const double *__restrict__ Lm, int n, int lda,
-                                                       const double *__restrict__ Dinv, double *b, int *flags,
-                                                       int epoch, int *err) {
-    __shared__ double part[4][NB];
-    __shared__ double accs[NB];
-    const int w = blockIdx.x;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int r = w * NB + lane;
-    double acc = 0.0;
-    for (int j = 0; j < w; j++) {
-        wait_flag(&flags[j], epoch, err);
-        // this wave: columns j*64 + wave*16 .. +16
-        for (int k = 0; k < 16; k++) {
-            const int c = j * NB + wave * 16 + k;
-            const double zc = __hip_atomic_load(&b[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (r < n) acc += Lm[(size_t)c * lda + r] * zc;
-        }
-    }
-    part[wave][lane] = acc;
-    __syncthreads();
-    if (threadIdx.x < NB) {
-        const int i = threadIdx.x;
-        const int gi = w * NB + i;
-        const double rhs = (gi < n) ? b[gi] : 0.0;
-        accs[i] = rhs - (((part[0][i] + part[1][i]) + part[2][i]) + part[3][i]);
-    }
-    __syncthreads();
-    if (threadIdx.x < NB) {
-        const int i = threadIdx.x;
-        const int gi = w * NB + i;
+// forward: z_w = L_ww^-1 acc_w;  acc_r -= L(r, block w) z_w for rows r >= (w+1)*64
+__global__ __launch_bounds__(256) void trsv_fwd_step(const double *__restrict__ L, int n, int lda, int w,
+                                                     double *__restrict__ acc, double *__restrict__ out) {
+    __shared__ double zs[NB];
+    const int t = threadIdx.x, lane = t & 63;
+    const int k0 = w * NB;
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    if (t < 64) {
+        // lane i holds row i of L_ww
+        double lrow[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            lrow[c] = (lane < kb && c < kb && c <= lane) ? L[(size_t)(k0 + c) * lda + k0 + lane] : 0.0;
+        double s = (lane < kb) ? acc[k0 + lane] : 0.0;
         double z = 0.0;
-        for (int k = 0; k <= i; k++) z += Dinv[(size_t)w * NB * NB + (size_t)k * NB + i] * accs[k];
-        if (gi < n) b[gi] = z;
-    }
-    post_flag(&flags[w], epoch);
-}
-
-// Backward sweep L^T x = z, block w = nblk-1-blockIdx.x, in place on b.
-__global__ __launch_bounds__(256) void trsv_bwd_kernel(const double *__restrict__ Lm, int n, int lda,
-                                                       const double *__restrict__ Dinv, double *b, int *flags,
-                                                       int epoch, int nblk, int *err) {
-    __shared__ double part[4][16];
-    __shared__ double accs[NB];
-    const int w = nblk - 1 - blockIdx.x;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    // wave owns output rows w*64 + wave*16 + q, q < 16; lanes run over k
-    double acc[16];
-    for (int q = 0; q < 16; q++) acc[q] = 0.0;
-    for (int j = w + 1; j < nblk; j++) {
-        wait_flag(&flags[j], epoch, err);
-        const int kr = j * NB + lane;
-        const double xk = (kr < n) ? __hip_atomic_load(&b[kr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        if (kr < n) {
-            for (int q = 0; q < 16; q++) {
-                const int c = w * NB + wave * 16 + q;
-                if (c < n) acc[q] += Lm[(size_t)c * lda + kr] * xk;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j < kb) {
+                const double zj = rdl(s, j) / rdl(lrow[j], j);
+                if (lane == j) z = zj;
+                s -= lrow[j] * zj;   // rows below j; row j itself is finished
             }
         }
-    }
-    for (int q = 0; q < 16; q++) {
-        double v = acc[q];
-        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) part[wave][q] = v;
+        zs[lane] = z;
+        if (blockIdx.x == 0 && lane < kb) out[k0 + lane] = z;
     }
     __syncthreads();
-    if (threadIdx.x < NB) {
-        const int i = threadIdx.x;
-        const int gi = w * NB + i;
-        const double rhs = (gi < n) ? b[gi] : 0.0;
-        accs[i] = rhs - part[i >> 4][i & 15];
-    }
-    __syncthreads();
-    if (threadIdx.x < NB) {
-        const int i = threadIdx.x;
-        const int gi = w * NB + i;
-        // x_i = sum_k Linv(k, i) accs[k], Linv(k, i) = Dinv[i*64 + k], k >= i
+    const int r = k0 + kb + blockIdx.x * 256 + t;
+    if (r >= n) return;
+    double v = 0.0;
+    for (int c = 0; c < kb; c++) v += L[(size_t)(k0 + c) * lda + r] * zs[c];
+    acc[r] -= v;
+}
+
+// backward: x_w = L_ww^-T acc_w;  acc_c -= sum_i L(w*64+i, c) x_w[i] for c < w*64
+__global__ __launch_bounds__(256) void trsv_bwd_step(const double *__restrict__ L, int n, int lda, int w,
+                                                     double *__restrict__ acc, double *__restrict__ out,
+                                                     int cols_per_wave) {
+    __shared__ double xs[NB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int k0 = w * NB;
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    if (t < 64) {
+        // lane i holds column i of L_ww (row i of L_ww^T)
+        double lcol[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+            lcol[k] = (lane < kb && k < kb && k >= lane) ? L[(size_t)(k0 + lane) * lda + k0 + k] : 0.0;
+        double s = (lane < kb) ? acc[k0 + lane] : 0.0;
         double x = 0.0;
-        for (int k = i; k < NB; k++) x += Dinv[(size_t)w * NB * NB + (size_t)i * NB + k] * accs[k];
-        if (gi < n) b[gi] = x;
+#pragma unroll
+        for (int jj = NB - 1; jj >= 0; jj--) {
+            if (jj < kb) {
+                const double xj = rdl(s, jj) / rdl(lcol[jj], jj);
+                if (lane == jj) x = xj;
+                s -= lcol[jj] * xj;   // lanes i < jj take L(jj, i) x_jj
+            }
+        }
+        xs[lane] = x;
+        if (blockIdx.x == 0 && lane < kb) out[k0 + lane] = x;
     }
-    post_flag(&flags[w], epoch);
+    __syncthreads();
+    const double xl = xs[lane];
+    const int c_begin = (blockIdx.x * 4 + wave) * cols_per_wave;
+    for (int q = 0; q < cols_per_wave; q++) {
+        const int c = c_begin + q;
+        if (c >= k0) break;
+        double v = (lane < kb) ? L[(size_t)c * lda + k0 + lane] * xl : 0.0;
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) acc[c] -= v;
+    }
+}
+
+// right-hand-side row of an augmented matrix <-> contiguous vector
+__global__ void gather_row(const double *__restrict__ A, int lda, int row, int n, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = A[(size_t)i * lda + row];
+}
+
+__global__ void scatter_row(double *__restrict__ A, int lda, int row, int n, const double *__restrict__ in) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(size_t)i * lda + row] = in[i];
+}
+
+void factor(double *A, int n, int nrows, int lda) {
+    int *info = info_buf();
+    irwls_dev_memset0(info, 4 * sizeof(int));
+    const int nblk = (n + NB - 1) / NB;
+    for (int kbk = 0; kbk < nblk; kbk++) {
+        const int k0 = kbk * NB;
+        const int kb = (n - k0 < NB) ? (n - k0) : NB;
+        const int prow = nrows - k0 - kb;
+        const int g1 = prow > 0 ? (prow + 255) / 256 : 1;
+        {
+            TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0 + (double)(prow > 0 ? prow : 0) * kb * kb,
+                          8.0 * (double)kb * kb + 16.0 * (prow > 0 ? prow : 0) * kb);
+            hipLaunchKernelGGL(chol_step_kernel, dim3(g1), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info);
+            IRWLS_LAUNCH_CHECK();
+        }
+        const int rest = n - k0 - kb;
+        if (rest <= 0) break;   // kernels return at once after a failure
+        const int tc = (rest + NB - 1) / NB;
+        const int tr = (nrows - k0 - kb + NB - 1) / NB;
+        TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
+                      8.0 * (double)rest * rest);
+        hipLaunchKernelGGL(syrk_kernel, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+// acc is consumed (left as scratch); the solution lands in out
+void sweep_forward(const double *L, int n, int lda, double *acc, double *out) {
+    const int nblk = (n + NB - 1) / NB;
+    for (int w = 0; w < nblk; w++) {
+        const int rows = n - (w + 1) * NB;
+        const int g = rows > 0 ? (rows + 255) / 256 : 1;
+        hipLaunchKernelGGL(trsv_fwd_step, dim3(g), dim3(256), 0, stream(), L, n, lda, w, acc, out);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+void sweep_backward(const double *L, int n, int lda, double *acc, double *out) {
+    const int nblk = (n + NB - 1) / NB;
+    const int cpw = 8;
+    for (int w = nblk - 1; w >= 0; w--) {
+        const int cols = w * NB;
+        const int g = cols > 0 ? (cols + 4 * cpw - 1) / (4 * cpw) : 1;
+        hipLaunchKernelGGL(trsv_bwd_step, dim3(g), dim3(256), 0, stream(), L, n, lda, w, acc, out, cpw);
+        IRWLS_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace
@@ -334,62 +339,39 @@ extern "C" {
 
 int irwls_dev_potrf(double *A, int n, int lda) {
     if (n <= 0) return 0;
-    const int nblk = (n + NB - 1) / NB;
-    ensure_work(nblk);
-    irwls_dev_memset0(g_cw.info, 4 * sizeof(int));
-    for (int kbk = 0; kbk < nblk; kbk++) {
-        const int k0 = kbk * NB;
-        const int kb = (n - k0 < NB) ? (n - k0) : NB;
-        {
-            TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0, 16.0 * kb * kb);
-            hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(256), 0, stream(), A, lda, k0, kb, g_cw.info,
-                               g_cw.Dinv + (size_t)kbk * NB * NB);
-            IRWLS_LAUNCH_CHECK();
-        }
-        const int rest = n - k0 - kb;
-        if (rest <= 0) break;   // trsm/syrk return at once after a failure
-        const int pt = (rest + NB - 1) / NB;
-        {
-            TimerScope ts(IRWLS_T_TRSM, 2.0 * rest * (double)kb * kb, 16.0 * rest * kb);
-            hipLaunchKernelGGL(trsm_kernel, dim3(pt), dim3(256), 0, stream(), A, n, lda, k0, kb, g_cw.info,
-                               g_cw.Dinv + (size_t)kbk * NB * NB);
-            IRWLS_LAUNCH_CHECK();
-        }
-        {
-            TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb, 8.0 * (double)rest * rest);
-            hipLaunchKernelGGL(syrk_kernel, dim3(pt * (pt + 1) / 2), dim3(256), 0, stream(), A, n, lda, k0, kb,
-                               g_cw.info);
-            IRWLS_LAUNCH_CHECK();
-        }
-    }
+    factor(A, n, n, lda);
     int info = 0;
-    irwls_dev_d2h(&info, g_cw.info, sizeof(int));
+    irwls_dev_d2h(&info, info_buf(), sizeof(int));
     return info;
 }
 
 void irwls_dev_potrs(const double *L, int n, int lda, double *b) {
     if (n <= 0) return;
-    const int nblk = (n + NB - 1) / NB;
-    ensure_work(nblk);
-    // potrs uses the diagonal-block inverses formed by the last potrf of the
-    // same matrix (also for blocks at or after a failed pivot).
-    g_cw.epoch += 1;
-    const int ep = g_cw.epoch;
-    {
-        TimerScope ts(IRWLS_T_POTRS, 2.0 * (double)n * n, 8.0 * (double)n * n);
-        hipLaunchKernelGGL(trsv_fwd_kernel, dim3(nblk), dim3(256), 0, stream(), L, n, lda, g_cw.Dinv, b,
-                           g_cw.flags, ep, g_cw.info + 1);
-        IRWLS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(trsv_bwd_kernel, dim3(nblk), dim3(256), 0, stream(), L, n, lda, g_cw.Dinv, b,
-                           g_cw.flags + g_cw.max_blocks, ep, nblk, g_cw.info + 1);
-        IRWLS_LAUNCH_CHECK();
-    }
-    int err = 0;
-    irwls_dev_d2h(&err, g_cw.info + 1, sizeof(int));
-    if (err) {
-        fprintf(stderr, "libirwls: triangular solve wait timed out\n");
+    double *tmp = (double *)scratch(9, (size_t)n * sizeof(double));
+    TimerScope ts(IRWLS_T_POTRS, 2.0 * (double)n * n, 8.0 * (double)n * n);
+    sweep_forward(L, n, lda, b, tmp);     // z in tmp, b is scratch
+    sweep_backward(L, n, lda, tmp, b);    // x in b
+}
+
+int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) {
+    if (n <= 0) return 0;
+    if (lda < n + 1) {
+        fprintf(stderr, "libirwls: irwls_dev_posv_aug needs lda >= n+1 (lda=%d, n=%d)\n", lda, n);
         exit(2);
     }
+    hipLaunchKernelGGL(scatter_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, rhs);
+    IRWLS_LAUNCH_CHECK();
+    factor(A, n, n + 1, lda);
+    double *z = (double *)scratch(9, (size_t)n * sizeof(double));
+    hipLaunchKernelGGL(gather_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, z);
+    IRWLS_LAUNCH_CHECK();
+    {
+        TimerScope ts(IRWLS_T_POTRS, 1.0 * (double)n * n, 4.0 * (double)n * n);
+        sweep_backward(A, n, lda, z, x);
+    }
+    int info = 0;
+    irwls_dev_d2h(&info, info_buf(), sizeof(int));
+    return info;
 }
 
 }  // extern "C"

@@ -12,10 +12,10 @@ using namespace irwls;
 
 namespace {
 
-// H column-major N x N (symmetric), N = nS1 + 1.
+// H column-major N x N (symmetric), N = nS1 + 1, leading dimension ldh.
 __global__ void build_kkt_kernel(const double *__restrict__ Kw, int ldk, const int32_t *__restrict__ S1, int nS1,
                                  const double *__restrict__ ys, const double *__restrict__ as,
-                                 double *__restrict__ H) {
+                                 double *__restrict__ H, int ldh) {
     const int N = nS1 + 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;   // row (contiguous)
     const int j = blockIdx.y;                               // column
@@ -31,7 +31,7 @@ __global__ void build_kkt_kernel(const double *__restrict__ Kw, int ldk, const i
     } else {
         v = 0.0;
     }
-    H[(size_t)j * N + i] = v;
+    H[(size_t)j * ldh + i] = v;
 }
 
 // One wave per output row; lanes stride the gathered columns.
@@ -90,11 +90,11 @@ void irwls_dev_gemv(const double *A, int m, int n, int lda, int trans, const dou
 
 
 void irwls_dev_build_kkt(const double *Kw, int ldk, const int32_t *S1, int nS1, const double *ys, const double *as,
-                         double *H) {
+                         double *H, int ldh) {
     const int N = nS1 + 1;
     TimerScope ts(IRWLS_T_KKT, 0.0, 8.0 * (double)N * N * 2.0);
     hipLaunchKernelGGL(build_kkt_kernel, dim3((N + 255) / 256, N), dim3(256), 0, stream(), Kw, ldk, S1, nS1, ys, as,
-                       H);
+                       H, ldh);
     IRWLS_LAUNCH_CHECK();
 }
 

@@ -244,7 +244,7 @@ void irwls_ws_reserve(irwls_ws *ws, int cap) {
     const size_t c = (size_t)(cap > 0 ? cap : 1);
     ws->cap = (int)c;
     ws->Kw = (double *)irwls_dev_alloc(c * c * sizeof(double));
-    ws->H = (double *)irwls_dev_alloc((c + 1) * (c + 1) * sizeof(double));
+    ws->H = (double *)irwls_dev_alloc((c + 2) * (c + 1) * sizeof(double));
     ws->rhs = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
     ws->esub = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
     ws->w = (double *)irwls_dev_alloc((c + 1) * sizeof(double));

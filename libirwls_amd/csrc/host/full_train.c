@@ -35,14 +35,14 @@ int *rpermute(int n) {
     return p;
 }
 
-/* Dense solve of the KKT system on the device: factor H (N x N) in place,
-   solve with rhs, copy the solution to x (host). */
+/* Dense solve of the KKT system on the device: H (N x N, leading dimension
+   N+1) is factored with the right-hand side riding in row N, then one
+   backward sweep; the solution is copied to x (host). */
 static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x) {
     double t0 = irwls_now();
     irwls_dev_h2d(ws->rhs, et, (size_t)N * sizeof(double));
-    (void)irwls_dev_potrf(ws->H, N, N);   /* failure code ignored, as the reference */
-    irwls_dev_potrs(ws->H, N, N, ws->rhs);
-    irwls_dev_d2h(x, ws->rhs, (size_t)N * sizeof(double));
+    (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);   /* failure code ignored, as the reference */
+    irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
     g_stats.solves++;
     g_stats.t_solve += irwls_now() - t0;
 }
@@ -94,7 +94,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l,
         irwls_dev_h2d(ws->S1, S1, (size_t)N * sizeof(int32_t));
         irwls_dev_h2d(ws->ys, hbuf, (size_t)N * sizeof(double));
         irwls_dev_h2d(ws->as, hbuf2, (size_t)N * sizeof(double));
-        irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H);
+        irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
         memset(bAux, 0, (size_t)N * sizeof(double));
         kkt_solve(ws, N, et, bAux);
 

@@ -36,19 +36,26 @@ void ParallelLinearSystem(double *matrix1, int r1, int c1, int ro1, int co1, dou
     (void)m; (void)rr; (void)cr; (void)ror; (void)cor; (void)nCores;
     if (n <= 0) return;
     irwls_dev_init();
-    const size_t nn = (size_t)n * n;
-    double *dA = (double *)irwls_dev_alloc(nn * sizeof(double));
+    /* device copy with one spare row per column for the right-hand side */
+    const int lda = n + 1;
+    double *h = (double *)irwls_xcalloc((size_t)lda * n, sizeof(double));
+    for (int j = 0; j < n; j++) memcpy(h + (size_t)j * lda, matrix1 + (size_t)j * n, (size_t)n * sizeof(double));
+    double *dA = (double *)irwls_dev_alloc((size_t)lda * n * sizeof(double));
     double *db = (double *)irwls_dev_alloc((size_t)n * sizeof(double));
-    irwls_dev_h2d(dA, matrix1, nn * sizeof(double));
+    double *dx = (double *)irwls_dev_alloc((size_t)n * sizeof(double));
+    irwls_dev_h2d(dA, h, (size_t)lda * n * sizeof(double));
     irwls_dev_h2d(db, matrix2, (size_t)n * sizeof(double));
-    (void)irwls_dev_potrf(dA, n, n);
-    irwls_dev_potrs(dA, n, n, db);
-    irwls_dev_d2h(matrix1, dA, nn * sizeof(double));
-    irwls_dev_d2h(result, db, (size_t)n * sizeof(double));
-    for (int j = 0; j < n; j++)
+    (void)irwls_dev_posv_aug(dA, n, lda, db, dx);
+    irwls_dev_d2h(h, dA, (size_t)lda * n * sizeof(double));
+    irwls_dev_d2h(result, dx, (size_t)n * sizeof(double));
+    for (int j = 0; j < n; j++) {
+        memcpy(matrix1 + (size_t)j * n, h + (size_t)j * lda, (size_t)n * sizeof(double));
         for (int k = 0; k < j; k++) matrix1[(size_t)j * n + k] = 0.0;
+    }
+    free(h);
     irwls_dev_free(dA);
     irwls_dev_free(db);
+    irwls_dev_free(dx);
 }
 
 static void vector_matrix(double *m1, int size, double *m2, double *result, int trans) {

@@ -96,6 +96,7 @@ def load():
         "irwls_dev_sync": (None, []),
         "irwls_dev_potrf": (C.c_int, [vp, C.c_int, C.c_int]),
         "irwls_dev_potrs": (None, [vp, C.c_int, C.c_int, vp]),
+        "irwls_dev_posv_aug": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "irwls_dev_kernel_matrix": (None, [P(irwls_rows), P(irwls_rows), C.c_int, C.c_int, C.c_double,
                                            vp, C.c_int]),
         "irwls_dev_kernel_matvec": (None, [P(irwls_rows), P(irwls_rows), C.c_int, C.c_int, C.c_double,

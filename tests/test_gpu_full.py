@@ -185,6 +185,35 @@ def test_kkt_failed_last_pivot(n):
     np.testing.assert_allclose(x, po.linear_system(H, r), rtol=1e-8, atol=1e-10)
 
 
+@pytest.mark.parametrize("n", [1, 2, 64, 65, 130, 300])
+def test_posv_aug_kkt(n):
+    """Factorization with the right-hand side riding in row n (the forward
+    sweep done by the factorization steps) plus one backward sweep, on the
+    KKT matrix with its failed last pivot."""
+    rng = np.random.default_rng(200 + n)
+    m = n - 1
+    H = np.zeros((n, n))
+    if m > 0:
+        X = rng.standard_normal((m, 5))
+        y = rng.choice([-1.0, 1.0], m)
+        H[:m, :m] = rbf(X, X, 0.2) * np.outer(y, y) + np.diag(rng.uniform(0.01, 1.0, m))
+        H[:m, m] = y
+        H[m, :m] = y
+    r = rng.standard_normal(n)
+    L = lib.load()
+    buf = np.zeros((n + 1, n), order="F")
+    buf[:n, :n] = H
+    Ad = Dev(buf.ravel(order="F"))
+    rd, xd = Dev(r), Dev(np.zeros(n))
+    info = L.irwls_dev_posv_aug(Ad.p, n, n + 1, rd.p, xd.p)
+    assert info == n
+    want = po.linear_system(H, r)
+    if m == 0:
+        assert not np.isfinite(xd.get()[0]) and not np.isfinite(want[0])
+    else:
+        np.testing.assert_allclose(xd.get(), want, rtol=1e-8, atol=1e-10)
+
+
 def test_parallel_linear_system_entry_point():
     rng = np.random.default_rng(5)
     n = 150
