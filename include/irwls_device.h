@@ -118,6 +118,35 @@ void irwls_dev_potrs(const double *d_L, int n, int lda, double *d_b);
    irwls_dev_potrf (the solve uses the factor as it stands).  Returns info. */
 int  irwls_dev_posv_aug(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
 
+/* ---- budgeted training (SGMA, parametric IRWLS) ------------------------- */
+/* C = alpha op(A) op(B) + beta C, all column-major (op = transpose when set). */
+void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, const double *d_A, int lda,
+                    const double *d_B, int ldb, double beta, double *d_C, int ldc);
+/* q[c] = sum_i (KSM[i*ldm + c] - sum_p KSC[i + p*n] Z[p + c*ldz])^2, c < ncand:
+   the SGMA residual of every candidate column and its squared norm. */
+void irwls_dev_sgma_gain(int n, int s, int ncand, const double *d_KSC, const double *d_Z, int ldz,
+                         const double *d_KSM, int ldm, double *d_q);
+/* eta[c] = 1 - sum_{e<s} KNC[c*ldk + e] Z[e + c*ldz] */
+void irwls_dev_sgma_eta(int s, int ncand, const double *d_KNC, int ldk, const double *d_Z, int ldz, double *d_eta);
+/* dst[i] = src[i*ld + col] */
+void irwls_dev_gather_col(const double *d_src, int ld, int col, int n, double *d_dst);
+/* M[c*ld + row] = v[c], c < count */
+void irwls_dev_scatter_row(double *d_M, int ld, int row, int count, const double *d_v);
+/* K1 (ld ldk) = KC + sum_q (sw_q Phi[a_q])^T (sw_q Phi[a_q]) over the nact
+   active rows a (NULL: rows 0..nact-1) of the row-major n x B matrix Phi;
+   sw NULL means weight 1. */
+void irwls_dev_wgram(int B, const double *d_Phi, const int32_t *d_act, const double *d_sw, int nact,
+                     const double *d_KC, double *d_K1, int ldk);
+/* K2[r] = sum_q (sw_q Phi[a_q][r]) (sw_q ymul y[a_q]) */
+void irwls_dev_wvec(int B, const double *d_Phi, const int32_t *d_act, const double *d_sw, const double *d_y,
+                    double ymul, int nact, double *d_K2);
+/* e = y - Phi beta */
+void irwls_dev_resid_rows(int n, int B, const double *d_Phi, const double *d_beta, const double *d_y, double *d_e);
+/* D_i = C/(y_i e_i) clipped to M (0 when y_i e_i < 0); active rows (D != 0)
+   compacted in row order into act with sw = sqrt(D).  Returns their count. */
+int  irwls_dev_irls_weights(int n, const double *d_e, const double *d_y, double C, double M, double *d_D,
+                            int32_t *d_act, double *d_sw);
+
 #ifdef __cplusplus
 }
 #endif

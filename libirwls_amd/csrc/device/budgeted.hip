@@ -1,0 +1,484 @@
+// Budgeted-training kernels for gfx950 (SGMA basis selection and the
+// parametric IRWLS), FP64 on the matrix cores where the work is GEMM-shaped.
+//
+//   gemm_f64       general C = alpha op(A) op(B) + beta C, column-major,
+//                  64x64 tiles staged through LDS, v_mfma_f64_16x16x4f64;
+//   sgma_gain      R = KSM - KSC Z fused with the per-candidate sum of R^2
+//                  (src/budgeted-train.c:195-198): KSC streams once per step;
+//   wgram          K1 += sum_i w_i phi_i phi_i^T over the active rows, split
+//                  over the row range with a deterministic reduction
+//                  (src/budgeted-train.c:401-416);
+//   resid_rows     e = y - Phi beta, one wave per row (src/budgeted-train.c:433-443);
+//   irls_weights   D_i = C/(y_i e_i) clipped to M, 0 on the wrong side, and the
+//                  active-row compaction in row order (src/budgeted-train.c:448-465).
+#include "common.h"
+
+using namespace irwls;
+
+namespace {
+
+constexpr int T = 64;
+constexpr int BK = 16;
+constexpr int LDP = BK + 1;
+
+// element (row, k) = M[(kc0 + k) * ld + r0 + row]   (k-outer layout)
+__device__ __forceinline__ void ld_kouter(double (*S)[LDP], const double *M, int ld, int r0, int rmax, int kc0,
+                                          int kmax) {
+    const int t = threadIdx.x;
+    const int row = t & 63;
+    const int kq = (t >> 6) * 4;
+    const bool rok = (r0 + row) < rmax;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int k = kc0 + kq + q;
+        S[row][kq + q] = (rok && k < kmax) ? M[(size_t)k * ld + r0 + row] : 0.0;
+    }
+}
+
+// element (row, k) = M[(r0 + row) * ld + kc0 + k]   (row-outer layout)
+__device__ __forceinline__ void ld_router(double (*S)[LDP], const double *M, int ld, int r0, int rmax, int kc0,
+                                          int kmax) {
+    const int t = threadIdx.x;
+    const int row = t >> 2;
+    const int kq = (t & 3) * 4;
+    const bool rok = (r0 + row) < rmax;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int k = kc0 + kq + q;
+        S[row][kq + q] = (rok && k < kmax) ? M[(size_t)(r0 + row) * ld + k] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void mfma_chunk(v4d acc[2][2], double (*As)[LDP], double (*Bs)[LDP]) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ks++) {
+        const int kk = ks * 4 + (lane >> 4);
+        const double a0 = As[wr * 32 + (lane & 15)][kk];
+        const double a1 = As[wr * 32 + 16 + (lane & 15)][kk];
+        const double b0 = Bs[wc * 32 + (lane & 15)][kk];
+        const double b1 = Bs[wc * 32 + 16 + (lane & 15)][kk];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+}
+
+// tile element owned by (mb, nb, r) of this lane
+__device__ __forceinline__ int tile_row(int mb, int r) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave >> 1) * 32 + mb * 16 + (lane >> 4) + 4 * r;
+}
+__device__ __forceinline__ int tile_col(int nb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave & 1) * 32 + nb * 16 + (lane & 15);
+}
+
+// ------------------------------------------------------------------- gemm ---
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int m, int n, int k, double alpha, const double *__restrict__ A,
+                                                       int lda, const double *__restrict__ B, int ldb, double beta,
+                                                       double *__restrict__ C, int ldc) {
+    __shared__ double As[T][LDP];
+    __shared__ double Bs[T][LDP];
+    const int r0 = blockIdx.x * T, c0 = blockIdx.y * T;
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int kc = 0; kc < k; kc += BK) {
+        // op(A)(i, p): !TA -> A[i + p*lda] (k-outer); TA -> A[p + i*lda] (row-outer)
+        if (!TA) ld_kouter(As, A, lda, r0, m, kc, k);
+        else ld_router(As, A, lda, r0, m, kc, k);
+        // op(B)(p, j) as (row j, k p): !TB -> B[p + j*ldb] (row-outer); TB -> B[j + p*ldb] (k-outer)
+        if (!TB) ld_router(Bs, B, ldb, c0, n, kc, k);
+        else ld_kouter(Bs, B, ldb, c0, n, kc, k);
+        __syncthreads();
+        mfma_chunk(acc, As, Bs);
+        __syncthreads();
+    }
+    for (int nb = 0; nb < 2; nb++) {
+        const int j = c0 + tile_col(nb);
+        if (j >= n) continue;
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) {
+                const int i = r0 + tile_row(mb, r);
+                if (i >= m) continue;
+                double *p = C + (size_t)j * ldc + i;
+                *p = (beta == 0.0) ? alpha * acc[mb][nb][r] : alpha * acc[mb][nb][r] + beta * (*p);
+            }
+    }
+}
+
+// -------------------------------------------------------------- sgma_gain ---
+// R(i, c) = KSM[i*ldm + c] - sum_p KSC[i + p*n] Z[p + c*ldz];
+// part[rowtile * ncand + c] = sum over the tile's rows of R(i, c)^2.
+__global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
+                                                        const double *__restrict__ Z, int ldz,
+                                                        const double *__restrict__ KSM, int ldm,
+                                                        double *__restrict__ part) {
+    __shared__ double As[T][LDP];
+    __shared__ double Bs[T][LDP];
+    __shared__ double red[2][T];
+    const int r0 = blockIdx.x * T, c0 = blockIdx.y * T;
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int kc = 0; kc < s; kc += BK) {
+        ld_kouter(As, KSC, n, r0, n, kc, s);
+        ld_router(Bs, Z, ldz, c0, ncand, kc, s);
+        __syncthreads();
+        mfma_chunk(acc, As, Bs);
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int nb = 0; nb < 2; nb++) {
+        const int c = c0 + tile_col(nb);
+        double q = 0.0;
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) {
+                const int i = r0 + tile_row(mb, r);
+                if (i < n && c < ncand) {
+                    const double v = KSM[(size_t)i * ldm + c] - acc[mb][nb][r];
+                    q += v * v;
+                }
+            }
+        // lanes l, l^16, l^32, l^48 hold the same column
+        q += __shfl_xor(q, 16);
+        q += __shfl_xor(q, 32);
+        if (lane < 16) red[wave >> 1][(wave & 1) * 32 + nb * 16 + lane] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x < T) {
+        const int c = c0 + threadIdx.x;
+        if (c < ncand) part[(size_t)blockIdx.x * ncand + c] = red[0][threadIdx.x] + red[1][threadIdx.x];
+    }
+}
+
+// fixed-order column sums of the partials
+__global__ void col_sum_kernel(const double *__restrict__ part, int nparts, int ncol, double *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[(size_t)p * ncol + c];
+    out[c] = s;
+}
+
+// eta_c = 1 - sum_e KNC[c*ldk + e] Z[e + c*ldz], one wave per candidate
+__global__ void sgma_eta_kernel(int s, int ncand, const double *__restrict__ KNC, int ldk,
+                                const double *__restrict__ Z, int ldz, double *__restrict__ eta) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncand) return;
+    double v = 0.0;
+    for (int e = lane; e < s; e += 64) v += KNC[(size_t)c * ldk + e] * Z[e + (size_t)c * ldz];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) eta[c] = 1.0 - v;
+}
+
+// dst[i] = src[i*ld + col]
+__global__ void gather_col_kernel(const double *__restrict__ src, int ld, int col, int n, double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(size_t)i * ld + col];
+}
+
+// M[c*ld + row] = v[c] for c < count
+__global__ void scatter_row_kernel(double *__restrict__ M, int ld, int row, int count, const double *__restrict__ v) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < count) M[(size_t)c * ld + row] = v[c];
+}
+
+// ----------------------------------------------------------------- wgram ---
+// Partial Gram matrices over row chunks: part[split][c*B + r] (lower tiles,
+// mirrored) = sum_{q in chunk} w_q Phi[a_q][r] Phi[a_q][c] with
+// w_q = sqrt(D[a_q])^2 taken as the product of the two scaled rows, as the
+// reference scales the rows by sqrt(D) before the product; a = active list
+// (NULL = all rows, weights 1).  Phi is row-major n x B.
+__device__ __forceinline__ void ld_rows_scaled(double (*S)[LDP], const double *Phi, int B, int r0, const int32_t *act,
+                                               const double *sw, int q0, int qmax) {
+    const int t = threadIdx.x;
+    const int row = t & 63;           // centroid index within the tile
+    const int kq = (t >> 6) * 4;      // 4 consecutive active rows
+    const bool rok = (r0 + row) < B;
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) {
+        const int q = q0 + kq + qq;
+        double v = 0.0;
+        if (rok && q < qmax) {
+            const int src = act ? act[q] : q;
+            v = Phi[(size_t)src * B + r0 + row];
+            if (sw) v *= sw[q];
+        }
+        S[row][kq + qq] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restrict__ Phi, const int32_t *__restrict__ act,
+                                                    const double *__restrict__ sw, int nact, int chunk,
+                                                    double *__restrict__ part) {
+    __shared__ double As[T][LDP];
+    __shared__ double Bs[T][LDP];
+    const int tj = blockIdx.x, ti = blockIdx.y;
+    if (ti < tj) return;
+    const int split = blockIdx.z;
+    const int q0 = split * chunk;
+    int q1 = q0 + chunk;
+    if (q1 > nact) q1 = nact;
+    const int rI = ti * T, rJ = tj * T;
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int q = q0; q < q1; q += BK) {
+        ld_rows_scaled(As, Phi, B, rI, act, sw, q, q1);
+        ld_rows_scaled(Bs, Phi, B, rJ, act, sw, q, q1);
+        __syncthreads();
+        mfma_chunk(acc, As, Bs);
+        __syncthreads();
+    }
+    double *P = part + (size_t)split * B * B;
+    for (int nb = 0; nb < 2; nb++) {
+        const int j = rJ + tile_col(nb);
+        if (j >= B) continue;
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) {
+                const int i = rI + tile_row(mb, r);
+                if (i >= B || i < j) continue;
+                P[(size_t)j * B + i] = acc[mb][nb][r];
+                P[(size_t)i * B + j] = acc[mb][nb][r];
+            }
+    }
+}
+
+// K1 = KC + sum_splits part (fixed order); K1 has leading dimension ldk
+__global__ void wgram_finish(int B, int nsplit, const double *__restrict__ part, const double *__restrict__ KC,
+                             double *__restrict__ K1, int ldk) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t BB = (size_t)B * B;
+    if (e >= BB) return;
+    double s = 0.0;
+    for (int p = 0; p < nsplit; p++) s += part[(size_t)p * BB + e];
+    const size_t c = e / B, r = e % B;
+    K1[c * ldk + r] = KC[e] + s;
+}
+
+// K2[r] = sum_q (sw_q Phi[a_q][r]) (sw_q y_q) over the active rows; one thread
+// per centroid, rows in order (deterministic).
+__global__ void wvec_kernel(int B, const double *__restrict__ Phi, const int32_t *__restrict__ act,
+                            const double *__restrict__ sw, const double *__restrict__ yv, double ymul, int nact,
+                            int chunk, double *__restrict__ part) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int split = blockIdx.y;
+    if (r >= B) return;
+    const int q0 = split * chunk;
+    int q1 = q0 + chunk;
+    if (q1 > nact) q1 = nact;
+    double s = 0.0;
+    for (int q = q0; q < q1; q++) {
+        const int src = act ? act[q] : q;
+        const double w = sw ? sw[q] : 1.0;
+        s += (w * Phi[(size_t)src * B + r]) * (w * (ymul * yv[src]));
+    }
+    part[(size_t)split * B + r] = s;
+}
+
+// e_i = y_i - sum_r Phi[i][r] beta_r, one wave per row
+__global__ __launch_bounds__(256) void resid_rows_kernel(int n, int B, const double *__restrict__ Phi,
+                                                         const double *__restrict__ beta, const double *__restrict__ yv,
+                                                         double *__restrict__ e) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double *row = Phi + (size_t)i * B;
+    double v = 0.0;
+    for (int r = lane; r < B; r += 64) v += beta[r] * row[r];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) e[i] = yv[i] - v;
+}
+
+// D_i from e (src/budgeted-train.c:448-457) and per-block active counts
+__global__ void irls_weights_kernel(int n, const double *__restrict__ e, const double *__restrict__ yv, double C,
+                                    double M, double *__restrict__ D, int *__restrict__ blockcount) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double d;
+        if (e[i] * yv[i] < 0.0) d = 0.0;
+        else d = 1.0 * C / (yv[i] * e[i]);
+        if (d > M) d = M;
+        D[i] = d;
+        if (d != 0.0) atomicAdd(&cnt, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) blockcount[blockIdx.x] = cnt;
+}
+
+// exclusive scan of the block counts (single work-group, sequential chunks)
+__global__ void scan_counts_kernel(int nb, const int *__restrict__ bc, int *__restrict__ off, int *__restrict__ total) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nb) ? bc[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int add = (threadIdx.x >= o) ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nb) off[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// ordered compaction: act[off_b + rank] = i, sw = sqrt(D_i)
+__global__ void compact_kernel(int n, const double *__restrict__ D, const int *__restrict__ off,
+                               int32_t *__restrict__ act, double *__restrict__ sw) {
+    __shared__ int flags[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = (i < n && D[i] != 0.0) ? 1 : 0;
+    flags[threadIdx.x] = f;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        int add = (threadIdx.x >= o) ? flags[threadIdx.x - o] : 0;
+        __syncthreads();
+        flags[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (f) {
+        const int pos = off[blockIdx.x] + flags[threadIdx.x] - 1;
+        act[pos] = i;
+        sw[pos] = sqrt(D[i]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, const double *A, int lda,
+                    const double *B, int ldb, double beta, double *C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    dim3 grid((m + T - 1) / T, (n + T - 1) / T);
+    if (!transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (!transA && transB) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
+                         int ldm, double *qout) {
+    if (n <= 0 || ncand <= 0) return;
+    const int rt = (n + T - 1) / T, ct = (ncand + T - 1) / T;
+    double *part = (double *)scratch(10, (size_t)rt * ncand * sizeof(double));
+    {
+        TimerScope ts(IRWLS_T_SGMA, 2.0 * n * (double)s * ncand, 8.0 * (double)n * s + 8.0 * (double)n * ncand);
+        hipLaunchKernelGGL(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz, KSM, ldm,
+                           part);
+        IRWLS_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(col_sum_kernel, dim3((ncand + 255) / 256), dim3(256), 0, stream(), part, rt, ncand, qout);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_eta(int s, int ncand, const double *KNC, int ldk, const double *Z, int ldz, double *eta) {
+    if (ncand <= 0) return;
+    hipLaunchKernelGGL(sgma_eta_kernel, dim3((ncand + 3) / 4), dim3(256), 0, stream(), s, ncand, KNC, ldk, Z, ldz, eta);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_gather_col(const double *src, int ld, int col, int n, double *dst) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather_col_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), src, ld, col, n, dst);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_scatter_row(double *M, int ld, int row, int count, const double *v) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(scatter_row_kernel, dim3((count + 255) / 256), dim3(256), 0, stream(), M, ld, row, count, v);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_wgram(int B, const double *Phi, const int32_t *act, const double *sw, int nact, const double *KC,
+                     double *K1, int ldk) {
+    const int tb = (B + T - 1) / T;
+    // enough splits to fill the device: ~4 work-groups per CU
+    int nsplit = 1;
+    const int tiles = tb * (tb + 1) / 2;
+    while (tiles * nsplit < 1024 && nact / (nsplit * 2) >= 256) nsplit *= 2;
+    int chunk = (nact + nsplit - 1) / nsplit;
+    chunk = (chunk + BK - 1) / BK * BK;
+    if (chunk < BK) chunk = BK;
+    nsplit = nact > 0 ? (nact + chunk - 1) / chunk : 1;
+    double *part = (double *)scratch(11, (size_t)nsplit * B * B * sizeof(double));
+    {
+        TimerScope ts(IRWLS_T_SYRKD, (double)nact * B * (double)(B + 1), 8.0 * (double)nact * B);
+        if (nact > 0) {
+            hipLaunchKernelGGL(wgram_kernel, dim3(tb, tb, nsplit), dim3(256), 0, stream(), B, Phi, act, sw, nact, chunk,
+                               part);
+            IRWLS_LAUNCH_CHECK();
+        } else {
+            irwls_dev_memset0(part, (size_t)B * B * sizeof(double));
+        }
+    }
+    const size_t BB = (size_t)B * B;
+    hipLaunchKernelGGL(wgram_finish, dim3((unsigned)((BB + 255) / 256)), dim3(256), 0, stream(), B,
+                       nact > 0 ? nsplit : 1, part, KC, K1, ldk);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_wvec(int B, const double *Phi, const int32_t *act, const double *sw, const double *y, double ymul,
+                    int nact, double *K2) {
+    int nsplit = 64;
+    int chunk = (nact + nsplit - 1) / nsplit;
+    if (chunk < 1) chunk = 1;
+    nsplit = nact > 0 ? (nact + chunk - 1) / chunk : 1;
+    double *part = (double *)scratch(12, (size_t)nsplit * B * sizeof(double));
+    if (nact > 0) {
+        hipLaunchKernelGGL(wvec_kernel, dim3((B + 255) / 256, nsplit), dim3(256), 0, stream(), B, Phi, act, sw, y, ymul,
+                           nact, chunk, part);
+        IRWLS_LAUNCH_CHECK();
+    } else {
+        irwls_dev_memset0(part, (size_t)B * sizeof(double));
+    }
+    hipLaunchKernelGGL(col_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, stream(), part, nsplit, B, K2);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_resid_rows(int n, int B, const double *Phi, const double *beta, const double *y, double *e) {
+    if (n <= 0) return;
+    TimerScope ts(IRWLS_T_MATVEC, 2.0 * n * (double)B, 8.0 * (double)n * B + 16.0 * n + 8.0 * B);
+    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), n, B, Phi, beta, y, e);
+    IRWLS_LAUNCH_CHECK();
+}
+
+int irwls_dev_irls_weights(int n, const double *e, const double *y, double C, double M, double *D, int32_t *act,
+                           double *sw) {
+    if (n <= 0) return 0;
+    const int nb = (n + 255) / 256;
+    int *bc = (int *)scratch(13, (size_t)(2 * nb + 2) * sizeof(int));
+    int *off = bc + nb;
+    int *total = bc + 2 * nb;
+    hipLaunchKernelGGL(irls_weights_kernel, dim3(nb), dim3(256), 0, stream(), n, e, y, C, M, D, bc);
+    IRWLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream(), nb, bc, off, total);
+    IRWLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(256), 0, stream(), n, D, off, act, sw);
+    IRWLS_LAUNCH_CHECK();
+    int h = 0;
+    irwls_dev_d2h(&h, total, sizeof(int));
+    return h;
+}
+
+}  // extern "C"

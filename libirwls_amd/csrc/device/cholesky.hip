@@ -2,12 +2,12 @@
 //
 // Column-major lower factor in place, block size 64.  One step per block
 // column k is two launches:
-//   chol_step: wave 0 of every work-group factors the 64x64 diagonal block in
-//              registers (lane i holds row i; column entries are broadcast
-//              with v_readlane), then every thread solves one panel row
-//              below it by forward substitution against the factor in LDS;
-//   syrk:      the trailing lower tiles take the rank-64 update on the FP64
-//              matrix cores (v_mfma_f64_16x16x4f64).
+//   trsm:  the panel below the diagonal tile is multiplied by the inverse of
+//          the factored tile (FP64 MFMA, v_mfma_f64_16x16x4f64);
+//   syrk:  the trailing lower tiles take the rank-64 update (FP64 MFMA); the
+//          work-group that owns the next diagonal tile then factors it and
+//          forms its inverse in LDS (look-ahead), so the serial part of step
+//          k+1 overlaps the trailing update of step k.
 // Right-hand sides can ride along as extra rows below the matrix
 // (rows n..nrows-1): the same steps then perform the forward sweep L z = r,
 // so a solve only adds the backward sweep.
@@ -46,76 +46,84 @@ __device__ __forceinline__ double rdl(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// ----------------------------------------------------------- chol_step ---
-// Diagonal block [k0, k0+kb) and panel rows [k0+kb, nrows).  n = factored
-// order (rows >= n are right-hand sides).
-__global__ __launch_bounds__(256) void chol_step_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
-                                                        int kb, int *__restrict__ info) {
-    __shared__ double Ls[NB][NB + 1];
-    __shared__ int s_fail;
-    const int prev = info[0];
-    if (prev != 0 && prev <= k0) return;   // stopped at an earlier block
-    const int t = threadIdx.x;
-    const int lane = t & 63;
-    if (t < 64) {
-        // lane i holds row i of the lower diagonal block
-        double a[NB];
-#pragma unroll
-        for (int c = 0; c < NB; c++)
-            a[c] = (lane < kb && c < kb && c <= lane) ? A[(size_t)(k0 + c) * lda + k0 + lane] : 0.0;
-        int fail = -1;
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-            if (j < kb && fail < 0) {
-                const double ajj = rdl(a[j], j);
-                if (!(ajj > 0.0)) {
-                    fail = j;           // uniform; lane j keeps the Schur value
-                } else {
-                    const double ljj = sqrt(ajj);
-                    a[j] = (lane == j) ? ljj : a[j] / ljj;
-#pragma unroll
-                    for (int c = j + 1; c < NB; c++) {
-                        if (c < kb) {
-                            const double lcj = rdl(a[j], c);
-                            a[c] -= a[j] * lcj;   // entries above the diagonal are never read
-                        }
-                    }
-                }
-            }
+// ------------------------------------------------- diagonal tile factor ---
+// Factor the lower kb x kb tile held in S (lower part) and form its inverse,
+// with all 256 threads: lanes run over rows (factor) or columns (inverse),
+// the four waves split every inner product, one barrier pair per column.
+// S packs both results: S[i][c] = L(i,c) for c <= i and S[c][i+1] =
+// Linv(i,c) for i >= c.  Returns the failing column or -1 (the Schur value
+// is left at L(j,j); the inverse is still formed from the factor as it
+// stands).
+__device__ int diag_factor_inverse(double (*S)[NB + 1], double (*part)[NB], int kb) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int fail = -1;
+    for (int j = 0; j < kb; j++) {
+        // s_i = A(i,j) - sum_{k<j} L(i,k) L(j,k), k split over the waves
+        double acc = 0.0;
+        const int q = (j + 3) >> 2;
+        const int kbeg = wave * q, kend = min(j, kbeg + q);
+        for (int k = kbeg; k < kend; k++) acc += S[lane][k] * S[j][k];
+        part[wave][lane] = acc;
+        __syncthreads();
+        const double sjj = S[j][j] - (((part[0][j] + part[1][j]) + part[2][j]) + part[3][j]);
+        if (!(sjj > 0.0)) {             // uniform
+            __syncthreads();
+            if (t == 0) S[j][j] = sjj;
+            fail = j;
+            __syncthreads();
+            break;
         }
-#pragma unroll
-        for (int c = 0; c < NB; c++) Ls[lane][c] = (c <= lane) ? a[c] : 0.0;
-        if (blockIdx.x == 0 && lane < kb) {
-#pragma unroll
-            for (int c = 0; c < NB; c++)
-                if (c < kb && c <= lane) A[(size_t)(k0 + c) * lda + k0 + lane] = a[c];
+        const double ljj = sqrt(sjj);
+        double si = 0.0;
+        if (wave == 0 && lane > j && lane < kb)
+            si = S[lane][j] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+        __syncthreads();
+        if (wave == 0 && lane >= j && lane < kb) S[lane][j] = (lane == j) ? ljj : si / ljj;
+        __syncthreads();
+    }
+    // inverse, column c = lane; row i: x(i,c) = (delta - sum_{c<=k<i} L(i,k) x(k,c)) / L(i,i)
+    for (int i = 0; i < kb; i++) {
+        const int c = lane;
+        double acc = 0.0;
+        if (c < i) {
+            const int len = i - c;
+            const int q = (len + 3) >> 2;
+            const int kbeg = c + wave * q, kend = min(i, kbeg + q);
+            for (int k = kbeg; k < kend; k++) acc += S[i][k] * S[c][k + 1];
         }
-        if (lane == 0) {
-            s_fail = fail;
-            if (blockIdx.x == 0 && fail >= 0) info[0] = k0 + fail + 1;
+        part[wave][lane] = acc;
+        __syncthreads();
+        if (wave == 0 && c <= i) {
+            const double s = ((c == i) ? 1.0 : 0.0) - (((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]);
+            S[c][i + 1] = s / S[i][i];
         }
+        __syncthreads();
+    }
+    return fail;
+}
+
+// Write the factored tile (lower part) back and the inverse to Dinv
+// (column-major 64x64, zero outside the lower kb x kb triangle).
+__device__ void store_factor(double (*S)[NB + 1], double *A, int lda, int k0, int kb, double *Dinv) {
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int i = e % NB, c = e / NB;
+        if (i < kb && c < kb && c <= i) A[(size_t)(k0 + c) * lda + k0 + i] = S[i][c];
+        Dinv[(size_t)c * NB + i] = (i < kb && c < kb && c <= i) ? S[c][i + 1] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void diag_first_kernel(double *__restrict__ A, int lda, int kb,
+                                                         int *__restrict__ info, double *__restrict__ Dinv) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ double part[4][NB];
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int i = e % NB, c = e / NB;
+        S[i][c] = (i < kb && c < kb && c <= i) ? A[(size_t)c * lda + i] : 0.0;
     }
     __syncthreads();
-    const int fail = s_fail;
-    // panel rows; after a failure only the right-hand-side rows continue
-    const int row0 = (fail >= 0) ? ((n > k0 + kb) ? n : k0 + kb) : k0 + kb;
-    const int r = k0 + kb + blockIdx.x * 256 + t;
-    if (r < row0 || r >= nrows) return;
-    double x[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) x[j] = (j < kb) ? A[(size_t)(k0 + j) * lda + r] : 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        if (j < kb) {
-            double s = x[j];
-#pragma unroll
-            for (int k = 0; k < j; k++) s -= x[k] * Ls[j][k];
-            x[j] = s / Ls[j][j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-        if (j < kb) A[(size_t)(k0 + j) * lda + r] = x[j];
+    const int fail = diag_factor_inverse(S, part, kb);
+    store_factor(S, A, lda, 0, kb, Dinv);
+    if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
 // Loads a 64 (rows) x 16 (k) chunk of a column-major matrix:
@@ -164,17 +172,56 @@ __device__ __forceinline__ void mfma_cm(v4d acc[2][2], double (*As)[NB][LDP], do
     }
 }
 
-// ----------------------------------------------------------------- syrk ---
-// Trailing tiles: A(I, J) -= P_I P_J^T, rows [t0, nrows) x columns [t0, n),
-// P = A(:, k0:k0+kb), t0 = k0 + kb; grid (column tiles, row tiles), tiles
-// strictly above the diagonal return at once.
-__global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
-                                                   int kb, const int *__restrict__ info) {
+// ----------------------------------------------------------------- trsm ---
+// Panel rows [k0+kb, nrows): P = A(:, k0:k0+kb) * Linv^T, in place.  After
+// the factorization has failed in this block only right-hand-side rows
+// (rows >= n) are solved; after a failure in an earlier block nothing is.
+__global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                   const int *__restrict__ info, const double *__restrict__ Dinv) {
     __shared__ double As[1][NB][LDP];
     __shared__ double Bs[1][NB][LDP];
+    const int inf = info[0];
+    if (inf != 0 && inf <= k0) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int r0 = k0 + kb + blockIdx.x * NB;
+    const int rmin = (inf != 0) ? n : 0;
+    if (r0 + NB <= rmin) return;
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    // Q(j, k) = Linv(j, k) = Dinv[k*64 + j]
+    mfma_cm(acc, As, Bs, A + (size_t)k0 * lda, lda, r0, nrows, Dinv, NB, 0, kb, 0, kb);
+    for (int nb = 0; nb < 2; nb++) {
+        const int c = wc * 32 + nb * 16 + (lane & 15);
+        if (c >= kb) continue;
+        for (int mb = 0; mb < 2; mb++)
+            for (int r = 0; r < 4; r++) {
+                const int i = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
+                if (i < nrows && i >= rmin) A[(size_t)(k0 + c) * lda + i] = acc[mb][nb][r];
+            }
+    }
+}
+
+// ----------------------------------------------------------- syrk + next ---
+// Trailing tiles: A(I, J) -= P_I P_J^T, rows [t0, nrows) x columns [t0, n),
+// P = A(:, k0:k0+kb) (solved), t0 = k0 + kb; grid (column tiles, row tiles),
+// tiles above the diagonal return at once.  Work-group (0,0) owns the next
+// diagonal tile: after its update it factors and inverts it (look-ahead), so
+// the next step needs no separate launch for that.
+__global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
+                                                        int kb, int *__restrict__ info, double *__restrict__ Dnext) {
+    __shared__ double As[1][NB][LDP];
+    __shared__ double Bs[1][NB][LDP];
+    __shared__ double S[NB][NB + 1];
+    __shared__ double part[4][NB];
     const int tj = blockIdx.x, ti = blockIdx.y;
-    if (ti < tj || info[0] != 0) return;
     const int t0 = k0 + kb;
+    // stop only for a failure in blocks <= k (pivot index < t0); work-group
+    // (0,0) may record a failure of block k+1 during this launch
+    const int inf = info[0];
+    if (ti < tj || (inf != 0 && inf <= t0)) return;
     const int rI = t0 + ti * NB, rJ = t0 + tj * NB;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -183,15 +230,29 @@ __global__ __launch_bounds__(256) void syrk_kernel(double *__restrict__ A, int n
     for (int a = 0; a < 2; a++)
         for (int c = 0; c < 2; c++) acc[a][c] = (v4d){0.0, 0.0, 0.0, 0.0};
     mfma_cm(acc, As, Bs, A, lda, rI, nrows, A, lda, rJ, n, k0, kb);
+    const bool diag = (ti == 0 && tj == 0);
+    const int kb2 = (n - t0 < NB) ? n - t0 : NB;
     for (int nb = 0; nb < 2; nb++) {
-        const int j = rJ + wc * 32 + nb * 16 + (lane & 15);
-        if (j >= n) continue;
+        const int jl = wc * 32 + nb * 16 + (lane & 15);
+        const int j = rJ + jl;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int i = rI + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
-                if (i < nrows && i >= j) A[(size_t)j * lda + i] -= acc[mb][nb][r];
+                const int il = wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
+                const int i = rI + il;
+                const bool in_tile = diag && i < t0 + kb2;   // matrix rows of the next diagonal tile
+                if (j < n && i < nrows && i >= j) {
+                    const double v = A[(size_t)j * lda + i] - acc[mb][nb][r];
+                    if (in_tile) S[il][jl] = v;
+                    else A[(size_t)j * lda + i] = v;   // other tiles and right-hand-side rows
+                }
+                if (diag && !(in_tile && j < n && i >= j)) S[il][jl] = 0.0;
             }
     }
+    if (!diag) return;
+    __syncthreads();
+    const int fail = diag_factor_inverse(S, part, kb2);
+    store_factor(S, A, lda, t0, kb2, Dnext);
+    if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
 // -------------------------------------------------------- triangular sweeps ---
@@ -285,19 +346,37 @@ __global__ void scatter_row(double *__restrict__ A, int lda, int row, int n, con
     if (i < n) A[(size_t)i * lda + row] = in[i];
 }
 
+double *g_dinv = nullptr;
+int g_dinv_blocks = 0;
+
+double *dinv_buf(int nblk) {
+    if (g_dinv_blocks < nblk) {
+        if (g_dinv) irwls_dev_free(g_dinv);
+        g_dinv_blocks = nblk < 16 ? 16 : nblk;
+        g_dinv = (double *)irwls_dev_alloc((size_t)g_dinv_blocks * NB * NB * sizeof(double));
+    }
+    return g_dinv;
+}
+
 void factor(double *A, int n, int nrows, int lda) {
     int *info = info_buf();
     irwls_dev_memset0(info, 4 * sizeof(int));
     const int nblk = (n + NB - 1) / NB;
+    double *Dinv = dinv_buf(nblk);
+    {
+        const int kb = (n < NB) ? n : NB;
+        TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0, 8.0 * (double)kb * kb);
+        hipLaunchKernelGGL(diag_first_kernel, dim3(1), dim3(256), 0, stream(), A, lda, kb, info, Dinv);
+        IRWLS_LAUNCH_CHECK();
+    }
     for (int kbk = 0; kbk < nblk; kbk++) {
         const int k0 = kbk * NB;
         const int kb = (n - k0 < NB) ? (n - k0) : NB;
         const int prow = nrows - k0 - kb;
-        const int g1 = prow > 0 ? (prow + 255) / 256 : 1;
-        {
-            TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0 + (double)(prow > 0 ? prow : 0) * kb * kb,
-                          8.0 * (double)kb * kb + 16.0 * (prow > 0 ? prow : 0) * kb);
-            hipLaunchKernelGGL(chol_step_kernel, dim3(g1), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info);
+        if (prow > 0) {
+            TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, 16.0 * prow * kb);
+            hipLaunchKernelGGL(trsm_kernel, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda, k0,
+                               kb, info, Dinv + (size_t)kbk * NB * NB);
             IRWLS_LAUNCH_CHECK();
         }
         const int rest = n - k0 - kb;
@@ -306,7 +385,8 @@ void factor(double *A, int n, int nrows, int lda) {
         const int tr = (nrows - k0 - kb + NB - 1) / NB;
         TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
                       8.0 * (double)rest * rest);
-        hipLaunchKernelGGL(syrk_kernel, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info);
+        hipLaunchKernelGGL(syrk_next_kernel, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info,
+                           Dinv + (size_t)(kbk + 1) * NB * NB);
         IRWLS_LAUNCH_CHECK();
     }
 }

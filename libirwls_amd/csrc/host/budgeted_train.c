@@ -1,0 +1,407 @@
+/*
+ * Budgeted training: SGMA basis selection and the parametric IRWLS, host
+ * control in C over the device layer.
+ *
+ * The host keeps the reference's control flow, its serial libc rand()
+ * draws (src/budgeted-train.c:84,160-165) and its strict-> argmax
+ * (:208-216); the device evaluates every kernel block and the dense algebra:
+ *   candidate kernel columns   :180        -> irwls_dev_kernel_matrix
+ *   z = K_C^-1 k_C, eta        :182-194    -> two GEMMs with L^-1, irwls_dev_sgma_eta
+ *   R = KSM - KSC z, |R|^2     :195-198    -> irwls_dev_sgma_gain (fused)
+ *   L, L^-1 growth             :247-279    -> GEMVs with L^-1
+ *   K1 = KC + Phi_A^T Phi_A    :401-416    -> irwls_dev_wgram / irwls_dev_wvec
+ *   solve                      :420-422    -> irwls_dev_posv_aug
+ *   e = y - Phi beta, D        :433-465    -> irwls_dev_resid_rows / irwls_dev_irls_weights
+ */
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include "internal.h"
+#include "budgeted-train.h"
+
+static irwls_gpu_stats g_bstats;
+
+
+int *randomCentroids(svm_dataset dataset, properties props) {
+    int *perm = (int *)irwls_xmalloc((size_t)(dataset.l > 0 ? dataset.l : 1) * sizeof(int));
+    for (int i = 0; i < dataset.l; ++i) perm[i] = i;
+    for (int i = dataset.l - 1; i >= 0; --i) {
+        int j = rand() % (i + 1);
+        int t = perm[i];
+        perm[i] = perm[j];
+        perm[j] = t;
+    }
+    int *c = (int *)irwls_xmalloc((size_t)(props.size > 0 ? props.size : 1) * sizeof(int));
+    for (int i = 0; i < props.size; i++) c[i] = perm[i];
+    free(perm);
+    return c;
+}
+
+/* k(x_a, x_b) of two device rows by a 1x1 kernel block */
+static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b, properties props, double *d_out) {
+    int32_t h[2] = {a, b};
+    irwls_dev_h2d(d_pair, h, sizeof(h));
+    irwls_rows r = irwls_devdata_rows(dd, d_pair, 1);
+    irwls_rows c = irwls_devdata_rows(dd, d_pair + 1, 1);
+    irwls_dev_kernel_matrix(&r, &c, dd->d, props.kernelType, props.Kgamma, d_out, 1);
+    double v = 0.0;
+    irwls_dev_d2h(&v, d_out, sizeof(double));
+    return v;
+}
+
+int *SGMA(svm_dataset dataset, properties props) {
+    const double t_start = irwls_now();
+    memset(&g_bstats, 0, sizeof(g_bstats));
+    irwls_dev_init();
+    const int n = dataset.l, B = props.size;
+    const int ncand = irwls_opts()->sgma_candidates;
+    irwls_devdata *dd = irwls_devdata_acquire(&dataset, 1, -1, 0);
+    int *cent = (int *)irwls_xmalloc((size_t)(B > 0 ? B : 1) * sizeof(int));
+    int32_t *cidx = (int32_t *)irwls_xmalloc((size_t)ncand * sizeof(int32_t));
+    double *eta = (double *)irwls_xmalloc((size_t)ncand * sizeof(double));
+    double *q = (double *)irwls_xmalloc((size_t)ncand * sizeof(double));
+    double *L2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
+    double *IL2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
+    double *row = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
+    const size_t Bz = (size_t)(B > 0 ? B : 1);
+    double *d_KSC = (double *)irwls_dev_alloc((size_t)n * Bz * sizeof(double));   /* n x B, column-major */
+    double *d_L = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));            /* B x B, column-major */
+    double *d_Li = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));
+    double *d_KSM = (double *)irwls_dev_alloc((size_t)n * ncand * sizeof(double)); /* n x ncand, row-major */
+    double *d_KNC = (double *)irwls_dev_alloc((size_t)ncand * Bz * sizeof(double)); /* ncand x B, row-major */
+    double *d_W = (double *)irwls_dev_alloc(Bz * ncand * sizeof(double));          /* B x ncand, column-major */
+    double *d_Z = (double *)irwls_dev_alloc(Bz * ncand * sizeof(double));
+    double *d_eta = (double *)irwls_dev_alloc((size_t)ncand * sizeof(double));
+    double *d_q = (double *)irwls_dev_alloc((size_t)ncand * sizeof(double));
+    double *d_v1 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
+    double *d_v2 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
+    double *d_tmp = (double *)irwls_dev_alloc(64 * sizeof(double));
+    int32_t *d_cidx = (int32_t *)irwls_dev_alloc((size_t)ncand * sizeof(int32_t));
+    int32_t *d_cent = (int32_t *)irwls_dev_alloc(Bz * sizeof(int32_t));
+    int32_t *d_pair = (int32_t *)irwls_dev_alloc(2 * sizeof(int32_t));
+    irwls_rows all = irwls_devdata_rows(dd, NULL, n);
+    const double *knc = NULL;       /* device row of k(new centroid, previous centroids) */
+    int best = 0;
+
+    for (int s = 0; s < B; s++) {
+        double value;
+        if (s > 1) {
+            g_bstats.sgma_steps++;
+            /* candidate draws in the reference's serial order */
+            for (int i = 0; i < ncand; i++) {
+                int ix = rand() % n;
+                while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
+                cidx[i] = ix;
+            }
+            irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
+            irwls_rows cr = irwls_devdata_rows(dd, d_cidx, ncand);
+            irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
+            /* KSM(:, c) = k(x_i, cand_c); KNC(c, e) = k(cand_c, centroid_e) */
+            irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, ncand);
+            irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
+            /* Z = L^-T L^-1 KNC^T */
+            irwls_dev_gemm(0, 0, s, ncand, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
+            irwls_dev_gemm(1, 0, s, ncand, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
+            irwls_dev_sgma_eta(s, ncand, d_KNC, B, d_Z, B, d_eta);
+            irwls_dev_sgma_gain(n, s, ncand, d_KSC, d_Z, B, d_KSM, ncand, d_q);
+            irwls_dev_d2h(eta, d_eta, (size_t)ncand * sizeof(double));
+            irwls_dev_d2h(q, d_q, (size_t)ncand * sizeof(double));
+            double bestv = 0.0;
+            for (int i = 0; i < ncand; i++) {
+                const double g = (eta[i] > 0.0) ? (1.0 / eta[i]) * q[i] : 0.0;
+                if (i == 0 || g > bestv) {
+                    bestv = g;
+                    best = i;
+                }
+            }
+            value = bestv;
+            cent[s] = cidx[best];
+            /* the chosen candidate's column is the new KSC column */
+            irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
+            knc = d_KNC + (size_t)best * B;
+        } else {
+            cent[s] = (s == 0) ? n : n + 1;
+            value = 1.0;
+            best = 0;
+        }
+        if (props.verbose == 1) {
+            if (s == 0) printf("Best Error Descent %f, Average of positive data is centroid %d\n", value, s);
+            if (s == 1) printf("Best Error Descent %f, Average of negative data is centroid %d\n", value, s);
+            if (s > 1) printf("Best Error Descent %f, Data with index %d is centroid %d\n", value, cent[s], s);
+        }
+        int32_t cs = cent[s];
+        irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
+        if (s <= 1) {
+            irwls_rows cc = irwls_devdata_rows(dd, d_cent + s, 1);
+            irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n, 1);
+        }
+        /* grow L and L^-1 of K_C + jitter by one row (src/budgeted-train.c:247-279) */
+        const double kcc = kernel_pair(dd, d_pair, cent[s], cent[s], props, d_tmp);
+        if (s == 0) {
+            const double l0 = sqrt(kcc + 0.000001);
+            const double il0 = 1.0 / l0;
+            irwls_dev_h2d(d_L, &l0, sizeof(double));
+            irwls_dev_h2d(d_Li, &il0, sizeof(double));
+        } else {
+            if (s == 1) {
+                const double k01 = kernel_pair(dd, d_pair, cent[0], cent[1], props, d_tmp);
+                irwls_dev_h2d(d_tmp, &k01, sizeof(double));
+                knc = d_tmp;
+            }
+            irwls_dev_gemv(d_Li, s, s, B, 0, knc, d_v1);        /* L2 = L^-1 k */
+            irwls_dev_gemv(d_Li, s, s, B, 1, d_v1, d_v2);       /* IL2 = L2^T L^-1 */
+            irwls_dev_d2h(L2, d_v1, (size_t)s * sizeof(double));
+            irwls_dev_d2h(IL2, d_v2, (size_t)s * sizeof(double));
+            double L3 = kcc + 0.00001;
+            for (int i = 0; i < s; i++) L3 = L3 - (L2[i] * L2[i]);
+            L3 = sqrt(L3);
+            const double IL3 = 1.0 / L3;
+            memcpy(row, L2, (size_t)s * sizeof(double));
+            row[s] = L3;
+            irwls_dev_h2d(d_v1, row, (size_t)(s + 1) * sizeof(double));
+            irwls_dev_scatter_row(d_L, B, s, s + 1, d_v1);
+            for (int i = 0; i < s; i++) row[i] = -IL3 * IL2[i];
+            row[s] = IL3;
+            irwls_dev_h2d(d_v2, row, (size_t)(s + 1) * sizeof(double));
+            irwls_dev_scatter_row(d_Li, B, s, s + 1, d_v2);
+        }
+    }
+    irwls_dev_free(d_KSC);
+    irwls_dev_free(d_L);
+    irwls_dev_free(d_Li);
+    irwls_dev_free(d_KSM);
+    irwls_dev_free(d_KNC);
+    irwls_dev_free(d_W);
+    irwls_dev_free(d_Z);
+    irwls_dev_free(d_eta);
+    irwls_dev_free(d_q);
+    irwls_dev_free(d_v1);
+    irwls_dev_free(d_v2);
+    irwls_dev_free(d_tmp);
+    irwls_dev_free(d_cidx);
+    irwls_dev_free(d_cent);
+    irwls_dev_free(d_pair);
+    irwls_devdata_release(dd);
+    free(cidx);
+    free(eta);
+    free(q);
+    free(L2);
+    free(IL2);
+    free(row);
+    g_bstats.t_total = irwls_now() - t_start;
+    irwls_stats_set(&g_bstats);
+    return cent;
+}
+
+double *IRWLSpar(svm_dataset dataset, int *indexes, properties props) {
+    const double t_start = irwls_now();
+    memset(&g_bstats, 0, sizeof(g_bstats));
+    irwls_dev_init();
+    const int n = dataset.l, B = props.size;
+    const double C = props.C;
+    irwls_devdata *dd = irwls_devdata_acquire(&dataset, 1, -1, 0);
+    const size_t Bz = (size_t)(B > 0 ? B : 1);
+    int32_t *hidx = (int32_t *)irwls_xmalloc(Bz * sizeof(int32_t));
+    for (int i = 0; i < B; i++) hidx[i] = indexes[i];
+    int32_t *d_idx = (int32_t *)irwls_dev_alloc(Bz * sizeof(int32_t));
+    irwls_dev_h2d(d_idx, hidx, (size_t)B * sizeof(int32_t));
+    double *d_KC = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));
+    double *d_Phi = (double *)irwls_dev_alloc((size_t)n * Bz * sizeof(double));   /* row-major n x B */
+    double *d_K1 = (double *)irwls_dev_alloc((Bz + 1) * Bz * sizeof(double));     /* ld B+1 */
+    double *d_K2 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
+    double *d_beta = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
+    double *d_e = (double *)irwls_dev_alloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double *d_D = (double *)irwls_dev_alloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double *d_sw = (double *)irwls_dev_alloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int32_t *d_act = (int32_t *)irwls_dev_alloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+
+    /* KC = K(C, C) + 1e-5 I (src/budgeted-train.c:336-343) */
+    irwls_rows rc = irwls_devdata_rows(dd, d_idx, B);
+    irwls_dev_kernel_matrix(&rc, &rc, dd->d, props.kernelType, props.Kgamma, d_KC, B);
+    {
+        double *kc = (double *)irwls_xmalloc(Bz * Bz * sizeof(double));
+        irwls_dev_d2h(kc, d_KC, Bz * Bz * sizeof(double));
+        for (int i = 0; i < B; i++) kc[(size_t)i * B + i] += pow(10, -5);
+        irwls_dev_h2d(d_KC, kc, Bz * Bz * sizeof(double));
+        free(kc);
+    }
+    /* Phi = K(X, C), row-major (src/budgeted-train.c:348-358) */
+    irwls_rows ra = irwls_devdata_rows(dd, NULL, n);
+    irwls_dev_kernel_matrix(&ra, &rc, dd->d, props.kernelType, props.Kgamma, d_Phi, B);
+
+    double M = 10000.0;
+    int iter = 0, since = 0;
+    double dW = 1e9, nW = 1.0, oldnorm = 0.0, bestDW = 1e9;
+    double *beta = (double *)irwls_xcalloc(Bz, sizeof(double));
+    double *bNew = (double *)irwls_xcalloc(Bz, sizeof(double));
+    double *bBest = (double *)irwls_xcalloc(Bz, sizeof(double));
+    int nA = n;
+    const int32_t *act = NULL;       /* first pass: all rows, unscaled */
+    const double *sw = NULL;
+    double ymul = M;
+    while (iter < 500 && dW / nW > 1e-6 && since < 5) {
+        /* K1 = KC + Phi_A^T Phi_A, K2 = Phi_A^T (sqrt(D) y) */
+        irwls_dev_wgram(B, d_Phi, act, sw, nA, d_KC, d_K1, B + 1);
+        if (nA > 0) irwls_dev_wvec(B, d_Phi, act, sw, dd->y, ymul, nA, d_K2);
+        else irwls_dev_memset0(d_K2, Bz * sizeof(double));
+        {
+            const double t0 = irwls_now();
+            (void)irwls_dev_posv_aug(d_K1, B, B + 1, d_K2, d_beta);
+            irwls_dev_d2h(bNew, d_beta, (size_t)B * sizeof(double));
+            g_bstats.solves++;
+            g_bstats.t_solve += irwls_now() - t0;
+        }
+        dW = 0.0;
+        nW = 0.0;
+        for (int i = 0; i < B; i++) {
+            const double t = bNew[i] - beta[i];
+            dW += t * t;
+            nW += bNew[i] * bNew[i];
+            beta[i] = bNew[i];
+        }
+        /* e = y - Phi beta; D, active rows (src/budgeted-train.c:433-465) */
+        irwls_dev_resid_rows(n, B, d_Phi, d_beta, dd->y, d_e);
+        nA = irwls_dev_irls_weights(n, d_e, dd->y, C, M, d_D, d_act, d_sw);
+        act = d_act;
+        sw = d_sw;
+        ymul = 1.0;
+        ++iter;
+        g_bstats.irwlspar_iters = iter;
+        if (props.verbose == 1) printf("Iteration %d, nSVs %d, ||deltaW||^2/||W||^2=%f\n", iter, nA, dW / nW);
+        if (iter > 10 && dW / nW > 100 * oldnorm) M = M / 10.0;
+        oldnorm = dW / nW;
+        if (dW / nW < bestDW) {
+            bestDW = dW / nW;
+            since = 0;
+            memcpy(bBest, bNew, (size_t)B * sizeof(double));
+        } else {
+            since++;
+        }
+    }
+    irwls_dev_free(d_idx);
+    irwls_dev_free(d_KC);
+    irwls_dev_free(d_Phi);
+    irwls_dev_free(d_K1);
+    irwls_dev_free(d_K2);
+    irwls_dev_free(d_beta);
+    irwls_dev_free(d_e);
+    irwls_dev_free(d_D);
+    irwls_dev_free(d_sw);
+    irwls_dev_free(d_act);
+    irwls_devdata_release(dd);
+    free(hidx);
+    free(beta);
+    free(bNew);
+    g_bstats.t_total = irwls_now() - t_start;
+    irwls_stats_set(&g_bstats);
+    return bBest;
+}
+
+model calculateBudgetedModel(properties props, svm_dataset dataset, int *centroids, double *beta) {
+    model m;
+    memset(&m, 0, sizeof(m));
+    m.Kgamma = props.Kgamma;
+    m.sparse = dataset.sparse;
+    m.maxdim = dataset.maxdim;
+    m.nSVs = props.size;
+    m.bias = 0.0;
+    m.kernelType = props.kernelType;
+    int nElem = 0;
+    for (int i = 0; i < props.size; i++) {
+        const svm_sample *s = dataset.x[centroids[i]];
+        while (s->index != -1) {
+            s++;
+            nElem++;
+        }
+        nElem++;
+    }
+    m.nElem = nElem;
+    m.weights = (double *)irwls_xcalloc((size_t)props.size, sizeof(double));
+    memcpy(m.weights, beta, (size_t)props.size * sizeof(double));
+    m.quadratic_value = (double *)irwls_xcalloc((size_t)props.size, sizeof(double));
+    m.x = (svm_sample **)irwls_xcalloc((size_t)props.size, sizeof(svm_sample *));
+    m.features = (svm_sample *)irwls_xcalloc((size_t)nElem, sizeof(svm_sample));
+    int f = 0;
+    for (int i = 0; i < props.size; i++) {
+        m.quadratic_value[i] = dataset.quadratic_value[centroids[i]];
+        m.x[i] = &m.features[f];
+        const svm_sample *s = dataset.x[centroids[i]];
+        for (; s->index != -1; s++, f++) {
+            m.features[f].index = s->index;
+            m.features[f].value = s->value;
+        }
+        m.features[f].index = -1;
+        f++;
+    }
+    return m;
+}
+
+properties parseTrainParameters(int *argc, char ***argv) {
+    properties props;
+    memset(&props, 0, sizeof(props));
+    props.Kgamma = 1.0;
+    props.C = 1.0;
+    props.Threads = 1;
+    props.MaxSize = 500;
+    props.Eta = 0.001;
+    props.size = 10;
+    props.algorithm = 1;
+    props.kernelType = 1;
+    props.file = 1;
+    props.separator = ",";
+    props.verbose = 1;
+    int i;
+    for (i = 1; i < *argc; ++i) {
+        if ((*argv)[i][0] != '-') break;
+        if (++i >= *argc) {
+            printBudgetedInstructions();
+            exit(1);
+        }
+        const char *name = &(*argv)[i - 1][1];
+        char *val = (*argv)[i];
+        if (strcmp(name, "g") == 0) props.Kgamma = atof(val);
+        else if (strcmp(name, "c") == 0) props.C = atof(val);
+        else if (strcmp(name, "e") == 0) props.Eta = atof(val);
+        else if (strcmp(name, "k") == 0) props.kernelType = atoi(val);
+        else if (strcmp(name, "t") == 0) props.Threads = atoi(val);
+        else if (strcmp(name, "w") == 0) props.MaxSize = atoi(val);
+        else if (strcmp(name, "s") == 0) props.size = atoi(val);
+        else if (strcmp(name, "a") == 0) props.algorithm = atoi(val);
+        else if (strcmp(name, "f") == 0) props.file = atoi(val);
+        else if (strcmp(name, "p") == 0) props.separator = val;
+        else if (strcmp(name, "v") == 0) props.verbose = atoi(val);
+        else {
+            fprintf(stderr, "Unknown parameter %s\n", name);
+            printBudgetedInstructions();
+            exit(2);
+        }
+    }
+    for (int j = 1; i + j - 1 < *argc; ++j) (*argv)[j] = (*argv)[i + j - 1];
+    *argc -= i - 1;
+    return props;
+}
+
+void printBudgetedInstructions(void) {
+    fprintf(stderr, "budgeted-train: This software train the sparse SVM on the given training set ");
+    fprintf(stderr, "and generages a model for futures prediction use.\n\n");
+    fprintf(stderr, "Usage: budgeted-train [options] training_set_file model_file\n\n");
+    fprintf(stderr, "Options:\n");
+    fprintf(stderr, "  -k kernel type: (default 1)\n");
+    fprintf(stderr, "       0 -- Linear kernel u'*v\n");
+    fprintf(stderr, "       1 -- radial basis function: exp(-gamma*|u-v|^2)\n");
+    fprintf(stderr, "  -g gamma: set gamma in radial basis kernel function (default 1)\n");
+    fprintf(stderr, "       radial basis K(u,v)= exp(-gamma*|u-v|^2)\n");
+    fprintf(stderr, "  -c Cost: set SVM Cost (default 1)\n");
+    fprintf(stderr, "  -t Threads: Number of threads (default 1)\n");
+    fprintf(stderr, "  -s Classifier size: Size of the classifier (default 1)\n");
+    fprintf(stderr, "  -a Algorithm: Algorithm for centroids selection (default 1)\n");
+    fprintf(stderr, "       0 -- Random Selection\n");
+    fprintf(stderr, "       1 -- SGMA (Sparse Greedy Matrix Approximation)\n");
+    fprintf(stderr, "  -f file format: (default 1)\n");
+    fprintf(stderr, "       0 -- CSV format (comma separator)\n");
+    fprintf(stderr, "       1 -- libsvm format\n");
+    fprintf(stderr, "  -p separator: csv separator character (default \",\" if csv format is selected)\n");
+    fprintf(stderr, "  -v verbose: (default 1)\n");
+    fprintf(stderr, "       0 -- No screen messages\n");
+    fprintf(stderr, "       1 -- Screen messages\n");
+}

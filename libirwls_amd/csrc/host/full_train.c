@@ -23,6 +23,8 @@ static irwls_gpu_stats g_stats;
 
 void irwls_gpu_last_stats(irwls_gpu_stats *out) { *out = g_stats; }
 
+void irwls_stats_set(const irwls_gpu_stats *s) { g_stats = *s; }
+
 int *rpermute(int n) {
     int *p = (int *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
     for (int k = 0; k < n; k++) p[k] = k;

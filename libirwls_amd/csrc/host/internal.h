@@ -74,6 +74,9 @@ void irwls_ws_free(irwls_ws *ws);
 double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l, const double *y,
                             properties props, const double *GIN, double *e, double *beta);
 
+/* Counters reported by irwls_gpu_last_stats (the last training call). */
+void irwls_stats_set(const irwls_gpu_stats *s);
+
 /* libc-compatible allocation with exit(2) on failure. */
 void *irwls_xmalloc(size_t bytes);
 void *irwls_xcalloc(size_t n, size_t sz);
