@@ -47,55 +47,130 @@ __device__ __forceinline__ double rdl(double v, int lane) {
 }
 
 // ------------------------------------------------- diagonal tile factor ---
-// Factor the lower kb x kb tile held in S (lower part) and form its inverse,
-// with all 256 threads: lanes run over rows (factor) or columns (inverse),
-// the four waves split every inner product, one barrier pair per column.
-// S packs both results: S[i][c] = L(i,c) for c <= i and S[c][i+1] =
-// Linv(i,c) for i >= c.  Returns the failing column or -1 (the Schur value
-// is left at L(j,j); the inverse is still formed from the factor as it
-// stands).
-__device__ int diag_factor_inverse(double (*S)[NB + 1], double (*part)[NB], int kb) {
+// Factor the lower kb x kb tile held in S (lower part) and form its inverse
+// in Inv, with all 256 threads.
+//   factor:  right-looking; per column one barrier publishes the scaled
+//            column, one publishes the rank-1 update (independent LDS
+//            updates, four threads per trailing row);
+//   inverse: the four 16x16 diagonal sub-blocks are inverted by the four
+//            waves in parallel (registers, one lane per column), then the
+//            off-diagonal blocks follow block row by block row:
+//            Linv_ij = -Linv_ii * sum_{j<=k<i} L_ik Linv_kj.
+// Returns the failing column or -1.  After a failure the Schur value is left
+// at L(j,j) (right-looking, so it is the fully updated pivot) and the inverse
+// is formed from the tile as it stands.
+__device__ int diag_factor_inverse(double (*S)[NB + 1], double (*Inv)[NB + 1], double *col, int kb) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     int fail = -1;
     for (int j = 0; j < kb; j++) {
-        // s_i = A(i,j) - sum_{k<j} L(i,k) L(j,k), k split over the waves
-        double acc = 0.0;
-        const int q = (j + 3) >> 2;
-        const int kbeg = wave * q, kend = min(j, kbeg + q);
-        for (int k = kbeg; k < kend; k++) acc += S[lane][k] * S[j][k];
-        part[wave][lane] = acc;
-        __syncthreads();
-        const double sjj = S[j][j] - (((part[0][j] + part[1][j]) + part[2][j]) + part[3][j]);
-        if (!(sjj > 0.0)) {             // uniform
-            __syncthreads();
-            if (t == 0) S[j][j] = sjj;
+        const double ajj = S[j][j];
+        if (!(ajj > 0.0)) {   // uniform
             fail = j;
-            __syncthreads();
             break;
         }
-        const double ljj = sqrt(sjj);
-        double si = 0.0;
-        if (wave == 0 && lane > j && lane < kb)
-            si = S[lane][j] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+        const double ljj = sqrt(ajj);
+        if (t < NB) col[t] = (t > j && t < kb) ? S[t][j] / ljj : ((t == j) ? ljj : 0.0);
         __syncthreads();
-        if (wave == 0 && lane >= j && lane < kb) S[lane][j] = (lane == j) ? ljj : si / ljj;
+        if (t < NB && t >= j && t < kb) S[t][j] = col[t];
+        const int i = j + 1 + (t >> 2);
+        if (i < kb) {
+            // all loads of this thread's share first, then the updates
+            const double li = col[i];
+            const int c0 = j + 1 + (t & 3);
+            double sv[16], cv[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = c0 + 4 * q;
+                const bool ok = c <= i;
+                sv[q] = ok ? S[i][c] : 0.0;
+                cv[q] = ok ? col[c] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = c0 + 4 * q;
+                if (c <= i) S[i][c] = sv[q] - li * cv[q];
+            }
+        }
         __syncthreads();
     }
-    // inverse, column c = lane; row i: x(i,c) = (delta - sum_{c<=k<i} L(i,k) x(k,c)) / L(i,i)
-    for (int i = 0; i < kb; i++) {
+    // zero the strict upper triangle of the tile, as the factor is read by
+    // block products below
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        if (c > r || r >= kb || c >= kb) S[r][c] = 0.0;
+    }
+    __syncthreads();
+    // diagonal 16x16 sub-block inverses: wave w inverts block w, lane = column
+    {
+        const int b0 = wave * 16;
         const int c = lane;
-        double acc = 0.0;
-        if (c < i) {
-            const int len = i - c;
-            const int q = (len + 3) >> 2;
-            const int kbeg = c + wave * q, kend = min(i, kbeg + q);
-            for (int k = kbeg; k < kend; k++) acc += S[i][k] * S[c][k + 1];
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            double lrow[16];
+#pragma unroll
+            for (int k = 0; k <= i; k++) lrow[k] = S[b0 + i][b0 + k];
+            double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; k++) s -= lrow[k] * x[k];
+            x[i] = (c < 16 && i >= c && b0 + i < kb) ? s / lrow[i] : 0.0;
         }
-        part[wave][lane] = acc;
+        if (c < 16) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) Inv[b0 + i][b0 + c] = x[i];
+        }
+    }
+    // zero the strict upper blocks of Inv
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        if ((c >> 4) > (r >> 4)) Inv[r][c] = 0.0;
+    }
+    __syncthreads();
+    // off-diagonal blocks, block row by block row; thread -> (r, c) of a 16x16 block
+    const int r = t >> 4, cc = t & 15;
+#pragma unroll
+    for (int bi = 1; bi < 4; bi++) {
+        // T_j = sum_{k=j}^{bi-1} L_{bi,k} Linv_{k,j} for j < bi, kept in registers
+        double tj[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int bj = 0; bj < bi; bj++) {
+            double s = 0.0;
+#pragma unroll
+            for (int bk = bj; bk < bi; bk++) {
+                double a[16], b[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    a[q] = S[bi * 16 + r][bk * 16 + q];
+                    b[q] = Inv[bk * 16 + q][bj * 16 + cc];
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q++) s += a[q] * b[q];
+            }
+            tj[bj] = s;
+        }
         __syncthreads();
-        if (wave == 0 && c <= i) {
-            const double s = ((c == i) ? 1.0 : 0.0) - (((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]);
-            S[c][i + 1] = s / S[i][i];
+        // stage T_j in the (still zero) strict upper blocks of Inv: block (bj, bi)
+#pragma unroll
+        for (int bj = 0; bj < bi; bj++) Inv[bj * 16 + r][bi * 16 + cc] = tj[bj];
+        __syncthreads();
+#pragma unroll
+        for (int bj = 0; bj < bi; bj++) {
+            double a[16], b[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                a[q] = Inv[bi * 16 + r][bi * 16 + q];
+                b[q] = Inv[bj * 16 + q][bi * 16 + cc];
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; q++) s += a[q] * b[q];
+            tj[bj] = -s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bj = 0; bj < bi; bj++) {
+            Inv[bi * 16 + r][bj * 16 + cc] = (bi * 16 + r < kb) ? tj[bj] : 0.0;
+            Inv[bj * 16 + r][bi * 16 + cc] = 0.0;
         }
         __syncthreads();
     }
@@ -104,25 +179,27 @@ __device__ int diag_factor_inverse(double (*S)[NB + 1], double (*part)[NB], int 
 
 // Write the factored tile (lower part) back and the inverse to Dinv
 // (column-major 64x64, zero outside the lower kb x kb triangle).
-__device__ void store_factor(double (*S)[NB + 1], double *A, int lda, int k0, int kb, double *Dinv) {
+__device__ void store_factor(double (*S)[NB + 1], double (*Inv)[NB + 1], double *A, int lda, int k0, int kb,
+                             double *Dinv) {
     for (int e = threadIdx.x; e < NB * NB; e += 256) {
         const int i = e % NB, c = e / NB;
         if (i < kb && c < kb && c <= i) A[(size_t)(k0 + c) * lda + k0 + i] = S[i][c];
-        Dinv[(size_t)c * NB + i] = (i < kb && c < kb && c <= i) ? S[c][i + 1] : 0.0;
+        Dinv[(size_t)c * NB + i] = (i < kb && c < kb && c <= i) ? Inv[i][c] : 0.0;
     }
 }
 
 __global__ __launch_bounds__(256) void diag_first_kernel(double *__restrict__ A, int lda, int kb,
                                                          int *__restrict__ info, double *__restrict__ Dinv) {
     __shared__ double S[NB][NB + 1];
-    __shared__ double part[4][NB];
+    __shared__ double Inv[NB][NB + 1];
+    __shared__ double col[NB];
     for (int e = threadIdx.x; e < NB * NB; e += 256) {
         const int i = e % NB, c = e / NB;
         S[i][c] = (i < kb && c < kb && c <= i) ? A[(size_t)c * lda + i] : 0.0;
     }
     __syncthreads();
-    const int fail = diag_factor_inverse(S, part, kb);
-    store_factor(S, A, lda, 0, kb, Dinv);
+    const int fail = diag_factor_inverse(S, Inv, col, kb);
+    store_factor(S, Inv, A, lda, 0, kb, Dinv);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
@@ -215,7 +292,8 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     __shared__ double As[1][NB][LDP];
     __shared__ double Bs[1][NB][LDP];
     __shared__ double S[NB][NB + 1];
-    __shared__ double part[4][NB];
+    __shared__ double Inv[NB][NB + 1];
+    __shared__ double colbuf[NB];
     const int tj = blockIdx.x, ti = blockIdx.y;
     const int t0 = k0 + kb;
     // stop only for a failure in blocks <= k (pivot index < t0); work-group
@@ -250,8 +328,8 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     }
     if (!diag) return;
     __syncthreads();
-    const int fail = diag_factor_inverse(S, part, kb2);
-    store_factor(S, A, lda, t0, kb2, Dnext);
+    const int fail = diag_factor_inverse(S, Inv, colbuf, kb2);
+    store_factor(S, Inv, A, lda, t0, kb2, Dnext);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
