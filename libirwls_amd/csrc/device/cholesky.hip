@@ -6,11 +6,12 @@
 //          the factored tile (FP64 MFMA, v_mfma_f64_16x16x4f64);
 //   syrk:  the trailing lower tiles take the rank-64 update (FP64 MFMA); the
 //          work-group that owns the next diagonal tile then factors it and
-//          forms its inverse in LDS (look-ahead), so the serial part of step
-//          k+1 overlaps the trailing update of step k.
+//          forms its inverse (look-ahead), so the serial part of step k+1
+//          overlaps the trailing update of step k.
 // Right-hand sides can ride along as extra rows below the matrix
 // (rows n..nrows-1): the same steps then perform the forward sweep L z = r,
-// so a solve only adds the backward sweep.
+// so a solve only adds the backward sweep, which multiplies by the stored
+// inverses of the diagonal tiles.
 //
 // A non-positive pivot ends the factorization as LAPACK potrf does: the Schur
 // value stays on the diagonal, the failing index is reported and later steps
@@ -24,13 +25,14 @@
 // boundary on the library stream.
 #include "common.h"
 
+#include <utility>
+
 using namespace irwls;
 
 namespace {
 
 constexpr int NB = 64;
-constexpr int BK = 16;
-constexpr int LDP = BK + 1;
+constexpr int TILE = NB * NB;
 
 int *g_info = nullptr;     // [0] = potrf info
 
@@ -47,205 +49,160 @@ __device__ __forceinline__ double rdl(double v, int lane) {
 }
 
 // ------------------------------------------------- diagonal tile factor ---
-// Factor the lower kb x kb tile held in S (lower part) and form its inverse
-// in Inv, with all 256 threads.
-//   factor:  right-looking; per column one barrier publishes the scaled
-//            column, one publishes the rank-1 update (independent LDS
-//            updates, four threads per trailing row);
-//   inverse: the four 16x16 diagonal sub-blocks are inverted by the four
-//            waves in parallel (registers, one lane per column), then the
-//            off-diagonal blocks follow block row by block row:
-//            Linv_ij = -Linv_ii * sum_{j<=k<i} L_ik Linv_kj.
-// Returns the failing column or -1.  After a failure the Schur value is left
-// at L(j,j) (right-looking, so it is the fully updated pivot) and the inverse
-// is formed from the tile as it stands.
-__device__ int diag_factor_inverse(double (*S)[NB + 1], double (*Inv)[NB + 1], double *col, int kb) {
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+// The 64 x 64 tile is held in registers: thread t owns row i = t/4 and the
+// sixteen columns c = (t%4) + 4q, q = 0..15, both of the tile (a[q]) and of
+// the inverse of its factor (x[q], starting from the identity).  Pivot j is
+// one barrier: the owners of column j publish it (rows > j, zeros above) and
+// the owners of row j of the inverse publish that row; every thread then
+// takes r = 1/sqrt(a_jj) and updates its elements
+//     a_ic -= (w_i r^2) w_c      (c > j: trailing matrix, l_i l_c)
+//     x_ic -= (w_i r^2) x_jc     (c <= j: rows below of the inverse)
+// while row j of the inverse becomes x_j r and column j of the tile w r
+// (LAPACK potf2 also scales by the reciprocal of the square root).  The
+// broadcast buffers alternate with the pivot parity, so a buffer is only
+// rewritten after every thread has passed the barrier that follows its last
+// read.  Rows and columns at or beyond kb hold the identity.
+constexpr int WLD = 20;   // row stride: the four column classes hit different banks, rows 16-byte aligned
+
+struct PivotBufs {
+    double w[2][4][WLD];   // column j of the trailing matrix: entry c at [c%4][c/4], zero for c <= j
+    double x[2][4][WLD];   // row j of the inverse, same layout
+    double piv[2][2];      // a_jj
+};
+
+// One pivot, j fixed at compile time so that every register index is static.
+// Returns false once the factorization is over (j >= kb or a failed pivot,
+// whose value is left in wfail).  Rows i <= j read w_i = 0, so their updates
+// are no-ops and need no branch.
+template <int j>
+__device__ __forceinline__ bool pivot_step(double (&a)[16], double (&x)[16], PivotBufs &B, int kb, int &fail,
+                                           double &wfail) {
+    if (j >= kb) return false;
+    const int t = threadIdx.x, i = t >> 2, c0 = t & 3;
+    constexpr int p = j & 1, cj = j & 3, qj = j >> 2;
+    if (c0 == cj) {
+        const double v = a[qj];
+        B.w[p][i & 3][i >> 2] = (i > j) ? v : 0.0;
+        if (i == j) B.piv[p][0] = v;
+    }
+    if (i == j) {
+#pragma unroll
+        for (int q = 0; q <= qj; q++) B.x[p][c0][q] = x[q];
+    }
+    __syncthreads();
+    const double wjj = B.piv[p][0];
+    if (!(wjj > 0.0)) {   // uniform
+        fail = j;
+        wfail = wjj;
+        return false;
+    }
+    const double r = rsqrt(wjj);
+    const double li = B.w[p][i & 3][i >> 2] * r;   // l_ij, zero for i <= j
+    const double m = li * r;
+#pragma unroll
+    for (int q = qj; q < 16; q++) a[q] = fma(-m, B.w[p][c0][q], a[q]);
+#pragma unroll
+    for (int q = 0; q <= qj; q++) x[q] = fma(-m, B.x[p][c0][q], x[q]);
+    if (c0 == cj && i > j) a[qj] = li;
+    if (i == j) {
+#pragma unroll
+        for (int q = 0; q <= qj; q++) x[q] = x[q] * r;
+        if (c0 == cj) a[qj] = wjj * r;
+    }
+    return true;
+}
+
+template <int... J>
+__device__ __forceinline__ void pivot_chain(double (&a)[16], double (&x)[16], PivotBufs &B, int kb, int &fail,
+                                            double &wfail, std::integer_sequence<int, J...>) {
+    (pivot_step<J>(a, x, B, kb, fail, wfail) && ...);
+}
+
+// Returns the failing pivot or -1.  After a failure at pivot j the Schur
+// value stays at a_jj, column j below it is left unscaled, row j of the
+// inverse is divided by the Schur value (as the triangular solves with the
+// factor as it stands do) and the rows below it are left unfinished.
+__device__ __forceinline__ int tile_factor(double (&a)[16], double (&x)[16], PivotBufs &B, int kb) {
     int fail = -1;
-    for (int j = 0; j < kb; j++) {
-        const double ajj = S[j][j];
-        if (!(ajj > 0.0)) {   // uniform
-            fail = j;
-            break;
-        }
-        const double ljj = sqrt(ajj);
-        if (t < NB) col[t] = (t > j && t < kb) ? S[t][j] / ljj : ((t == j) ? ljj : 0.0);
-        __syncthreads();
-        if (t < NB && t >= j && t < kb) S[t][j] = col[t];
-        const int i = j + 1 + (t >> 2);
-        if (i < kb) {
-            // all loads of this thread's share first, then the updates
-            const double li = col[i];
-            const int c0 = j + 1 + (t & 3);
-            double sv[16], cv[16];
+    double wfail = 0.0;
+    pivot_chain(a, x, B, kb, fail, wfail, std::make_integer_sequence<int, NB>{});
+    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    if (fail >= 0 && i == fail) {
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int c = c0 + 4 * q;
-                const bool ok = c <= i;
-                sv[q] = ok ? S[i][c] : 0.0;
-                cv[q] = ok ? col[c] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int c = c0 + 4 * q;
-                if (c <= i) S[i][c] = sv[q] - li * cv[q];
-            }
-        }
-        __syncthreads();
-    }
-    // zero the strict upper triangle of the tile, as the factor is read by
-    // block products below
-    for (int e = t; e < NB * NB; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        if (c > r || r >= kb || c >= kb) S[r][c] = 0.0;
-    }
-    __syncthreads();
-    // diagonal 16x16 sub-block inverses: wave w inverts block w, lane = column
-    {
-        const int b0 = wave * 16;
-        const int c = lane;
-        double x[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            double lrow[16];
-#pragma unroll
-            for (int k = 0; k <= i; k++) lrow[k] = S[b0 + i][b0 + k];
-            double s = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < i; k++) s -= lrow[k] * x[k];
-            x[i] = (c < 16 && i >= c && b0 + i < kb) ? s / lrow[i] : 0.0;
-        }
-        if (c < 16) {
-#pragma unroll
-            for (int i = 0; i < 16; i++) Inv[b0 + i][b0 + c] = x[i];
-        }
-    }
-    // zero the strict upper blocks of Inv
-    for (int e = t; e < NB * NB; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        if ((c >> 4) > (r >> 4)) Inv[r][c] = 0.0;
-    }
-    __syncthreads();
-    // off-diagonal blocks, block row by block row; thread -> (r, c) of a 16x16 block
-    const int r = t >> 4, cc = t & 15;
-#pragma unroll
-    for (int bi = 1; bi < 4; bi++) {
-        // T_j = sum_{k=j}^{bi-1} L_{bi,k} Linv_{k,j} for j < bi, kept in registers
-        double tj[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int bj = 0; bj < bi; bj++) {
-            double s = 0.0;
-#pragma unroll
-            for (int bk = bj; bk < bi; bk++) {
-                double a[16], b[16];
-#pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    a[q] = S[bi * 16 + r][bk * 16 + q];
-                    b[q] = Inv[bk * 16 + q][bj * 16 + cc];
-                }
-#pragma unroll
-                for (int q = 0; q < 16; q++) s += a[q] * b[q];
-            }
-            tj[bj] = s;
-        }
-        __syncthreads();
-        // stage T_j in the (still zero) strict upper blocks of Inv: block (bj, bi)
-#pragma unroll
-        for (int bj = 0; bj < bi; bj++) Inv[bj * 16 + r][bi * 16 + cc] = tj[bj];
-        __syncthreads();
-#pragma unroll
-        for (int bj = 0; bj < bi; bj++) {
-            double a[16], b[16];
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                a[q] = Inv[bi * 16 + r][bi * 16 + q];
-                b[q] = Inv[bj * 16 + q][bi * 16 + cc];
-            }
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 16; q++) s += a[q] * b[q];
-            tj[bj] = -s;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int bj = 0; bj < bi; bj++) {
-            Inv[bi * 16 + r][bj * 16 + cc] = (bi * 16 + r < kb) ? tj[bj] : 0.0;
-            Inv[bj * 16 + r][bi * 16 + cc] = 0.0;
-        }
-        __syncthreads();
+        for (int q = 0; q < 16; q++)
+            if (c0 + 4 * q <= fail) x[q] = x[q] / wfail;
     }
     return fail;
 }
 
-// Write the factored tile (lower part) back and the inverse to Dinv
-// (column-major 64x64, zero outside the lower kb x kb triangle).
-__device__ void store_factor(double (*S)[NB + 1], double (*Inv)[NB + 1], double *A, int lda, int k0, int kb,
-                             double *Dinv) {
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
-        const int i = e % NB, c = e / NB;
-        if (i < kb && c < kb && c <= i) A[(size_t)(k0 + c) * lda + k0 + i] = S[i][c];
-        Dinv[(size_t)c * NB + i] = (i < kb && c < kb && c <= i) ? Inv[i][c] : 0.0;
+__device__ __forceinline__ void tile_identity_inverse(double (&x)[16]) {
+    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] = (c0 + 4 * q == i) ? 1.0 : 0.0;
+}
+
+// Write the factored tile (lower kb x kb part) back to A(k0.., k0..), the
+// inverse of the factor to Dinv (column-major: Dinv[c*64 + i] = Linv(i, c))
+// and its transpose to DinvT (DinvT[c*64 + i] = Linv(c, i)); both are zero
+// outside the lower kb x kb triangle.
+__device__ __forceinline__ void tile_store(const double (&a)[16], const double (&x)[16], double *A, int lda, int k0,
+                                           int kb, double *Dinv, double *DinvT) {
+    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = c0 + 4 * q;
+        const bool low = (i < kb && c <= i);
+        if (low) A[(size_t)(k0 + c) * lda + k0 + i] = a[q];
+        const double v = low ? x[q] : 0.0;
+        Dinv[c * NB + i] = v;
+        DinvT[i * NB + c] = v;
     }
 }
 
 __global__ __launch_bounds__(256) void diag_first_kernel(double *__restrict__ A, int lda, int kb,
                                                          int *__restrict__ info, double *__restrict__ Dinv) {
-    __shared__ double S[NB][NB + 1];
-    __shared__ double Inv[NB][NB + 1];
-    __shared__ double col[NB];
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
-        const int i = e % NB, c = e / NB;
-        S[i][c] = (i < kb && c < kb && c <= i) ? A[(size_t)c * lda + i] : 0.0;
+    __shared__ PivotBufs B;
+    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    double a[16], x[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = c0 + 4 * q;
+        a[q] = (i < kb && c <= i) ? A[(size_t)c * lda + i] : (c == i ? 1.0 : 0.0);
     }
-    __syncthreads();
-    const int fail = diag_factor_inverse(S, Inv, col, kb);
-    store_factor(S, Inv, A, lda, 0, kb, Dinv);
+    tile_identity_inverse(x);
+    const int fail = tile_factor(a, x, B, kb);
+    tile_store(a, x, A, lda, 0, kb, Dinv, Dinv + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
-// Loads a 64 (rows) x 16 (k) chunk of a column-major matrix:
-// element (row, k) = M[(kc0 + k) * ld + r0 + row], zero outside [rows, kmax).
-__device__ __forceinline__ void load_cm(double (*S)[LDP], const double *M, int ld, int r0, int rmax, int kc0,
-                                        int kmax) {
-    const int t = threadIdx.x;
-    const int row = t & 63;
-    const int kq = (t >> 6) * 4;
-    const bool rok = (r0 + row) < rmax;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int k = kc0 + kq + q;
-        double v = 0.0;
-        if (rok && k < kmax) v = M[(size_t)k * ld + r0 + row];
-        S[row][kq + q] = v;
-    }
-}
-
-// acc[mb][nb] += sum_k P(i, k) Q(j, k) over k in [kc0, kc0+K), P and Q
-// column-major with row offsets pr0 / qr0.
-__device__ __forceinline__ void mfma_cm(v4d acc[2][2], double (*As)[NB][LDP], double (*Bs)[NB][LDP],
-                                        const double *P, int ldp, int pr0, int prmax, const double *Q, int ldq,
-                                        int qr0, int qrmax, int kc0, int K) {
+// ------------------------------------------------------------ MFMA tiles ---
+// acc[mb][nb] += sum_{k < K} P(pr + 16 mb + row, k) Q(qr + 16 nb + col, k)
+// for the wave's 32 x 32 block, P(r, k) = P[k*ldp + r] and Q likewise
+// (column-major); rows at or beyond prmax / qrmax and k >= K read as zero.
+// The operands go straight from memory into the MFMA source registers (A:
+// lane l = (row l%16, k l/16); B: lane l = (k l/16, col l%16)), all loads
+// issued before the first MFMA.
+__device__ __forceinline__ void mfma_rows(v4d (&acc)[2][2], const double *__restrict__ P, int ldp, int pr, int prmax,
+                                          const double *__restrict__ Q, int ldq, int qr, int qrmax, int K) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int nchunk = (K + BK - 1) / BK;
-    for (int ch = 0; ch < nchunk; ch++) {
-        load_cm(As[0], P, ldp, pr0, prmax, kc0 + ch * BK, kc0 + K);
-        load_cm(Bs[0], Q, ldq, qr0, qrmax, kc0 + ch * BK, kc0 + K);
-        __syncthreads();
+    const int rr = lane & 15, kq = lane >> 4;
+    double a[2][16], b[2][16];
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ks++) {
-            const int kk = ks * 4 + (lane >> 4);
-            double a0 = As[0][wr * 32 + (lane & 15)][kk];
-            double a1 = As[0][wr * 32 + 16 + (lane & 15)][kk];
-            double b0 = Bs[0][wc * 32 + (lane & 15)][kk];
-            double b1 = Bs[0][wc * 32 + 16 + (lane & 15)][kk];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int s = 0; s < 16; s++) {
+        const int k = 4 * s + kq;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ri = pr + h * 16 + rr, rj = qr + h * 16 + rr;
+            a[h][s] = (k < K && ri < prmax) ? P[(size_t)k * ldp + ri] : 0.0;
+            b[h][s] = (k < K && rj < qrmax) ? Q[(size_t)k * ldq + rj] : 0.0;
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b[0][s], acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b[1][s], acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b[0][s], acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b[1][s], acc[1][1], 0, 0, 0);
     }
 }
 
@@ -255,8 +212,6 @@ __device__ __forceinline__ void mfma_cm(v4d acc[2][2], double (*As)[NB][LDP], do
 // (rows >= n) are solved; after a failure in an earlier block nothing is.
 __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
                                                    const int *__restrict__ info, const double *__restrict__ Dinv) {
-    __shared__ double As[1][NB][LDP];
-    __shared__ double Bs[1][NB][LDP];
     const int inf = info[0];
     if (inf != 0 && inf <= k0) return;
     const int lane = threadIdx.x & 63;
@@ -269,7 +224,8 @@ __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
     // Q(j, k) = Linv(j, k) = Dinv[k*64 + j]
-    mfma_cm(acc, As, Bs, A + (size_t)k0 * lda, lda, r0, nrows, Dinv, NB, 0, kb, 0, kb);
+    mfma_rows(acc, A + (size_t)k0 * lda, lda, r0 + wr * 32, nrows, Dinv, NB, wc * 32, kb, kb);
+    __syncthreads();   // both column halves read the whole panel row before it is overwritten
     for (int nb = 0; nb < 2; nb++) {
         const int c = wc * 32 + nb * 16 + (lane & 15);
         if (c >= kb) continue;
@@ -289,11 +245,8 @@ __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n
 // the next step needs no separate launch for that.
 __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
                                                         int kb, int *__restrict__ info, double *__restrict__ Dnext) {
-    __shared__ double As[1][NB][LDP];
-    __shared__ double Bs[1][NB][LDP];
     __shared__ double S[NB][NB + 1];
-    __shared__ double Inv[NB][NB + 1];
-    __shared__ double colbuf[NB];
+    __shared__ PivotBufs B;
     const int tj = blockIdx.x, ti = blockIdx.y;
     const int t0 = k0 + kb;
     // stop only for a failure in blocks <= k (pivot index < t0); work-group
@@ -307,7 +260,7 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     v4d acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int c = 0; c < 2; c++) acc[a][c] = (v4d){0.0, 0.0, 0.0, 0.0};
-    mfma_cm(acc, As, Bs, A, lda, rI, nrows, A, lda, rJ, n, k0, kb);
+    mfma_rows(acc, A + (size_t)k0 * lda, lda, rI + wr * 32, nrows, A + (size_t)k0 * lda, lda, rJ + wc * 32, n, kb);
     const bool diag = (ti == 0 && tj == 0);
     const int kb2 = (n - t0 < NB) ? n - t0 : NB;
     for (int nb = 0; nb < 2; nb++) {
@@ -317,26 +270,33 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
             for (int r = 0; r < 4; r++) {
                 const int il = wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
                 const int i = rI + il;
-                const bool in_tile = diag && i < t0 + kb2;   // matrix rows of the next diagonal tile
+                const bool in_tile = diag && il < kb2;   // matrix rows of the next diagonal tile
                 if (j < n && i < nrows && i >= j) {
                     const double v = A[(size_t)j * lda + i] - acc[mb][nb][r];
                     if (in_tile) S[il][jl] = v;
                     else A[(size_t)j * lda + i] = v;   // other tiles and right-hand-side rows
                 }
-                if (diag && !(in_tile && j < n && i >= j)) S[il][jl] = 0.0;
             }
     }
     if (!diag) return;
     __syncthreads();
-    const int fail = diag_factor_inverse(S, Inv, colbuf, kb2);
-    store_factor(S, Inv, A, lda, t0, kb2, Dnext);
+    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    double a[16], x[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = c0 + 4 * q;
+        a[q] = (i < kb2 && c <= i) ? S[i][c] : (c == i ? 1.0 : 0.0);
+    }
+    tile_identity_inverse(x);
+    const int fail = tile_factor(a, x, B, kb2);
+    tile_store(a, x, A, lda, t0, kb2, Dnext, Dnext + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
 // -------------------------------------------------------- triangular sweeps ---
 // One launch per 64-row block w.  Every work-group first solves the diagonal
-// block for block w of the solution from the (final) accumulator with wave 0
-// and work-group 0 stores it in `out`; then the work-groups update the
+// block for block w of the solution from the (final) accumulator and
+// work-group 0 stores it in `out`; then the work-groups update the
 // accumulators of the blocks still to come.  Block w of `acc` is only read in
 // launch w, so no work-group reads what another one writes.
 
@@ -374,12 +334,27 @@ __global__ __launch_bounds__(256) void trsv_fwd_step(const double *__restrict__ 
     acc[r] -= v;
 }
 
-// backward: x_w = L_ww^-T acc_w;  acc_c -= sum_i L(w*64+i, c) x_w[i] for c < w*64
+// acc_c -= sum_i L(k0+i, c) xs_i for the columns c < k0 of this wave
+__device__ __forceinline__ void bwd_update(const double *__restrict__ L, int lda, int k0, int kb, const double *xs,
+                                           double *__restrict__ acc, int cols_per_wave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double xl = xs[lane];
+    const int c_begin = (blockIdx.x * 4 + wave) * cols_per_wave;
+    for (int q = 0; q < cols_per_wave; q++) {
+        const int c = c_begin + q;
+        if (c >= k0) break;
+        double v = (lane < kb) ? L[(size_t)c * lda + k0 + lane] * xl : 0.0;
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) acc[c] -= v;
+    }
+}
+
+// backward: x_w = L_ww^-T acc_w by substitution;  acc_c -= sum_i L(w*64+i, c) x_w[i] for c < w*64
 __global__ __launch_bounds__(256) void trsv_bwd_step(const double *__restrict__ L, int n, int lda, int w,
                                                      double *__restrict__ acc, double *__restrict__ out,
                                                      int cols_per_wave) {
     __shared__ double xs[NB];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, lane = t & 63;
     const int k0 = w * NB;
     const int kb = (n - k0 < NB) ? n - k0 : NB;
     if (t < 64) {
@@ -402,15 +377,36 @@ __global__ __launch_bounds__(256) void trsv_bwd_step(const double *__restrict__ 
         if (blockIdx.x == 0 && lane < kb) out[k0 + lane] = x;
     }
     __syncthreads();
-    const double xl = xs[lane];
-    const int c_begin = (blockIdx.x * 4 + wave) * cols_per_wave;
-    for (int q = 0; q < cols_per_wave; q++) {
-        const int c = c_begin + q;
-        if (c >= k0) break;
-        double v = (lane < kb) ? L[(size_t)c * lda + k0 + lane] * xl : 0.0;
-        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) acc[c] -= v;
+    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave);
+}
+
+// backward with the stored inverse of the diagonal tile:
+// x_w[i] = sum_k Linv(k, i) acc_w[k], Linv(k, i) = DinvT[k*64 + i]
+__global__ __launch_bounds__(256) void trsv_bwd_inv_step(const double *__restrict__ L, int n, int lda, int w,
+                                                         const double *__restrict__ DinvT, double *__restrict__ acc,
+                                                         double *__restrict__ out, int cols_per_wave) {
+    __shared__ double zs[NB], part[4][NB], xs[NB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int k0 = w * NB;
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    if (t < NB) zs[t] = (t < kb) ? acc[k0 + t] : 0.0;
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+        const int kk = wave * 16 + k;
+        s0 = fma(DinvT[kk * NB + lane], zs[kk], s0);
+        s1 = fma(DinvT[(kk + 1) * NB + lane], zs[kk + 1], s1);
     }
+    part[wave][lane] = s0 + s1;
+    __syncthreads();
+    if (t < NB) {
+        const double x = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+        xs[t] = x;
+        if (blockIdx.x == 0 && t < kb) out[k0 + t] = x;
+    }
+    __syncthreads();
+    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave);
 }
 
 // right-hand-side row of an augmented matrix <-> contiguous vector
@@ -424,6 +420,8 @@ __global__ void scatter_row(double *__restrict__ A, int lda, int row, int n, con
     if (i < n) A[(size_t)i * lda + row] = in[i];
 }
 
+// inverses of the diagonal tiles of the last factorization: block b holds
+// Dinv at [2b * 4096] and its transpose at [(2b+1) * 4096]
 double *g_dinv = nullptr;
 int g_dinv_blocks = 0;
 
@@ -431,12 +429,12 @@ double *dinv_buf(int nblk) {
     if (g_dinv_blocks < nblk) {
         if (g_dinv) irwls_dev_free(g_dinv);
         g_dinv_blocks = nblk < 16 ? 16 : nblk;
-        g_dinv = (double *)irwls_dev_alloc((size_t)g_dinv_blocks * NB * NB * sizeof(double));
+        g_dinv = (double *)irwls_dev_alloc((size_t)g_dinv_blocks * 2 * TILE * sizeof(double));
     }
     return g_dinv;
 }
 
-void factor(double *A, int n, int nrows, int lda) {
+double *factor(double *A, int n, int nrows, int lda) {
     int *info = info_buf();
     irwls_dev_memset0(info, 4 * sizeof(int));
     const int nblk = (n + NB - 1) / NB;
@@ -454,7 +452,7 @@ void factor(double *A, int n, int nrows, int lda) {
         if (prow > 0) {
             TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, 16.0 * prow * kb);
             hipLaunchKernelGGL(trsm_kernel, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda, k0,
-                               kb, info, Dinv + (size_t)kbk * NB * NB);
+                               kb, info, Dinv + (size_t)kbk * 2 * TILE);
             IRWLS_LAUNCH_CHECK();
         }
         const int rest = n - k0 - kb;
@@ -464,9 +462,10 @@ void factor(double *A, int n, int nrows, int lda) {
         TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
                       8.0 * (double)rest * rest);
         hipLaunchKernelGGL(syrk_next_kernel, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info,
-                           Dinv + (size_t)(kbk + 1) * NB * NB);
+                           Dinv + (size_t)(kbk + 1) * 2 * TILE);
         IRWLS_LAUNCH_CHECK();
     }
+    return Dinv;
 }
 
 // acc is consumed (left as scratch); the solution lands in out
@@ -480,13 +479,26 @@ void sweep_forward(const double *L, int n, int lda, double *acc, double *out) {
     }
 }
 
+constexpr int BWD_CPW = 8;   // columns per wave in the backward update
+
 void sweep_backward(const double *L, int n, int lda, double *acc, double *out) {
     const int nblk = (n + NB - 1) / NB;
-    const int cpw = 8;
     for (int w = nblk - 1; w >= 0; w--) {
         const int cols = w * NB;
-        const int g = cols > 0 ? (cols + 4 * cpw - 1) / (4 * cpw) : 1;
-        hipLaunchKernelGGL(trsv_bwd_step, dim3(g), dim3(256), 0, stream(), L, n, lda, w, acc, out, cpw);
+        const int g = cols > 0 ? (cols + 4 * BWD_CPW - 1) / (4 * BWD_CPW) : 1;
+        hipLaunchKernelGGL(trsv_bwd_step, dim3(g), dim3(256), 0, stream(), L, n, lda, w, acc, out, BWD_CPW);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+// backward sweep with the diagonal-tile inverses left by factor()
+void sweep_backward_inv(const double *L, int n, int lda, const double *Dinv, double *acc, double *out) {
+    const int nblk = (n + NB - 1) / NB;
+    for (int w = nblk - 1; w >= 0; w--) {
+        const int cols = w * NB;
+        const int g = cols > 0 ? (cols + 4 * BWD_CPW - 1) / (4 * BWD_CPW) : 1;
+        hipLaunchKernelGGL(trsv_bwd_inv_step, dim3(g), dim3(256), 0, stream(), L, n, lda, w,
+                           Dinv + (size_t)(2 * w + 1) * TILE, acc, out, BWD_CPW);
         IRWLS_LAUNCH_CHECK();
     }
 }
@@ -519,13 +531,13 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
     }
     hipLaunchKernelGGL(scatter_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, rhs);
     IRWLS_LAUNCH_CHECK();
-    factor(A, n, n + 1, lda);
+    const double *Dinv = factor(A, n, n + 1, lda);
     double *z = (double *)scratch(9, (size_t)n * sizeof(double));
     hipLaunchKernelGGL(gather_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, z);
     IRWLS_LAUNCH_CHECK();
     {
         TimerScope ts(IRWLS_T_POTRS, 1.0 * (double)n * n, 4.0 * (double)n * n);
-        sweep_backward(A, n, lda, z, x);
+        sweep_backward_inv(A, n, lda, Dinv, z, x);
     }
     int info = 0;
     irwls_dev_d2h(&info, info_buf(), sizeof(int));
