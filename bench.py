@@ -7,11 +7,12 @@ working set 2000, eta=0.001), with the dataset resident in HBM before the
 timed region.  `value` is seconds per training run (lower is better).
 
 The roofline object covers the kernel with the largest share of device time
-in the timed region; its duration comes from HIP events recorded around every
-launch on the library's stream.  The CPU baseline times the oracle (the FP64
-C restatement of the reference, OpenMP over the host cores) on a bounded
-sample: the first outer iterations of the same run, extrapolated to the GPU
-run's outer-iteration count.
+(found in the last warmup step with every timer on); during the timed steps
+HIP events are recorded on the library's stream around that kernel's
+launches only.  The CPU baseline times the oracle (the FP64 C restatement of
+the reference, OpenMP over the host cores) on the same workload: the whole
+run by default, or its first --cpu-outer outer iterations extrapolated to the
+GPU run's outer-iteration count.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -61,7 +62,7 @@ def cpu_baseline(wl, X, y, gpu_outer, max_outer):
     from oracle import pyoracle as po
     props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
     ds = po.Dataset.dense(X, y)
-    po.set_limits(max_outer, 0, 0)
+    po.set_limits(max(max_outer, 0), 0, 0)
     try:
         t0 = time.time()
         po.train_full(ds, props, seed=0)
@@ -70,15 +71,19 @@ def cpu_baseline(wl, X, y, gpu_outer, max_outer):
     finally:
         po.set_limits(0, 0, 0)
     per_outer = dt / max(st["outer_iters"], 1)
+    if max_outer < 0:
+        per_outer = dt / max(gpu_outer, 1)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": per_outer * gpu_outer,
         "unit": "s",
         "cores": cores,
         "kind": "port",
-        "sample": "oracle trainFULL, first %d outer iterations (%d inner) in %.1f s; %.2f s/outer x %d GPU "
-                  "outer iterations (extrapolated)" % (st["outer_iters"], st["inner_iters"], dt, per_outer,
-                                                       gpu_outer),
+        "sample": ("oracle trainFULL, whole run: %d outer / %d inner iterations in %.1f s"
+                   % (st["outer_iters"], st["inner_iters"], dt)) if max_outer < 0 else
+                  ("oracle trainFULL, first %d outer iterations (%d inner) in %.1f s; %.2f s/outer x %d GPU "
+                   "outer iterations (extrapolated)" % (st["outer_iters"], st["inner_iters"], dt, per_outer,
+                                                        gpu_outer)),
     }
 
 
@@ -88,7 +93,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="four_gaussians_n50000_d128", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample (0 = skip)")
+    ap.add_argument("--cpu-outer", type=int, default=-1,
+                    help="outer iterations in the CPU sample (0 = skip, -1 = the whole run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
@@ -104,11 +110,22 @@ def main():
     L.irwls_dev_init()
     ds = lib.Dataset.dense(X, y)
     ds.pin(with_means=False)            # inputs resident in HBM before timing
+    # the last warmup step runs with every timer on and names the kernel with
+    # the largest share of device time; the timed steps record HIP events
+    # around that kernel's launches only
+    dom = None
     for i in range(args.warmup):
+        prof = (i == args.warmup - 1)
+        if prof:
+            lib.timing(True)
         t0 = time.time()
         lib.train_full(ds, props, seed=0)
         log("bench: warmup %d done in %.2f s" % (i, time.time() - t0))
-    lib.timing(True)
+        if prof:
+            tw = lib.timers()
+            dom = max(tw, key=lambda k: tw[k]["ms"])
+            lib.timing(False)
+    lib.timing(True, kinds=None if dom is None else [dom])
     times = []
     beta = None
     for i in range(args.steps):
@@ -128,8 +145,9 @@ def main():
     acc = float(np.mean(pred == yt))
     nsv = int(np.count_nonzero(beta[:-1]))
 
-    total_ms = sum(v["ms"] for v in tm.values())
-    dom = max(tm, key=lambda k: tm[k]["ms"])
+    value = float(np.mean(times))
+    if dom is None:
+        dom = max(tm, key=lambda k: tm[k]["ms"])
     t = tm[dom]
     avg_ms = t["ms"] / max(t["launches"], 1)
     if dom in MFMA_KINDS:
@@ -143,14 +161,13 @@ def main():
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_ms
     roof["launches_per_step"] = t["launches"] / args.steps
-    roof["device_share"] = t["ms"] / total_ms if total_ms else None
+    roof["share_of_step"] = (t["ms"] / args.steps) / (value * 1e3)
 
     cpu = None
-    if args.cpu_outer > 0:
+    if args.cpu_outer != 0:
         log("bench: cpu baseline (oracle, %d outer iterations)" % args.cpu_outer)
         cpu = cpu_baseline(wl, X, y, stats["outer_iters"], args.cpu_outer)
 
-    value = float(np.mean(times))
     out = {
         "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
         "value": value,
@@ -172,6 +189,7 @@ def main():
         "inner_iters": stats["inner_iters"],
         "solve_s": stats["t_solve"],
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in tm.items() if v["launches"]},
+        "warmup_kernel_ms": None if not args.warmup else {k: v["ms"] for k, v in tw.items() if v["launches"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -36,6 +36,8 @@ void irwls_gpu_unpin_dataset(svm_dataset dataset);
 
 /* Per-kernel timers (HIP events on the library stream). */
 void irwls_gpu_timing(int enable);
+/* Restrict timing to the kinds whose bit is set (bit k = timer kind k; default all). */
+void irwls_gpu_timing_mask(unsigned mask);
 void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double *bytes);
 
 /* Counters of the last training call. */

@@ -49,47 +49,44 @@ __device__ __forceinline__ double rdl(double v, int lane) {
 }
 
 // ------------------------------------------------- diagonal tile factor ---
-// The 64 x 64 tile is held in registers: thread t owns row i = t/4 and the
-// sixteen columns c = (t%4) + 4q, q = 0..15, both of the tile (a[q]) and of
+// The 64 x 64 tile is held in registers: wave c0 owns the columns
+// c = c0 + 4q (q = 0..15) and lane i row i, both of the tile (a[q]) and of
 // the inverse of its factor (x[q], starting from the identity).  Pivot j is
-// one barrier: the owners of column j publish it (rows > j, zeros above) and
-// the owners of row j of the inverse publish that row; every thread then
-// takes r = 1/sqrt(a_jj) and updates its elements
+// one barrier: the wave that owns column j publishes it through LDS (rows > j,
+// zeros above) with the pivot value; every wave then takes r = 1/sqrt(a_jj)
+// and updates its elements
 //     a_ic -= (w_i r^2) w_c      (c > j: trailing matrix, l_i l_c)
 //     x_ic -= (w_i r^2) x_jc     (c <= j: rows below of the inverse)
-// while row j of the inverse becomes x_j r and column j of the tile w r
-// (LAPACK potf2 also scales by the reciprocal of the square root).  The
-// broadcast buffers alternate with the pivot parity, so a buffer is only
-// rewritten after every thread has passed the barrier that follows its last
-// read.  Rows and columns at or beyond kb hold the identity.
-constexpr int WLD = 20;   // row stride: the four column classes hit different banks, rows 16-byte aligned
-
+// where w_c comes from lane c of the published column and x_jc from lane j
+// of the wave's own inverse columns, both by v_readlane (wave-uniform
+// operands, no LDS traffic); row j of the inverse becomes x_j r and column
+// j of the tile w r (LAPACK potf2 also scales by the reciprocal of the
+// square root).  Rows i <= j read w_i = 0, so their updates are no-ops and
+// need no branch.  The broadcast buffers alternate with the pivot parity, so
+// a buffer is only rewritten after every thread has passed the barrier that
+// follows its last read.  Rows and columns at or beyond kb hold the
+// identity.  The pivot index is a template parameter so that every register
+// index is static.
 struct PivotBufs {
-    double w[2][4][WLD];   // column j of the trailing matrix: entry c at [c%4][c/4], zero for c <= j
-    double x[2][4][WLD];   // row j of the inverse, same layout
-    double piv[2][2];      // a_jj
+    double w[2][NB];     // column j of the trailing matrix, zero for rows <= j
+    double piv[2][2];    // a_jj
 };
 
-// One pivot, j fixed at compile time so that every register index is static.
-// Returns false once the factorization is over (j >= kb or a failed pivot,
-// whose value is left in wfail).  Rows i <= j read w_i = 0, so their updates
-// are no-ops and need no branch.
+__device__ __forceinline__ int tile_class() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 template <int j>
-__device__ __forceinline__ bool pivot_step(double (&a)[16], double (&x)[16], PivotBufs &B, int kb, int &fail,
+__device__ __forceinline__ bool pivot_step(double (&a)[16], double (&x)[16], PivotBufs &B, int c0, int kb, int &fail,
                                            double &wfail) {
     if (j >= kb) return false;
-    const int t = threadIdx.x, i = t >> 2, c0 = t & 3;
+    const int i = threadIdx.x & 63;
     constexpr int p = j & 1, cj = j & 3, qj = j >> 2;
-    if (c0 == cj) {
+    if (c0 == cj) {   // wave-uniform
         const double v = a[qj];
-        B.w[p][i & 3][i >> 2] = (i > j) ? v : 0.0;
+        B.w[p][i] = (i > j) ? v : 0.0;
         if (i == j) B.piv[p][0] = v;
     }
-    if (i == j) {
-#pragma unroll
-        for (int q = 0; q <= qj; q++) B.x[p][c0][q] = x[q];
-    }
     __syncthreads();
+    const double wv = B.w[p][i];
     const double wjj = B.piv[p][0];
     if (!(wjj > 0.0)) {   // uniform
         fail = j;
@@ -97,25 +94,25 @@ __device__ __forceinline__ bool pivot_step(double (&a)[16], double (&x)[16], Piv
         return false;
     }
     const double r = rsqrt(wjj);
-    const double li = B.w[p][i & 3][i >> 2] * r;   // l_ij, zero for i <= j
+    const double li = wv * r;   // l_ij, zero for i <= j
     const double m = li * r;
 #pragma unroll
-    for (int q = qj; q < 16; q++) a[q] = fma(-m, B.w[p][c0][q], a[q]);
+    for (int q = qj; q < 16; q++) a[q] = fma(-m, rdl(wv, c0 + 4 * q), a[q]);
 #pragma unroll
-    for (int q = 0; q <= qj; q++) x[q] = fma(-m, B.x[p][c0][q], x[q]);
-    if (c0 == cj && i > j) a[qj] = li;
+    for (int q = 0; q <= qj; q++) x[q] = fma(-m, rdl(x[q], j), x[q]);
     if (i == j) {
 #pragma unroll
         for (int q = 0; q <= qj; q++) x[q] = x[q] * r;
-        if (c0 == cj) a[qj] = wjj * r;
     }
+    const double nv = (i == j) ? wjj * r : li;
+    if (c0 == cj && i >= j) a[qj] = nv;
     return true;
 }
 
 template <int... J>
-__device__ __forceinline__ void pivot_chain(double (&a)[16], double (&x)[16], PivotBufs &B, int kb, int &fail,
+__device__ __forceinline__ void pivot_chain(double (&a)[16], double (&x)[16], PivotBufs &B, int c0, int kb, int &fail,
                                             double &wfail, std::integer_sequence<int, J...>) {
-    (pivot_step<J>(a, x, B, kb, fail, wfail) && ...);
+    (pivot_step<J>(a, x, B, c0, kb, fail, wfail) && ...);
 }
 
 // Returns the failing pivot or -1.  After a failure at pivot j the Schur
@@ -123,10 +120,11 @@ __device__ __forceinline__ void pivot_chain(double (&a)[16], double (&x)[16], Pi
 // inverse is divided by the Schur value (as the triangular solves with the
 // factor as it stands do) and the rows below it are left unfinished.
 __device__ __forceinline__ int tile_factor(double (&a)[16], double (&x)[16], PivotBufs &B, int kb) {
+    const int c0 = tile_class();
     int fail = -1;
     double wfail = 0.0;
-    pivot_chain(a, x, B, kb, fail, wfail, std::make_integer_sequence<int, NB>{});
-    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    pivot_chain(a, x, B, c0, kb, fail, wfail, std::make_integer_sequence<int, NB>{});
+    const int i = threadIdx.x & 63;
     if (fail >= 0 && i == fail) {
 #pragma unroll
         for (int q = 0; q < 16; q++)
@@ -136,7 +134,7 @@ __device__ __forceinline__ int tile_factor(double (&a)[16], double (&x)[16], Piv
 }
 
 __device__ __forceinline__ void tile_identity_inverse(double (&x)[16]) {
-    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 16; q++) x[q] = (c0 + 4 * q == i) ? 1.0 : 0.0;
 }
@@ -147,7 +145,7 @@ __device__ __forceinline__ void tile_identity_inverse(double (&x)[16]) {
 // outside the lower kb x kb triangle.
 __device__ __forceinline__ void tile_store(const double (&a)[16], const double (&x)[16], double *A, int lda, int k0,
                                            int kb, double *Dinv, double *DinvT) {
-    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int c = c0 + 4 * q;
@@ -162,7 +160,7 @@ __device__ __forceinline__ void tile_store(const double (&a)[16], const double (
 __global__ __launch_bounds__(256) void diag_first_kernel(double *__restrict__ A, int lda, int kb,
                                                          int *__restrict__ info, double *__restrict__ Dinv) {
     __shared__ PivotBufs B;
-    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
     double a[16], x[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -280,7 +278,7 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     }
     if (!diag) return;
     __syncthreads();
-    const int i = threadIdx.x >> 2, c0 = threadIdx.x & 3;
+    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
     double a[16], x[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {

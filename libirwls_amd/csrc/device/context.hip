@@ -18,6 +18,7 @@ hipStream_t stream() {
 // pairs are resolved (elapsed time read) when the pool fills or the host asks
 // for the totals, so timing does not serialise the stream.
 static bool g_timing = false;
+static unsigned g_timing_mask = ~0u;   // kinds that are timed while timing is on
 static long g_launch[IRWLS_T_COUNT];
 static double g_ms[IRWLS_T_COUNT], g_flops[IRWLS_T_COUNT], g_bytes[IRWLS_T_COUNT];
 static const int kPool = 4096;
@@ -37,7 +38,8 @@ static void resolve_pool() {
     g_evn = 0;
 }
 
-TimerScope::TimerScope(int k, double f, double b) : kind(k), flops(f), bytes(b), on(g_timing) {
+TimerScope::TimerScope(int k, double f, double b)
+    : kind(k), flops(f), bytes(b), on(g_timing && ((g_timing_mask >> k) & 1u)) {
     if (!on) return;
     if (g_evn == kPool) resolve_pool();
     IRWLS_HIP(hipEventRecord(g_evs[g_evn][0], stream()));
@@ -95,8 +97,16 @@ __global__ void fill_zero_kernel(unsigned char *p, size_t n) {
 
 extern "C" {
 
+// The HIP runtime may draw from libc's random() state while it starts up.
+// The training entry points reproduce the reference's rand() sequence from
+// the caller's srand(), which the reference's drivers call before
+// initMemory(), so start-up runs on a private state and the caller's state,
+// position included, is put back afterwards.
+static char g_rand_private[256];
+
 int irwls_dev_init(void) {
     if (g_init) return 0;
+    char *caller_state = initstate(1u, g_rand_private, sizeof(g_rand_private));
     int ndev = 0;
     IRWLS_HIP(hipGetDeviceCount(&ndev));
     if (ndev < 1) {
@@ -113,6 +123,14 @@ int irwls_dev_init(void) {
         g_pool_ready = true;
     }
     g_init = true;
+    // first allocation and first launch (code-object load) inside the guard too
+    void *probe = nullptr;
+    IRWLS_HIP(hipMalloc(&probe, 256));
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, g_stream, (unsigned char *)probe, (size_t)256);
+    IRWLS_LAUNCH_CHECK();
+    IRWLS_HIP(hipStreamSynchronize(g_stream));
+    IRWLS_HIP(hipFree(probe));
+    setstate(caller_state);
     return 0;
 }
 
@@ -184,6 +202,8 @@ void irwls_dev_trim(void) {
         g_scr_sz[s] = 0;
     }
 }
+
+void irwls_dev_timing_mask(unsigned mask) { g_timing_mask = mask; }
 
 void irwls_dev_timing(int enable) {
     irwls_dev_init();

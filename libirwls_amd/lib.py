@@ -85,6 +85,7 @@ def load():
         "irwls_gpu_pin_dataset": (None, [svm_dataset, C.c_int]),
         "irwls_gpu_unpin_dataset": (None, [svm_dataset]),
         "irwls_gpu_timing": (None, [C.c_int]),
+        "irwls_gpu_timing_mask": (None, [C.c_uint]),
         "irwls_gpu_timer": (None, [C.c_int, P(C.c_long), dp, dp, dp]),
         "irwls_gpu_last_stats": (None, [P(irwls_gpu_stats)]),
         "irwls_gpu_shutdown": (None, []),
@@ -299,8 +300,12 @@ def set_options(**kw):
     return o
 
 
-def timing(enable=True):
-    load().irwls_gpu_timing(1 if enable else 0)
+def timing(enable=True, kinds=None):
+    """Per-kernel HIP-event timers; `kinds` restricts them to those timer names."""
+    L = load()
+    mask = 0xFFFFFFFF if kinds is None else sum(1 << TIMER_KINDS.index(k) for k in kinds)
+    L.irwls_gpu_timing_mask(mask)
+    L.irwls_gpu_timing(1 if enable else 0)
 
 
 def timers():
