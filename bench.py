@@ -2,9 +2,13 @@
 """Benchmark of the IRWLS training hot path on one MI355X.
 
 A step is one full-training run (trainFULL through the C entry point) over
-the four-Gaussian workload n=50000, d=128 (FP64, RBF gamma=1/128, C=1,
-working set 2000, eta=0.001), with the dataset resident in HBM before the
-timed region.  `value` is seconds per training run (lower is better).
+the four-Gaussian workload n=50000, d=128 (RBF gamma=1/128, C=1, working
+set 2000, eta=0.001), with the dataset resident in HBM before the timed
+region.  `value` is seconds per training run (lower is better) of the
+mixed-precision trainer (BF16 hi/lo x3 kernel products, FP32 Cholesky with
+FP64 diagonal tiles and FP64 refinement); the plain FP64 trainer runs on the
+same data in the same process and its time, accuracy and the differences
+are reported beside it (--mode plain|mixed runs one of them only).
 
 The roofline object covers the kernel with the largest share of device time
 (found in the last warmup step with every timer on); during the timed steps
@@ -28,9 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 # timer kinds whose bound is the matrix cores vs HBM
-MFMA_KINDS = {"kmv", "kmat", "trsm", "syrk", "potf2", "sgma", "syrkd", "csr"}
+MFMA_KINDS = {"kmv", "kmat", "trsm", "syrk", "potf2", "sgma", "syrkd", "csr", "bf16"}
 
 WORKLOADS = {
     "four_gaussians_n50000_d128": dict(n=50000, n_test=10000, d=128, gamma=1 / 128, C=1.0, W=2000, eta=0.001,
@@ -38,6 +43,9 @@ WORKLOADS = {
     "four_gaussians_n10000_d128": dict(n=10000, n_test=2000, d=128, gamma=1 / 128, C=1.0, W=2000, eta=0.001,
                                        seed=1),
     "two_gaussians_n4000_d16": dict(n=4000, n_test=1000, d=16, gamma=1 / 16, C=10.0, W=500, eta=0.001, seed=0),
+    # mixed-precision workload: same generator, n=200000, d=256, working set 4000
+    "four_gaussians_n200000_d256": dict(n=200000, n_test=20000, d=256, gamma=1 / 256, C=1.0, W=4000, eta=0.001,
+                                        seed=1, cpu_outer=1),
 }
 
 
@@ -87,14 +95,74 @@ def cpu_baseline(wl, X, y, gpu_outer, max_outer):
     }
 
 
+def run_mode(lib, L, ds, props, precision, warmup, steps):
+    """Warmup and timed trainFULL runs in one precision mode.  The last
+    warmup step runs with every timer on and names the kernel with the
+    largest share of device time; the timed steps record HIP events around
+    that kernel's launches only."""
+    lib.set_options(precision=precision)
+    dom, tw = None, None
+    for i in range(warmup):
+        prof = (i == warmup - 1)
+        if prof:
+            lib.timing(True)
+        t0 = time.time()
+        lib.train_full(ds, props, seed=0)
+        log("bench: precision %d warmup %d done in %.2f s" % (precision, i, time.time() - t0))
+        if prof:
+            tw = lib.timers()
+            dom = max(tw, key=lambda k: tw[k]["ms"])
+            lib.timing(False)
+    lib.timing(True, kinds=None if dom is None else [dom])
+    times = []
+    beta = None
+    for i in range(steps):
+        L.irwls_dev_sync()
+        t0 = time.perf_counter()
+        beta = lib.train_full(ds, props, seed=0)
+        L.irwls_dev_sync()
+        times.append(time.perf_counter() - t0)
+        log("bench: precision %d step %d: %.3f s %s" % (precision, i, times[-1], lib.last_stats()))
+    stats = lib.last_stats()
+    tm = lib.timers()
+    lib.timing(False)
+    lib.set_options(precision=0)
+    if dom is None:
+        dom = max(tm, key=lambda k: tm[k]["ms"])
+    return dict(times=times, value=float(np.mean(times)), beta=beta, stats=stats, tm=tm, tw=tw, dom=dom)
+
+
+def roofline(r, steps):
+    t = r["tm"][r["dom"]]
+    avg_ms = t["ms"] / max(t["launches"], 1)
+    dom = r["dom"]
+    if dom in MFMA_KINDS:
+        peak = PEAK_BF16_TFLOPS if dom == "bf16" else PEAK_FP64_TFLOPS
+        ach = (t["flops"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": None}
+    else:
+        ach = (t["bytes"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": None}
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches_per_step"] = t["launches"] / steps
+    roof["share_of_step"] = (t["ms"] / steps) / (r["value"] * 1e3)
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="four_gaussians_n50000_d128", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-outer", type=int, default=-1,
-                    help="outer iterations in the CPU sample (0 = skip, -1 = the whole run)")
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"],
+                    help="plain FP64, mixed precision, or both (value = mixed, plain reported beside it)")
+    ap.add_argument("--cpu-outer", type=int, default=None,
+                    help="outer iterations in the CPU sample (0 = skip, -1 = the whole run; "
+                         "default: the workload's own setting)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
@@ -103,6 +171,7 @@ def main():
 
     from libirwls_amd import abi, lib
     wl = WORKLOADS[args.workload]
+    cpu_outer = wl.get("cpu_outer", -1) if args.cpu_outer is None else args.cpu_outer
     log("bench: generating %s" % args.workload)
     X, y, Xt, yt = make_data(wl)
     props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
@@ -110,89 +179,66 @@ def main():
     L.irwls_dev_init()
     ds = lib.Dataset.dense(X, y)
     ds.pin(with_means=False)            # inputs resident in HBM before timing
-    # the last warmup step runs with every timer on and names the kernel with
-    # the largest share of device time; the timed steps record HIP events
-    # around that kernel's launches only
-    dom = None
-    for i in range(args.warmup):
-        prof = (i == args.warmup - 1)
-        if prof:
-            lib.timing(True)
-        t0 = time.time()
-        lib.train_full(ds, props, seed=0)
-        log("bench: warmup %d done in %.2f s" % (i, time.time() - t0))
-        if prof:
-            tw = lib.timers()
-            dom = max(tw, key=lambda k: tw[k]["ms"])
-            lib.timing(False)
-    lib.timing(True, kinds=None if dom is None else [dom])
-    times = []
-    beta = None
-    for i in range(args.steps):
-        L.irwls_dev_sync()
-        t0 = time.perf_counter()
-        beta = lib.train_full(ds, props, seed=0)
-        L.irwls_dev_sync()
-        times.append(time.perf_counter() - t0)
-        log("bench: step %d: %.3f s %s" % (i, times[-1], lib.last_stats()))
-    stats = lib.last_stats()
-    tm = lib.timers()
-    lib.timing(False)
-    # accuracy of the trained model on the held-out set
-    m = lib.full_model(props, ds, beta)
     td = lib.Dataset.dense(Xt, yt)
-    pred = lib.hard_test(td, m, abi.make_predict_properties(labels=0))
-    acc = float(np.mean(pred == yt))
-    nsv = int(np.count_nonzero(beta[:-1]))
+    pp = abi.make_predict_properties(labels=0)
 
-    value = float(np.mean(times))
-    if dom is None:
-        dom = max(tm, key=lambda k: tm[k]["ms"])
-    t = tm[dom]
-    avg_ms = t["ms"] / max(t["launches"], 1)
-    if dom in MFMA_KINDS:
-        ach = (t["flops"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
-    else:
-        ach = (t["bytes"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None}
-    roof["kernel"] = dom
-    roof["avg_launch_ms"] = avg_ms
-    roof["launches_per_step"] = t["launches"] / args.steps
-    roof["share_of_step"] = (t["ms"] / args.steps) / (value * 1e3)
+    def evaluate(r):
+        m = lib.full_model(props, ds, r["beta"])
+        f = lib.soft_test(td, m, pp)
+        r["decision"] = f
+        r["acc"] = float(np.mean(np.where(f >= 0, 1.0, -1.0) == yt))
+        r["nsv"] = int(np.count_nonzero(r["beta"][:-1]))
+
+    runs = {}
+    for mode in (["plain", "mixed"] if args.mode == "both" else [args.mode]):
+        runs[mode] = run_mode(lib, L, ds, props, 1 if mode == "mixed" else 0, args.warmup, args.steps)
+        evaluate(runs[mode])
+    main_mode = "mixed" if "mixed" in runs else "plain"
+    r = runs[main_mode]
 
     cpu = None
-    if args.cpu_outer != 0:
-        log("bench: cpu baseline (oracle, %d outer iterations)" % args.cpu_outer)
-        cpu = cpu_baseline(wl, X, y, stats["outer_iters"], args.cpu_outer)
+    if cpu_outer != 0:
+        log("bench: cpu baseline (oracle, %s outer iterations)" % ("all" if cpu_outer < 0 else cpu_outer))
+        cpu = cpu_baseline(wl, X, y, runs.get("plain", r)["stats"]["outer_iters"], cpu_outer)
+
+    def summary(x):
+        return {"s": x["value"], "test_accuracy": x["acc"], "n_sv": x["nsv"], "outer_iters": x["stats"]["outer_iters"],
+                "inner_iters": x["stats"]["inner_iters"], "solve_s": x["stats"]["t_solve"],
+                "factorizations": x["stats"]["factorizations"], "pcg_iters": x["stats"]["pcg_iters"],
+                "fp64_fallbacks": x["stats"]["fp64_fallbacks"]}
 
     out = {
         "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
-        "value": value,
+        "value": r["value"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": value * 1e3,
+        "ms_per_step": r["value"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3+f32+f64" if main_mode == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "mode": "plain_fp64", "n": wl["n"], "d": wl["d"], "W": wl["W"],
-                   "gamma": wl["gamma"], "C": wl["C"], "eta": wl["eta"], "n_test": wl["n_test"]},
-        "test_accuracy": acc,
-        "n_sv": nsv,
-        "outer_iters": stats["outer_iters"],
-        "inner_iters": stats["inner_iters"],
-        "solve_s": stats["t_solve"],
-        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in tm.items() if v["launches"]},
-        "warmup_kernel_ms": None if not args.warmup else {k: v["ms"] for k, v in tw.items() if v["launches"]},
-        "roofline": roof,
+        "config": {"workload": args.workload, "mode": main_mode, "n": wl["n"], "d": wl["d"], "W": wl["W"],
+                   "gamma": wl["gamma"], "C": wl["C"], "eta": wl["eta"], "n_test": wl["n_test"],
+                   "seed": wl["seed"]},
+        "test_accuracy": r["acc"],
+        "runs": {k: summary(v) for k, v in runs.items()},
+        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
+        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
+        "roofline": roofline(r, args.steps),
         "cpu_baseline": cpu,
     }
+    if "plain" in runs and "mixed" in runs:
+        p, m = runs["plain"], runs["mixed"]
+        out["speedup_vs_plain"] = p["value"] / m["value"]
+        out["mixed_vs_plain"] = {
+            "accuracy_delta": m["acc"] - p["acc"],
+            "weight_rel_err": float(np.linalg.norm(m["beta"] - p["beta"]) / np.linalg.norm(p["beta"])),
+            "decision_max_err": float(np.abs(m["decision"] - p["decision"]).max()),
+            "decision_rms_err": float(np.sqrt(np.mean((m["decision"] - p["decision"]) ** 2))),
+        }
     print(json.dumps(out), flush=True)
 
 

@@ -49,6 +49,8 @@ enum irwls_timer_kind {
     IRWLS_T_CSR,          /* CSR kernel columns */
     IRWLS_T_SYRKD,        /* IRWLSpar weighted Gram update */
     IRWLS_T_MATVEC,       /* IRWLSpar e = y - Phi beta */
+    IRWLS_T_PCG,          /* mixed-precision refinement (preconditioned CG) */
+    IRWLS_T_BF16,         /* BF16 hi/lo kernel tiles (mixed precision) */
     IRWLS_T_COUNT
 };
 void irwls_dev_timing(int enable);          /* enable/disable and reset */
@@ -118,6 +120,48 @@ void irwls_dev_potrs(const double *d_L, int n, int lda, double *d_b);
    factorization), then one backward sweep gives x.  Failure semantics as
    irwls_dev_potrf (the solve uses the factor as it stands).  Returns info. */
 int  irwls_dev_posv_aug(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
+
+/* Mixed precision: dense rows as BF16 hi/lo pairs (x = hi + lo to about 16
+   bits), row-major with ld a multiple of 32 (zero padded), and FP32 squared
+   norms sum(hi^2 + 2 hi lo). */
+typedef struct irwls_rows_bf16 {
+    const uint16_t *Xh, *Xl;
+    int ld;
+    const float *nrm;      /* by original row */
+    const int32_t *idx;    /* row selection, may be NULL */
+    int n;
+} irwls_rows_bf16;
+
+void irwls_dev_bf16_split(const double *d_X, int ldx, int n, int d, uint16_t *d_Xh, uint16_t *d_Xl, int ldb,
+                          float *d_nrm);
+/* As irwls_dev_kernel_matvec / irwls_dev_kernel_matrix, dot products from
+   three BF16 MFMA products with FP32 accumulation (dense rows only). */
+void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf16 *c, int kernel, double gamma,
+                                  const double *d_w, double a, double b, double cst, const double *d_rs,
+                                  double *d_out);
+void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf16 *c, int kernel, double gamma,
+                                  double *d_K, int ldk);
+
+/* Mixed precision.  FP32 Cholesky of the lower triangle of the FP64 matrix A
+   (n x n, leading dimension lda), FP64 diagonal tiles, with the identity
+   appended as rows n..2n-1 of F (FP32, column-major, ldf >= 2n): on return
+   F holds L in rows 0..n-1 and Z = L^-T in rows n..2n-1, and ZT (n x n,
+   ldz >= n) holds Z^T.  Returns the potrf info (0: positive definite). */
+int  irwls_dev_potrf_inv_f32(const double *d_A, int n, int lda, float *d_F, int ldf, float *d_ZT, int ldz);
+/* Preconditioned CG in FP64 for A X = B with two right-hand sides (B, X:
+   device [2][n]); preconditioner Z Z^T from irwls_dev_potrf_inv_f32 (of this
+   or an earlier matrix).  Stops when |b_c - A x_c| <= tol |b_c| for both
+   columns (relres[c] on return).  Returns the iteration count, negated when
+   maxit was reached without convergence. */
+int  irwls_dev_pcg_spd2(const double *d_A, int n, int lda, const double *d_B, const float *d_F, int ldf,
+                        const float *d_ZT, int ldz, double tol, int maxit, double *d_X, double *relres);
+/* The same with a factor over another index set of size nf (ldf >= 2 nf,
+   ldz >= nf): d_map[i] = factor position of row i or -1 (n entries),
+   d_inv[f] = row at factor position f or -1 (nf entries).  Rows without a
+   factor position are preconditioned by their diagonal. */
+int  irwls_dev_pcg_spd2_mapped(const double *d_A, int n, int lda, const double *d_B, const float *d_F, int ldf,
+                               const float *d_ZT, int ldz, int nf, const int32_t *d_map, const int32_t *d_inv,
+                               double tol, int maxit, double *d_X, double *relres);
 
 /* ---- budgeted training (SGMA, parametric IRWLS) ------------------------- */
 /* C = alpha op(A) op(B) + beta C, all column-major (op = transpose when set). */

@@ -16,7 +16,8 @@ extern "C" {
 #endif
 
 typedef struct irwls_gpu_options {
-    int precision;        /* 0 = FP64 (default), 1 = mixed (BF16x3 kernel GEMM, FP32 Cholesky + refinement) */
+    int precision;        /* 0 = FP64 (default), 1 = mixed (BF16x3 kernel GEMM, FP32 Cholesky + refinement);
+                             2 = only the mixed solver, 3 = only the BF16x3 kernel GEMM (ablations) */
     int sgma_candidates;  /* SGMA candidates per step (default 64, as the reference) */
     int proposal;         /* 0 = plain SGMA draws (default), 1 = learned shortlist */
     int proposal_keep;    /* learned proposal: candidates kept for exact scoring */
@@ -49,6 +50,9 @@ typedef struct irwls_gpu_stats {
     int irwlspar_iters;
     double t_total;        /* seconds, host wall clock */
     double t_solve;        /* seconds inside the dense solver */
+    int factorizations;    /* Cholesky factorizations (mixed precision: FP32 factors) */
+    int pcg_iters;         /* mixed precision: CG iterations over all solves */
+    int fp64_fallbacks;    /* mixed precision: solves done by the FP64 path instead */
 } irwls_gpu_stats;
 void irwls_gpu_last_stats(irwls_gpu_stats *out);
 

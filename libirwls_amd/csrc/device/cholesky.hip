@@ -142,65 +142,88 @@ __device__ __forceinline__ void tile_identity_inverse(double (&x)[16]) {
 // Write the factored tile (lower kb x kb part) back to A(k0.., k0..), the
 // inverse of the factor to Dinv (column-major: Dinv[c*64 + i] = Linv(i, c))
 // and its transpose to DinvT (DinvT[c*64 + i] = Linv(c, i)); both are zero
-// outside the lower kb x kb triangle.
-__device__ __forceinline__ void tile_store(const double (&a)[16], const double (&x)[16], double *A, int lda, int k0,
-                                           int kb, double *Dinv, double *DinvT) {
+// outside the lower kb x kb triangle.  T is the storage type of the matrix.
+template <typename T>
+__device__ __forceinline__ void tile_store(const double (&a)[16], const double (&x)[16], T *A, int lda, int k0,
+                                           int kb, T *Dinv, T *DinvT) {
     const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int c = c0 + 4 * q;
         const bool low = (i < kb && c <= i);
-        if (low) A[(size_t)(k0 + c) * lda + k0 + i] = a[q];
-        const double v = low ? x[q] : 0.0;
+        if (low) A[(size_t)(k0 + c) * lda + k0 + i] = (T)a[q];
+        const T v = low ? (T)x[q] : (T)0;
         Dinv[c * NB + i] = v;
         DinvT[i * NB + c] = v;
     }
 }
 
-__global__ __launch_bounds__(256) void diag_first_kernel(double *__restrict__ A, int lda, int kb,
-                                                         int *__restrict__ info, double *__restrict__ Dinv) {
+template <typename T>
+__global__ __launch_bounds__(256) void diag_first_kernel(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
+                                                         T *__restrict__ Dinv) {
     __shared__ PivotBufs B;
     const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
     double a[16], x[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int c = c0 + 4 * q;
-        a[q] = (i < kb && c <= i) ? A[(size_t)c * lda + i] : (c == i ? 1.0 : 0.0);
+        a[q] = (i < kb && c <= i) ? (double)A[(size_t)c * lda + i] : (c == i ? 1.0 : 0.0);
     }
     tile_identity_inverse(x);
     const int fail = tile_factor(a, x, B, kb);
-    tile_store(a, x, A, lda, 0, kb, Dinv, Dinv + TILE);
+    tile_store<T>(a, x, A, lda, 0, kb, Dinv, Dinv + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
 // ------------------------------------------------------------ MFMA tiles ---
+// FP64: v_mfma_f64_16x16x4f64, accumulator register r of lane l holds
+// C[(l/16) + 4r][l%16].  FP32: v_mfma_f32_16x16x4f32, register r of lane l
+// holds C[4(l/16) + r][l%16].  Both take A as lane l = (row l%16, k l/16)
+// and B as lane l = (k l/16, col l%16).
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 // acc[mb][nb] += sum_{k < K} P(pr + 16 mb + row, k) Q(qr + 16 nb + col, k)
 // for the wave's 32 x 32 block, P(r, k) = P[k*ldp + r] and Q likewise
 // (column-major); rows at or beyond prmax / qrmax and k >= K read as zero.
-// The operands go straight from memory into the MFMA source registers (A:
-// lane l = (row l%16, k l/16); B: lane l = (k l/16, col l%16)), all loads
-// issued before the first MFMA.
-__device__ __forceinline__ void mfma_rows(v4d (&acc)[2][2], const double *__restrict__ P, int ldp, int pr, int prmax,
-                                          const double *__restrict__ Q, int ldq, int qr, int qrmax, int K) {
+// The operands go straight from memory into the MFMA source registers, all
+// loads issued before the first MFMA.
+template <typename T>
+__device__ __forceinline__ void mfma_rows(typename Mma<T>::acc_t (&acc)[2][2], const T *__restrict__ P, int ldp,
+                                          int pr, int prmax, const T *__restrict__ Q, int ldq, int qr, int qrmax,
+                                          int K) {
     const int lane = threadIdx.x & 63;
     const int rr = lane & 15, kq = lane >> 4;
-    double a[2][16], b[2][16];
+    T a[2][16], b[2][16];
 #pragma unroll
     for (int s = 0; s < 16; s++) {
         const int k = 4 * s + kq;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int ri = pr + h * 16 + rr, rj = qr + h * 16 + rr;
-            a[h][s] = (k < K && ri < prmax) ? P[(size_t)k * ldp + ri] : 0.0;
-            b[h][s] = (k < K && rj < qrmax) ? Q[(size_t)k * ldq + rj] : 0.0;
+            a[h][s] = (k < K && ri < prmax) ? P[(size_t)k * ldp + ri] : (T)0;
+            b[h][s] = (k < K && rj < qrmax) ? Q[(size_t)k * ldq + rj] : (T)0;
         }
     }
 #pragma unroll
     for (int s = 0; s < 16; s++) {
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b[0][s], acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b[1][s], acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b[0][s], acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b[1][s], acc[1][1], 0, 0, 0);
+        acc[0][0] = Mma<T>::mma(a[0][s], b[0][s], acc[0][0]);
+        acc[0][1] = Mma<T>::mma(a[0][s], b[1][s], acc[0][1]);
+        acc[1][0] = Mma<T>::mma(a[1][s], b[0][s], acc[1][0]);
+        acc[1][1] = Mma<T>::mma(a[1][s], b[1][s], acc[1][1]);
     }
 }
 
@@ -208,8 +231,9 @@ __device__ __forceinline__ void mfma_rows(v4d (&acc)[2][2], const double *__rest
 // Panel rows [k0+kb, nrows): P = A(:, k0:k0+kb) * Linv^T, in place.  After
 // the factorization has failed in this block only right-hand-side rows
 // (rows >= n) are solved; after a failure in an earlier block nothing is.
-__global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
-                                                   const int *__restrict__ info, const double *__restrict__ Dinv) {
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                   const int *__restrict__ info, const T *__restrict__ Dinv) {
     const int inf = info[0];
     if (inf != 0 && inf <= k0) return;
     const int lane = threadIdx.x & 63;
@@ -218,18 +242,18 @@ __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n
     const int r0 = k0 + kb + blockIdx.x * NB;
     const int rmin = (inf != 0) ? n : 0;
     if (r0 + NB <= rmin) return;
-    v4d acc[2][2];
+    typename Mma<T>::acc_t acc[2][2];
     for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 2; b++) acc[a][b] = (typename Mma<T>::acc_t){0, 0, 0, 0};
     // Q(j, k) = Linv(j, k) = Dinv[k*64 + j]
-    mfma_rows(acc, A + (size_t)k0 * lda, lda, r0 + wr * 32, nrows, Dinv, NB, wc * 32, kb, kb);
+    mfma_rows<T>(acc, A + (size_t)k0 * lda, lda, r0 + wr * 32, nrows, Dinv, NB, wc * 32, kb, kb);
     __syncthreads();   // both column halves read the whole panel row before it is overwritten
     for (int nb = 0; nb < 2; nb++) {
         const int c = wc * 32 + nb * 16 + (lane & 15);
         if (c >= kb) continue;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int i = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
+                const int i = r0 + wr * 32 + mb * 16 + Mma<T>::row(lane, r);
                 if (i < nrows && i >= rmin) A[(size_t)(k0 + c) * lda + i] = acc[mb][nb][r];
             }
     }
@@ -239,10 +263,11 @@ __global__ __launch_bounds__(256) void trsm_kernel(double *__restrict__ A, int n
 // Trailing tiles: A(I, J) -= P_I P_J^T, rows [t0, nrows) x columns [t0, n),
 // P = A(:, k0:k0+kb) (solved), t0 = k0 + kb; grid (column tiles, row tiles),
 // tiles above the diagonal return at once.  Work-group (0,0) owns the next
-// diagonal tile: after its update it factors and inverts it (look-ahead), so
-// the next step needs no separate launch for that.
-__global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, int n, int nrows, int lda, int k0,
-                                                        int kb, int *__restrict__ info, double *__restrict__ Dnext) {
+// diagonal tile: after its update it factors and inverts it in FP64
+// (look-ahead), so the next step needs no separate launch for that.
+template <typename T>
+__global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                        int *__restrict__ info, T *__restrict__ Dnext) {
     __shared__ double S[NB][NB + 1];
     __shared__ PivotBufs B;
     const int tj = blockIdx.x, ti = blockIdx.y;
@@ -255,10 +280,11 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    v4d acc[2][2];
+    typename Mma<T>::acc_t acc[2][2];
     for (int a = 0; a < 2; a++)
-        for (int c = 0; c < 2; c++) acc[a][c] = (v4d){0.0, 0.0, 0.0, 0.0};
-    mfma_rows(acc, A + (size_t)k0 * lda, lda, rI + wr * 32, nrows, A + (size_t)k0 * lda, lda, rJ + wc * 32, n, kb);
+        for (int c = 0; c < 2; c++) acc[a][c] = (typename Mma<T>::acc_t){0, 0, 0, 0};
+    mfma_rows<T>(acc, A + (size_t)k0 * lda, lda, rI + wr * 32, nrows, A + (size_t)k0 * lda, lda, rJ + wc * 32, n,
+                 kb);
     const bool diag = (ti == 0 && tj == 0);
     const int kb2 = (n - t0 < NB) ? n - t0 : NB;
     for (int nb = 0; nb < 2; nb++) {
@@ -266,12 +292,12 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
         const int j = rJ + jl;
         for (int mb = 0; mb < 2; mb++)
             for (int r = 0; r < 4; r++) {
-                const int il = wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
+                const int il = wr * 32 + mb * 16 + Mma<T>::row(lane, r);
                 const int i = rI + il;
                 const bool in_tile = diag && il < kb2;   // matrix rows of the next diagonal tile
                 if (j < n && i < nrows && i >= j) {
-                    const double v = A[(size_t)j * lda + i] - acc[mb][nb][r];
-                    if (in_tile) S[il][jl] = v;
+                    const T v = A[(size_t)j * lda + i] - acc[mb][nb][r];
+                    if (in_tile) S[il][jl] = (double)v;
                     else A[(size_t)j * lda + i] = v;   // other tiles and right-hand-side rows
                 }
             }
@@ -287,7 +313,7 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(double *__restrict__ A, 
     }
     tile_identity_inverse(x);
     const int fail = tile_factor(a, x, B, kb2);
-    tile_store(a, x, A, lda, t0, kb2, Dnext, Dnext + TILE);
+    tile_store<T>(a, x, A, lda, t0, kb2, Dnext, Dnext + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
@@ -420,27 +446,33 @@ __global__ void scatter_row(double *__restrict__ A, int lda, int row, int n, con
 
 // inverses of the diagonal tiles of the last factorization: block b holds
 // Dinv at [2b * 4096] and its transpose at [(2b+1) * 4096]
-double *g_dinv = nullptr;
-int g_dinv_blocks = 0;
+template <typename T> struct DinvPool {
+    static T *buf;
+    static int blocks;
+};
+template <typename T> T *DinvPool<T>::buf = nullptr;
+template <typename T> int DinvPool<T>::blocks = 0;
 
-double *dinv_buf(int nblk) {
-    if (g_dinv_blocks < nblk) {
-        if (g_dinv) irwls_dev_free(g_dinv);
-        g_dinv_blocks = nblk < 16 ? 16 : nblk;
-        g_dinv = (double *)irwls_dev_alloc((size_t)g_dinv_blocks * 2 * TILE * sizeof(double));
+template <typename T>
+T *dinv_buf(int nblk) {
+    if (DinvPool<T>::blocks < nblk) {
+        if (DinvPool<T>::buf) irwls_dev_free(DinvPool<T>::buf);
+        DinvPool<T>::blocks = nblk < 16 ? 16 : nblk;
+        DinvPool<T>::buf = (T *)irwls_dev_alloc((size_t)DinvPool<T>::blocks * 2 * TILE * sizeof(T));
     }
-    return g_dinv;
+    return DinvPool<T>::buf;
 }
 
-double *factor(double *A, int n, int nrows, int lda) {
+template <typename T>
+T *factor(T *A, int n, int nrows, int lda) {
     int *info = info_buf();
     irwls_dev_memset0(info, 4 * sizeof(int));
     const int nblk = (n + NB - 1) / NB;
-    double *Dinv = dinv_buf(nblk);
+    T *Dinv = dinv_buf<T>(nblk);
     {
         const int kb = (n < NB) ? n : NB;
         TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0, 8.0 * (double)kb * kb);
-        hipLaunchKernelGGL(diag_first_kernel, dim3(1), dim3(256), 0, stream(), A, lda, kb, info, Dinv);
+        hipLaunchKernelGGL(diag_first_kernel<T>, dim3(1), dim3(256), 0, stream(), A, lda, kb, info, Dinv);
         IRWLS_LAUNCH_CHECK();
     }
     for (int kbk = 0; kbk < nblk; kbk++) {
@@ -448,9 +480,9 @@ double *factor(double *A, int n, int nrows, int lda) {
         const int kb = (n - k0 < NB) ? (n - k0) : NB;
         const int prow = nrows - k0 - kb;
         if (prow > 0) {
-            TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, 16.0 * prow * kb);
-            hipLaunchKernelGGL(trsm_kernel, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda, k0,
-                               kb, info, Dinv + (size_t)kbk * 2 * TILE);
+            TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, (double)sizeof(T) * 2.0 * prow * kb);
+            hipLaunchKernelGGL(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda,
+                               k0, kb, info, Dinv + (size_t)kbk * 2 * TILE);
             IRWLS_LAUNCH_CHECK();
         }
         const int rest = n - k0 - kb;
@@ -458,12 +490,43 @@ double *factor(double *A, int n, int nrows, int lda) {
         const int tc = (rest + NB - 1) / NB;
         const int tr = (nrows - k0 - kb + NB - 1) / NB;
         TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
-                      8.0 * (double)rest * rest);
-        hipLaunchKernelGGL(syrk_next_kernel, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info,
+                      (double)sizeof(T) * (double)rest * rest);
+        hipLaunchKernelGGL(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info,
                            Dinv + (size_t)(kbk + 1) * 2 * TILE);
         IRWLS_LAUNCH_CHECK();
     }
     return Dinv;
+}
+
+// FP32 working copy for the mixed-precision factor: F(r, c) = A(r, c) for
+// c <= r < n (the lower triangle is all the factorization reads) and the
+// identity in rows n..2n-1, which the factorization turns into L^-T.
+__global__ void to_f32_with_identity(const double *__restrict__ A, int n, int lda, float *__restrict__ F, int ldf) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= 2 * n) return;
+    float v;
+    if (r < n) v = (c <= r) ? (float)A[(size_t)c * lda + r] : 0.0f;
+    else v = (r - n == c) ? 1.0f : 0.0f;
+    F[(size_t)c * ldf + r] = v;
+}
+
+// ZT(c, i) = Z(i, c) = F(n + i, c) for c >= i (Z = L^-T is upper
+// triangular), zero below; 64 x 64 tiles through LDS for coalescing.
+__global__ __launch_bounds__(256) void transpose_z(const float *__restrict__ F, int n, int ldf, float *__restrict__ ZT,
+                                                   int ldz) {
+    __shared__ float tile[64][65];
+    const int i0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int cc = ty; cc < 64; cc += 4) {   // read F columns c0+cc, rows n+i0+tx
+        const int c = c0 + cc, i = i0 + tx;
+        tile[cc][tx] = (c < n && i < n && c >= i) ? F[(size_t)c * ldf + n + i] : 0.0f;
+    }
+    __syncthreads();
+    for (int ii = ty; ii < 64; ii += 4) {   // write ZT columns i0+ii, rows c0+tx
+        const int i = i0 + ii, c = c0 + tx;
+        if (i < n && c < n) ZT[(size_t)i * ldz + c] = tile[tx][ii];
+    }
 }
 
 // acc is consumed (left as scratch); the solution lands in out
@@ -507,7 +570,7 @@ extern "C" {
 
 int irwls_dev_potrf(double *A, int n, int lda) {
     if (n <= 0) return 0;
-    factor(A, n, n, lda);
+    factor<double>(A, n, n, lda);
     int info = 0;
     irwls_dev_d2h(&info, info_buf(), sizeof(int));
     return info;
@@ -529,7 +592,7 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
     }
     hipLaunchKernelGGL(scatter_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, rhs);
     IRWLS_LAUNCH_CHECK();
-    const double *Dinv = factor(A, n, n + 1, lda);
+    const double *Dinv = factor<double>(A, n, n + 1, lda);
     double *z = (double *)scratch(9, (size_t)n * sizeof(double));
     hipLaunchKernelGGL(gather_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, z);
     IRWLS_LAUNCH_CHECK();
@@ -537,6 +600,24 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
         TimerScope ts(IRWLS_T_POTRS, 1.0 * (double)n * n, 4.0 * (double)n * n);
         sweep_backward_inv(A, n, lda, Dinv, z, x);
     }
+    int info = 0;
+    irwls_dev_d2h(&info, info_buf(), sizeof(int));
+    return info;
+}
+
+int irwls_dev_potrf_inv_f32(const double *A, int n, int lda, float *F, int ldf, float *ZT, int ldz) {
+    if (n <= 0) return 0;
+    if (ldf < 2 * n || ldz < n) {
+        fprintf(stderr, "libirwls: irwls_dev_potrf_inv_f32 needs ldf >= 2n and ldz >= n (n=%d, ldf=%d, ldz=%d)\n", n,
+                ldf, ldz);
+        exit(2);
+    }
+    hipLaunchKernelGGL(to_f32_with_identity, dim3((2 * n + 255) / 256, n), dim3(256), 0, stream(), A, n, lda, F, ldf);
+    IRWLS_LAUNCH_CHECK();
+    factor<float>(F, n, 2 * n, ldf);
+    const int nt = (n + 63) / 64;
+    hipLaunchKernelGGL(transpose_z, dim3(nt, nt), dim3(256), 0, stream(), F, n, ldf, ZT, ldz);
+    IRWLS_LAUNCH_CHECK();
     int info = 0;
     irwls_dev_d2h(&info, info_buf(), sizeof(int));
     return info;

@@ -1,0 +1,339 @@
+// Mixed-precision SPD solve for two right-hand sides: preconditioned
+// conjugate gradients in FP64 on the FP64 matrix, with the FP32 Cholesky
+// factor (FP64 diagonal tiles) as preconditioner M^-1 = Z Z^T, Z = L^-T.
+//
+// The FP64 iteration recovers FP64 residuals from the FP32 factor; CG
+// converges for any SPD preconditioner, so a factor of an earlier matrix
+// (same index set, other diagonal) still serves, at the cost of more
+// iterations.  Every matrix-vector product is one wave per output entry over
+// a contiguous column: A is symmetric (column i = row i), Z^T r reads the
+// columns of Z from the factor's identity rows and Z W the columns of Z^T.
+// Dot products leave one partial per work-group; the next kernel reduces
+// them (each work-group redundantly), so no launch is spent on reductions.
+// Vectors with two columns are stored [2][n].
+#include "common.h"
+
+using namespace irwls;
+
+namespace {
+
+// out_c(i) = sum_{k in range(i)} M[i*ld + off + k] v_c(k), c = 0, 1, with
+// range [0, n) (mode 0), [0, i] (mode 1) or [i, n) (mode 2).  One wave per i.
+// When dot_with is set, work-group partials of sum_i dot_with_c(i) out_c(i)
+// go to part[2*blockIdx.x + c].
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void coldot2(const T *__restrict__ M, int ld, size_t off, int n,
+                                               const double *__restrict__ v, double *__restrict__ out,
+                                               const double *__restrict__ dot_with, double *__restrict__ part) {
+    __shared__ double wp[4][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wave;
+    double s0 = 0.0, s1 = 0.0;
+    if (i < n) {
+        const int lo = (MODE == 2) ? i : 0;
+        const int hi = (MODE == 1) ? i + 1 : n;
+        const T *col = M + (size_t)i * ld + off;
+        for (int k = lo + lane; k < hi; k += 64) {
+            const double m = (double)col[k];
+            s0 = fma(m, v[k], s0);
+            s1 = fma(m, v[n + k], s1);
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            s0 += __shfl_xor(s0, o);
+            s1 += __shfl_xor(s1, o);
+        }
+        if (lane == 0) {
+            out[i] = s0;
+            out[n + i] = s1;
+        }
+    }
+    if (dot_with) {
+        if (lane == 0) {
+            wp[wave][0] = (i < n) ? dot_with[i] * s0 : 0.0;
+            wp[wave][1] = (i < n) ? dot_with[n + i] * s1 : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            const int c = threadIdx.x;
+            part[2 * blockIdx.x + c] = (wp[0][c] + wp[1][c]) + (wp[2][c] + wp[3][c]);
+        }
+    }
+}
+
+// Sum of np partial pairs (part[2p + c]) into red[c], by all 256 threads.
+__device__ __forceinline__ void reduce_parts(const double *__restrict__ part, int np, double *red) {
+    __shared__ double sh[2][256];
+    double a0 = 0.0, a1 = 0.0;
+    for (int p = threadIdx.x; p < np; p += 256) {
+        a0 += part[2 * p];
+        a1 += part[2 * p + 1];
+    }
+    sh[0][threadIdx.x] = a0;
+    sh[1][threadIdx.x] = a1;
+    __syncthreads();
+    for (int s = 128; s >= 1; s >>= 1) {
+        if (threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    red[0] = sh[0][0];
+    red[1] = sh[1][0];
+    __syncthreads();
+}
+
+// alpha_c = rz_c / (p_c . q_c);  x_c += alpha_c p_c;  r_c -= alpha_c q_c;
+// work-group partials of |r_c|^2 to rr_part.  A column whose rz is zero
+// (solved, or a zero right-hand side) is left as it is.
+__global__ __launch_bounds__(256) void update_xr(int n, const double *__restrict__ pq_part, int npq,
+                                                 const double *__restrict__ rz, const double *__restrict__ p,
+                                                 const double *__restrict__ q, double *__restrict__ x,
+                                                 double *__restrict__ r, double *__restrict__ rr_part) {
+    __shared__ double wsum[2][4];
+    double pq[2];
+    reduce_parts(pq_part, npq, pq);
+    const double al0 = (rz[0] != 0.0 && pq[0] != 0.0) ? rz[0] / pq[0] : 0.0;
+    const double al1 = (rz[1] != 0.0 && pq[1] != 0.0) ? rz[1] / pq[1] : 0.0;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double r0 = 0.0, r1 = 0.0;
+    if (i < n) {
+        x[i] = fma(al0, p[i], x[i]);
+        x[n + i] = fma(al1, p[n + i], x[n + i]);
+        r0 = fma(-al0, q[i], r[i]);
+        r1 = fma(-al1, q[n + i], r[n + i]);
+        r[i] = r0;
+        r[n + i] = r1;
+    }
+    double s0 = r0 * r0, s1 = r1 * r1;
+    for (int o = 32; o >= 1; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wsum[0][wave] = s0;
+        wsum[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int c = threadIdx.x;
+        rr_part[2 * blockIdx.x + c] = (wsum[c][0] + wsum[c][1]) + (wsum[c][2] + wsum[c][3]);
+    }
+}
+
+// rz_new_c = sum of partials; beta_c = rz_new_c / rz_old_c (0 on the first
+// call); p_c = z_c + beta_c p_c; work-group 0 stores rz_new in rz_out (a
+// different slot from rz_old, so no work-group reads what another writes).
+__global__ __launch_bounds__(256) void update_p(int n, const double *__restrict__ rz_part, int nrz,
+                                                const double *__restrict__ rz_old, int first,
+                                                double *__restrict__ rz_out, const double *__restrict__ z,
+                                                double *__restrict__ p) {
+    double rzn[2];
+    reduce_parts(rz_part, nrz, rzn);
+    const double b0 = (first || rz_old[0] == 0.0) ? 0.0 : rzn[0] / rz_old[0];
+    const double b1 = (first || rz_old[1] == 0.0) ? 0.0 : rzn[1] / rz_old[1];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        p[i] = fma(b0, p[i], z[i]);
+        p[n + i] = fma(b1, p[n + i], z[n + i]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rz_out[0] = rzn[0];
+        rz_out[1] = rzn[1];
+    }
+}
+
+// Preconditioner of a system whose index set differs from the factored
+// one.  inv[f] = row of the system at factor position f (or -1), map[i] =
+// factor position of row i (or -1).  r is scattered into the factor's index
+// space (zeros where the system has no row), Z Z^T applied there, and the
+// result gathered back; rows the factor lacks take the Jacobi step r_i/A_ii.
+// For rows removed from a factored SPD matrix the operator (A_old^-1)_CC is
+// SPD and differs from (A_CC)^-1 by a term of rank |removed|.
+__global__ void scatter_to_factor(int n, int nf, const int32_t *__restrict__ inv, const double *__restrict__ r,
+                                  double *__restrict__ rf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int i = inv[f];
+    rf[f] = (i >= 0) ? r[i] : 0.0;
+    rf[nf + f] = (i >= 0) ? r[n + i] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void gather_from_factor(int n, int nf, const int32_t *__restrict__ map,
+                                                          const double *__restrict__ zf, const double *__restrict__ r,
+                                                          const double *__restrict__ A, int lda,
+                                                          double *__restrict__ z, double *__restrict__ part) {
+    __shared__ double wsum[2][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double p0 = 0.0, p1 = 0.0;
+    if (i < n) {
+        const int f = map[i];
+        double z0, z1;
+        if (f >= 0) {
+            z0 = zf[f];
+            z1 = zf[nf + f];
+        } else {
+            const double dii = A[(size_t)i * lda + i];
+            z0 = r[i] / dii;
+            z1 = r[n + i] / dii;
+        }
+        z[i] = z0;
+        z[n + i] = z1;
+        p0 = r[i] * z0;
+        p1 = r[n + i] * z1;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        p0 += __shfl_xor(p0, o);
+        p1 += __shfl_xor(p1, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wsum[0][wave] = p0;
+        wsum[1][wave] = p1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int c = threadIdx.x;
+        part[2 * blockIdx.x + c] = (wsum[c][0] + wsum[c][1]) + (wsum[c][2] + wsum[c][3]);
+    }
+}
+
+struct PcgBufs {
+    int cap = 0;
+    double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *w = nullptr;
+    double *rf = nullptr, *zf = nullptr;
+    double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr, *rz = nullptr;
+};
+PcgBufs g_pcg;
+
+void pcg_reserve(int n) {
+    if (g_pcg.cap >= n) return;
+    double **vs[] = {&g_pcg.r, &g_pcg.p, &g_pcg.q, &g_pcg.z, &g_pcg.w, &g_pcg.rf, &g_pcg.zf};
+    for (double **v : vs) {
+        if (*v) irwls_dev_free(*v);
+        *v = (double *)irwls_dev_alloc((size_t)2 * n * sizeof(double));
+    }
+    double **ps[] = {&g_pcg.part_a, &g_pcg.part_b, &g_pcg.part_c};
+    for (double **v : ps) {
+        if (*v) irwls_dev_free(*v);
+        *v = (double *)irwls_dev_alloc((size_t)2 * ((n + 3) / 4 + 1) * sizeof(double));
+    }
+    if (!g_pcg.rz) g_pcg.rz = (double *)irwls_dev_alloc(4 * sizeof(double));
+    g_pcg.cap = n;
+}
+
+// z = M^-1 r, with partials of r . z in part (one pair per block of 4 rows
+// without a map, per block of 256 rows with one; npart returns the count).
+// Without a map the factor's index set is the system's (nf = n).
+struct Precond {
+    const float *F;
+    int ldf;
+    const float *ZT;
+    int ldz;
+    int nf;
+    const int32_t *map, *inv;
+};
+
+int precond(const Precond &M, int n, const double *A, int lda, const double *r, double *w, double *z,
+            double *part) {
+    const int nf = M.map ? M.nf : n;
+    const int g = (nf + 3) / 4;
+    if (!M.map) {
+        hipLaunchKernelGGL((coldot2<float, 1>), dim3(g), dim3(256), 0, stream(), M.F, M.ldf, (size_t)nf, nf, r, w,
+                           (const double *)nullptr, (double *)nullptr);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL((coldot2<float, 2>), dim3(g), dim3(256), 0, stream(), M.ZT, M.ldz, (size_t)0, nf, w, z, r,
+                           part);
+        IRWLS_LAUNCH_CHECK();
+        return g;
+    }
+    hipLaunchKernelGGL(scatter_to_factor, dim3((nf + 255) / 256), dim3(256), 0, stream(), n, nf, M.inv, r, g_pcg.rf);
+    IRWLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL((coldot2<float, 1>), dim3(g), dim3(256), 0, stream(), M.F, M.ldf, (size_t)nf, nf, g_pcg.rf,
+                       w, (const double *)nullptr, (double *)nullptr);
+    IRWLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL((coldot2<float, 2>), dim3(g), dim3(256), 0, stream(), M.ZT, M.ldz, (size_t)0, nf, w, g_pcg.zf,
+                       (const double *)nullptr, (double *)nullptr);
+    IRWLS_LAUNCH_CHECK();
+    const int gv = (n + 255) / 256;
+    hipLaunchKernelGGL(gather_from_factor, dim3(gv), dim3(256), 0, stream(), n, nf, M.map, g_pcg.zf, r, A, lda, z,
+                       part);
+    IRWLS_LAUNCH_CHECK();
+    return gv;
+}
+
+}  // namespace
+
+extern "C" {
+
+int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, const float *F, int ldf,
+                              const float *ZT, int ldz, int nf, const int32_t *map, const int32_t *inv, double tol,
+                              int maxit, double *X, double *relres) {
+    if (n <= 0) return 0;
+    pcg_reserve(n > nf ? n : nf);
+    const Precond M = {F, ldf, ZT, ldz, nf, map, inv};
+    const size_t vb = (size_t)2 * n * sizeof(double);
+    const int gcol = (n + 3) / 4;           // work-groups of the one-wave-per-entry kernels
+    const int gvec = (n + 255) / 256;       // work-groups of the elementwise kernels
+    double bn[2] = {0.0, 0.0};
+    {
+        double *hb = (double *)malloc(vb);
+        irwls_dev_d2h(hb, B, vb);
+        for (int i = 0; i < n; i++) {
+            bn[0] += hb[i] * hb[i];
+            bn[1] += hb[n + i] * hb[n + i];
+        }
+        free(hb);
+    }
+    TimerScope ts(IRWLS_T_PCG, 0.0, 0.0);
+    irwls_dev_memset0(X, vb);
+    irwls_dev_d2d(g_pcg.r, B, vb);
+    int np = precond(M, n, A, lda, g_pcg.r, g_pcg.w, g_pcg.z, g_pcg.part_c);
+    hipLaunchKernelGGL(update_p, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_c, np, g_pcg.rz, 1, g_pcg.rz,
+                       g_pcg.z, g_pcg.p);
+    IRWLS_LAUNCH_CHECK();
+    double *rr_host = (double *)malloc((size_t)2 * gvec * sizeof(double));
+    int it = 0, done = 0;
+    relres[0] = relres[1] = 1.0;
+    for (it = 1; it <= maxit; it++) {
+        double *rz_cur = g_pcg.rz + 2 * ((it - 1) & 1);
+        double *rz_next = g_pcg.rz + 2 * (it & 1);
+        hipLaunchKernelGGL((coldot2<double, 0>), dim3(gcol), dim3(256), 0, stream(), A, lda, (size_t)0, n, g_pcg.p,
+                           g_pcg.q, g_pcg.p, g_pcg.part_a);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(update_xr, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_a, gcol, rz_cur, g_pcg.p,
+                           g_pcg.q, X, g_pcg.r, g_pcg.part_b);
+        IRWLS_LAUNCH_CHECK();
+        // the residual is read back (a host synchronisation) every other
+        // iteration and at the last one; extra iterations after convergence
+        // only shrink it further
+        if ((it & 1) == 0 || it == maxit) {
+            irwls_dev_d2h(rr_host, g_pcg.part_b, (size_t)2 * gvec * sizeof(double));
+            double rr[2] = {0.0, 0.0};
+            for (int g = 0; g < gvec; g++) {
+                rr[0] += rr_host[2 * g];
+                rr[1] += rr_host[2 * g + 1];
+            }
+            for (int c = 0; c < 2; c++) relres[c] = (bn[c] > 0.0) ? sqrt(rr[c] / bn[c]) : 0.0;
+            if (relres[0] <= tol && relres[1] <= tol) {
+                done = 1;
+                break;
+            }
+            if (!(relres[0] < 1e30 && relres[1] < 1e30)) break;   // diverged or NaN
+        }
+        np = precond(M, n, A, lda, g_pcg.r, g_pcg.w, g_pcg.z, g_pcg.part_c);
+        hipLaunchKernelGGL(update_p, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_c, np, rz_cur, 0, rz_next,
+                           g_pcg.z, g_pcg.p);
+        IRWLS_LAUNCH_CHECK();
+    }
+    free(rr_host);
+    return done ? it : -it;
+}
+
+int irwls_dev_pcg_spd2(const double *A, int n, int lda, const double *B, const float *F, int ldf, const float *ZT,
+                       int ldz, double tol, int maxit, double *X, double *relres) {
+    return irwls_dev_pcg_spd2_mapped(A, n, lda, B, F, ldf, ZT, ldz, n, nullptr, nullptr, tol, maxit, X, relres);
+}
+
+}  // extern "C"

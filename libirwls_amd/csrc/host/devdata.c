@@ -161,6 +161,9 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
 }
 
 static void destroy(irwls_devdata *dd) {
+    if (dd->Xh) irwls_dev_free(dd->Xh);
+    if (dd->Xl) irwls_dev_free(dd->Xl);
+    if (dd->nrm32) irwls_dev_free(dd->nrm32);
     irwls_dev_free(dd->X);
     irwls_dev_free(dd->rowptr);
     irwls_dev_free(dd->col);
@@ -204,6 +207,51 @@ irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int
     return r;
 }
 
+static int use_bf16(const irwls_devdata *dd) {
+    const int p = irwls_opts()->precision;
+    return (p == 1 || p == 3) && dd->dense && dd->d > 0;
+}
+
+static irwls_rows_bf16 rows_bf16(irwls_devdata *dd, const int32_t *d_idx, int n) {
+    if (!dd->Xh) {
+        dd->ldb = (dd->d + 31) / 32 * 32;
+        dd->Xh = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
+        dd->Xl = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
+        dd->nrm32 = (float *)irwls_dev_alloc((size_t)dd->rows * sizeof(float));
+        irwls_dev_bf16_split(dd->X, dd->d, dd->rows, dd->d, dd->Xh, dd->Xl, dd->ldb, dd->nrm32);
+    }
+    irwls_rows_bf16 r;
+    r.Xh = dd->Xh;
+    r.Xl = dd->Xl;
+    r.ld = dd->ldb;
+    r.nrm = dd->nrm32;
+    r.idx = d_idx;
+    r.n = n;
+    return r;
+}
+
+void irwls_kmv(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t *d_cidx, int cn, int kernel,
+               double gamma, const double *d_w, double a, double b, double cst, const double *d_rs, double *d_out) {
+    if (use_bf16(dd)) {
+        irwls_rows_bf16 r = rows_bf16(dd, d_ridx, rn), c = rows_bf16(dd, d_cidx, cn);
+        irwls_dev_kernel_matvec_bf16(&r, &c, kernel, gamma, d_w, a, b, cst, d_rs, d_out);
+    } else {
+        irwls_rows r = irwls_devdata_rows(dd, d_ridx, rn), c = irwls_devdata_rows(dd, d_cidx, cn);
+        irwls_dev_kernel_matvec(&r, &c, dd->d, kernel, gamma, d_w, a, b, cst, d_rs, d_out);
+    }
+}
+
+void irwls_kmat(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t *d_cidx, int cn, int kernel,
+                double gamma, double *d_K, int ldk) {
+    if (use_bf16(dd)) {
+        irwls_rows_bf16 r = rows_bf16(dd, d_ridx, rn), c = rows_bf16(dd, d_cidx, cn);
+        irwls_dev_kernel_matrix_bf16(&r, &c, kernel, gamma, d_K, ldk);
+    } else {
+        irwls_rows r = irwls_devdata_rows(dd, d_ridx, rn), c = irwls_devdata_rows(dd, d_cidx, cn);
+        irwls_dev_kernel_matrix(&r, &c, dd->d, kernel, gamma, d_K, ldk);
+    }
+}
+
 void irwls_gpu_pin_dataset(svm_dataset ds, int with_means) {
     irwls_dev_init();
     for (int i = 0; i < MAX_PINNED; i++)
@@ -235,6 +283,7 @@ void irwls_gpu_shutdown(void) {
             g_pinned[i] = NULL;
         }
     }
+    irwls_mixed_release();
     irwls_dev_shutdown();
 }
 

@@ -37,19 +37,30 @@ int *rpermute(int n) {
     return p;
 }
 
-/* Dense solve of the KKT system on the device: H (N x N, leading dimension
-   N+1) is factored with the right-hand side riding in row N, then one
-   backward sweep; the solution is copied to x (host). */
-static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x) {
+/* Dense solve of the KKT system on the device.  FP64: H (N x N, leading
+   dimension N+1) is factored with the right-hand side riding in row N, then
+   one backward sweep.  Mixed precision: the closed form of the same result
+   from two SPD solves (mixed_kkt.c), FP64 when that declines.  The solution
+   is copied to x (host). */
+static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x, const int32_t *S1, const double *ys,
+                      long gen) {
     double t0 = irwls_now();
-    irwls_dev_h2d(ws->rhs, et, (size_t)N * sizeof(double));
-    (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);   /* failure code ignored, as the reference */
-    irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
+    int done = 0;
+    if (irwls_opts()->precision == 1 || irwls_opts()->precision == 2) {
+        done = irwls_kkt_solve_mixed(ws->H, N - 1, S1, gen, ys, et, x, &g_stats);
+        if (!done && N - 1 >= 128) g_stats.fp64_fallbacks++;
+    }
+    if (!done) {
+        irwls_dev_h2d(ws->rhs, et, (size_t)N * sizeof(double));
+        (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);   /* failure code ignored, as the reference */
+        irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
+        g_stats.factorizations++;
+    }
     g_stats.solves++;
     g_stats.t_solve += irwls_now() - t0;
 }
 
-double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l, const double *y,
+double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_widx, int l, const double *y,
                             properties props, const double *GIN, double *e, double *beta) {
     const double C = props.C;
     irwls_ws_reserve(ws, l);
@@ -68,9 +79,10 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l,
     double maxb = 0.0, minb = 0.0;
     int iter = 0, since = 0, nS1 = 0, nS3 = 0;
     double dW = 1e9, nW = 1.0, bestDW = 1e9;
+    const long gen = irwls_mixed_new_generation();
 
     /* the working-set kernel block, evaluated once */
-    if (l > 0) irwls_dev_kernel_matrix(wrows, wrows, d, props.kernelType, props.Kgamma, ws->Kw, l);
+    if (l > 0) irwls_kmat(dd, d_widx, l, d_widx, l, props.kernelType, props.Kgamma, ws->Kw, l);
     irwls_dev_h2d(ws->esub, e, (size_t)l * sizeof(double));
 
     /* initial weights and groups (src/full-train.c:131-155) */
@@ -98,7 +110,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l,
         irwls_dev_h2d(ws->as, hbuf2, (size_t)N * sizeof(double));
         irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
         memset(bAux, 0, (size_t)N * sizeof(double));
-        kkt_solve(ws, N, et, bAux);
+        kkt_solve(ws, N, et, bAux, S1, hbuf, gen);
 
         /* weights, bounds and change (src/full-train.c:205-230) */
         dW = 0.0;
@@ -197,12 +209,14 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l,
 }
 
 double *subIRWLS(svm_dataset dataset, properties props, double *GIN, double *e, double *beta) {
+    const double t_start = irwls_now();
     irwls_dev_init();
+    memset(&g_stats, 0, sizeof(g_stats));
     irwls_devdata *dd = irwls_devdata_acquire(&dataset, 0, -1, 0);
-    irwls_rows rows = irwls_devdata_rows(dd, NULL, dataset.l);
     irwls_ws ws;
     memset(&ws, 0, sizeof(ws));
-    double *b = irwls_sub_irwls_dev(&ws, &rows, dd->d, dataset.l, dataset.y, props, GIN, e, beta);
+    double *b = irwls_sub_irwls_dev(&ws, dd, NULL, dataset.l, dataset.y, props, GIN, e, beta);
+    g_stats.t_total = irwls_now() - t_start;
     irwls_ws_free(&ws);
     irwls_devdata_release(dd);
     return b;
@@ -272,10 +286,8 @@ double *trainFULL(svm_dataset dataset, properties props) {
                 irwls_dev_h2d(d_cols, cols, (size_t)nc * sizeof(int32_t));
                 irwls_dev_h2d(d_w, wv, (size_t)nc * sizeof(double));
                 irwls_dev_h2d(d_ysw, ysub, (size_t)nSW * sizeof(double));
-                irwls_rows r = irwls_devdata_rows(dd, d_SW, nSW);
-                irwls_rows c = irwls_devdata_rows(dd, d_cols, nc);
-                irwls_dev_kernel_matvec(&r, &c, dd->d, props.kernelType, props.Kgamma, d_w, 0.0, 1.0, 0.0, d_ysw,
-                                        d_GIN);
+                irwls_kmv(dd, d_SW, nSW, d_cols, nc, props.kernelType, props.Kgamma, d_w, 0.0, 1.0, 0.0, d_ysw,
+                          d_GIN);
                 irwls_dev_d2h(GIN, d_GIN, (size_t)nSW * sizeof(double));
             }
             GIN[nSW] = gsum;
@@ -287,8 +299,7 @@ double *trainFULL(svm_dataset dataset, properties props) {
             esub[i] = e[SW[i]];
         }
         bsub[nSW] = beta[n];
-        irwls_rows wr = irwls_devdata_rows(dd, d_SW, nSW);
-        double *bt = irwls_sub_irwls_dev(&ws, &wr, dd->d, nSW, ysub, props, GIN, esub, bsub);
+        double *bt = irwls_sub_irwls_dev(&ws, dd, d_SW, nSW, ysub, props, GIN, esub, bsub);
         memcpy(bNew, beta, (size_t)(n + 1) * sizeof(double));
         for (int i = 0; i < nSW; i++) bNew[SW[i]] = bt[i];
         bNew[n] = bt[nSW];
@@ -306,10 +317,8 @@ double *trainFULL(svm_dataset dataset, properties props) {
             }
             irwls_dev_h2d(d_cols, cols, (size_t)nc * sizeof(int32_t));
             irwls_dev_h2d(d_w, wv, (size_t)nc * sizeof(double));
-            irwls_rows r = irwls_devdata_rows(dd, NULL, n);
-            irwls_rows c = irwls_devdata_rows(dd, d_cols, nc);
-            irwls_dev_kernel_matvec(&r, &c, dd->d, props.kernelType, props.Kgamma, d_w, 1.0, -1.0,
-                                    -(bNew[n] - beta[n]), NULL, d_e);
+            irwls_kmv(dd, NULL, n, d_cols, nc, props.kernelType, props.Kgamma, d_w, 1.0, -1.0, -(bNew[n] - beta[n]),
+                      NULL, d_e);
             irwls_dev_d2h(e, d_e, (size_t)n * sizeof(double));
         }
         double dW = 0.0, nW = 0.0;

@@ -36,6 +36,11 @@ typedef struct irwls_devdata {
     double *qv;
     double *y;
     int pinned;                /* kept resident between calls */
+    /* mixed precision (dense only, built on first use): BF16 hi/lo rows,
+       ldb = d rounded up to 32, FP32 norms of the split representation */
+    uint16_t *Xh, *Xl;
+    int ldb;
+    float *nrm32;
 } irwls_devdata;
 
 /* 1 when every feature index of rows [0, rows) is >= 1, else 0. */
@@ -50,6 +55,15 @@ irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int 
 void irwls_devdata_release(irwls_devdata *dd);
 /* Row view of the whole device dataset (idx may select rows). */
 irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int n);
+
+/* Kernel block x vector and materialised kernel block over rows of a device
+   dataset (row selections d_ridx / d_cidx may be NULL), in the current
+   precision: BF16 hi/lo products for dense data in mixed mode, FP64
+   otherwise.  Arguments as irwls_dev_kernel_matvec / _matrix. */
+void irwls_kmv(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t *d_cidx, int cn, int kernel,
+               double gamma, const double *d_w, double a, double b, double cst, const double *d_rs, double *d_out);
+void irwls_kmat(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t *d_cidx, int cn, int kernel,
+                double gamma, double *d_K, int ldk);
 
 /* Current options (environment defaults merged with irwls_gpu_set_options). */
 const irwls_gpu_options *irwls_opts(void);
@@ -71,8 +85,15 @@ void irwls_ws_free(irwls_ws *ws);
    contract as subIRWLS (src/full-train.c:96-362): y = labels of the working
    set (host), GIN has l+1 entries, e and beta (l and l+1 entries) are updated
    in place; returns betaBest (malloc, l+1). */
-double *irwls_sub_irwls_dev(irwls_ws *ws, const irwls_rows *wrows, int d, int l, const double *y,
+double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_widx, int l, const double *y,
                             properties props, const double *GIN, double *e, double *beta);
+
+/* Mixed precision (mixed_kkt.c).  A new generation marks a new kernel
+   block (one per subIRWLS call): factors of an older one are never reused. */
+long irwls_mixed_new_generation(void);
+int irwls_kkt_solve_mixed(const double *d_H, int nS1, const int32_t *S1, long gen, const double *ys,
+                          const double *et, double *x, irwls_gpu_stats *st);
+void irwls_mixed_release(void);
 
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
 void irwls_stats_set(const irwls_gpu_stats *s);

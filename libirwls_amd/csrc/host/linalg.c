@@ -17,6 +17,7 @@ void initMemory(int Threads, int size) {
 /* reference: src/ParallelAlgorithms.c:118-131 */
 void freeMemory(int Threads) {
     (void)Threads;
+    irwls_mixed_release();
     irwls_dev_trim();
 }
 

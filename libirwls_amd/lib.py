@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libirwls.so")
 
 # timer kinds (enum irwls_timer_kind in include/irwls_device.h)
 TIMER_KINDS = ["kmv", "kmat", "potf2", "trsm", "syrk", "potrs", "kkt", "gemv", "sgma", "csr",
-               "syrkd", "matvec"]
+               "syrkd", "matvec", "pcg", "bf16"]
 
 
 class irwls_gpu_options(C.Structure):
@@ -28,12 +28,18 @@ class irwls_gpu_options(C.Structure):
 class irwls_gpu_stats(C.Structure):
     _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("solves", C.c_int),
                 ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double),
-                ("t_solve", C.c_double)]
+                ("t_solve", C.c_double), ("factorizations", C.c_int), ("pcg_iters", C.c_int),
+                ("fp64_fallbacks", C.c_int)]
 
 
 class irwls_rows(C.Structure):
     _fields_ = [("X", C.c_void_p), ("ldx", C.c_int), ("rowptr", C.c_void_p), ("col", C.c_void_p),
                 ("val", C.c_void_p), ("nrm", C.c_void_p), ("idx", C.c_void_p), ("n", C.c_int)]
+
+
+class irwls_rows_bf16(C.Structure):
+    _fields_ = [("Xh", C.c_void_p), ("Xl", C.c_void_p), ("ld", C.c_int), ("nrm", C.c_void_p),
+                ("idx", C.c_void_p), ("n", C.c_int)]
 
 
 _lib = None

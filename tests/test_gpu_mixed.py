@@ -1,0 +1,240 @@
+"""GPU tests of the mixed-precision solver: the FP32 Cholesky with FP64
+diagonal tiles and its L^-T, preconditioned CG in FP64 to a 1e-10 relative
+residual (fresh and stale preconditioner), and subIRWLS / trainFULL in mixed
+mode against the FP64 path.
+
+The mixed mode has no reference counterpart.  Its pins are the residual of
+the FP64 matrix (here against numpy in FP64) and the end-to-end agreement
+with the FP64 path, which is itself pinned to the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import Dev, rbf
+from libirwls_amd import abi, lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _bind():
+    L = lib.load()
+    L.irwls_dev_potrf_inv_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.irwls_dev_pcg_spd2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
+    return L
+
+
+def _kernel_spd(n, seed, dmin=1e-3):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, 6))
+    y = rng.choice([-1.0, 1.0], n)
+    return rbf(X, X, 0.3) * np.outer(y, y) + np.diag(rng.uniform(dmin, 1.0, n)), rng
+
+
+@pytest.mark.parametrize("n", [64, 200, 513])
+def test_potrf_inv_f32(n):
+    A, _ = _kernel_spd(n, 300 + n, dmin=0.05)
+    L = _bind()
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    Fd = Dev(np.zeros(2 * n * n, np.float32))
+    Zd = Dev(np.zeros(n * n, np.float32))
+    info = L.irwls_dev_potrf_inv_f32(Ad.p, n, n, Fd.p, 2 * n, Zd.p, n)
+    assert info == 0
+    F = Fd.get().reshape((2 * n, n), order="F").astype(np.float64)
+    Lf = np.tril(F[:n])
+    Z = F[n:]
+    ZT = Zd.get().reshape((n, n), order="F").astype(np.float64)
+    Lx = np.linalg.cholesky(A)
+    rel = np.linalg.norm(Lf - Lx) / np.linalg.norm(Lx)
+    print("n", n, "factor rel err", rel)
+    assert rel < 1e-5
+    # Z = L^-T (upper triangular), ZT its transpose
+    np.testing.assert_allclose(np.triu(Z), Z, atol=0)
+    zi = np.linalg.norm(Z.T @ Lf - np.eye(n)) / np.sqrt(n)
+    print("n", n, "|Z^T L - I|", zi)
+    assert zi < 1e-4
+    np.testing.assert_array_equal(ZT, Z.T)
+
+
+@pytest.mark.parametrize("n,dmin", [(300, 1e-2), (1000, 1e-4)])
+def test_pcg_two_rhs_to_1e10(n, dmin):
+    A, rng = _kernel_spd(n, 400 + n, dmin=dmin)
+    print("cond(A) = %.3g" % np.linalg.cond(A))
+    B = rng.standard_normal((2, n))
+    L = _bind()
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    Fd = Dev(np.zeros(2 * n * n, np.float32))
+    Zd = Dev(np.zeros(n * n, np.float32))
+    assert L.irwls_dev_potrf_inv_f32(Ad.p, n, n, Fd.p, 2 * n, Zd.p, n) == 0
+    Bd, Xd = Dev(B), Dev(np.zeros((2, n)))
+    rel = (C.c_double * 2)()
+    it = L.irwls_dev_pcg_spd2(Ad.p, n, n, Bd.p, Fd.p, 2 * n, Zd.p, n, 1e-10, 200, Xd.p, rel)
+    X = Xd.get()
+    print("iterations", it, "relres", list(rel))
+    assert it > 0
+    for c in range(2):
+        true_rel = np.linalg.norm(B[c] - A @ X[c]) / np.linalg.norm(B[c])
+        print("column", c, "true relative residual", true_rel)
+        assert true_rel <= 2e-10
+        np.testing.assert_allclose(X[c], np.linalg.solve(A, B[c]), rtol=1e-6, atol=1e-8 * np.abs(X[c]).max())
+
+
+def test_pcg_stale_preconditioner():
+    """A factor of an earlier matrix (other diagonal) still preconditions CG
+    to 1e-10 on the new one."""
+    n = 400
+    A1, rng = _kernel_spd(n, 501, dmin=1e-2)
+    A2 = A1 + np.diag(rng.uniform(0.0, 0.2, n) * np.diag(A1))   # stays SPD
+    L = _bind()
+    A1d = Dev(np.asfortranarray(A1).ravel(order="F"))
+    A2d = Dev(np.asfortranarray(A2).ravel(order="F"))
+    Fd = Dev(np.zeros(2 * n * n, np.float32))
+    Zd = Dev(np.zeros(n * n, np.float32))
+    assert L.irwls_dev_potrf_inv_f32(A1d.p, n, n, Fd.p, 2 * n, Zd.p, n) == 0
+    B = rng.standard_normal((2, n))
+    Bd, Xd = Dev(B), Dev(np.zeros((2, n)))
+    rel = (C.c_double * 2)()
+    it = L.irwls_dev_pcg_spd2(A2d.p, n, n, Bd.p, Fd.p, 2 * n, Zd.p, n, 1e-10, 200, Xd.p, rel)
+    X = Xd.get()
+    print("iterations with the stale factor", it)
+    assert it > 0
+    for c in range(2):
+        assert np.linalg.norm(B[c] - A2 @ X[c]) / np.linalg.norm(B[c]) <= 2e-10
+
+
+def _sub_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    X, y = synth.two_gaussians(n, 8, seed=seed)
+    GIN = rng.standard_normal(n + 1) * 0.1
+    return X, y, GIN
+
+
+def test_sub_irwls_mixed_solver_matches_fp64():
+    """subIRWLS with only the mixed solver (closed form of the failed-pivot
+    solve from two PCG solves, FP64 kernel block) against FP64 subIRWLS on
+    the same problem: weights within 1e-7 relative."""
+    n = 600
+    X, y, GIN = _sub_problem(n, 41)
+    props = abi.make_properties(gamma=0.2, C_=5.0, W=n)
+    gds = lib.Dataset.dense(X, y)
+    beta0 = np.zeros(n + 1)
+    lib.set_options(precision=0)
+    b64, e64, _ = lib.sub_irwls(gds, props, GIN, y.copy(), beta0)
+    s64 = lib.last_stats()
+    try:
+        lib.set_options(precision=2)
+        bmx, emx, _ = lib.sub_irwls(gds, props, GIN, y.copy(), beta0)
+        smx = lib.last_stats()
+    finally:
+        lib.set_options(precision=0)
+    print("fp64", s64)
+    print("mixed", smx)
+    assert smx["fp64_fallbacks"] == 0 and smx["pcg_iters"] > 0
+    rel = np.linalg.norm(bmx - b64) / np.linalg.norm(b64)
+    print("relative weight difference", rel)
+    assert rel < 1e-7
+    np.testing.assert_allclose(emx, e64, rtol=1e-6, atol=1e-8)
+
+
+def test_train_full_mixed_accuracy():
+    """Mixed trainFULL against FP64 trainFULL: test accuracy within 0.1%
+    (absolute), weights and decision values close."""
+    n, d = 4000, 32
+    X, y = synth.four_gaussians(n, d, seed=7, means_seed=7)
+    Xt, yt = synth.four_gaussians(2000, d, seed=107, means_seed=7)
+    props = abi.make_properties(gamma=1.0 / d, C_=1.0, W=1000, eta=0.001)
+    ds = lib.Dataset.dense(X, y)
+    td = lib.Dataset.dense(Xt, yt)
+    pp = abi.make_predict_properties(labels=0)
+    lib.set_options(precision=0)
+    b64 = lib.train_full(ds, props, seed=0)
+    m64 = lib.full_model(props, ds, b64)
+    f64 = lib.soft_test(td, m64, pp)
+    try:
+        lib.set_options(precision=1)
+        bmx = lib.train_full(ds, props, seed=0)
+        st = lib.last_stats()
+        mmx = lib.full_model(props, ds, bmx)
+        fmx = lib.soft_test(td, mmx, pp)
+    finally:
+        lib.set_options(precision=0)
+    acc64 = np.mean(np.where(f64 >= 0, 1.0, -1.0) == yt)
+    accmx = np.mean(np.where(fmx >= 0, 1.0, -1.0) == yt)
+    wrel = np.linalg.norm(bmx - b64) / np.linalg.norm(b64)
+    print("mixed stats", st)
+    print("accuracy fp64 %.4f mixed %.4f; weight rel err %.3g; decision max err %.3g rms %.3g"
+          % (acc64, accmx, wrel, np.abs(fmx - f64).max(), np.sqrt(np.mean((fmx - f64) ** 2))))
+    assert abs(accmx - acc64) <= 0.001
+    assert wrel < 5e-2
+
+
+def _bf16_rows(Xd, n, d):
+    """Device BF16 hi/lo copy of n dense FP64 rows (width d), padded to 32."""
+    L = lib.load()
+    ldb = (d + 31) // 32 * 32
+    Xh = Dev(np.zeros(n * ldb, np.uint16))
+    Xl = Dev(np.zeros(n * ldb, np.uint16))
+    nrm = Dev(np.zeros(n, np.float32))
+    L.irwls_dev_bf16_split.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_void_p]
+    L.irwls_dev_bf16_split(Xd.p, d, n, d, Xh.p, Xl.p, ldb, nrm.p)
+    r = lib.irwls_rows_bf16()
+    r.Xh, r.Xl, r.ld, r.nrm, r.idx, r.n = Xh.p, Xl.p, ldb, nrm.p, None, n
+    return r, (Xh, Xl, nrm)
+
+
+@pytest.mark.parametrize("n,m,d", [(300, 200, 256), (257, 129, 100), (64, 1000, 16)])
+def test_bf16_kernel_matrix_error_bound(n, m, d):
+    """RBF block from BF16 hi/lo x3 products (FP32 accumulation) against the
+    FP64 kernel: |x|^2 + |z|^2 - 2 x.z is off by at most about
+    2^-15 (|x|^2 + |z|^2) (split error 2^-16 per factor, FP32 sums), so
+    |k - k64| <= gamma * 2^-15 * (|x|^2 + |z|^2) (k <= 1)."""
+    rng = np.random.default_rng(600 + d)
+    X = rng.standard_normal((n, d)) + 0.5
+    Z = rng.standard_normal((m, d)) - 0.25
+    g = 1.0 / d
+    L = lib.load()
+    L.irwls_dev_kernel_matrix_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int]
+    Xd, Zd = Dev(X), Dev(Z)
+    r, keep_r = _bf16_rows(Xd, n, d)
+    c, keep_c = _bf16_rows(Zd, m, d)
+    Kd = Dev(np.zeros(n * m))
+    L.irwls_dev_kernel_matrix_bf16(C.byref(r), C.byref(c), 1, g, Kd.p, m)
+    K = Kd.get().reshape(n, m)
+    K64 = rbf(X, Z, g)
+    nx = (X ** 2).sum(1)[:, None] + (Z ** 2).sum(1)[None, :]
+    bound = g * 2.0 ** -15 * nx
+    err = np.abs(K - K64)
+    print("n m d", n, m, d, "max err %.3g, max err/bound %.3f" % (err.max(), (err / bound).max()))
+    assert np.all(err <= bound)
+    # the same rows against themselves: k(x, x) = 1 up to the accumulation error
+    Kd2 = Dev(np.zeros(n * n))
+    L.irwls_dev_kernel_matrix_bf16(C.byref(r), C.byref(r), 1, g, Kd2.p, n)
+    diag = np.diag(Kd2.get().reshape(n, n))
+    assert np.all(np.abs(diag - 1.0) <= 1e-5)
+
+
+@pytest.mark.parametrize("n,m,d", [(5000, 700, 128), (300, 3000, 64)])
+def test_bf16_kernel_matvec(n, m, d):
+    """out = a out + rs (b K w) + cst with the BF16 block, against FP64
+    numpy; also covers the split-column path (many column tiles)."""
+    rng = np.random.default_rng(700 + n)
+    X = rng.standard_normal((n, d))
+    Z = rng.standard_normal((m, d))
+    w = rng.standard_normal(m)
+    rs = rng.choice([-1.0, 1.0], n)
+    out0 = rng.standard_normal(n)
+    g = 0.5 / d
+    L = lib.load()
+    L.irwls_dev_kernel_matvec_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_double,
+                                               C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+    Xd, Zd = Dev(X), Dev(Z)
+    r, keep_r = _bf16_rows(Xd, n, d)
+    c, keep_c = _bf16_rows(Zd, m, d)
+    wd, rsd, outd = Dev(w), Dev(rs), Dev(out0)
+    L.irwls_dev_kernel_matvec_bf16(C.byref(r), C.byref(c), 1, g, wd.p, 0.5, -1.0, 0.25, rsd.p, outd.p)
+    want = 0.5 * out0 + rs * (-(rbf(X, Z, g) @ w)) + 0.25
+    err = np.abs(outd.get() - want).max()
+    print("matvec max abs err", err, "scale", np.abs(w).sum())
+    assert err <= 1e-4 * np.abs(w).sum()
