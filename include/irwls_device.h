@@ -51,6 +51,7 @@ enum irwls_timer_kind {
     IRWLS_T_MATVEC,       /* IRWLSpar e = y - Phi beta */
     IRWLS_T_PCG,          /* mixed-precision refinement (preconditioned CG) */
     IRWLS_T_BF16,         /* BF16 hi/lo kernel tiles (mixed precision) */
+    IRWLS_T_MLP,          /* proposal network training / inference */
     IRWLS_T_COUNT
 };
 void irwls_dev_timing(int enable);          /* enable/disable and reset */
@@ -121,6 +122,27 @@ void irwls_dev_potrs(const double *d_L, int n, int lda, double *d_b);
    irwls_dev_potrf (the solve uses the factor as it stands).  Returns info. */
 int  irwls_dev_posv_aug(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
 
+/* Learned SGMA proposal: a 32 -> 128 -> 128 -> 1 MLP (ReLU).  Parameters
+   are IRWLS_MLP_NPARAMS floats: W1[32][128], b1[128], W2[128][128], b2[128],
+   w3[128], b3.  Loss: mean squared error over a batch. */
+#define IRWLS_MLP_NPARAMS 20865
+int  irwls_mlp_nparams(void);
+int  irwls_mlp_npacked(void);
+/* Gradient (grad, NPARAMS floats) and loss (1 float) of nb samples X [nb][32], targets T [nb]. */
+void irwls_dev_mlp_grad(const float *d_P, const float *d_X, const float *d_T, int nb, float *d_grad, float *d_loss);
+/* Adam step `step` (from 1), beta1 0.9, beta2 0.999, eps 1e-8. */
+void irwls_dev_mlp_adam(float *d_P, const float *d_grad, float *d_m, float *d_v, int step, float lr);
+void irwls_dev_mlp_forward_f32(const float *d_P, const float *d_X, int nb, float *d_Y);
+/* BF16 inference: pack the weight matrices once (irwls_mlp_npacked() uint16), then score. */
+void irwls_dev_mlp_pack_bf16(const float *d_P, uint16_t *d_Q);
+void irwls_dev_mlp_infer_bf16(const uint16_t *d_Q, const float *d_P, const float *d_X, int nb, float *d_Y);
+/* 32 features per candidate of a pool of P <= 1024 into F [P][32]. */
+void irwls_dev_sgma_features(int P, int s, int B, const double *d_KNC, int ldk, const double *d_eta,
+                             const double *d_gsub, const double *d_gsub2, const double *d_ycand,
+                             const double *d_qvcand, float pos_frac, float *d_F);
+/* dst[j] = src[idx[j]], j < m */
+void irwls_dev_gather_idx(const double *d_src, const int32_t *d_idx, int m, double *d_dst);
+
 /* Mixed precision: dense rows as BF16 hi/lo pairs (x = hi + lo to about 16
    bits), row-major with ld a multiple of 32 (zero padded), and FP32 squared
    norms sum(hi^2 + 2 hi lo). */
@@ -172,6 +194,10 @@ void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, c
 void irwls_dev_sgma_gain(int n, int s, int ncand, const double *d_KSC, const double *d_Z, int ldz,
                          const double *d_KSM, int ldm, double *d_q);
 /* eta[c] = 1 - sum_{e<s} KNC[c*ldk + e] Z[e + c*ldz] */
+/* As irwls_dev_sgma_gain, plus qhalf = the same sums over the first half of
+   the 64-row tiles. */
+void irwls_dev_sgma_gain_half(int n, int s, int ncand, const double *d_KSC, const double *d_Z, int ldz,
+                              const double *d_KSM, int ldm, double *d_q, double *d_qhalf);
 void irwls_dev_sgma_eta(int s, int ncand, const double *d_KNC, int ldk, const double *d_Z, int ldz, double *d_eta);
 /* dst[i] = src[i*ld + col] */
 void irwls_dev_gather_col(const double *d_src, int ld, int col, int n, double *d_dst);

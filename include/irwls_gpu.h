@@ -20,8 +20,10 @@ typedef struct irwls_gpu_options {
                              2 = only the mixed solver, 3 = only the BF16x3 kernel GEMM (ablations) */
     int sgma_candidates;  /* SGMA candidates per step (default 64, as the reference) */
     int proposal;         /* 0 = plain SGMA draws (default), 1 = learned shortlist */
-    int proposal_keep;    /* learned proposal: candidates kept for exact scoring */
+    int proposal_keep;    /* learned proposal: candidates kept for exact scoring (0: the network's choice) */
     double refine_tol;    /* mixed precision: relative residual target (default 1e-10) */
+    int proposal_pool;    /* learned proposal: candidates drawn per step (0: 4 x sgma_candidates, at most 1024) */
+    int proposal_warmup;  /* learned proposal: plain SGMA steps that train the network (0: B/10 in [20, 100]) */
 } irwls_gpu_options;
 
 /* Defaults, overridable by IRWLS_PRECISION=fp64|mixed, IRWLS_SGMA_CAND=<n>,
@@ -53,6 +55,9 @@ typedef struct irwls_gpu_stats {
     int factorizations;    /* Cholesky factorizations (mixed precision: FP32 factors) */
     int pcg_iters;         /* mixed precision: CG iterations over all solves */
     int fp64_fallbacks;    /* mixed precision: solves done by the FP64 path instead */
+    int learned_steps;     /* SGMA steps chosen through the learned proposal */
+    double mlp_loss0;      /* proposal network: loss before / after its fit */
+    double mlp_loss;
 } irwls_gpu_stats;
 void irwls_gpu_last_stats(irwls_gpu_stats *out);
 

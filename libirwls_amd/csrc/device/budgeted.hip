@@ -392,6 +392,19 @@ void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const doubl
     IRWLS_LAUNCH_CHECK();
 }
 
+void irwls_dev_sgma_gain_half(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
+                              int ldm, double *qout, double *qhalf) {
+    if (n <= 0 || ncand <= 0) return;
+    irwls_dev_sgma_gain(n, s, ncand, KSC, Z, ldz, KSM, ldm, qout);
+    // the partials of the call above: one per 64-row tile; the first half of
+    // the tiles gives the sum over rows [0, 64 * (rt / 2))
+    const int rt = (n + T - 1) / T;
+    const double *part = (const double *)scratch(10, (size_t)rt * ncand * sizeof(double));
+    const int h = rt / 2 > 0 ? rt / 2 : 1;
+    hipLaunchKernelGGL(col_sum_kernel, dim3((ncand + 255) / 256), dim3(256), 0, stream(), part, h, ncand, qhalf);
+    IRWLS_LAUNCH_CHECK();
+}
+
 void irwls_dev_sgma_eta(int s, int ncand, const double *KNC, int ldk, const double *Z, int ldz, double *eta) {
     if (ncand <= 0) return;
     hipLaunchKernelGGL(sgma_eta_kernel, dim3((ncand + 3) / 4), dim3(256), 0, stream(), s, ncand, KNC, ldk, Z, ldz, eta);

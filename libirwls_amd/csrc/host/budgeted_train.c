@@ -49,92 +49,191 @@ static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b
     return v;
 }
 
+/* Exact scoring of nc candidates d_c at step s: KSM(:, c) = k(x_i, cand_c)
+   (n x nc row-major), KNC(c, e) = k(cand_c, centroid_e) (ld B),
+   Z = L^-T L^-1 KNC^T (ld B), eta_c = 1 - KNC_c . Z_c and
+   q_c = |KSM(:, c) - KSC Z_c|^2 (src/budgeted-train.c:176-203). */
+static void score_exact(irwls_devdata *dd, properties props, int n, int B, int s, int nc, const int32_t *d_c,
+                        const int32_t *d_cent, const double *d_Li, const double *d_KSC, double *d_KSM, double *d_KNC,
+                        double *d_W, double *d_Z, double *d_eta, double *d_q) {
+    irwls_rows all = irwls_devdata_rows(dd, NULL, n);
+    irwls_rows cr = irwls_devdata_rows(dd, d_c, nc);
+    irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
+    irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, nc);
+    irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
+    irwls_dev_gemm(0, 0, s, nc, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
+    irwls_dev_gemm(1, 0, s, nc, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
+    irwls_dev_sgma_eta(s, nc, d_KNC, B, d_Z, B, d_eta);
+    irwls_dev_sgma_gain(n, s, nc, d_KSC, d_Z, B, d_KSM, nc, d_q);
+}
+
+/* argmax of q/eta with the first maximum winning (src/budgeted-train.c:208-216) */
+static int best_gain(int nc, const double *q, const double *eta, double *value) {
+    double bestv = 0.0;
+    int best = 0;
+    for (int i = 0; i < nc; i++) {
+        const double g = (eta[i] > 0.0) ? (1.0 / eta[i]) * q[i] : 0.0;
+        if (i == 0 || g > bestv) {
+            bestv = g;
+            best = i;
+        }
+    }
+    *value = bestv;
+    return best;
+}
+
 int *SGMA(svm_dataset dataset, properties props) {
     const double t_start = irwls_now();
     memset(&g_bstats, 0, sizeof(g_bstats));
     irwls_dev_init();
     const int n = dataset.l, B = props.size;
-    const int ncand = irwls_opts()->sgma_candidates;
+    const irwls_gpu_options *opt = irwls_opts();
+    const int ncand = opt->sgma_candidates;
+    /* learned proposal: pool size, shortlist and warm-up */
+    const int learned = (opt->proposal == 1) && B > 2 && n > 0;
+    int pool = opt->proposal_pool > 0 ? opt->proposal_pool : 4 * ncand;
+    if (pool > 1024) pool = 1024;
+    if (pool < ncand) pool = ncand;
+    int keep = opt->proposal_keep;
+    if (keep > pool) keep = pool;
+    int warm = opt->proposal_warmup;
+    if (warm <= 0) {
+        warm = B / 10;
+        if (warm < 20) warm = 20;
+        if (warm > 100) warm = 100;
+    }
+    if (warm > B - 2) warm = B - 2;
+    const int cmax = learned ? pool : ncand;
     irwls_devdata *dd = irwls_devdata_acquire(&dataset, 1, -1, 0);
     int *cent = (int *)irwls_xmalloc((size_t)(B > 0 ? B : 1) * sizeof(int));
-    int32_t *cidx = (int32_t *)irwls_xmalloc((size_t)ncand * sizeof(int32_t));
-    double *eta = (double *)irwls_xmalloc((size_t)ncand * sizeof(double));
-    double *q = (double *)irwls_xmalloc((size_t)ncand * sizeof(double));
+    int32_t *cidx = (int32_t *)irwls_xmalloc((size_t)cmax * sizeof(int32_t));
+    int32_t *cidx2 = (int32_t *)irwls_xmalloc((size_t)cmax * sizeof(int32_t));
+    double *eta = (double *)irwls_xmalloc((size_t)cmax * sizeof(double));
+    double *q = (double *)irwls_xmalloc((size_t)cmax * sizeof(double));
+    float *score = (float *)irwls_xmalloc((size_t)cmax * sizeof(float));
+    char *taken = (char *)irwls_xmalloc((size_t)cmax);
     double *L2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     double *IL2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     double *row = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     const size_t Bz = (size_t)(B > 0 ? B : 1);
+    const int kmax = ncand > keep ? ncand : keep;   /* exact-scored candidates per step */
     double *d_KSC = (double *)irwls_dev_alloc((size_t)n * Bz * sizeof(double));   /* n x B, column-major */
     double *d_L = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));            /* B x B, column-major */
     double *d_Li = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));
-    double *d_KSM = (double *)irwls_dev_alloc((size_t)n * ncand * sizeof(double)); /* n x ncand, row-major */
-    double *d_KNC = (double *)irwls_dev_alloc((size_t)ncand * Bz * sizeof(double)); /* ncand x B, row-major */
-    double *d_W = (double *)irwls_dev_alloc(Bz * ncand * sizeof(double));          /* B x ncand, column-major */
-    double *d_Z = (double *)irwls_dev_alloc(Bz * ncand * sizeof(double));
-    double *d_eta = (double *)irwls_dev_alloc((size_t)ncand * sizeof(double));
-    double *d_q = (double *)irwls_dev_alloc((size_t)ncand * sizeof(double));
+    double *d_KSM = (double *)irwls_dev_alloc((size_t)n * (kmax > 0 ? kmax : 1) * sizeof(double)); /* row-major */
+    double *d_KNC = (double *)irwls_dev_alloc((size_t)cmax * Bz * sizeof(double)); /* cmax x B, row-major */
+    double *d_W = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));          /* B x cmax, column-major */
+    double *d_Z = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));
+    double *d_eta = (double *)irwls_dev_alloc((size_t)cmax * sizeof(double));
+    double *d_q = (double *)irwls_dev_alloc((size_t)cmax * sizeof(double));
     double *d_v1 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
     double *d_v2 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
     double *d_tmp = (double *)irwls_dev_alloc(64 * sizeof(double));
-    int32_t *d_cidx = (int32_t *)irwls_dev_alloc((size_t)ncand * sizeof(int32_t));
+    int32_t *d_cidx = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
     int32_t *d_cent = (int32_t *)irwls_dev_alloc(Bz * sizeof(int32_t));
     int32_t *d_pair = (int32_t *)irwls_dev_alloc(2 * sizeof(int32_t));
     irwls_rows all = irwls_devdata_rows(dd, NULL, n);
+    irwls_proposal *lp = learned ? irwls_proposal_begin(n, B, cmax, warm * ncand) : NULL;
     const double *knc = NULL;       /* device row of k(new centroid, previous centroids) */
-    int best = 0;
+    int npos = 0;
 
     for (int s = 0; s < B; s++) {
         double value;
         if (s > 1) {
             g_bstats.sgma_steps++;
-            /* candidate draws in the reference's serial order */
-            for (int i = 0; i < ncand; i++) {
-                int ix = rand() % n;
-                while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
-                cidx[i] = ix;
-            }
-            irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
-            irwls_rows cr = irwls_devdata_rows(dd, d_cidx, ncand);
-            irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
-            /* KSM(:, c) = k(x_i, cand_c); KNC(c, e) = k(cand_c, centroid_e) */
-            irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, ncand);
-            irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
-            /* Z = L^-T L^-1 KNC^T */
-            irwls_dev_gemm(0, 0, s, ncand, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
-            irwls_dev_gemm(1, 0, s, ncand, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
-            irwls_dev_sgma_eta(s, ncand, d_KNC, B, d_Z, B, d_eta);
-            irwls_dev_sgma_gain(n, s, ncand, d_KSC, d_Z, B, d_KSM, ncand, d_q);
-            irwls_dev_d2h(eta, d_eta, (size_t)ncand * sizeof(double));
-            irwls_dev_d2h(q, d_q, (size_t)ncand * sizeof(double));
-            double bestv = 0.0;
-            for (int i = 0; i < ncand; i++) {
-                const double g = (eta[i] > 0.0) ? (1.0 / eta[i]) * q[i] : 0.0;
-                if (i == 0 || g > bestv) {
-                    bestv = g;
-                    best = i;
+            if (!irwls_proposal_trained(lp)) {
+                /* candidate draws in the reference's serial order */
+                for (int i = 0; i < ncand; i++) {
+                    int ix = rand() % n;
+                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
+                    cidx[i] = ix;
                 }
+                irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
+                score_exact(dd, props, n, B, s, ncand, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
+                            d_q);
+                irwls_dev_d2h(eta, d_eta, (size_t)ncand * sizeof(double));
+                irwls_dev_d2h(q, d_q, (size_t)ncand * sizeof(double));
+                const int best = best_gain(ncand, q, eta, &value);
+                cent[s] = cidx[best];
+                /* the chosen candidate's column is the new KSC column */
+                irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
+                knc = d_KNC + (size_t)best * B;
+                if (lp) {
+                    /* warm-up: log the candidates' features with their exact gains */
+                    irwls_proposal_features(lp, dd, &dataset, props, s, ncand, d_cidx, cidx, d_KNC, d_Z, d_eta,
+                                            (float)npos / (float)s);
+                    irwls_proposal_log(lp, ncand, q, eta);
+                    if (s - 1 == warm) (void)irwls_proposal_train(lp, 300, 3e-3f, &g_bstats);
+                }
+            } else {
+                /* learned proposal: score a larger pool, then exact gains of the shortlist */
+                for (int i = 0; i < pool; i++) {
+                    int ix = rand() % n;
+                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
+                    cidx[i] = ix;
+                }
+                irwls_dev_h2d(d_cidx, cidx, (size_t)pool * sizeof(int32_t));
+                irwls_rows cr = irwls_devdata_rows(dd, d_cidx, pool);
+                irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
+                irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
+                irwls_dev_gemm(0, 0, s, pool, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
+                irwls_dev_gemm(1, 0, s, pool, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
+                irwls_dev_sgma_eta(s, pool, d_KNC, B, d_Z, B, d_eta);
+                irwls_proposal_features(lp, dd, &dataset, props, s, pool, d_cidx, cidx, d_KNC, d_Z, d_eta,
+                                        (float)npos / (float)s);
+                irwls_proposal_score(lp, pool, score);
+                /* shortlist: the highest scores, ties to the earlier draw */
+                const int k = keep > 0 ? keep : 1;
+                memset(taken, 0, (size_t)pool);
+                for (int r = 0; r < k; r++) {
+                    int bi = -1;
+                    for (int i = 0; i < pool; i++)
+                        if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i;
+                    taken[bi] = 1;
+                    cidx2[r] = cidx[bi];
+                    if (r == 0 && keep == 0) {
+                        /* the network's choice without exact scoring */
+                        cent[s] = cidx[bi];
+                        knc = d_KNC + (size_t)bi * B;
+                        value = score[bi];
+                    }
+                }
+                if (keep > 0) {
+                    irwls_dev_h2d(d_cidx, cidx2, (size_t)keep * sizeof(int32_t));
+                    score_exact(dd, props, n, B, s, keep, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
+                                d_q);
+                    irwls_dev_d2h(eta, d_eta, (size_t)keep * sizeof(double));
+                    irwls_dev_d2h(q, d_q, (size_t)keep * sizeof(double));
+                    const int best = best_gain(keep, q, eta, &value);
+                    cent[s] = cidx2[best];
+                    irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
+                    knc = d_KNC + (size_t)best * B;
+                } else {
+                    const int32_t c1 = cent[s];
+                    irwls_dev_h2d(d_pair, &c1, sizeof(int32_t));
+                    irwls_rows cc = irwls_devdata_rows(dd, d_pair, 1);
+                    irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n,
+                                            1);
+                }
+                g_bstats.learned_steps++;
             }
-            value = bestv;
-            cent[s] = cidx[best];
-            /* the chosen candidate's column is the new KSC column */
-            irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
-            knc = d_KNC + (size_t)best * B;
         } else {
             cent[s] = (s == 0) ? n : n + 1;
             value = 1.0;
-            best = 0;
         }
         if (props.verbose == 1) {
             if (s == 0) printf("Best Error Descent %f, Average of positive data is centroid %d\n", value, s);
             if (s == 1) printf("Best Error Descent %f, Average of negative data is centroid %d\n", value, s);
             if (s > 1) printf("Best Error Descent %f, Data with index %d is centroid %d\n", value, cent[s], s);
         }
+        if (cent[s] == n || (cent[s] < n && dataset.y[cent[s]] > 0)) npos++;
         int32_t cs = cent[s];
         irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
         if (s <= 1) {
             irwls_rows cc = irwls_devdata_rows(dd, d_cent + s, 1);
             irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n, 1);
         }
+        if (lp) irwls_proposal_add_centroid(lp, s, d_KSC + (size_t)s * n);
         /* grow L and L^-1 of K_C + jitter by one row (src/budgeted-train.c:247-279) */
         const double kcc = kernel_pair(dd, d_pair, cent[s], cent[s], props, d_tmp);
         if (s == 0) {
@@ -182,7 +281,11 @@ int *SGMA(svm_dataset dataset, properties props) {
     irwls_dev_free(d_cent);
     irwls_dev_free(d_pair);
     irwls_devdata_release(dd);
+    irwls_proposal_end(lp);
     free(cidx);
+    free(cidx2);
+    free(score);
+    free(taken);
     free(eta);
     free(q);
     free(L2);

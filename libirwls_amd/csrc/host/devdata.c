@@ -43,6 +43,8 @@ static void opts_defaults(void) {
     g_opts.proposal = 0;
     g_opts.proposal_keep = 8;
     g_opts.refine_tol = 1e-10;
+    g_opts.proposal_pool = 0;
+    g_opts.proposal_warmup = 0;
     const char *p = getenv("IRWLS_PRECISION");
     if (p && strcasecmp(p, "mixed") == 0) g_opts.precision = 1;
     const char *c = getenv("IRWLS_SGMA_CAND");
@@ -67,7 +69,10 @@ void irwls_gpu_set_options(const irwls_gpu_options *o) {
     g_opts = *o;
     if (g_opts.sgma_candidates < 1) g_opts.sgma_candidates = 1;
     if (g_opts.sgma_candidates > 1024) g_opts.sgma_candidates = 1024;
-    if (g_opts.proposal_keep < 1) g_opts.proposal_keep = 1;
+    if (g_opts.proposal_keep < 0) g_opts.proposal_keep = 0;
+    if (g_opts.proposal_pool < 0) g_opts.proposal_pool = 0;
+    if (g_opts.proposal_pool > 1024) g_opts.proposal_pool = 1024;
+    if (g_opts.proposal_warmup < 0) g_opts.proposal_warmup = 0;
 }
 
 void irwls_gpu_timing(int enable) { irwls_dev_timing(enable); }

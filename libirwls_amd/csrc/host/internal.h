@@ -95,6 +95,19 @@ int irwls_kkt_solve_mixed(const double *d_H, int nS1, const int32_t *S1, long ge
                           const double *et, double *x, irwls_gpu_stats *st);
 void irwls_mixed_release(void);
 
+/* Learned SGMA proposal (proposal.c). */
+typedef struct irwls_proposal irwls_proposal;
+irwls_proposal *irwls_proposal_begin(int n, int B, int pmax, int log_cap);
+void irwls_proposal_end(irwls_proposal *lp);
+int irwls_proposal_trained(const irwls_proposal *lp);
+void irwls_proposal_add_centroid(irwls_proposal *lp, int s, const double *d_col);
+void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_dataset *ds, properties props,
+                             int s, int P, const int32_t *d_c, const int32_t *cidx, const double *d_KNC,
+                             const double *d_Z, const double *d_eta, float pos_frac);
+void irwls_proposal_log(irwls_proposal *lp, int P, const double *q, const double *eta);
+int irwls_proposal_train(irwls_proposal *lp, int steps, float lr, irwls_gpu_stats *st);
+void irwls_proposal_score(irwls_proposal *lp, int P, float *score);
+
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
 void irwls_stats_set(const irwls_gpu_stats *s);
 

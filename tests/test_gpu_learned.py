@@ -1,0 +1,147 @@
+"""GPU tests of the learned SGMA proposal: the hand-written training kernels
+against the CPU oracle of the network (gradient, Adam step, forward), the
+BF16 inference kernel against the FP32 network, and an ablation of SGMA with
+and without the trained network.
+
+The learned proposal has no reference counterpart; plain SGMA (pinned to the
+oracle's index sequence in test_gpu_budgeted.py) is what it is judged by:
+the budgeted objective (the Nystrom residual of the chosen basis on held-out
+rows) and the test accuracy after IRWLSpar."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import Dev, rbf
+from libirwls_amd import abi, lib, synth
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _bind():
+    L = lib.load()
+    fp = C.c_void_p
+    L.irwls_dev_mlp_grad.argtypes = [fp, fp, fp, C.c_int, fp, fp]
+    L.irwls_dev_mlp_adam.argtypes = [fp, fp, fp, fp, C.c_int, C.c_float]
+    L.irwls_dev_mlp_forward_f32.argtypes = [fp, fp, C.c_int, fp]
+    L.irwls_dev_mlp_pack_bf16.argtypes = [fp, fp]
+    L.irwls_dev_mlp_infer_bf16.argtypes = [fp, fp, fp, C.c_int, fp]
+    return L
+
+
+def _net(seed, nb):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(M.NPARAMS)
+    for lo, hi, fan in [(M.OW1, M.OB1, 32), (M.OW2, M.OB2, 128), (M.OW3, M.OB3, 128)]:
+        p[lo:hi] = rng.uniform(-1, 1, hi - lo) * np.sqrt(6.0 / fan)
+    p[M.OB1:M.OW2] = rng.standard_normal(M.NH) * 0.1
+    p[M.OB2:M.OW3] = rng.standard_normal(M.NH) * 0.1
+    X = rng.standard_normal((nb, M.NIN))
+    T = rng.standard_normal(nb)
+    return p.astype(np.float32), X.astype(np.float32), T.astype(np.float32)
+
+
+@pytest.mark.parametrize("nb", [1, 64, 200, 1000])
+def test_mlp_gradient_against_oracle(nb):
+    p, X, T = _net(10 + nb, nb)
+    L = _bind()
+    assert L.irwls_mlp_nparams() == M.NPARAMS
+    Pd, Xd, Td = Dev(p), Dev(X), Dev(T)
+    Gd, lossd = Dev(np.zeros(M.NPARAMS, np.float32)), Dev(np.zeros(1, np.float32))
+    L.irwls_dev_mlp_grad(Pd.p, Xd.p, Td.p, nb, Gd.p, lossd.p)
+    g = Gd.get().astype(np.float64)
+    want = M.grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    rel = np.linalg.norm(g - want) / np.linalg.norm(want)
+    print("nb", nb, "gradient relative error", rel)
+    assert rel < 1e-5
+    for lo, hi in [(M.OW1, M.OB1), (M.OB1, M.OW2), (M.OW2, M.OB2), (M.OB2, M.OW3), (M.OW3, M.OB3)]:
+        np.testing.assert_allclose(g[lo:hi], want[lo:hi], rtol=1e-3, atol=1e-5 * np.abs(want).max())
+    assert abs(lossd.get()[0] - M.loss(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))) < 1e-4
+
+
+def test_mlp_adam_step_against_oracle():
+    p, X, T = _net(30, 300)
+    L = _bind()
+    g = M.grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64)).astype(np.float32)
+    m = (np.random.default_rng(1).standard_normal(M.NPARAMS) * 1e-3).astype(np.float32)
+    v = (np.random.default_rng(2).random(M.NPARAMS) * 1e-4).astype(np.float32)
+    Pd, Gd, Md, Vd = Dev(p), Dev(g), Dev(m), Dev(v)
+    L.irwls_dev_mlp_adam(Pd.p, Gd.p, Md.p, Vd.p, 3, 2e-3)
+    pw, mw, vw = M.adam(p.astype(np.float64), g.astype(np.float64), m.astype(np.float64), v.astype(np.float64), 3,
+                        2e-3)
+    np.testing.assert_allclose(Pd.get(), pw, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(Md.get(), mw, rtol=1e-5, atol=1e-9)
+    # the kernel forms 1 - beta2 in FP32 (1.0f - 0.999f carries a relative error of ~5e-5)
+    np.testing.assert_allclose(Vd.get(), vw, rtol=1e-4, atol=1e-12)
+
+
+def test_mlp_forward_and_bf16_inference():
+    nb = 300
+    p, X, _ = _net(40, nb)
+    L = _bind()
+    Pd, Xd = Dev(p), Dev(X)
+    Yd = Dev(np.zeros(nb, np.float32))
+    L.irwls_dev_mlp_forward_f32(Pd.p, Xd.p, nb, Yd.p)
+    want, _ = M.forward(p.astype(np.float64), X.astype(np.float64))
+    np.testing.assert_allclose(Yd.get(), want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+    Qd = Dev(np.zeros(L.irwls_mlp_npacked(), np.uint16))
+    L.irwls_dev_mlp_pack_bf16(Pd.p, Qd.p)
+    Bd = Dev(np.zeros(nb, np.float32))
+    L.irwls_dev_mlp_infer_bf16(Qd.p, Pd.p, Xd.p, nb, Bd.p)
+    yb = Bd.get()
+    err = np.abs(yb - want).max() / np.abs(want).max()
+    corr = np.corrcoef(yb, want)[0, 1]
+    print("bf16 inference: max error / max |y| = %.3g, correlation %.6f" % (err, corr))
+    assert err < 0.05 and corr > 0.999
+
+
+def _nystrom_residual(Xaug, cent, Xt, g):
+    """Mean of k(x,x) - k_C(x)^T K_C^-1 k_C(x) over held-out rows (RBF, k(x,x) = 1)."""
+    V = Xaug[np.asarray(cent)]
+    KC = rbf(V, V, g) + 1e-8 * np.eye(len(cent))
+    Kt = rbf(Xt, V, g)
+    sol = np.linalg.solve(KC, Kt.T)
+    return float(np.mean(1.0 - np.einsum("ij,ji->i", Kt, sol)))
+
+
+def _sgma_run(ds, props, seed, **opts):
+    lib.set_options(**opts)
+    try:
+        cent = lib.sgma(ds, props, seed=seed)
+        st = lib.last_stats()
+        beta = lib.irwls_par(ds, cent, props)
+    finally:
+        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_warmup=0)
+    return cent, beta, st
+
+
+def test_learned_proposal_ablation():
+    """SGMA with the trained network (pool of 236, exact gains for its 8
+    best) against SGMA with the same exact-scoring budget and no network
+    (8 random candidates), and against the reference's plain SGMA with 59
+    candidates: the network's basis has a lower held-out Nystrom residual
+    than the random 8, and its test accuracy is within 1% of plain SGMA's."""
+    n, d, B, g = 3000, 16, 150, 0.1
+    X, y = synth.two_gaussians(n, d, seed=50)
+    Xt, yt = synth.two_gaussians(1000, d, seed=150)
+    Xaug = np.vstack([X, X[y > 0].mean(0), X[y < 0].mean(0)])
+    ds = lib.Dataset.dense(X, y)
+    td = lib.Dataset.dense(Xt, yt)
+    props = abi.make_properties(gamma=g, C_=10.0, size=B)
+    pp = abi.make_predict_properties(labels=0)
+    res = {}
+    for name, opts in [("plain59", dict(sgma_candidates=59, proposal=0)),
+                       ("learned", dict(sgma_candidates=59, proposal=1, proposal_keep=8, proposal_pool=236,
+                                        proposal_warmup=30)),
+                       ("random8", dict(sgma_candidates=8, proposal=0))]:
+        cent, beta, st = _sgma_run(ds, props, 7, **opts)
+        m = lib.budgeted_model(props, ds, cent, beta)
+        acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
+        res[name] = (_nystrom_residual(Xaug, cent, Xt, g), acc, st)
+        print(name, "held-out Nystrom residual %.5f accuracy %.4f" % res[name][:2],
+              {k: st[k] for k in ("sgma_steps", "learned_steps", "mlp_loss0", "mlp_loss")})
+    assert res["learned"][2]["learned_steps"] == B - 2 - 30
+    assert res["learned"][2]["mlp_loss"] < res["learned"][2]["mlp_loss0"]
+    assert res["learned"][0] < res["random8"][0]
+    assert res["learned"][1] >= res["plain59"][1] - 0.01
