@@ -164,6 +164,21 @@ void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
 void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf16 *c, int kernel, double gamma,
                                   double *d_K, int ldk);
 
+/* Batched irwls_dev_posv_aug: nprob systems (augmented matrices A[p] with
+   leading dimension lda[p] >= n[p]+1, right-hand sides rhs[p], solutions
+   x[p], all device) factored and solved together, each with the
+   arithmetic of the single-system call; info[p] (may be NULL) receives the
+   factorization info.  The launch sequence is captured per batch shape
+   (count, largest size rounded up to 256) and replayed; its tile shape (64,
+   128 or 256 panel rows per work-group) is picked on the first call of a
+   shape by timing each on SPD systems of the same sizes.  Returns 0. */
+int  irwls_dev_posv_aug_batched(int nprob, double *const *d_A, const int *n, const int *lda,
+                                const double *const *d_rhs, double *const *d_x, int *info);
+void irwls_dev_batched_autotune(int enable);
+/* Rows per work-group (in 64-row tiles) chosen for a batch shape, 0 if none
+   yet; ms3 (may be NULL) receives the three timings. */
+int  irwls_dev_batched_choice(int nprob, int nmax, double *ms3);
+
 /* Mixed precision.  FP32 Cholesky of the lower triangle of the FP64 matrix A
    (n x n, leading dimension lda), FP64 diagonal tiles, with the identity
    appended as rows n..2n-1 of F (FP32, column-major, ldf >= 2n): on return

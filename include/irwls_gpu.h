@@ -58,6 +58,7 @@ typedef struct irwls_gpu_stats {
     int learned_steps;     /* SGMA steps chosen through the learned proposal */
     double mlp_loss0;      /* proposal network: loss before / after its fit */
     double mlp_loss;
+    int batch_rounds;      /* trainFULLBatched: batched solve rounds */
 } irwls_gpu_stats;
 void irwls_gpu_last_stats(irwls_gpu_stats *out);
 
@@ -68,6 +69,14 @@ void irwls_gpu_last_stats(irwls_gpu_stats *out);
 svm_dataset irwls_dataset_from_dense(const double *X, const double *y, int n, int d);
 svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, const double *val,
                                    const double *y, int n);
+
+/* Batched full training: nprob independent problems (datasets[p],
+   props[p]); out[p] receives trainFULL's result for problem p (l_p + 1
+   doubles, malloc'd, the caller frees it).  Each problem equals its
+   standalone trainFULL run after srand(0) (the reference's full-train main),
+   bit for bit; their dense solves run batched.  Returns 0.  No counterpart
+   in the reference. */
+int trainFULLBatched(const svm_dataset *datasets, const properties *props, int nprob, double **out);
 
 /* Release the device context (initMemory/freeMemory do this for callers of
    the reference interface). */

@@ -21,7 +21,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC,
              # packed FP32 ops stay off on this target
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-I" + INC, "-Wall", "-Wno-unused-result",
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-pthread", "-I" + INC, "-Wall", "-Wno-unused-result",
            "-D__HIP_PLATFORM_AMD__"]
 
 
@@ -57,7 +57,7 @@ def build(verbose=False):
             if o.strip():
                 print(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [j[1] for j in jobs] + ["-lm"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [j[1] for j in jobs] + ["-lm", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -159,9 +159,8 @@ __device__ __forceinline__ void tile_store(const double (&a)[16], const double (
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void diag_first_kernel(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
-                                                         T *__restrict__ Dinv) {
-    __shared__ PivotBufs B;
+__device__ __forceinline__ void diag_first_body(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
+                                                T *__restrict__ Dinv, PivotBufs &B) {
     const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
     double a[16], x[16];
 #pragma unroll
@@ -173,6 +172,13 @@ __global__ __launch_bounds__(256) void diag_first_kernel(T *__restrict__ A, int 
     const int fail = tile_factor(a, x, B, kb);
     tile_store<T>(a, x, A, lda, 0, kb, Dinv, Dinv + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_first_kernel(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
+                                                         T *__restrict__ Dinv) {
+    __shared__ PivotBufs B;
+    diag_first_body<T>(A, lda, kb, info, Dinv, B);
 }
 
 // ------------------------------------------------------------ MFMA tiles ---
@@ -232,14 +238,15 @@ __device__ __forceinline__ void mfma_rows(typename Mma<T>::acc_t (&acc)[2][2], c
 // the factorization has failed in this block only right-hand-side rows
 // (rows >= n) are solved; after a failure in an earlier block nothing is.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
-                                                   const int *__restrict__ info, const T *__restrict__ Dinv) {
+__device__ __forceinline__ void trsm_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                          const int *__restrict__ info, const T *__restrict__ Dinv, int bx) {
     const int inf = info[0];
     if (inf != 0 && inf <= k0) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int r0 = k0 + kb + blockIdx.x * NB;
+    const int r0 = k0 + kb + bx * NB;
+    if (r0 >= nrows) return;
     const int rmin = (inf != 0) ? n : 0;
     if (r0 + NB <= rmin) return;
     typename Mma<T>::acc_t acc[2][2];
@@ -259,6 +266,12 @@ __global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                   const int *__restrict__ info, const T *__restrict__ Dinv) {
+    trsm_body<T>(A, n, nrows, lda, k0, kb, info, Dinv, blockIdx.x);
+}
+
 // ----------------------------------------------------------- syrk + next ---
 // Trailing tiles: A(I, J) -= P_I P_J^T, rows [t0, nrows) x columns [t0, n),
 // P = A(:, k0:k0+kb) (solved), t0 = k0 + kb; grid (column tiles, row tiles),
@@ -266,12 +279,11 @@ __global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int
 // diagonal tile: after its update it factors and inverts it in FP64
 // (look-ahead), so the next step needs no separate launch for that.
 template <typename T>
-__global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
-                                                        int *__restrict__ info, T *__restrict__ Dnext) {
-    __shared__ double S[NB][NB + 1];
-    __shared__ PivotBufs B;
-    const int tj = blockIdx.x, ti = blockIdx.y;
+__device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                               int *__restrict__ info, T *__restrict__ Dnext, int tj, int ti,
+                                               double (*S)[NB + 1], PivotBufs &B) {
     const int t0 = k0 + kb;
+    if (t0 + tj * NB >= n || t0 + ti * NB >= nrows) return;
     // stop only for a failure in blocks <= k (pivot index < t0); work-group
     // (0,0) may record a failure of block k+1 during this launch
     const int inf = info[0];
@@ -317,6 +329,14 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                        int *__restrict__ info, T *__restrict__ Dnext) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ PivotBufs B;
+    syrk_next_body<T>(A, n, nrows, lda, k0, kb, info, Dnext, blockIdx.x, blockIdx.y, S, B);
+}
+
 // -------------------------------------------------------- triangular sweeps ---
 // One launch per 64-row block w.  Every work-group first solves the diagonal
 // block for block w of the solution from the (final) accumulator and
@@ -360,10 +380,10 @@ __global__ __launch_bounds__(256) void trsv_fwd_step(const double *__restrict__ 
 
 // acc_c -= sum_i L(k0+i, c) xs_i for the columns c < k0 of this wave
 __device__ __forceinline__ void bwd_update(const double *__restrict__ L, int lda, int k0, int kb, const double *xs,
-                                           double *__restrict__ acc, int cols_per_wave) {
+                                           double *__restrict__ acc, int cols_per_wave, int bx) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double xl = xs[lane];
-    const int c_begin = (blockIdx.x * 4 + wave) * cols_per_wave;
+    const int c_begin = (bx * 4 + wave) * cols_per_wave;
     for (int q = 0; q < cols_per_wave; q++) {
         const int c = c_begin + q;
         if (c >= k0) break;
@@ -401,14 +421,14 @@ __global__ __launch_bounds__(256) void trsv_bwd_step(const double *__restrict__ 
         if (blockIdx.x == 0 && lane < kb) out[k0 + lane] = x;
     }
     __syncthreads();
-    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave);
+    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave, blockIdx.x);
 }
 
 // backward with the stored inverse of the diagonal tile:
 // x_w[i] = sum_k Linv(k, i) acc_w[k], Linv(k, i) = DinvT[k*64 + i]
-__global__ __launch_bounds__(256) void trsv_bwd_inv_step(const double *__restrict__ L, int n, int lda, int w,
-                                                         const double *__restrict__ DinvT, double *__restrict__ acc,
-                                                         double *__restrict__ out, int cols_per_wave) {
+__device__ __forceinline__ void bwd_inv_body(const double *__restrict__ L, int n, int lda, int w,
+                                             const double *__restrict__ DinvT, double *__restrict__ acc,
+                                             double *__restrict__ out, int cols_per_wave, int bx) {
     __shared__ double zs[NB], part[4][NB], xs[NB];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int k0 = w * NB;
@@ -427,10 +447,16 @@ __global__ __launch_bounds__(256) void trsv_bwd_inv_step(const double *__restric
     if (t < NB) {
         const double x = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
         xs[t] = x;
-        if (blockIdx.x == 0 && t < kb) out[k0 + t] = x;
+        if (bx == 0 && t < kb) out[k0 + t] = x;
     }
     __syncthreads();
-    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave);
+    bwd_update(L, lda, k0, kb, xs, acc, cols_per_wave, bx);
+}
+
+__global__ __launch_bounds__(256) void trsv_bwd_inv_step(const double *__restrict__ L, int n, int lda, int w,
+                                                         const double *__restrict__ DinvT, double *__restrict__ acc,
+                                                         double *__restrict__ out, int cols_per_wave) {
+    bwd_inv_body(L, n, lda, w, DinvT, acc, out, cols_per_wave, blockIdx.x);
 }
 
 // right-hand-side row of an augmented matrix <-> contiguous vector
@@ -564,6 +590,218 @@ void sweep_backward_inv(const double *L, int n, int lda, const double *Dinv, dou
     }
 }
 
+int nblk_of_host(int n) { return (n + NB - 1) / NB; }
+
+// ---------------------------------------------------------------- batched ---
+// Many KKT-style systems of different sizes solved together: problem p's
+// augmented matrix (right-hand side in row n_p), tile inverses, info and
+// vectors are listed in a device table; every kernel takes its problem from
+// blockIdx.z and skips the steps and tiles the problem does not have, so one
+// sequence of launches (sized for the largest problem) factors and solves
+// them all, and can be captured once and replayed with new table contents.
+// R = 64-row tiles per work-group in the panel solve and trailing update
+// (the tile shape the autotuner picks).
+struct BatchProb {
+    double *A;
+    int n, lda;
+    double *Dinv;          // 2 tiles per 64-block (inverse, transpose)
+    int *info;
+    const double *rhs;
+    double *z, *x;
+};
+
+__device__ __forceinline__ int nblk_of(int n) { return (n + NB - 1) / NB; }
+
+__global__ void b_prepare(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p.n) p.A[(size_t)i * p.lda + p.n] = p.rhs[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.info[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void b_diag_first(const BatchProb *__restrict__ tab) {
+    __shared__ PivotBufs B;
+    const BatchProb p = tab[blockIdx.z];
+    if (p.n <= 0) return;
+    diag_first_body<double>(p.A, p.lda, p.n < NB ? p.n : NB, p.info, p.Dinv, B);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void b_trsm(const BatchProb *__restrict__ tab, int kbk) {
+    const BatchProb p = tab[blockIdx.z];
+    if (kbk >= nblk_of(p.n)) return;
+    const int k0 = kbk * NB, kb = (p.n - k0 < NB) ? p.n - k0 : NB;
+    for (int r = 0; r < R; r++) {
+        trsm_body<double>(p.A, p.n, p.n + 1, p.lda, k0, kb, p.info, p.Dinv + (size_t)kbk * 2 * TILE,
+                          blockIdx.x * R + r);
+        __syncthreads();
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void b_syrk(const BatchProb *__restrict__ tab, int kbk) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ PivotBufs B;
+    const BatchProb p = tab[blockIdx.z];
+    if (kbk + 1 >= nblk_of(p.n)) return;
+    const int k0 = kbk * NB;
+    for (int r = 0; r < R; r++) {
+        syrk_next_body<double>(p.A, p.n, p.n + 1, p.lda, k0, NB, p.info, p.Dinv + (size_t)(kbk + 1) * 2 * TILE,
+                               blockIdx.x, blockIdx.y * R + r, S, B);
+        __syncthreads();
+    }
+}
+
+__global__ void b_gather_z(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p.n) p.z[i] = p.A[(size_t)i * p.lda + p.n];
+}
+
+__global__ __launch_bounds__(256) void b_bwd(const BatchProb *__restrict__ tab, int w, int cpw) {
+    const BatchProb p = tab[blockIdx.z];
+    if (w >= nblk_of(p.n)) return;
+    if ((int)blockIdx.x * 4 * cpw >= (w > 0 ? w * NB : 1)) return;   // no columns left for this work-group
+    bwd_inv_body(p.A, p.n, p.lda, w, p.Dinv + (size_t)(2 * w + 1) * TILE, p.z, p.x, cpw, blockIdx.x);
+}
+
+// SPD test systems for the autotuner: A = I * (n+1) + 1/(1+|i-j|), rhs = 1.
+__global__ void b_fill_test(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i > p.n || j >= p.n) return;
+    double v = (i == p.n) ? 1.0 : 1.0 / (1.0 + (double)(i > j ? i - j : j - i));
+    if (i == j) v += (double)(p.n + 1);
+    p.A[(size_t)j * p.lda + i] = v;
+}
+
+constexpr int B_CPW = 8;
+
+// The launch sequence of one batched solve, sized for problems up to ncap.
+void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStream_t st) {
+    const int nblk = (ncap + NB - 1) / NB;
+    const int vb = (ncap + 1 + 255) / 256;
+    hipLaunchKernelGGL(b_prepare, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
+    hipLaunchKernelGGL(b_diag_first, dim3(1, 1, nprob), dim3(256), 0, st, d_tab);
+    for (int k = 0; k < nblk; k++) {
+        const int rt = (ncap + 1 - k * NB - NB + NB - 1) / NB;   // panel row tiles below block k (rhs row included)
+        if (rt > 0) {
+            const int g = (rt + R - 1) / R;
+            if (R == 1) hipLaunchKernelGGL(b_trsm<1>, dim3(g, 1, nprob), dim3(256), 0, st, d_tab, k);
+            else if (R == 2) hipLaunchKernelGGL(b_trsm<2>, dim3(g, 1, nprob), dim3(256), 0, st, d_tab, k);
+            else hipLaunchKernelGGL(b_trsm<4>, dim3(g, 1, nprob), dim3(256), 0, st, d_tab, k);
+        }
+        if (k + 1 < nblk) {
+            const int tc = nblk - k - 1;
+            const int tr = (ncap + 1 - (k + 1) * NB + NB - 1) / NB;
+            const int g = (tr + R - 1) / R;
+            if (R == 1) hipLaunchKernelGGL(b_syrk<1>, dim3(tc, g, nprob), dim3(256), 0, st, d_tab, k);
+            else if (R == 2) hipLaunchKernelGGL(b_syrk<2>, dim3(tc, g, nprob), dim3(256), 0, st, d_tab, k);
+            else hipLaunchKernelGGL(b_syrk<4>, dim3(tc, g, nprob), dim3(256), 0, st, d_tab, k);
+        }
+    }
+    hipLaunchKernelGGL(b_gather_z, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
+    for (int w = nblk - 1; w >= 0; w--) {
+        const int cols = w * NB;
+        const int g = cols > 0 ? (cols + 4 * B_CPW - 1) / (4 * B_CPW) : 1;
+        hipLaunchKernelGGL(b_bwd, dim3(g, 1, nprob), dim3(256), 0, st, d_tab, w, B_CPW);
+    }
+}
+
+// Per batch shape (problem count, capacity): the captured graph of each tile
+// shape, the device table it reads, and the autotuned choice.
+struct BatchPlan {
+    int nprob = 0, ncap = 0;
+    BatchProb *d_tab = nullptr;
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    int best = -1;            // index into kShapes
+    double ms[3] = {0, 0, 0};
+};
+constexpr int kShapes[3] = {1, 2, 4};
+constexpr int kMaxPlans = 16;
+BatchPlan g_plans[kMaxPlans];
+int g_nplans = 0;
+int g_autotune = 1;
+
+BatchPlan *plan_for(int nprob, int ncap) {
+    for (int i = 0; i < g_nplans; i++)
+        if (g_plans[i].nprob == nprob && g_plans[i].ncap == ncap) return &g_plans[i];
+    BatchPlan *pl;
+    if (g_nplans < kMaxPlans) {
+        pl = &g_plans[g_nplans++];
+    } else {   // evict the oldest
+        pl = &g_plans[0];
+        for (int e = 0; e < 3; e++)
+            if (pl->exec[e]) IRWLS_HIP(hipGraphExecDestroy(pl->exec[e]));
+        irwls_dev_free(pl->d_tab);
+        *pl = BatchPlan();
+    }
+    pl->nprob = nprob;
+    pl->ncap = ncap;
+    pl->d_tab = (BatchProb *)irwls_dev_alloc((size_t)nprob * sizeof(BatchProb));
+    return pl;
+}
+
+hipGraphExec_t graph_for(BatchPlan *pl, int e) {
+    if (pl->exec[e]) return pl->exec[e];
+    hipGraph_t g;
+    IRWLS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    batched_sequence(pl->d_tab, pl->nprob, pl->ncap, kShapes[e], stream());
+    IRWLS_HIP(hipStreamEndCapture(stream(), &g));
+    IRWLS_HIP(hipGraphInstantiate(&pl->exec[e], g, nullptr, nullptr, 0));
+    IRWLS_HIP(hipGraphDestroy(g));
+    return pl->exec[e];
+}
+
+// Times each tile shape on SPD test systems of the batch's sizes (in scratch
+// buffers of the same shapes) and keeps the fastest.
+void autotune(BatchPlan *pl, const BatchProb *host_tab) {
+    const int np = pl->nprob;
+    BatchProb *t = (BatchProb *)malloc((size_t)np * sizeof(BatchProb));
+    for (int i = 0; i < np; i++) {
+        t[i] = host_tab[i];
+        const int nb = nblk_of_host(t[i].n);
+        t[i].A = (double *)irwls_dev_alloc((size_t)t[i].lda * (t[i].n > 0 ? t[i].n : 1) * sizeof(double));
+        t[i].Dinv = (double *)irwls_dev_alloc((size_t)(nb + 1) * 2 * TILE * sizeof(double));
+        t[i].info = (int *)irwls_dev_alloc(4 * sizeof(int));
+        t[i].z = (double *)irwls_dev_alloc((size_t)(t[i].n + 1) * sizeof(double));
+        t[i].x = (double *)irwls_dev_alloc((size_t)(t[i].n + 1) * sizeof(double));
+        t[i].rhs = t[i].z;
+    }
+    irwls_dev_h2d(pl->d_tab, t, (size_t)np * sizeof(BatchProb));
+    hipEvent_t e0, e1;
+    IRWLS_HIP(hipEventCreate(&e0));
+    IRWLS_HIP(hipEventCreate(&e1));
+    for (int e = 0; e < 3; e++) {
+        hipGraphExec_t ex = graph_for(pl, e);
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            hipLaunchKernelGGL(b_fill_test, dim3((pl->ncap + 1 + 255) / 256, pl->ncap, np), dim3(256), 0, stream(),
+                               pl->d_tab);
+            IRWLS_HIP(hipEventRecord(e0, stream()));
+            IRWLS_HIP(hipGraphLaunch(ex, stream()));
+            IRWLS_HIP(hipEventRecord(e1, stream()));
+            IRWLS_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            IRWLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) best = ms;
+        }
+        pl->ms[e] = best;
+        if (pl->best < 0 || best < pl->ms[pl->best]) pl->best = e;
+    }
+    IRWLS_HIP(hipEventDestroy(e0));
+    IRWLS_HIP(hipEventDestroy(e1));
+    for (int i = 0; i < np; i++) {
+        irwls_dev_free(t[i].A);
+        irwls_dev_free(t[i].Dinv);
+        irwls_dev_free(t[i].info);
+        irwls_dev_free(t[i].z);
+        irwls_dev_free(t[i].x);
+    }
+    free(t);
+}
+
 }  // namespace
 
 extern "C" {
@@ -603,6 +841,79 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
     int info = 0;
     irwls_dev_d2h(&info, info_buf(), sizeof(int));
     return info;
+}
+
+int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const int *lda, const double *const *rhs,
+                               double *const *x, int *info) {
+    if (nprob <= 0) return 0;
+    int nmax = 0;
+    for (int p = 0; p < nprob; p++) {
+        if (lda[p] < n[p] + 1) {
+            fprintf(stderr, "libirwls: irwls_dev_posv_aug_batched: problem %d needs lda >= n+1 (lda=%d, n=%d)\n", p,
+                    lda[p], n[p]);
+            exit(2);
+        }
+        if (n[p] > nmax) nmax = n[p];
+    }
+    const int ncap = (nmax + 255) / 256 * 256;   // shape class: graphs and tuning are shared within it
+    BatchPlan *pl = plan_for(nprob, ncap);
+    // per-problem tile inverses, info and forward-sweep vectors
+    static double *pool = nullptr;
+    static size_t pool_sz = 0;
+    size_t need = 0;
+    for (int p = 0; p < nprob; p++) need += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE + (size_t)n[p] + 8;
+    if (pool_sz < need) {
+        if (pool) irwls_dev_free(pool);
+        pool = (double *)irwls_dev_alloc(need * sizeof(double));
+        pool_sz = need;
+    }
+    BatchProb *t = (BatchProb *)malloc((size_t)nprob * sizeof(BatchProb));
+    size_t off = 0;
+    for (int p = 0; p < nprob; p++) {
+        t[p].A = A[p];
+        t[p].n = n[p];
+        t[p].lda = lda[p];
+        t[p].Dinv = pool + off;
+        off += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE;
+        t[p].info = (int *)(pool + off);
+        off += 4;
+        t[p].z = pool + off;
+        off += (size_t)n[p] + 4;
+        t[p].rhs = rhs[p];
+        t[p].x = x[p];
+    }
+    if (pl->best < 0) {
+        if (g_autotune) autotune(pl, t);
+        else pl->best = 0;
+    }
+    irwls_dev_h2d(pl->d_tab, t, (size_t)nprob * sizeof(BatchProb));
+    {
+        double fl = 0.0;
+        for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
+        TimerScope ts(IRWLS_T_SYRK, fl, 0.0);
+        IRWLS_HIP(hipGraphLaunch(graph_for(pl, pl->best), stream()));
+    }
+    int rc = 0;
+    for (int p = 0; p < nprob; p++) {
+        int v = 0;
+        irwls_dev_d2h(&v, t[p].info, sizeof(int));
+        if (info) info[p] = v;
+    }
+    free(t);
+    return rc;
+}
+
+void irwls_dev_batched_autotune(int enable) { g_autotune = enable != 0; }
+
+int irwls_dev_batched_choice(int nprob, int nmax, double *ms3) {
+    const int ncap = (nmax + 255) / 256 * 256;
+    for (int i = 0; i < g_nplans; i++)
+        if (g_plans[i].nprob == nprob && g_plans[i].ncap == ncap) {
+            if (ms3)
+                for (int e = 0; e < 3; e++) ms3[e] = g_plans[i].ms[e];
+            return g_plans[i].best >= 0 ? kShapes[g_plans[i].best] : 0;
+        }
+    return 0;
 }
 
 int irwls_dev_potrf_inv_f32(const double *A, int n, int lda, float *F, int ldf, float *ZT, int ldz) {

@@ -1,0 +1,186 @@
+/*
+ * Batched full training: many independent trainFULL problems (one-vs-rest
+ * tasks, C/gamma grid points) in one call, their dense solves batched.
+ *
+ * Each problem runs the unchanged trainFULL control flow on a host thread of
+ * its own; the threads take turns on the device under one lock (all device
+ * work goes to the library's single stream).  A problem that reaches its
+ * KKT solve (src/full-train.c:192-199) queues it and waits; when every
+ * problem still running is waiting, one batched factorization and solve runs
+ * for all of them (irwls_dev_posv_aug_batched: one replay of a captured
+ * launch sequence, tile shape chosen by its autotuner), and they continue.
+ * The per-problem arithmetic is that of the standalone solve, so each
+ * problem's result is bit-identical to its standalone trainFULL run.
+ *
+ * Each problem draws its working-set permutations (src/full-train.c:631)
+ * from a private stream with glibc's rand() algorithm seeded like srand(0)
+ * (the reference's full-train main calls srand(0) before trainFULL), so it
+ * does not depend on the other problems' progress.
+ */
+#define _GNU_SOURCE
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+#include "full-train.h"
+
+typedef struct {
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    int nprob, active, npend, rounds;
+    int *pending;
+    double **A;
+    int *n, *lda;
+    const double **rhs;
+    double **x;
+} batch_sched;
+
+typedef struct {
+    batch_sched *s;
+    int slot;
+    svm_dataset ds;
+    properties props;
+    double *out;
+    irwls_gpu_stats stats;
+    struct random_data rd;
+    char rstate[128];
+} batch_job;
+
+static __thread batch_sched *t_sched;
+static __thread int t_slot;
+static __thread struct random_data *t_rng;
+
+int irwls_rand(void) {
+    if (t_rng) {
+        int32_t r = 0;
+        random_r(t_rng, &r);
+        return r;
+    }
+    return rand();
+}
+
+/* Runs the pending solves (called with the lock held). */
+static void run_batch(batch_sched *s) {
+    int k = 0;
+    double **A = (double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
+    int *n = (int *)irwls_xmalloc((size_t)s->npend * sizeof(int));
+    int *lda = (int *)irwls_xmalloc((size_t)s->npend * sizeof(int));
+    const double **rhs = (const double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
+    double **x = (double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
+    for (int i = 0; i < s->nprob; i++) {
+        if (!s->pending[i]) continue;
+        A[k] = s->A[i];
+        n[k] = s->n[i];
+        lda[k] = s->lda[i];
+        rhs[k] = s->rhs[i];
+        x[k] = s->x[i];
+        k++;
+    }
+    irwls_dev_posv_aug_batched(k, A, n, lda, rhs, x, NULL);
+    irwls_dev_sync();
+    for (int i = 0; i < s->nprob; i++) s->pending[i] = 0;
+    s->npend = 0;
+    s->rounds++;
+    free(A);
+    free(n);
+    free(lda);
+    free(rhs);
+    free(x);
+    pthread_cond_broadcast(&s->cv);
+}
+
+int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x) {
+    batch_sched *s = t_sched;
+    if (!s) return 0;
+    const int me = t_slot;
+    s->A[me] = d_A;
+    s->n[me] = n;
+    s->lda[me] = lda;
+    s->rhs[me] = d_rhs;
+    s->x[me] = d_x;
+    s->pending[me] = 1;
+    s->npend++;
+    if (s->npend == s->active) run_batch(s);
+    while (s->pending[me]) pthread_cond_wait(&s->cv, &s->mu);
+    return 1;
+}
+
+static void *worker(void *arg) {
+    batch_job *j = (batch_job *)arg;
+    batch_sched *s = j->s;
+    pthread_mutex_lock(&s->mu);
+    t_sched = s;
+    t_slot = j->slot;
+    t_rng = &j->rd;
+    j->out = trainFULL(j->ds, j->props);
+    irwls_gpu_last_stats(&j->stats);
+    t_sched = NULL;
+    t_rng = NULL;
+    s->active--;
+    if (s->npend > 0 && s->npend == s->active) run_batch(s);
+    pthread_mutex_unlock(&s->mu);
+    return NULL;
+}
+
+int trainFULLBatched(const svm_dataset *datasets, const properties *props, int nprob, double **out) {
+    if (nprob <= 0) return 0;
+    const double t0 = irwls_now();
+    irwls_dev_init();
+    batch_sched s;
+    memset(&s, 0, sizeof(s));
+    pthread_mutex_init(&s.mu, NULL);
+    pthread_cond_init(&s.cv, NULL);
+    s.nprob = nprob;
+    s.active = nprob;
+    s.pending = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
+    s.A = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
+    s.n = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
+    s.lda = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
+    s.rhs = (const double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
+    s.x = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
+    batch_job *jobs = (batch_job *)irwls_xcalloc((size_t)nprob, sizeof(batch_job));
+    pthread_t *th = (pthread_t *)irwls_xcalloc((size_t)nprob, sizeof(pthread_t));
+    for (int p = 0; p < nprob; p++) {
+        jobs[p].s = &s;
+        jobs[p].slot = p;
+        jobs[p].ds = datasets[p];
+        jobs[p].props = props[p];
+        memset(&jobs[p].rd, 0, sizeof(jobs[p].rd));
+        initstate_r(0u, jobs[p].rstate, sizeof(jobs[p].rstate), &jobs[p].rd);
+    }
+    for (int p = 0; p < nprob; p++) {
+        if (pthread_create(&th[p], NULL, worker, &jobs[p]) != 0) {
+            fprintf(stderr, "libirwls: trainFULLBatched: cannot start a worker thread\n");
+            exit(2);
+        }
+    }
+    for (int p = 0; p < nprob; p++) pthread_join(th[p], NULL);
+    irwls_gpu_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    for (int p = 0; p < nprob; p++) {
+        out[p] = jobs[p].out;
+        tot.outer_iters += jobs[p].stats.outer_iters;
+        tot.inner_iters += jobs[p].stats.inner_iters;
+        tot.solves += jobs[p].stats.solves;
+        tot.factorizations += jobs[p].stats.factorizations;
+        tot.pcg_iters += jobs[p].stats.pcg_iters;
+        tot.fp64_fallbacks += jobs[p].stats.fp64_fallbacks;
+        tot.t_solve += jobs[p].stats.t_solve;
+    }
+    tot.batch_rounds = s.rounds;
+    tot.t_total = irwls_now() - t0;
+    irwls_stats_set(&tot);
+    pthread_cond_destroy(&s.cv);
+    pthread_mutex_destroy(&s.mu);
+    free(s.pending);
+    free(s.A);
+    free(s.n);
+    free(s.lda);
+    free(s.rhs);
+    free(s.x);
+    free(jobs);
+    free(th);
+    return 0;
+}

@@ -19,7 +19,8 @@
 #include "internal.h"
 #include "full-train.h"
 
-static irwls_gpu_stats g_stats;
+/* per thread: the batched trainer runs one problem per host thread */
+static __thread irwls_gpu_stats g_stats;
 
 void irwls_gpu_last_stats(irwls_gpu_stats *out) { *out = g_stats; }
 
@@ -29,7 +30,7 @@ int *rpermute(int n) {
     int *p = (int *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
     for (int k = 0; k < n; k++) p[k] = k;
     for (int k = n - 1; k > 0; k--) {
-        int j = rand() % (k + 1);
+        int j = irwls_rand() % (k + 1);
         int t = p[j];
         p[j] = p[k];
         p[k] = t;
@@ -52,7 +53,9 @@ static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x, const in
     }
     if (!done) {
         irwls_dev_h2d(ws->rhs, et, (size_t)N * sizeof(double));
-        (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);   /* failure code ignored, as the reference */
+        /* failure code ignored, as the reference; in a batched run the solve
+           joins the other problems' pending solves */
+        if (!irwls_batch_solve(ws->H, N, N + 1, ws->rhs, ws->w)) (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);
         irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
         g_stats.factorizations++;
     }

@@ -108,6 +108,14 @@ void irwls_proposal_log(irwls_proposal *lp, int P, const double *q, const double
 int irwls_proposal_train(irwls_proposal *lp, int steps, float lr, irwls_gpu_stats *st);
 void irwls_proposal_score(irwls_proposal *lp, int P, float *score);
 
+/* Batched training (batch.c).  irwls_rand is libc rand(), or the calling
+   worker's private glibc-compatible stream in a batched run.
+   irwls_batch_solve joins a batched run's pending KKT solves (device
+   augmented matrix A, lda >= n+1, right-hand side and solution on the
+   device) and returns 1 when done, or returns 0 outside a batched run. */
+int irwls_rand(void);
+int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
+
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
 void irwls_stats_set(const irwls_gpu_stats *s);
 

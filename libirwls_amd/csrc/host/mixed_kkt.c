@@ -42,8 +42,9 @@ typedef struct {
     int mark_cap;
 } mixed_cache;
 
-static mixed_cache g_mc;
-static long g_gen;
+/* per thread: the batched trainer runs one problem per host thread */
+static __thread mixed_cache g_mc;
+static __thread long g_gen;
 
 long irwls_mixed_new_generation(void) { return ++g_gen; }
 

@@ -31,7 +31,7 @@ class irwls_gpu_stats(C.Structure):
                 ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double),
                 ("t_solve", C.c_double), ("factorizations", C.c_int), ("pcg_iters", C.c_int),
                 ("fp64_fallbacks", C.c_int), ("learned_steps", C.c_int), ("mlp_loss0", C.c_double),
-                ("mlp_loss", C.c_double)]
+                ("mlp_loss", C.c_double), ("batch_rounds", C.c_int)]
 
 
 class irwls_rows(C.Structure):
@@ -341,3 +341,18 @@ def linear_system(A, b):
     x = np.zeros(n)
     load().ParallelLinearSystem(_dptr(buf), n, n, 0, 0, _dptr(rhs), n, 1, 0, 0, n, 1, _dptr(x), n, 1, 0, 0, 1)
     return x, buf.reshape((n, n), order="F")
+
+
+def train_full_batched(datasets, props_list):
+    """trainFULLBatched over [(Dataset, properties)]: one beta array (l+1) per
+    problem, each equal to a standalone trainFULL run after srand(0)."""
+    n = len(datasets)
+    L = load()
+    DS = (svm_dataset * n)(*[d.ds for d in datasets])
+    P = (properties * n)(*props_list)
+    out = (C.POINTER(C.c_double) * n)()
+    L.trainFULLBatched.argtypes = [C.POINTER(svm_dataset), C.POINTER(properties), C.c_int,
+                                   C.POINTER(C.POINTER(C.c_double))]
+    L.trainFULLBatched.restype = C.c_int
+    L.trainFULLBatched(DS, P, n, out)
+    return [take_doubles(out[i], datasets[i].ds.l + 1) for i in range(n)]
