@@ -21,6 +21,7 @@ GPU run's outer-iteration count.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -46,6 +47,15 @@ WORKLOADS = {
     # mixed-precision workload: same generator, n=200000, d=256, working set 4000
     "four_gaussians_n200000_d256": dict(n=200000, n_test=20000, d=256, gamma=1 / 256, C=1.0, W=4000, eta=0.001,
                                         seed=1, cpu_outer=1),
+    # budgeted workloads (SGMA + IRWLSpar): CSR, d=2000, 1% density, values U[0,1], labels from an RBF
+    # expansion over 200 anchors; budget 1000, 59 candidates, gamma 0.5, C 10
+    "sparse_rbf_n1000000_d2000": dict(kind="budgeted", n=1000000, n_test=100000, d=2000, density=0.01, gamma=0.5,
+                                      C=10.0, B=1000, cand=59, seed=3, cpu_sgma=3),
+    "sparse_rbf_n100000_d2000": dict(kind="budgeted", n=100000, n_test=20000, d=2000, density=0.01, gamma=0.5,
+                                     C=10.0, B=500, cand=59, seed=3, cpu_sgma=3),
+    # batched workload: 12 one-vs-rest tasks of a 12-class mixture (n log-uniform in [1000, 30000], d in
+    # {16, 32, 64, 128}) x 8 points C in {0.1, 1, 10, 100}, gamma in {0.5/d, 2/d}; W = min(n/4, 1500)
+    "ovr12_grid8_batched": dict(kind="batched", n_test=2000, seed=4, n=None, d=None),
 }
 
 
@@ -152,14 +162,222 @@ def roofline(r, steps):
     return roof
 
 
+def run_budgeted(args, wl, lib, abi):
+    """SGMA + IRWLSpar (the reference budgeted-train's timed window) with the
+    plain and the learned proposal on the same CSR data."""
+    from libirwls_amd import synth
+    log("bench: generating %s" % args.workload)
+    rp, col, val, y = synth.sparse_rbf_labels(wl["n"] + wl["n_test"], wl["d"], wl["density"], gamma=wl["gamma"],
+                                              seed=wl["seed"])
+    n = wl["n"]
+    tr = rp[:n + 1]
+    te = rp[n:] - rp[n]
+    ds = lib.Dataset.csr(tr, col[:rp[n]], val[:rp[n]], y[:n])
+    td = lib.Dataset.csr(te, col[rp[n]:], val[rp[n]:], y[n:])
+    yt = y[n:]
+    L = lib.load()
+    L.irwls_dev_init()
+    ds.pin(with_means=True)
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], size=wl["B"])
+    pp = abi.make_predict_properties(labels=0)
+    modes = {"plain": dict(sgma_candidates=wl["cand"], proposal=0),
+             "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"])}
+    runs = {}
+    sel = ["plain", "learned"] if args.mode == "both" else [{"plain": "plain"}.get(args.mode, "learned")]
+    for mode in sel:
+        lib.set_options(**modes[mode])
+        dom, tw = None, None
+        for i in range(args.warmup):
+            lib.timing(i == args.warmup - 1)
+            t0 = time.time()
+            c = lib.sgma(ds, props, seed=1)
+            lib.irwls_par(ds, c, props)
+            log("bench: %s warmup %d done in %.2f s" % (mode, i, time.time() - t0))
+            if i == args.warmup - 1:
+                tw = lib.timers()
+                dom = max(tw, key=lambda k: tw[k]["ms"])
+                lib.timing(False)
+        lib.timing(True, kinds=None if dom is None else [dom])
+        times, t_sgma = [], []
+        for i in range(args.steps):
+            L.irwls_dev_sync()
+            t0 = time.perf_counter()
+            cent = lib.sgma(ds, props, seed=1)
+            sst = lib.last_stats()
+            t1 = time.perf_counter()
+            beta = lib.irwls_par(ds, cent, props)
+            L.irwls_dev_sync()
+            times.append(time.perf_counter() - t0)
+            t_sgma.append(t1 - t0)
+            pst = lib.last_stats()
+            log("bench: %s step %d: %.3f s (SGMA %.3f s) %s %s" % (mode, i, times[-1], t_sgma[-1], sst, pst))
+        tm = lib.timers()
+        lib.timing(False)
+        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0)
+        m = lib.budgeted_model(props, ds, cent, beta)
+        acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
+        if dom is None:
+            dom = max(tm, key=lambda k: tm[k]["ms"])
+        runs[mode] = dict(value=float(np.mean(times)), sgma_s=float(np.mean(t_sgma)), acc=acc, tm=tm, tw=tw,
+                          dom=dom, sgma_stats=sst, par_stats=pst)
+    main_mode = "learned" if "learned" in runs else "plain"
+    r = runs[main_mode]
+    cpu = None
+    if wl.get("cpu_sgma", 0) > 0 and args.cpu_outer != 0:
+        cpu = cpu_baseline_budgeted(wl, tr, col[:rp[n]], val[:rp[n]], y[:n], props, r)
+    out = {
+        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
+        "value": r["value"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+bf16-mlp" if main_mode == "learned" else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "mode": main_mode, "n": n, "d": wl["d"], "B": wl["B"],
+                   "candidates": wl["cand"], "gamma": wl["gamma"], "C": wl["C"], "n_test": wl["n_test"],
+                   "seed": wl["seed"]},
+        "test_accuracy": r["acc"],
+        "runs": {k: {"s": v["value"], "sgma_s": v["sgma_s"], "test_accuracy": v["acc"],
+                     "sgma_steps": v["sgma_stats"]["sgma_steps"], "learned_steps": v["sgma_stats"]["learned_steps"],
+                     "mlp_loss0": v["sgma_stats"]["mlp_loss0"], "mlp_loss": v["sgma_stats"]["mlp_loss"],
+                     "irwlspar_iters": v["par_stats"]["irwlspar_iters"]} for k, v in runs.items()},
+        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
+        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
+        "roofline": roofline(r, args.steps),
+        "cpu_baseline": cpu,
+    }
+    if "plain" in runs and "learned" in runs:
+        out["speedup_vs_plain"] = runs["plain"]["value"] / runs["learned"]["value"]
+        out["learned_vs_plain"] = {"accuracy_delta": runs["learned"]["acc"] - runs["plain"]["acc"]}
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args, wl, lib, abi):
+    """96 problems: sequential plain FP64 trainFULL runs against
+    trainFULLBatched with BF16 hi/lo kernel products and batched FP64 solves."""
+    from libirwls_amd import synth
+    log("bench: generating %s" % args.workload)
+    tasks = synth.twelve_class_tasks(seed=wl["seed"], n_test=wl["n_test"])
+    L = lib.load()
+    L.irwls_dev_init()
+    probs = []
+    for t, (X, y, d, n, Xt, yt) in enumerate(tasks):
+        ds = lib.Dataset.dense(X, y)
+        ds.pin(with_means=False)
+        td = lib.Dataset.dense(Xt, yt)
+        for C_ in (0.1, 1.0, 10.0, 100.0):
+            for gm in (0.5, 2.0):
+                probs.append(dict(ds=ds, td=td, yt=yt, props=abi.make_properties(gamma=gm / d, C_=C_,
+                                  W=min(n // 4, 1500), eta=0.001), task=t, n=n, d=d))
+    pp = abi.make_predict_properties(labels=0)
+
+    def accuracies(betas):
+        out = []
+        for pr, b in zip(probs, betas):
+            m = lib.full_model(pr["props"], pr["ds"], b)
+            out.append(float(np.mean(lib.hard_test(pr["td"], m, pp) == pr["yt"])))
+        return np.array(out)
+
+    runs = {}
+    sel = ["plain", "batched"] if args.mode == "both" else (["plain"] if args.mode == "plain" else ["batched"])
+    for mode in sel:
+        lib.set_options(precision=0 if mode == "plain" else 3)
+        dom, tw = None, None
+        for i in range(args.warmup):
+            lib.timing(i == args.warmup - 1)
+            t0 = time.time()
+            if mode == "plain":
+                for pr in probs:
+                    lib.train_full(pr["ds"], pr["props"], seed=0)
+            else:
+                lib.train_full_batched([pr["ds"] for pr in probs], [pr["props"] for pr in probs])
+            log("bench: %s warmup %d done in %.2f s" % (mode, i, time.time() - t0))
+            if i == args.warmup - 1:
+                tw = lib.timers()
+                dom = max(tw, key=lambda k: tw[k]["ms"])
+                lib.timing(False)
+        lib.timing(True, kinds=None if dom is None else [dom])
+        times = []
+        for i in range(args.steps):
+            L.irwls_dev_sync()
+            t0 = time.perf_counter()
+            if mode == "plain":
+                betas = [lib.train_full(pr["ds"], pr["props"], seed=0) for pr in probs]
+            else:
+                betas = lib.train_full_batched([pr["ds"] for pr in probs], [pr["props"] for pr in probs])
+            L.irwls_dev_sync()
+            times.append(time.perf_counter() - t0)
+            log("bench: %s step %d: %.3f s %s" % (mode, i, times[-1], lib.last_stats() if mode != "plain" else ""))
+        st = lib.last_stats() if mode != "plain" else None
+        tm = lib.timers()
+        lib.timing(False)
+        lib.set_options(precision=0)
+        if dom is None:
+            dom = max(tm, key=lambda k: tm[k]["ms"])
+        runs[mode] = dict(value=float(np.mean(times)), betas=betas, acc=accuracies(betas), tm=tm, tw=tw, dom=dom,
+                          stats=st)
+    main_mode = "batched" if "batched" in runs else "plain"
+    r = runs[main_mode]
+    nmax = max(min(pr["n"] // 4, 1500) for pr in probs) + 1
+    ms = (C.c_double * 3)()
+    L.irwls_dev_batched_choice.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out = {
+        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
+        "value": r["value"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f64" if main_mode == "batched" else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "mode": main_mode, "problems": len(probs), "tasks": len(tasks),
+                   "n": [t[3] for t in tasks], "d": [t[2] for t in tasks], "seed": wl["seed"],
+                   "n_test_per_task": wl["n_test"]},
+        "test_accuracy": float(r["acc"].mean()),
+        "runs": {k: {"s": v["value"], "mean_test_accuracy": float(v["acc"].mean())} for k, v in runs.items()},
+        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
+        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
+        "roofline": roofline(r, args.steps),
+        "cpu_baseline": None,
+    }
+    if r["stats"]:
+        out["runs"][main_mode].update({"batch_rounds": r["stats"]["batch_rounds"], "solves": r["stats"]["solves"]})
+    if "plain" in runs and "batched" in runs:
+        dacc = runs["batched"]["acc"] - runs["plain"]["acc"]
+        out["speedup_vs_plain"] = runs["plain"]["value"] / runs["batched"]["value"]
+        out["batched_vs_plain"] = {"accuracy_delta_mean": float(dacc.mean()), "accuracy_delta_max_abs":
+                                   float(np.abs(dacc).max()), "within_0.1pct": int(np.sum(np.abs(dacc) <= 0.001))}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_budgeted(wl, rp, col, val, y, props, r):
+    """Oracle SGMA for its first cpu_sgma steps and IRWLSpar for one
+    iteration on the host cores, extrapolated linearly to the GPU run's step
+    and iteration counts (a lower bound: later SGMA steps cost more)."""
+    from oracle import pyoracle as po
+    ods = po.Dataset.csr(rp, col, val, y)
+    k = wl["cpu_sgma"]
+    po.set_limits(0, k, 1)
+    try:
+        t0 = time.time()
+        cent = po.sgma(ods, props, ncand=wl["cand"], seed=1)
+        t1 = time.time()
+        po.irwls_par(ods, np.asarray(cent), props)
+        t2 = time.time()
+    finally:
+        po.set_limits(0, 0, 0)
+    steps = max(props.size - 2, 1)
+    per_step = (t1 - t0) / k
+    per_iter = t2 - t1
+    iters = max(r["par_stats"]["irwlspar_iters"], 1)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": per_step * steps + per_iter * iters, "unit": "s", "cores": cores, "kind": "port",
+            "sample": "oracle SGMA first %d steps in %.1f s and IRWLSpar first iteration in %.1f s, extrapolated to "
+                      "%d steps and %d iterations (lower bound)" % (k, t1 - t0, t2 - t1, steps, iters)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="four_gaussians_n50000_d128", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"],
-                    help="plain FP64, mixed precision, or both (value = mixed, plain reported beside it)")
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "learned", "batched", "both"],
+                    help="plain FP64; mixed precision (full training) or learned proposal (budgeted); "
+                         "both: value = the mixed / learned trainer, plain reported beside it")
     ap.add_argument("--cpu-outer", type=int, default=None,
                     help="outer iterations in the CPU sample (0 = skip, -1 = the whole run; "
                          "default: the workload's own setting)")
@@ -171,6 +389,12 @@ def main():
 
     from libirwls_amd import abi, lib
     wl = WORKLOADS[args.workload]
+    if wl.get("kind") == "budgeted":
+        run_budgeted(args, wl, lib, abi)
+        return
+    if wl.get("kind") == "batched":
+        run_batched(args, wl, lib, abi)
+        return
     cpu_outer = wl.get("cpu_outer", -1) if args.cpu_outer is None else args.cpu_outer
     log("bench: generating %s" % args.workload)
     X, y, Xt, yt = make_data(wl)

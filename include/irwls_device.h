@@ -164,6 +164,18 @@ void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
 void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf16 *c, int kernel, double gamma,
                                   double *d_K, int ldk);
 
+/* Worker threads: irwls_dev_thread_begin gives the calling host thread a
+   stream of its own (and its own scratch, staging and solver buffers), so
+   several threads can drive the device at once; irwls_dev_thread_end waits
+   for that stream and releases them.  Timers only record on the main
+   thread's stream. */
+void irwls_dev_thread_begin(void);
+void irwls_dev_thread_end(void);
+void irwls_dev_release_thread_buffers(void);
+void irwls_dev_release_cholesky_buffers(void);
+void irwls_dev_release_pcg_buffers(void);
+void irwls_dev_release_mlp_buffers(void);
+
 /* Batched irwls_dev_posv_aug: nprob systems (augmented matrices A[p] with
    leading dimension lda[p] >= n[p]+1, right-hand sides rhs[p], solutions
    x[p], all device) factored and solved together, each with the

@@ -34,7 +34,7 @@ namespace {
 constexpr int NB = 64;
 constexpr int TILE = NB * NB;
 
-int *g_info = nullptr;     // [0] = potrf info
+thread_local int *g_info = nullptr;     // [0] = potrf info
 
 int *info_buf() {
     if (!g_info) g_info = (int *)irwls_dev_alloc(4 * sizeof(int));
@@ -473,11 +473,11 @@ __global__ void scatter_row(double *__restrict__ A, int lda, int row, int n, con
 // inverses of the diagonal tiles of the last factorization: block b holds
 // Dinv at [2b * 4096] and its transpose at [(2b+1) * 4096]
 template <typename T> struct DinvPool {
-    static T *buf;
-    static int blocks;
+    static thread_local T *buf;
+    static thread_local int blocks;
 };
-template <typename T> T *DinvPool<T>::buf = nullptr;
-template <typename T> int DinvPool<T>::blocks = 0;
+template <typename T> thread_local T *DinvPool<T>::buf = nullptr;
+template <typename T> thread_local int DinvPool<T>::blocks = 0;
 
 template <typename T>
 T *dinv_buf(int nblk) {
@@ -805,6 +805,17 @@ void autotune(BatchPlan *pl, const BatchProb *host_tab) {
 }  // namespace
 
 extern "C" {
+
+void irwls_dev_release_cholesky_buffers(void) {
+    if (g_info) irwls_dev_free(g_info);
+    g_info = nullptr;
+    if (DinvPool<double>::buf) irwls_dev_free(DinvPool<double>::buf);
+    DinvPool<double>::buf = nullptr;
+    DinvPool<double>::blocks = 0;
+    if (DinvPool<float>::buf) irwls_dev_free(DinvPool<float>::buf);
+    DinvPool<float>::buf = nullptr;
+    DinvPool<float>::blocks = 0;
+}
 
 int irwls_dev_potrf(double *A, int n, int lda) {
     if (n <= 0) return 0;

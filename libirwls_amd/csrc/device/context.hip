@@ -7,8 +7,12 @@ namespace irwls {
 
 static bool g_init = false;
 static hipStream_t g_stream = nullptr;
+// A host thread of a batched run works on a stream of its own (and its own
+// scratch, staging ring and solver buffers, all thread-local below).
+static thread_local hipStream_t t_stream = nullptr;
 
 hipStream_t stream() {
+    if (t_stream) return t_stream;
     if (!g_init) irwls_dev_init();
     return g_stream;
 }
@@ -39,7 +43,7 @@ static void resolve_pool() {
 }
 
 TimerScope::TimerScope(int k, double f, double b)
-    : kind(k), flops(f), bytes(b), on(g_timing && ((g_timing_mask >> k) & 1u)) {
+    : kind(k), flops(f), bytes(b), on(g_timing && !t_stream && ((g_timing_mask >> k) & 1u)) {
     if (!on) return;
     if (g_evn == kPool) resolve_pool();
     IRWLS_HIP(hipEventRecord(g_evs[g_evn][0], stream()));
@@ -57,8 +61,8 @@ TimerScope::~TimerScope() {
 
 // ---- scratch slots -------------------------------------------------------
 static const int kSlots = 16;
-static void *g_scr[kSlots];
-static size_t g_scr_sz[kSlots];
+static thread_local void *g_scr[kSlots];
+static thread_local size_t g_scr_sz[kSlots];
 
 void *scratch(int slot, size_t bytes) {
     if (slot < 0 || slot >= kSlots) {
@@ -163,16 +167,43 @@ void irwls_dev_free(void *p) {
     IRWLS_HIP(hipFree(p));
 }
 
+// Small host-to-device copies go through a pinned staging ring: the caller's
+// buffer is copied into the ring at once (so it may be reused on return) and
+// the transfer is queued on the library stream without waiting, stream order
+// keeping it ahead of the kernels that read it.  The ring restarts whenever
+// the stream has drained (device-to-host copies and explicit
+// synchronisations wait for it), or, when full, after a synchronisation.
+static const size_t kStage = 16u << 20;
+static const size_t kStageMax = 1u << 20;   // larger copies go directly and wait
+static thread_local char *g_stage = nullptr;
+static thread_local size_t g_stage_head = 0;
+
+static void stage_reset() { g_stage_head = 0; }
+
 void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
-    IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
-    IRWLS_HIP(hipStreamSynchronize(stream()));
+    if (!g_stage) IRWLS_HIP(hipHostMalloc((void **)&g_stage, kStage, hipHostMallocDefault));
+    if (bytes > kStageMax) {
+        IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+        IRWLS_HIP(hipStreamSynchronize(stream()));
+        stage_reset();
+        return;
+    }
+    if (g_stage_head + bytes > kStage) {
+        IRWLS_HIP(hipStreamSynchronize(stream()));
+        stage_reset();
+    }
+    char *slot = g_stage + g_stage_head;
+    memcpy(slot, src, bytes);
+    IRWLS_HIP(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, stream()));
+    g_stage_head += (bytes + 255) & ~(size_t)255;
 }
 
 void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
     IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
     IRWLS_HIP(hipStreamSynchronize(stream()));
+    stage_reset();
 }
 
 void irwls_dev_d2d(void *dst, const void *src, size_t bytes) {
@@ -191,7 +222,10 @@ void irwls_dev_memset0(void *dst, size_t bytes) {
     IRWLS_LAUNCH_CHECK();
 }
 
-void irwls_dev_sync(void) { IRWLS_HIP(hipStreamSynchronize(stream())); }
+void irwls_dev_sync(void) {
+    IRWLS_HIP(hipStreamSynchronize(stream()));
+    stage_reset();
+}
 
 void irwls_dev_trim(void) {
     if (!g_init) return;
@@ -225,6 +259,34 @@ void irwls_dev_timer_get(int kind, long *launches, double *ms, double *flops, do
     *ms = g_ms[kind];
     *flops = g_flops[kind];
     *bytes = g_bytes[kind];
+}
+
+void irwls_dev_release_thread_buffers(void) {
+    irwls_dev_release_cholesky_buffers();
+    irwls_dev_release_pcg_buffers();
+    irwls_dev_release_mlp_buffers();
+}
+
+void irwls_dev_thread_begin(void) {
+    irwls_dev_init();
+    if (t_stream) return;
+    IRWLS_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+}
+
+void irwls_dev_thread_end(void) {
+    if (!t_stream) return;
+    IRWLS_HIP(hipStreamSynchronize(t_stream));
+    irwls_dev_release_thread_buffers();
+    for (int s = 0; s < kSlots; s++) {
+        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
+        g_scr[s] = nullptr;
+        g_scr_sz[s] = 0;
+    }
+    if (g_stage) IRWLS_HIP(hipHostFree(g_stage));
+    g_stage = nullptr;
+    g_stage_head = 0;
+    IRWLS_HIP(hipStreamDestroy(t_stream));
+    t_stream = nullptr;
 }
 
 }  // extern "C"

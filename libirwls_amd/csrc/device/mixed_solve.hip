@@ -205,7 +205,7 @@ struct PcgBufs {
     double *rf = nullptr, *zf = nullptr;
     double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr, *rz = nullptr;
 };
-PcgBufs g_pcg;
+thread_local PcgBufs g_pcg;
 
 void pcg_reserve(int n) {
     if (g_pcg.cap >= n) return;
@@ -334,6 +334,16 @@ int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, 
 int irwls_dev_pcg_spd2(const double *A, int n, int lda, const double *B, const float *F, int ldf, const float *ZT,
                        int ldz, double tol, int maxit, double *X, double *relres) {
     return irwls_dev_pcg_spd2_mapped(A, n, lda, B, F, ldf, ZT, ldz, n, nullptr, nullptr, tol, maxit, X, relres);
+}
+
+void irwls_dev_release_pcg_buffers(void) {
+    double **vs[] = {&g_pcg.r, &g_pcg.p, &g_pcg.q, &g_pcg.z, &g_pcg.w, &g_pcg.rf, &g_pcg.zf,
+                     &g_pcg.part_a, &g_pcg.part_b, &g_pcg.part_c, &g_pcg.rz};
+    for (double **v : vs) {
+        if (*v) irwls_dev_free(*v);
+        *v = nullptr;
+    }
+    g_pcg.cap = 0;
 }
 
 }  // extern "C"

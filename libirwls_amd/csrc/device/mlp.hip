@@ -390,7 +390,7 @@ struct MlpBufs {
     int cap = 0;
     float *gpart = nullptr, *lpart = nullptr;
 };
-MlpBufs g_mlp;
+thread_local MlpBufs g_mlp;
 
 void mlp_reserve(int nblk) {
     if (g_mlp.cap >= nblk) return;
@@ -462,6 +462,14 @@ void irwls_dev_gather_idx(const double *src, const int32_t *idx, int m, double *
     if (m <= 0) return;
     hipLaunchKernelGGL(gather_idx_kernel, dim3((m + 255) / 256), dim3(256), 0, stream(), src, idx, m, dst);
     IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_release_mlp_buffers(void) {
+    if (g_mlp.gpart) irwls_dev_free(g_mlp.gpart);
+    if (g_mlp.lpart) irwls_dev_free(g_mlp.lpart);
+    g_mlp.gpart = nullptr;
+    g_mlp.lpart = nullptr;
+    g_mlp.cap = 0;
 }
 
 }  // extern "C"

@@ -3,12 +3,14 @@
  * tasks, C/gamma grid points) in one call, their dense solves batched.
  *
  * Each problem runs the unchanged trainFULL control flow on a host thread of
- * its own; the threads take turns on the device under one lock (all device
- * work goes to the library's single stream).  A problem that reaches its
- * KKT solve (src/full-train.c:192-199) queues it and waits; when every
- * problem still running is waiting, one batched factorization and solve runs
- * for all of them (irwls_dev_posv_aug_batched: one replay of a captured
- * launch sequence, tile shape chosen by its autotuner), and they continue.
+ * its own, with a device stream, scratch and staging buffers of its own
+ * (irwls_dev_thread_begin), so the problems' kernel-block products, KKT
+ * assembly and vector updates overlap on the device.  A problem that
+ * reaches its KKT solve (src/full-train.c:192-199) finishes its stream's
+ * work, queues the solve and waits; when every problem still running is
+ * waiting, one batched factorization and solve runs for all of them
+ * (irwls_dev_posv_aug_batched: one replay of a captured launch sequence,
+ * tile shape chosen by its autotuner), and they continue.
  * The per-problem arithmetic is that of the standalone solve, so each
  * problem's result is bit-identical to its standalone trainFULL run.
  *
@@ -95,6 +97,8 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
     batch_sched *s = t_sched;
     if (!s) return 0;
     const int me = t_slot;
+    irwls_dev_sync();   /* the matrix and right-hand side are complete on this thread's stream */
+    pthread_mutex_lock(&s->mu);
     s->A[me] = d_A;
     s->n[me] = n;
     s->lda[me] = lda;
@@ -104,13 +108,14 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
     s->npend++;
     if (s->npend == s->active) run_batch(s);
     while (s->pending[me]) pthread_cond_wait(&s->cv, &s->mu);
+    pthread_mutex_unlock(&s->mu);
     return 1;
 }
 
 static void *worker(void *arg) {
     batch_job *j = (batch_job *)arg;
     batch_sched *s = j->s;
-    pthread_mutex_lock(&s->mu);
+    irwls_dev_thread_begin();
     t_sched = s;
     t_slot = j->slot;
     t_rng = &j->rd;
@@ -118,6 +123,9 @@ static void *worker(void *arg) {
     irwls_gpu_last_stats(&j->stats);
     t_sched = NULL;
     t_rng = NULL;
+    irwls_mixed_release();
+    irwls_dev_thread_end();
+    pthread_mutex_lock(&s->mu);
     s->active--;
     if (s->npend > 0 && s->npend == s->active) run_batch(s);
     pthread_mutex_unlock(&s->mu);

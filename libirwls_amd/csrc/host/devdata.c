@@ -2,6 +2,7 @@
  * Device residency of datasets, options, timers and small host utilities.
  */
 #define _POSIX_C_SOURCE 200809L
+#include <pthread.h>
 #include <stdio.h>
 #include <string.h>
 #include <strings.h>
@@ -85,6 +86,11 @@ void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double
 /* ---- residency --------------------------------------------------------- */
 #define MAX_PINNED 16
 static irwls_devdata *g_pinned[MAX_PINNED];
+/* The dataset table and the lazily built device copies are shared by the
+   worker threads of a batched run: changes happen under this lock, and a
+   thread finishes (synchronises) the device work of a build before it lets
+   the others see the result. */
+static pthread_mutex_t g_dd_mu = PTHREAD_MUTEX_INITIALIZER;
 
 int irwls_dataset_base(const svm_dataset *ds, int rows) {
     for (int i = 0; i < rows; i++)
@@ -181,12 +187,17 @@ static void destroy(irwls_devdata *dd) {
 
 irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols) {
     irwls_dev_init();
+    pthread_mutex_lock(&g_dd_mu);
     for (int i = 0; i < MAX_PINNED; i++) {
         irwls_devdata *p = g_pinned[i];
         if (p && p->key == ds->features && p->l == ds->l && p->dense == (ds->sparse == 0) &&
-            (p->rows >= ds->l + (with_means ? 2 : 0)) && (base < 0 || p->base == base) && p->d >= min_cols)
+            (p->rows >= ds->l + (with_means ? 2 : 0)) && (base < 0 || p->base == base) && p->d >= min_cols) {
+            pthread_mutex_unlock(&g_dd_mu);
             return p;
+        }
     }
+    pthread_mutex_unlock(&g_dd_mu);
+    /* an unpinned copy belongs to the calling thread alone */
     return upload(ds, with_means, base, min_cols);
 }
 
@@ -218,13 +229,19 @@ static int use_bf16(const irwls_devdata *dd) {
 }
 
 static irwls_rows_bf16 rows_bf16(irwls_devdata *dd, const int32_t *d_idx, int n) {
+    pthread_mutex_lock(&g_dd_mu);
     if (!dd->Xh) {
         dd->ldb = (dd->d + 31) / 32 * 32;
-        dd->Xh = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
-        dd->Xl = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
-        dd->nrm32 = (float *)irwls_dev_alloc((size_t)dd->rows * sizeof(float));
-        irwls_dev_bf16_split(dd->X, dd->d, dd->rows, dd->d, dd->Xh, dd->Xl, dd->ldb, dd->nrm32);
+        uint16_t *Xh = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
+        uint16_t *Xl = (uint16_t *)irwls_dev_alloc((size_t)dd->rows * dd->ldb * sizeof(uint16_t));
+        float *nrm32 = (float *)irwls_dev_alloc((size_t)dd->rows * sizeof(float));
+        irwls_dev_bf16_split(dd->X, dd->d, dd->rows, dd->d, Xh, Xl, dd->ldb, nrm32);
+        irwls_dev_sync();   /* complete before another thread's stream reads it */
+        dd->Xl = Xl;
+        dd->nrm32 = nrm32;
+        dd->Xh = Xh;
     }
+    pthread_mutex_unlock(&g_dd_mu);
     irwls_rows_bf16 r;
     r.Xh = dd->Xh;
     r.Xl = dd->Xl;
@@ -259,12 +276,19 @@ void irwls_kmat(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t 
 
 void irwls_gpu_pin_dataset(svm_dataset ds, int with_means) {
     irwls_dev_init();
+    pthread_mutex_lock(&g_dd_mu);
     for (int i = 0; i < MAX_PINNED; i++)
-        if (g_pinned[i] && g_pinned[i]->key == ds.features) return;
+        if (g_pinned[i] && g_pinned[i]->key == ds.features) {
+            pthread_mutex_unlock(&g_dd_mu);
+            return;
+        }
     for (int i = 0; i < MAX_PINNED; i++) {
         if (!g_pinned[i]) {
-            g_pinned[i] = upload(&ds, with_means, -1, 0);
-            g_pinned[i]->pinned = 1;
+            irwls_devdata *dd = upload(&ds, with_means, -1, 0);
+            irwls_dev_sync();
+            dd->pinned = 1;
+            g_pinned[i] = dd;
+            pthread_mutex_unlock(&g_dd_mu);
             return;
         }
     }
@@ -273,12 +297,14 @@ void irwls_gpu_pin_dataset(svm_dataset ds, int with_means) {
 }
 
 void irwls_gpu_unpin_dataset(svm_dataset ds) {
+    pthread_mutex_lock(&g_dd_mu);
     for (int i = 0; i < MAX_PINNED; i++) {
         if (g_pinned[i] && g_pinned[i]->key == ds.features) {
             destroy(g_pinned[i]);
             g_pinned[i] = NULL;
         }
     }
+    pthread_mutex_unlock(&g_dd_mu);
 }
 
 void irwls_gpu_shutdown(void) {

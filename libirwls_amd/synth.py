@@ -107,21 +107,22 @@ def sparse_rbf_labels(n, d=2000, density=0.01, anchors=200, gamma=0.5, seed=3, c
     return rowptr, col, val, y
 
 
-def twelve_class_tasks(seed=4, n_tasks=12):
+def twelve_class_tasks(seed=4, n_tasks=12, n_test=0):
     """Per one-vs-rest task: n log-uniform in [1000, 30000], d from
     {16, 32, 64, 128}; a 12-class mixture with means N(0, I_d), covariance
     0.5*I; task t relabels class t as +1 and the rest -1.  Returns a list of
-    (X, y, d, n) tuples."""
+    (X, y, d, n, Xt, yt) tuples; Xt, yt are n_test held-out points of the
+    same task (empty when n_test is 0)."""
     rng = np.random.default_rng(seed)
     out = []
     for t in range(n_tasks):
         n = int(np.exp(rng.uniform(np.log(1000), np.log(30000))))
         d = int(rng.choice([16, 32, 64, 128]))
         mu = rng.standard_normal((12, d))
-        cls = rng.integers(0, 12, size=n)
-        X = mu[cls] + np.sqrt(0.5) * rng.standard_normal((n, d))
+        cls = rng.integers(0, 12, size=n + n_test)
+        X = mu[cls] + np.sqrt(0.5) * rng.standard_normal((n + n_test, d))
         y = np.where(cls == t, 1.0, -1.0)
-        out.append((np.ascontiguousarray(X), y, d, n))
+        out.append((np.ascontiguousarray(X[:n]), y[:n], d, n, np.ascontiguousarray(X[n:]), y[n:]))
     return out
 
 
