@@ -36,7 +36,7 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 # timer kinds whose bound is the matrix cores vs HBM
-MFMA_KINDS = {"kmv", "kmat", "trsm", "syrk", "potf2", "sgma", "syrkd", "csr", "bf16"}
+MFMA_KINDS = {"kmv", "kmat", "trsm", "syrk", "potf2", "sgma", "syrkd", "bf16"}
 
 WORKLOADS = {
     "four_gaussians_n50000_d128": dict(n=50000, n_test=10000, d=128, gamma=1 / 128, C=1.0, W=2000, eta=0.001,

@@ -76,6 +76,7 @@ typedef struct irwls_rows {
     const double  *nrm;      /* |x|^2 per row of X / CSR (before index mapping) */
     const int32_t *idx;      /* optional row selection */
     int            n;        /* number of selected rows */
+    double         nnz_per_row; /* mean nonzeros per CSR row (traffic accounting only; 0 if unknown) */
 } irwls_rows;
 
 /* |x|^2 of n dense rows, summed sequentially in feature order. */
@@ -222,7 +223,7 @@ void irwls_dev_sgma_gain(int n, int s, int ncand, const double *d_KSC, const dou
                          const double *d_KSM, int ldm, double *d_q);
 /* eta[c] = 1 - sum_{e<s} KNC[c*ldk + e] Z[e + c*ldz] */
 /* As irwls_dev_sgma_gain, plus qhalf = the same sums over the first half of
-   the 64-row tiles. */
+   the kernel's row tiles (64 rows each for ncand <= 16, else 128). */
 void irwls_dev_sgma_gain_half(int n, int s, int ncand, const double *d_KSC, const double *d_Z, int ldz,
                               const double *d_KSM, int ldm, double *d_q, double *d_qhalf);
 void irwls_dev_sgma_eta(int s, int ncand, const double *d_KNC, int ldk, const double *d_Z, int ldz, double *d_eta);

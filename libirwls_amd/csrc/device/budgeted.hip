@@ -113,57 +113,191 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int m, int n, int k, doub
 }
 
 // -------------------------------------------------------------- sgma_gain ---
-// R(i, c) = KSM[i*ldm + c] - sum_p KSC[i + p*n] Z[p + c*ldz];
-// part[rowtile * ncand + c] = sum over the tile's rows of R(i, c)^2.
+// R(i, c) = KSM[i*ldm + c] - sum_p KSC[i + p*n] Z[p + c*ldz] over a tile of
+// GR = 128 rows x 64 candidates; part[rowtile * ncand + c] = sum over the
+// tile's rows of R(i, c)^2.  Each wave owns 32 rows and every candidate of
+// the tile (2 x 4 blocks of v_mfma_f64_16x16x4f64).  Its KSC operands come
+// from HBM straight into registers one 16-k chunk ahead of the products
+// (KSC is k-outer, so one block's 16 rows at one k are 128 contiguous
+// bytes); the Z chunk, shared by the four waves, is double-buffered in LDS,
+// one barrier per chunk.  KSC streams once per column tile: 16 flops per
+// byte, at the FP64 matrix-core ridge of the part.
+constexpr int GR = 128, GK = 16;
+
+__device__ __forceinline__ void gain_load_a(double a[4][2], const double *__restrict__ KSC, int n, int s, int kc,
+                                            int ra0, bool ok0, bool ok1) {
+    const int kq = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int k = kc + 4 * u + kq;
+        const bool kok = k < s;
+        a[u][0] = (ok0 && kok) ? KSC[(size_t)k * n + ra0] : 0.0;
+        a[u][1] = (ok1 && kok) ? KSC[(size_t)k * n + ra0 + 16] : 0.0;
+    }
+}
+
 __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
                                                         const double *__restrict__ Z, int ldz,
                                                         const double *__restrict__ KSM, int ldm,
                                                         double *__restrict__ part) {
-    __shared__ double As[T][LDP];
-    __shared__ double Bs[T][LDP];
-    __shared__ double red[2][T];
-    const int r0 = blockIdx.x * T, c0 = blockIdx.y * T;
-    v4d acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    for (int kc = 0; kc < s; kc += BK) {
-        ld_kouter(As, KSC, n, r0, n, kc, s);
-        ld_router(Bs, Z, ldz, c0, ncand, kc, s);
-        __syncthreads();
-        mfma_chunk(acc, As, Bs);
-        __syncthreads();
-    }
+    __shared__ double Zs[2][GK][T];
+    __shared__ double red[4][T];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int nb = 0; nb < 2; nb++) {
-        const int c = c0 + tile_col(nb);
+    const int li = lane & 15, kq = lane >> 4;
+    const int r0 = blockIdx.x * GR + wave * 32;
+    const int c0 = blockIdx.y * T;
+    const int ra0 = r0 + li;
+    const bool ok0 = ra0 < n, ok1 = ra0 + 16 < n;
+    // Z staging: thread t carries candidate c0 + t/4, k = 4 (t % 4) .. +3 of a chunk
+    const int zc = threadIdx.x >> 2, zk = (threadIdx.x & 3) * 4;
+    const bool zok = c0 + zc < ncand;
+    const double *zp = Z + (size_t)(zok ? c0 + zc : 0) * ldz;
+    double z[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) z[q] = (zok && zk + q < s) ? zp[zk + q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) Zs[0][zk + q][zc] = z[q];
+    double a[4][2], an[4][2];
+    gain_load_a(a, KSC, n, s, 0, ra0, ok0, ok1);
+    v4d acc[2][4];
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    int buf = 0;
+    for (int kc = 0; kc < s; kc += GK) {
+        const bool more = kc + GK < s;
+        if (more) {
+            gain_load_a(an, KSC, n, s, kc + GK, ra0, ok0, ok1);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = kc + GK + zk + q;
+                z[q] = (zok && k < s) ? zp[k] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            double bv[4];
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++) bv[nb] = Zs[buf][4 * u + kq][nb * 16 + li];
+#pragma unroll
+            for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 4; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][mb], bv[nb], acc[mb][nb], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) Zs[buf ^ 1][zk + q][zc] = z[q];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                a[u][0] = an[u][0];
+                a[u][1] = an[u][1];
+            }
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+    // accumulator register r of lane l: (row l/16 + 4r, candidate l%16) of its block
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int c = c0 + nb * 16 + li;
         double q = 0.0;
+#pragma unroll
         for (int mb = 0; mb < 2; mb++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int i = r0 + tile_row(mb, r);
+                const int i = r0 + mb * 16 + kq + 4 * r;
                 if (i < n && c < ncand) {
                     const double v = KSM[(size_t)i * ldm + c] - acc[mb][nb][r];
                     q += v * v;
                 }
             }
-        // lanes l, l^16, l^32, l^48 hold the same column
+        // lanes l, l^16, l^32, l^48 hold the same candidate
         q += __shfl_xor(q, 16);
         q += __shfl_xor(q, 32);
-        if (lane < 16) red[wave >> 1][(wave & 1) * 32 + nb * 16 + lane] = q;
+        if (lane < 16) red[wave][nb * 16 + lane] = q;
     }
     __syncthreads();
     if (threadIdx.x < T) {
         const int c = c0 + threadIdx.x;
-        if (c < ncand) part[(size_t)blockIdx.x * ncand + c] = red[0][threadIdx.x] + red[1][threadIdx.x];
+        if (c < ncand)
+            part[(size_t)blockIdx.x * ncand + c] =
+                (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
     }
 }
 
-// fixed-order column sums of the partials
-__global__ void col_sum_kernel(const double *__restrict__ part, int nparts, int ncol, double *__restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
+// Gain sums for at most 16 candidates: one 16 x 16 MFMA block per wave (16
+// rows, every candidate), the whole k range in registers, operands loaded
+// straight from KSC (k-outer, so 16 consecutive rows per k) and Z; the
+// work-group covers 64 rows.  Bound by the one pass over KSC.
+__global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
+                                                               const double *__restrict__ Z, int ldz,
+                                                               const double *__restrict__ KSM, int ldm,
+                                                               double *__restrict__ part) {
+    __shared__ double red[4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * T + wave * 16;
+    const int ra = row0 + (lane & 15);        // A operand row
+    const int cb = lane & 15;                 // B operand column (candidate)
+    const int kq = lane >> 4;
+    const bool rok = ra < n, cok = cb < ncand;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    int k0 = 0;
+    for (; k0 + 16 <= s; k0 += 16) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int k = k0 + 4 * u + kq;
+            a[u] = rok ? KSC[(size_t)k * n + ra] : 0.0;
+            b[u] = cok ? Z[(size_t)cb * ldz + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+    for (; k0 < s; k0 += 4) {
+        const int k = k0 + kq;
+        const double a = (rok && k < s) ? KSC[(size_t)k * n + ra] : 0.0;
+        const double b = (cok && k < s) ? Z[(size_t)cb * ldz + k] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    // accumulator register r of lane l: (row (l/16) + 4r, candidate l%16)
+    double q = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = row0 + (lane >> 4) + 4 * r;
+        if (i < n && cok) {
+            const double v = KSM[(size_t)i * ldm + cb] - acc[r];
+            q += v * v;
+        }
+    }
+    q += __shfl_xor(q, 16);
+    q += __shfl_xor(q, 32);
+    if (lane < 16) red[wave][lane] = q;
+    __syncthreads();
+    if (threadIdx.x < 16 && threadIdx.x < ncand) {
+        const int c = threadIdx.x;
+        part[(size_t)blockIdx.x * ncand + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+    }
+}
+
+// fixed-order column sums of the partials: one work-group per column, each
+// thread a strided subset in order, then a tree in a fixed order (the same
+// sum on every run)
+__global__ __launch_bounds__(256) void col_sum_kernel(const double *__restrict__ part, int nparts, int ncol,
+                                                      double *__restrict__ out) {
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t)p * ncol + c];
-    out[c] = s;
+    for (int p = threadIdx.x; p < nparts; p += 256) s += part[(size_t)p * ncol + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
 }
 
 // eta_c = 1 - sum_e KNC[c*ldk + e] Z[e + c*ldz], one wave per candidate
@@ -377,18 +511,25 @@ void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, c
     IRWLS_LAUNCH_CHECK();
 }
 
+// Rows per partial sum of the gain kernel chosen for ncand candidates.
+static int gain_tile_rows(int ncand) { return ncand <= 16 ? T : GR; }
+
 void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
                          int ldm, double *qout) {
     if (n <= 0 || ncand <= 0) return;
-    const int rt = (n + T - 1) / T, ct = (ncand + T - 1) / T;
+    const int rt = (n + gain_tile_rows(ncand) - 1) / gain_tile_rows(ncand), ct = (ncand + T - 1) / T;
     double *part = (double *)scratch(10, (size_t)rt * ncand * sizeof(double));
     {
         TimerScope ts(IRWLS_T_SGMA, 2.0 * n * (double)s * ncand, 8.0 * (double)n * s + 8.0 * (double)n * ncand);
-        hipLaunchKernelGGL(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz, KSM, ldm,
-                           part);
+        if (ncand <= 16)
+            hipLaunchKernelGGL(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz,
+                               KSM, ldm, part);
+        else
+            hipLaunchKernelGGL(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz, KSM,
+                               ldm, part);
         IRWLS_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(col_sum_kernel, dim3((ncand + 255) / 256), dim3(256), 0, stream(), part, rt, ncand, qout);
+    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, rt, ncand, qout);
     IRWLS_LAUNCH_CHECK();
 }
 
@@ -396,12 +537,12 @@ void irwls_dev_sgma_gain_half(int n, int s, int ncand, const double *KSC, const 
                               int ldm, double *qout, double *qhalf) {
     if (n <= 0 || ncand <= 0) return;
     irwls_dev_sgma_gain(n, s, ncand, KSC, Z, ldz, KSM, ldm, qout);
-    // the partials of the call above: one per 64-row tile; the first half of
-    // the tiles gives the sum over rows [0, 64 * (rt / 2))
-    const int rt = (n + T - 1) / T;
+    // the partials of the call above, one per row tile: the first half of the
+    // tiles gives the sum over their rows
+    const int rt = (n + gain_tile_rows(ncand) - 1) / gain_tile_rows(ncand);
     const double *part = (const double *)scratch(10, (size_t)rt * ncand * sizeof(double));
     const int h = rt / 2 > 0 ? rt / 2 : 1;
-    hipLaunchKernelGGL(col_sum_kernel, dim3((ncand + 255) / 256), dim3(256), 0, stream(), part, h, ncand, qhalf);
+    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, h, ncand, qhalf);
     IRWLS_LAUNCH_CHECK();
 }
 
@@ -465,7 +606,7 @@ void irwls_dev_wvec(int B, const double *Phi, const int32_t *act, const double *
     } else {
         irwls_dev_memset0(part, (size_t)B * sizeof(double));
     }
-    hipLaunchKernelGGL(col_sum_kernel, dim3((B + 255) / 256), dim3(256), 0, stream(), part, nsplit, B, K2);
+    hipLaunchKernelGGL(col_sum_kernel, dim3(B), dim3(256), 0, stream(), part, nsplit, B, K2);
     IRWLS_LAUNCH_CHECK();
 }
 

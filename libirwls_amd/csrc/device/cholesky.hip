@@ -41,13 +41,6 @@ int *info_buf() {
     return g_info;
 }
 
-__device__ __forceinline__ double rdl(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 // ------------------------------------------------- diagonal tile factor ---
 // The 64 x 64 tile is held in registers: wave c0 owns the columns
 // c = c0 + 4q (q = 0..15) and lane i row i, both of the tile (a[q]) and of

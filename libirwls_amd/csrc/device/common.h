@@ -26,6 +26,14 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace irwls {
 
+// Lane `lane`'s value of v, wave-uniform (v_readlane on both halves).
+__device__ __forceinline__ double rdl(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // The library stream: every kernel and copy of the library is issued here.
 hipStream_t stream();
 

@@ -199,6 +199,42 @@ __global__ __launch_bounds__(256) void kmat_dense_kernel(DenseSide rs, DenseSide
     }
 }
 
+// Materialised block for at most 16 columns (a shortlist of candidates):
+// one thread per CSR row with every column's dot product in registers, so a
+// narrow block costs one pass over the rows' nonzeros instead of one wave per
+// row with most lanes idle.  A nonzero whose feature no column has (mask 0)
+// adds exact zeros and is skipped, as in csr_row_dot.
+constexpr int NARROW = 16;
+
+__global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
+    const double *__restrict__ XcT, int ldc, int d, const unsigned long long *__restrict__ cmask,
+    const double *__restrict__ cnrm, int ncols, int kernel, double gamma, double *__restrict__ K, int ldk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int src = ridx ? ridx[i] : i;
+    const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+    double acc[NARROW];
+#pragma unroll
+    for (int c = 0; c < NARROW; c++) acc[c] = 0.0;
+    for (int64_t p = p0; p < p1; p++) {
+        const int f = col[p];
+        if (f >= d) continue;
+        const unsigned m = (unsigned)cmask[f];
+        if (m == 0u) continue;
+        const double v = val[p];
+        const double *xr = XcT + (size_t)f * ldc;
+#pragma unroll
+        for (int c = 0; c < NARROW; c++)
+            if ((m >> c) & 1u) acc[c] = fma(v, xr[c], acc[c]);
+    }
+    const double nr = rnrm ? rnrm[src] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NARROW; c++)
+        if (c < ncols) K[(size_t)i * ldk + c] = kfun(kernel, gamma, acc[c], nr, cnrm ? cnrm[c] : 0.0);
+}
+
 // Deterministic split reduction + output epilogue.
 __global__ void kmv_finish_kernel(const double *__restrict__ part, int n, int nsplit, double a, double b,
                                   double cst, const double *__restrict__ rsc, double *__restrict__ out) {
@@ -211,86 +247,110 @@ __global__ void kmv_finish_kernel(const double *__restrict__ part, int n, int ns
 }
 
 // ---- CSR rows x transposed dense columns ---------------------------------
-// One wave per row; lanes over 64 columns per chunk.  XcT is d x ldc.
+// XcT is d x ldc; cmask[(j / 64) * d + f] has bit j % 64 set when column j's
+// feature f is nonzero.  One wave per row, lanes over 64 columns per chunk.
+// The row's nonzeros and their features' masks are fetched 64 at a time with
+// one coalesced load each and broadcast lane by lane from registers; a
+// nonzero whose feature no column of the chunk has is skipped by the whole
+// wave, and otherwise only the lanes whose column has it load XcT.  The
+// products are summed in nonzero order, one fused multiply-add each; the
+// skipped products are exact zeros, so the sum is that of the full
+// dense-column dot product, bit for bit.
+__device__ __forceinline__ double csr_row_dot(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                              int64_t p0, int64_t p1, const double *__restrict__ XcT, int ldc,
+                                              int d, const unsigned long long *__restrict__ cmask, int j) {
+    const int lane = threadIdx.x & 63;
+    double dot = 0.0;
+    for (int64_t b = p0; b < p1; b += 64) {
+        const int cnt = (int)min((int64_t)64, p1 - b);
+        int fl = 0;
+        double vl = 0.0;
+        unsigned long long ml = 0;
+        if (lane < cnt) {
+            fl = col[b + lane];
+            vl = val[b + lane];
+            if (fl < d) ml = cmask[fl];
+        }
+        for (int q = 0; q < cnt; q++) {
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)ml, q);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(ml >> 32), q);
+            if ((lo | hi) == 0u) continue;
+            const unsigned mine = lane < 32 ? (lo >> lane) : (hi >> (lane - 32));
+            const int f = __builtin_amdgcn_readlane(fl, q);
+            const double v = rdl(vl, q);
+            if (mine & 1u) dot = fma(v, XcT[(size_t)f * ldc + j], dot);
+        }
+    }
+    return dot;
+}
+
 __global__ __launch_bounds__(256) void kmv_csr_kernel(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
-    const double *__restrict__ XcT, int ldc, int d, const double *__restrict__ cnrm, int ncols,
-    int kernel, double gamma, const double *__restrict__ w, int rows_per_block,
+    const double *__restrict__ XcT, int ldc, int d, const unsigned long long *__restrict__ cmask,
+    const double *__restrict__ cnrm, int ncols, int kernel, double gamma, const double *__restrict__ w,
     double a, double b, double cst, const double *__restrict__ rsc, double *__restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int rbeg = blockIdx.x * rows_per_block;
-    const int rend = min(nrows, rbeg + rows_per_block);
-    for (int i = rbeg + wave; i < rend; i += 4) {
-        const int src = ridx ? ridx[i] : i;
-        const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
-        const double nr = rnrm ? rnrm[src] : 0.0;
-        double acc = 0.0;
-        for (int c0 = 0; c0 < ncols; c0 += 64) {
-            const int j = c0 + lane;
-            if (j < ncols) {
-                double dot = 0.0;
-                for (int64_t p = p0; p < p1; p++) {
-                    const int f = col[p];
-                    if (f < d) dot += val[p] * XcT[(size_t)f * ldc + j];
-                }
-                acc += w[j] * kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
-            }
-        }
-        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) {
-            const double sc = rsc ? rsc[i] : 1.0;
-            out[i] = (a == 0.0 ? 0.0 : a * out[i]) + sc * (b * acc) + cst;
-        }
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nrows) return;
+    const int src = ridx ? ridx[i] : i;
+    const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+    const double nr = rnrm ? rnrm[src] : 0.0;
+    double acc = 0.0;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        const int j = c0 + lane;
+        const bool jok = j < ncols;
+        const double dot = csr_row_dot(col, val, p0, p1, XcT, ldc, d, cmask + (size_t)(c0 >> 6) * d, j);
+        if (jok) acc += w[j] * kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
+    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        const double sc = rsc ? rsc[i] : 1.0;
+        out[i] = (a == 0.0 ? 0.0 : a * out[i]) + sc * (b * acc) + cst;
     }
 }
 
 __global__ __launch_bounds__(256) void kmat_csr_kernel(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
-    const double *__restrict__ XcT, int ldc, int d, const double *__restrict__ cnrm, int ncols,
-    int kernel, double gamma, int rows_per_block, double *__restrict__ K, int ldk) {
+    const double *__restrict__ XcT, int ldc, int d, const unsigned long long *__restrict__ cmask,
+    const double *__restrict__ cnrm, int ncols, int kernel, double gamma, double *__restrict__ K, int ldk) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int rbeg = blockIdx.x * rows_per_block;
-    const int rend = min(nrows, rbeg + rows_per_block);
-    for (int i = rbeg + wave; i < rend; i += 4) {
-        const int src = ridx ? ridx[i] : i;
-        const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
-        const double nr = rnrm ? rnrm[src] : 0.0;
-        for (int c0 = 0; c0 < ncols; c0 += 64) {
-            const int j = c0 + lane;
-            if (j < ncols) {
-                double dot = 0.0;
-                for (int64_t p = p0; p < p1; p++) {
-                    const int f = col[p];
-                    if (f < d) dot += val[p] * XcT[(size_t)f * ldc + j];
-                }
-                K[(size_t)i * ldk + j] = kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
-            }
-        }
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nrows) return;
+    const int src = ridx ? ridx[i] : i;
+    const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+    const double nr = rnrm ? rnrm[src] : 0.0;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        const int j = c0 + lane;
+        const bool jok = j < ncols;
+        const double dot = csr_row_dot(col, val, p0, p1, XcT, ldc, d, cmask + (size_t)(c0 >> 6) * d, j);
+        if (jok) K[(size_t)i * ldk + j] = kfun(kernel, gamma, dot, nr, cnrm ? cnrm[j] : 0.0);
     }
 }
 
 // Column block as a transposed dense matrix XcT (d x nc) + compact norms.
 __global__ void cols_T_from_dense(const double *__restrict__ X, int ldx, const int32_t *__restrict__ idx,
                                   const double *__restrict__ nrm, int nc, int d, double *__restrict__ XcT,
-                                  double *__restrict__ cn) {
+                                  double *__restrict__ cn, unsigned long long *__restrict__ mask) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     if (j >= nc) return;
     const int src = idx ? idx[j] : j;
-    XcT[(size_t)f * nc + j] = X[(size_t)src * ldx + f];
+    const double x = X[(size_t)src * ldx + f];
+    XcT[(size_t)f * nc + j] = x;
+    if (x != 0.0) atomicOr(&mask[(size_t)(j >> 6) * d + f], 1ull << (j & 63));
     if (f == 0) cn[j] = nrm ? nrm[src] : 0.0;
 }
 
 // Scatter CSR rows into a zero-filled dense block: transposed (d x nc) when
-// trans != 0, else row-major (nc x d).
+// trans != 0, else row-major (nc x d); with mask, set the feature masks of
+// the transposed block (bit j % 64 of mask[(j / 64) * d + f]).
 __global__ void cols_from_csr(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                               const double *__restrict__ val, const int32_t *__restrict__ idx,
                               const double *__restrict__ nrm, int nc, int d, int trans,
-                              double *__restrict__ out, double *__restrict__ cn) {
+                              double *__restrict__ out, double *__restrict__ cn,
+                              unsigned long long *__restrict__ mask) {
     const int j = blockIdx.x;
     if (j >= nc) return;
     const int src = idx ? idx[j] : j;
@@ -298,7 +358,10 @@ __global__ void cols_from_csr(const int64_t *__restrict__ rowptr, const int32_t 
     for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         const int f = col[p];
         if (f < d) {
-            if (trans) out[(size_t)f * nc + j] = val[p];
+            if (trans) {
+                out[(size_t)f * nc + j] = val[p];
+                if (mask) atomicOr(&mask[(size_t)(j >> 6) * d + f], 1ull << (j & 63));
+            }
             else out[(size_t)j * d + f] = val[p];
         }
     }
@@ -340,7 +403,7 @@ DenseSide dense_cols(const irwls_rows *c, int d, int slot_vals, int slot_nrm) {
     irwls_dev_memset0(buf, (size_t)c->n * d * sizeof(double));
     if (c->n > 0) {
         hipLaunchKernelGGL(cols_from_csr, dim3(c->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val,
-                           c->idx, c->nrm, c->n, d, 0, buf, cn);
+                           c->idx, c->nrm, c->n, d, 0, buf, cn, nullptr);
         IRWLS_LAUNCH_CHECK();
     }
     DenseSide s;
@@ -352,18 +415,22 @@ DenseSide dense_cols(const irwls_rows *c, int d, int slot_vals, int slot_nrm) {
     return s;
 }
 
-// Transposed dense column block for the CSR row kernels.
-void cols_T(const irwls_rows *c, int d, double **XcT, double **cn) {
+// Transposed dense column block for the CSR row kernels, with the feature
+// masks of its 64-column chunks.
+void cols_T(const irwls_rows *c, int d, double **XcT, double **cn, unsigned long long **mask) {
+    const size_t nchunk = (size_t)(c->n + 63) / 64;
     *XcT = (double *)scratch(4, (size_t)c->n * d * sizeof(double) + 8);
     *cn = (double *)scratch(5, (size_t)c->n * sizeof(double) + 8);
+    *mask = (unsigned long long *)scratch(15, nchunk * d * sizeof(unsigned long long) + 8);
     if (c->n == 0) return;
+    irwls_dev_memset0(*mask, nchunk * d * sizeof(unsigned long long));
     if (c->X) {
         hipLaunchKernelGGL(cols_T_from_dense, dim3((c->n + 255) / 256, d), dim3(256), 0, stream(), c->X,
-                           c->ldx, c->idx, c->nrm, c->n, d, *XcT, *cn);
+                           c->ldx, c->idx, c->nrm, c->n, d, *XcT, *cn, *mask);
     } else {
         irwls_dev_memset0(*XcT, (size_t)c->n * d * sizeof(double));
         hipLaunchKernelGGL(cols_from_csr, dim3(c->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val,
-                           c->idx, c->nrm, c->n, d, 1, *XcT, *cn);
+                           c->idx, c->nrm, c->n, d, 1, *XcT, *cn, *mask);
     }
     IRWLS_LAUNCH_CHECK();
 }
@@ -411,13 +478,13 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
         }
     } else {
         double *XcT, *cn;
-        cols_T(c, d, &XcT, &cn);
-        const int rpb = 16;
-        const double bytes = 8.0 * ((double)c->n * d + 2.0 * r->n);
-        TimerScope ts(IRWLS_T_CSR, flops, bytes);
-        hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + rpb - 1) / rpb), dim3(256), 0, stream(), r->rowptr,
-                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cn, c->n, kernel, gamma, w, rpb, a, b,
-                           cst, rs, out);
+        unsigned long long *cm;
+        cols_T(c, d, &XcT, &cn, &cm);
+        // algorithmic: the rows' nonzeros (value + column), their row pointers, the column table, the output
+        const double nz = r->nnz_per_row * r->n;
+        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 24.0 * r->n);
+        hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col, r->val,
+                           r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, w, a, b, cst, rs, out);
         IRWLS_LAUNCH_CHECK();
     }
 }
@@ -434,14 +501,23 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TM - 1) / TM, (c->n + TN - 1) / TN), dim3(256), 0,
                            stream(), rsd, csd, d, kernel, gamma, K, ldk);
         IRWLS_LAUNCH_CHECK();
+    } else if (c->n <= NARROW) {
+        double *XcT, *cn;
+        unsigned long long *cm;
+        cols_T(c, d, &XcT, &cn, &cm);
+        const double nz = r->nnz_per_row * r->n;
+        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
+        hipLaunchKernelGGL(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->rowptr,
+                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
+        IRWLS_LAUNCH_CHECK();
     } else {
         double *XcT, *cn;
-        cols_T(c, d, &XcT, &cn);
-        const int rpb = 16;
-        const double bytes = 8.0 * ((double)c->n * d + (double)r->n * c->n);
-        TimerScope ts(IRWLS_T_CSR, flops, bytes);
-        hipLaunchKernelGGL(kmat_csr_kernel, dim3((r->n + rpb - 1) / rpb), dim3(256), 0, stream(), r->rowptr,
-                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cn, c->n, kernel, gamma, rpb, K, ldk);
+        unsigned long long *cm;
+        cols_T(c, d, &XcT, &cn, &cm);
+        const double nz = r->nnz_per_row * r->n;
+        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
+        hipLaunchKernelGGL(kmat_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col, r->val,
+                           r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
         IRWLS_LAUNCH_CHECK();
     }
 }

@@ -166,6 +166,7 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
         free(rp);
         free(cl);
         free(vl);
+        dd->nnz_per_row = rows > 0 ? (double)nnz / rows : 0.0;
         dd->nrm = dd->qv;
     }
     return dd;
@@ -216,6 +217,7 @@ irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int
         r.rowptr = dd->rowptr;
         r.col = dd->col;
         r.val = dd->val;
+        r.nnz_per_row = dd->nnz_per_row;
     }
     r.nrm = dd->nrm;
     r.idx = d_idx;

@@ -36,6 +36,7 @@ typedef struct irwls_devdata {
     double *qv;
     double *y;
     int pinned;                /* kept resident between calls */
+    double nnz_per_row;        /* CSR: mean nonzeros per uploaded row */
     /* mixed precision (dense only, built on first use): BF16 hi/lo rows,
        ldb = d rounded up to 32, FP32 norms of the split representation */
     uint16_t *Xh, *Xl;

@@ -36,7 +36,8 @@ class irwls_gpu_stats(C.Structure):
 
 class irwls_rows(C.Structure):
     _fields_ = [("X", C.c_void_p), ("ldx", C.c_int), ("rowptr", C.c_void_p), ("col", C.c_void_p),
-                ("val", C.c_void_p), ("nrm", C.c_void_p), ("idx", C.c_void_p), ("n", C.c_int)]
+                ("val", C.c_void_p), ("nrm", C.c_void_p), ("idx", C.c_void_p), ("n", C.c_int),
+                ("nnz_per_row", C.c_double)]
 
 
 class irwls_rows_bf16(C.Structure):
