@@ -172,6 +172,9 @@ void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
    thread's stream. */
 void irwls_dev_thread_begin(void);
 void irwls_dev_thread_end(void);
+/* Seconds the calling thread has spent waiting for its stream (read-backs
+   and synchronisations) since its last reset. */
+double irwls_dev_wait_seconds(int reset);
 void irwls_dev_release_thread_buffers(void);
 void irwls_dev_release_cholesky_buffers(void);
 void irwls_dev_release_pcg_buffers(void);
@@ -181,15 +184,17 @@ void irwls_dev_release_mlp_buffers(void);
    leading dimension lda[p] >= n[p]+1, right-hand sides rhs[p], solutions
    x[p], all device) factored and solved together, each with the
    arithmetic of the single-system call; info[p] (may be NULL) receives the
-   factorization info.  The launch sequence is captured per batch shape
-   (count, largest size rounded up to 256) and replayed; its tile shape (64,
-   128 or 256 panel rows per work-group) is picked on the first call of a
-   shape by timing each on SPD systems of the same sizes.  Returns 0. */
+   factorization info.  The launch sequence is captured per batch class
+   (count rounded up to a multiple of 16, largest size rounded up to 256)
+   and replayed; its tile shape (64, 128 or 256 panel rows per work-group) is
+   picked on the first call with a given rounded largest size by timing each
+   on SPD systems of that batch's sizes.  Returns 0. */
 int  irwls_dev_posv_aug_batched(int nprob, double *const *d_A, const int *n, const int *lda,
                                 const double *const *d_rhs, double *const *d_x, int *info);
 void irwls_dev_batched_autotune(int enable);
-/* Rows per work-group (in 64-row tiles) chosen for a batch shape, 0 if none
-   yet; ms3 (may be NULL) receives the three timings. */
+/* Rows per work-group (in 64-row tiles) chosen for batches whose largest
+   size rounds up to the same multiple of 256 as nmax, 0 if none yet; ms3
+   (may be NULL) receives the three timings.  nprob is not used. */
 int  irwls_dev_batched_choice(int nprob, int nmax, double *ms3);
 
 /* Mixed precision.  FP32 Cholesky of the lower triangle of the FP64 matrix A

@@ -59,6 +59,8 @@ typedef struct irwls_gpu_stats {
     double mlp_loss0;      /* proposal network: loss before / after its fit */
     double mlp_loss;
     int batch_rounds;      /* trainFULLBatched: batched solve rounds */
+    double t_wait;         /* seconds the host waited for the device (read-backs, synchronisations;
+                              trainFULLBatched: summed over the workers, batched solves excluded) */
 } irwls_gpu_stats;
 void irwls_gpu_last_stats(irwls_gpu_stats *out);
 

@@ -78,23 +78,29 @@ __device__ __forceinline__ int tile_col(int nb) {
 }
 
 // ------------------------------------------------------------------- gemm ---
+// Split over k along z when the tile grid is too small to fill the device:
+// work-group z sums k in [z * kper, (z + 1) * kper) and, with more than one
+// slice, stores its raw tile to part[z][j][i] (m x n, column-major) for
+// gemm_splitk_finish, which adds the slices in slice order.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int m, int n, int k, double alpha, const double *__restrict__ A,
                                                        int lda, const double *__restrict__ B, int ldb, double beta,
-                                                       double *__restrict__ C, int ldc) {
+                                                       double *__restrict__ C, int ldc, int kper,
+                                                       double *__restrict__ part) {
     __shared__ double As[T][LDP];
     __shared__ double Bs[T][LDP];
     const int r0 = blockIdx.x * T, c0 = blockIdx.y * T;
+    const int kb = blockIdx.z * kper, ke = min(k, kb + kper);
     v4d acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    for (int kc = 0; kc < k; kc += BK) {
+    for (int kc = kb; kc < ke; kc += BK) {
         // op(A)(i, p): !TA -> A[i + p*lda] (k-outer); TA -> A[p + i*lda] (row-outer)
-        if (!TA) ld_kouter(As, A, lda, r0, m, kc, k);
-        else ld_router(As, A, lda, r0, m, kc, k);
+        if (!TA) ld_kouter(As, A, lda, r0, m, kc, ke);
+        else ld_router(As, A, lda, r0, m, kc, ke);
         // op(B)(p, j) as (row j, k p): !TB -> B[p + j*ldb] (row-outer); TB -> B[j + p*ldb] (k-outer)
-        if (!TB) ld_router(Bs, B, ldb, c0, n, kc, k);
-        else ld_kouter(Bs, B, ldb, c0, n, kc, k);
+        if (!TB) ld_router(Bs, B, ldb, c0, n, kc, ke);
+        else ld_kouter(Bs, B, ldb, c0, n, kc, ke);
         __syncthreads();
         mfma_chunk(acc, As, Bs);
         __syncthreads();
@@ -106,10 +112,25 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int m, int n, int k, doub
             for (int r = 0; r < 4; r++) {
                 const int i = r0 + tile_row(mb, r);
                 if (i >= m) continue;
-                double *p = C + (size_t)j * ldc + i;
-                *p = (beta == 0.0) ? alpha * acc[mb][nb][r] : alpha * acc[mb][nb][r] + beta * (*p);
+                if (gridDim.z > 1) {
+                    part[((size_t)blockIdx.z * n + j) * m + i] = acc[mb][nb][r];
+                } else {
+                    double *p = C + (size_t)j * ldc + i;
+                    *p = (beta == 0.0) ? alpha * acc[mb][nb][r] : alpha * acc[mb][nb][r] + beta * (*p);
+                }
             }
     }
+}
+
+__global__ void gemm_splitk_finish(int m, int n, int nz, const double *__restrict__ part, double alpha, double beta,
+                                   double *__restrict__ C, int ldc) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)m * n) return;
+    const int i = (int)(e % m), j = (int)(e / m);
+    double s = 0.0;
+    for (int z = 0; z < nz; z++) s += part[(size_t)z * m * n + e];
+    double *p = C + (size_t)j * ldc + i;
+    *p = (beta == 0.0) ? alpha * s : alpha * s + beta * (*p);
 }
 
 // -------------------------------------------------------------- sgma_gain ---
@@ -229,13 +250,18 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
 }
 
 // Gain sums for at most 16 candidates: one 16 x 16 MFMA block per wave (16
-// rows, every candidate), the whole k range in registers, operands loaded
-// straight from KSC (k-outer, so 16 consecutive rows per k) and Z; the
-// work-group covers 64 rows.  Bound by the one pass over KSC.
+// rows, every candidate); the work-group covers 64 rows.  KSC operands come
+// straight from HBM into registers one 16-k chunk ahead of the products
+// (k-outer, so 16 consecutive rows per k); the 16 x 16 chunk of Z, shared by
+// the four waves, is double-buffered in LDS (one load per thread, one
+// barrier per chunk), so the Z reads cost no per-candidate cache lines in
+// the vector memory path.  Past k = s the operands are zero, which adds
+// exact zeros.  Bound by the one pass over KSC.
 __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
                                                                const double *__restrict__ Z, int ldz,
                                                                const double *__restrict__ KSM, int ldm,
                                                                double *__restrict__ part) {
+    __shared__ double Zs[2][16][16];     // [k][candidate]
     __shared__ double red[4][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = blockIdx.x * T + wave * 16;
@@ -243,24 +269,43 @@ __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int
     const int cb = lane & 15;                 // B operand column (candidate)
     const int kq = lane >> 4;
     const bool rok = ra < n, cok = cb < ncand;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    int k0 = 0;
-    for (; k0 + 16 <= s; k0 += 16) {
-        double a[4], b[4];
+    const double *ap = KSC + (rok ? ra : 0);
+    // Z staging: thread t carries candidate t / 16, k = t % 16 of a chunk
+    const int zc = threadIdx.x >> 4, zk = threadIdx.x & 15;
+    const bool zok = zc < ncand;
+    const double *zp = Z + (size_t)(zok ? zc : 0) * ldz;
+    double a[4], an[4];
+    double zv = (zok && zk < s) ? zp[zk] : 0.0;
+    Zs[0][zk][zc] = zv;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int k = k0 + 4 * u + kq;
-            a[u] = rok ? KSC[(size_t)k * n + ra] : 0.0;
-            b[u] = cok ? Z[(size_t)cb * ldz + k] : 0.0;
+    for (int u = 0; u < 4; u++) {
+        const int k = 4 * u + kq;
+        a[u] = (rok && k < s) ? ap[(size_t)k * n] : 0.0;
+    }
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < s; k0 += 16) {
+        const bool more = k0 + 16 < s;
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + 16 + 4 * u + kq;
+                an[u] = (rok && k < s) ? ap[(size_t)k * n] : 0.0;
+            }
+            const int k = k0 + 16 + zk;
+            zv = (zok && k < s) ? zp[k] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
-    }
-    for (; k0 < s; k0 += 4) {
-        const int k = k0 + kq;
-        const double a = (rok && k < s) ? KSC[(size_t)k * n + ra] : 0.0;
-        const double b = (cok && k < s) ? Z[(size_t)cb * ldz + k] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        for (int u = 0; u < 4; u++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], Zs[buf][4 * u + kq][cb], acc, 0, 0, 0);
+        if (more) {
+            Zs[buf ^ 1][zk][zc] = zv;
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] = an[u];
+        }
+        __syncthreads();
+        buf ^= 1;
     }
     // accumulator register r of lane l: (row (l/16) + 4r, candidate l%16)
     double q = 0.0;
@@ -330,57 +375,115 @@ __global__ void scatter_row_kernel(double *__restrict__ M, int ld, int row, int 
 // w_q = sqrt(D[a_q])^2 taken as the product of the two scaled rows, as the
 // reference scales the rows by sqrt(D) before the product; a = active list
 // (NULL = all rows, weights 1).  Phi is row-major n x B.
-__device__ __forceinline__ void ld_rows_scaled(double (*S)[LDP], const double *Phi, int B, int r0, const int32_t *act,
-                                               const double *sw, int q0, int qmax) {
-    const int t = threadIdx.x;
-    const int row = t & 63;           // centroid index within the tile
-    const int kq = (t >> 6) * 4;      // 4 consecutive active rows
-    const bool rok = (r0 + row) < B;
-#pragma unroll
-    for (int qq = 0; qq < 4; qq++) {
-        const int q = q0 + kq + qq;
-        double v = 0.0;
-        if (rok && q < qmax) {
-            const int src = act ? act[q] : q;
-            v = Phi[(size_t)src * B + r0 + row];
-            if (sw) v *= sw[q];
-        }
-        S[row][kq + qq] = v;
-    }
-}
+//
+// A work-group takes a 128 x 128 lower tile (ti >= tj) and one row chunk;
+// its waves own 64 x 64 quarters (4 x 4 blocks of v_mfma_f64_16x16x4f64).
+// The next 16 rows are loaded into registers (each thread 8 consecutive
+// centroids of one row, both tile sides) while the current 16 are
+// multiplied out of LDS, double-buffered, one barrier per 16 rows.  With
+// xcd_map, work-group b runs on XCD b % 8 and takes split
+// (b % 8) + 8 * (b / (8 * ntiles)), so the tiles that read one row chunk
+// share that XCD's L2 and the chunk streams from HBM about once.
+constexpr int WT = 128, WK = 16, WLP = WK + 1;
 
 __global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restrict__ Phi, const int32_t *__restrict__ act,
-                                                    const double *__restrict__ sw, int nact, int chunk,
-                                                    double *__restrict__ part) {
-    __shared__ double As[T][LDP];
-    __shared__ double Bs[T][LDP];
-    const int tj = blockIdx.x, ti = blockIdx.y;
-    if (ti < tj) return;
-    const int split = blockIdx.z;
+                                                    const double *__restrict__ sw, int nact, int chunk, int ntiles,
+                                                    int xcd_map, double *__restrict__ part) {
+    __shared__ double Sa[2][WT][WLP];
+    __shared__ double Sb[2][WT][WLP];
+    const int bid = blockIdx.x;
+    int tile, split;
+    if (xcd_map) {
+        const int r = bid >> 3;
+        tile = r % ntiles;
+        split = (bid & 7) + 8 * (r / ntiles);
+    } else {
+        tile = bid % ntiles;
+        split = bid / ntiles;
+    }
+    int ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= tile) ti++;
+    const int tj = tile - ti * (ti + 1) / 2;
+    const bool diag = ti == tj;
+    const int rI = ti * WT, rJ = tj * WT;
     const int q0 = split * chunk;
-    int q1 = q0 + chunk;
-    if (q1 > nact) q1 = nact;
-    const int rI = ti * T, rJ = tj * T;
-    v4d acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    for (int q = q0; q < q1; q += BK) {
-        ld_rows_scaled(As, Phi, B, rI, act, sw, q, q1);
-        ld_rows_scaled(Bs, Phi, B, rJ, act, sw, q, q1);
+    const int q1 = min(nact, q0 + chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int ql = threadIdx.x >> 4, cl = (threadIdx.x & 15) * 8;
+    double pa[8], pb[8];
+    auto load = [&](int q) {
+        const int qq = q + ql;
+        const bool ok = qq < q1;
+        const int src = ok ? (act ? act[qq] : qq) : 0;
+        const double *row = Phi + (size_t)src * B;
+        const double w = (ok && sw) ? sw[qq] : 1.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int ca = rI + cl + u, cb = rJ + cl + u;
+            double va = (ok && ca < B) ? row[ca] : 0.0;
+            double vb = (ok && !diag && cb < B) ? row[cb] : 0.0;
+            if (sw) {
+                va *= w;
+                vb *= w;
+            }
+            pa[u] = va;
+            pb[u] = vb;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) Sa[buf][cl + u][ql] = pa[u];
+        if (!diag) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) Sb[buf][cl + u][ql] = pb[u];
+        }
+    };
+    v4d acc[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+    load(q0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int q = q0; q < q1; q += WK) {
+        const bool more = q + WK < q1;
+        if (more) load(q + WK);
+        double (*A)[WLP] = Sa[buf];
+        double (*Bm)[WLP] = diag ? Sa[buf] : Sb[buf];
+#pragma unroll
+        for (int ks = 0; ks < WK / 4; ks++) {
+            const int kk = ks * 4 + (lane >> 4);
+            double av[4], bv[4];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) av[mb] = A[wr * 64 + mb * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++) bv[nb] = Bm[wc * 64 + nb * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 4; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mb], bv[nb], acc[mb][nb], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
         __syncthreads();
-        mfma_chunk(acc, As, Bs);
-        __syncthreads();
+        buf ^= 1;
     }
     double *P = part + (size_t)split * B * B;
-    for (int nb = 0; nb < 2; nb++) {
-        const int j = rJ + tile_col(nb);
-        if (j >= B) continue;
-        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int j = rJ + wc * 64 + nb * 16 + (lane & 15);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int i = rI + tile_row(mb, r);
-                if (i >= B || i < j) continue;
-                P[(size_t)j * B + i] = acc[mb][nb][r];
-                P[(size_t)i * B + j] = acc[mb][nb][r];
+                const int i = rI + wr * 64 + mb * 16 + (lane >> 4) + 4 * r;
+                if (i < B && j < B && i >= j) {
+                    P[(size_t)j * B + i] = acc[mb][nb][r];
+                    P[(size_t)i * B + j] = acc[mb][nb][r];
+                }
             }
     }
 }
@@ -503,12 +606,25 @@ extern "C" {
 void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, const double *A, int lda,
                     const double *B, int ldb, double beta, double *C, int ldc) {
     if (m <= 0 || n <= 0) return;
-    dim3 grid((m + T - 1) / T, (n + T - 1) / T);
-    if (!transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (!transA && transB) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    const int tiles = ((m + T - 1) / T) * ((n + T - 1) / T);
+    // slices of at least 64 k until the grid has ~256 work-groups
+    int nz = 1;
+    while (tiles * nz * 2 <= 256 && k / (nz * 2) >= 64) nz *= 2;
+    const int kper = nz > 1 ? (((k + nz - 1) / nz + BK - 1) / BK) * BK : (k > 0 ? k : 1);
+    nz = (k + kper - 1) / kper > 1 ? (k + kper - 1) / kper : 1;
+    double *part = nz > 1 ? (double *)scratch(1, (size_t)nz * m * n * sizeof(double)) : nullptr;
+    dim3 grid((m + T - 1) / T, (n + T - 1) / T, nz);
+    if (!transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kper, part);
+    else if (transA && !transB) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kper, part);
+    else if (!transA && transB) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kper, part);
+    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, stream(), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kper, part);
     IRWLS_LAUNCH_CHECK();
+    if (nz > 1) {
+        const size_t e = (size_t)m * n;
+        hipLaunchKernelGGL(gemm_splitk_finish, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, stream(), m, n, nz,
+                           part, alpha, beta, C, ldc);
+        IRWLS_LAUNCH_CHECK();
+    }
 }
 
 // Rows per partial sum of the gain kernel chosen for ncand candidates.
@@ -566,27 +682,30 @@ void irwls_dev_scatter_row(double *M, int ld, int row, int count, const double *
 
 void irwls_dev_wgram(int B, const double *Phi, const int32_t *act, const double *sw, int nact, const double *KC,
                      double *K1, int ldk) {
-    const int tb = (B + T - 1) / T;
-    // enough splits to fill the device: ~4 work-groups per CU
-    int nsplit = 1;
-    const int tiles = tb * (tb + 1) / 2;
-    while (tiles * nsplit < 1024 && nact / (nsplit * 2) >= 256) nsplit *= 2;
+    const int tb = (B + WT - 1) / WT;
+    const int ntiles = tb * (tb + 1) / 2;
+    // about two work-groups per CU, at least 256 rows per split, partials
+    // under 1 GiB; a multiple of 8 splits when there are 8 or more
+    int nsplit = (512 + ntiles - 1) / ntiles;
+    if (nsplit > nact / 256) nsplit = nact / 256;
+    const size_t BB = (size_t)B * B;
+    while (nsplit > 1 && (size_t)nsplit * BB * sizeof(double) > ((size_t)1 << 30)) nsplit--;
+    if (nsplit < 1) nsplit = 1;
+    if (nsplit >= 8) nsplit &= ~7;
     int chunk = (nact + nsplit - 1) / nsplit;
-    chunk = (chunk + BK - 1) / BK * BK;
-    if (chunk < BK) chunk = BK;
-    nsplit = nact > 0 ? (nact + chunk - 1) / chunk : 1;
-    double *part = (double *)scratch(11, (size_t)nsplit * B * B * sizeof(double));
+    chunk = (chunk + WK - 1) / WK * WK;
+    if (chunk < WK) chunk = WK;
+    double *part = (double *)scratch(11, (size_t)nsplit * BB * sizeof(double));
     {
         TimerScope ts(IRWLS_T_SYRKD, (double)nact * B * (double)(B + 1), 8.0 * (double)nact * B);
         if (nact > 0) {
-            hipLaunchKernelGGL(wgram_kernel, dim3(tb, tb, nsplit), dim3(256), 0, stream(), B, Phi, act, sw, nact, chunk,
-                               part);
+            hipLaunchKernelGGL(wgram_kernel, dim3(ntiles * nsplit), dim3(256), 0, stream(), B, Phi, act, sw, nact,
+                               chunk, ntiles, nsplit % 8 == 0 ? 1 : 0, part);
             IRWLS_LAUNCH_CHECK();
         } else {
-            irwls_dev_memset0(part, (size_t)B * B * sizeof(double));
+            irwls_dev_memset0(part, BB * sizeof(double));
         }
     }
-    const size_t BB = (size_t)B * B;
     hipLaunchKernelGGL(wgram_finish, dim3((unsigned)((BB + 255) / 256)), dim3(256), 0, stream(), B,
                        nact > 0 ? nsplit : 1, part, KC, K1, ldk);
     IRWLS_LAUNCH_CHECK();

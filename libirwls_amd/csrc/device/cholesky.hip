@@ -702,24 +702,36 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStr
     }
 }
 
-// Per batch shape (problem count, capacity): the captured graph of each tile
-// shape, the device table it reads, and the autotuned choice.
+// Per batch shape class: the captured graph of each tile shape and the
+// device table it reads.  A class is (problem slots, capacity): the problem
+// count rounded up to a multiple of 16 (unused slots hold empty problems,
+// which every kernel skips) and the largest size rounded up to 256, so the
+// graphs of one class serve every batch in it.  The tile shape is tuned once
+// per capacity, on the first batch that has it.
 struct BatchPlan {
-    int nprob = 0, ncap = 0;
+    int nslot = 0, ncap = 0;
     BatchProb *d_tab = nullptr;
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+};
+struct BatchTune {
+    int ncap = 0;
     int best = -1;            // index into kShapes
     double ms[3] = {0, 0, 0};
 };
 constexpr int kShapes[3] = {1, 2, 4};
-constexpr int kMaxPlans = 16;
+constexpr int kMaxPlans = 64;
+constexpr int kMaxTunes = 64;
 BatchPlan g_plans[kMaxPlans];
 int g_nplans = 0;
+BatchTune g_tunes[kMaxTunes];
+int g_ntunes = 0;
 int g_autotune = 1;
 
-BatchPlan *plan_for(int nprob, int ncap) {
+int slots_for(int nprob) { return (nprob + 15) / 16 * 16; }
+
+BatchPlan *plan_for(int nslot, int ncap) {
     for (int i = 0; i < g_nplans; i++)
-        if (g_plans[i].nprob == nprob && g_plans[i].ncap == ncap) return &g_plans[i];
+        if (g_plans[i].nslot == nslot && g_plans[i].ncap == ncap) return &g_plans[i];
     BatchPlan *pl;
     if (g_nplans < kMaxPlans) {
         pl = &g_plans[g_nplans++];
@@ -730,17 +742,26 @@ BatchPlan *plan_for(int nprob, int ncap) {
         irwls_dev_free(pl->d_tab);
         *pl = BatchPlan();
     }
-    pl->nprob = nprob;
+    pl->nslot = nslot;
     pl->ncap = ncap;
-    pl->d_tab = (BatchProb *)irwls_dev_alloc((size_t)nprob * sizeof(BatchProb));
+    pl->d_tab = (BatchProb *)irwls_dev_alloc((size_t)nslot * sizeof(BatchProb));
     return pl;
+}
+
+BatchTune *tune_for(int ncap) {
+    for (int i = 0; i < g_ntunes; i++)
+        if (g_tunes[i].ncap == ncap) return &g_tunes[i];
+    BatchTune *t = g_ntunes < kMaxTunes ? &g_tunes[g_ntunes++] : &g_tunes[0];
+    *t = BatchTune();
+    t->ncap = ncap;
+    return t;
 }
 
 hipGraphExec_t graph_for(BatchPlan *pl, int e) {
     if (pl->exec[e]) return pl->exec[e];
     hipGraph_t g;
     IRWLS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-    batched_sequence(pl->d_tab, pl->nprob, pl->ncap, kShapes[e], stream());
+    batched_sequence(pl->d_tab, pl->nslot, pl->ncap, kShapes[e], stream());
     IRWLS_HIP(hipStreamEndCapture(stream(), &g));
     IRWLS_HIP(hipGraphInstantiate(&pl->exec[e], g, nullptr, nullptr, 0));
     IRWLS_HIP(hipGraphDestroy(g));
@@ -749,8 +770,8 @@ hipGraphExec_t graph_for(BatchPlan *pl, int e) {
 
 // Times each tile shape on SPD test systems of the batch's sizes (in scratch
 // buffers of the same shapes) and keeps the fastest.
-void autotune(BatchPlan *pl, const BatchProb *host_tab) {
-    const int np = pl->nprob;
+void autotune(BatchPlan *pl, BatchTune *tu, const BatchProb *host_tab) {
+    const int np = pl->nslot;
     BatchProb *t = (BatchProb *)malloc((size_t)np * sizeof(BatchProb));
     for (int i = 0; i < np; i++) {
         t[i] = host_tab[i];
@@ -780,8 +801,8 @@ void autotune(BatchPlan *pl, const BatchProb *host_tab) {
             IRWLS_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best) best = ms;
         }
-        pl->ms[e] = best;
-        if (pl->best < 0 || best < pl->ms[pl->best]) pl->best = e;
+        tu->ms[e] = best;
+        if (tu->best < 0 || best < tu->ms[tu->best]) tu->best = e;
     }
     IRWLS_HIP(hipEventDestroy(e0));
     IRWLS_HIP(hipEventDestroy(e1));
@@ -859,63 +880,67 @@ int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const 
         }
         if (n[p] > nmax) nmax = n[p];
     }
-    const int ncap = (nmax + 255) / 256 * 256;   // shape class: graphs and tuning are shared within it
-    BatchPlan *pl = plan_for(nprob, ncap);
-    // per-problem tile inverses, info and forward-sweep vectors
+    const int ncap = (nmax + 255) / 256 * 256;
+    const int nslot = slots_for(nprob);
+    BatchPlan *pl = plan_for(nslot, ncap);
+    BatchTune *tu = tune_for(ncap);
+    // per-problem info words (together, read back with one copy), tile
+    // inverses and forward-sweep vectors
     static double *pool = nullptr;
     static size_t pool_sz = 0;
-    size_t need = 0;
+    size_t need = (size_t)nslot * 2 + 8;   // 4 info ints per slot
     for (int p = 0; p < nprob; p++) need += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE + (size_t)n[p] + 8;
     if (pool_sz < need) {
         if (pool) irwls_dev_free(pool);
         pool = (double *)irwls_dev_alloc(need * sizeof(double));
         pool_sz = need;
     }
-    BatchProb *t = (BatchProb *)malloc((size_t)nprob * sizeof(BatchProb));
-    size_t off = 0;
-    for (int p = 0; p < nprob; p++) {
+    int *d_info = (int *)pool;   // slot p: d_info[4 p]
+    BatchProb *t = (BatchProb *)calloc((size_t)nslot, sizeof(BatchProb));
+    size_t off = (size_t)nslot * 2 + 8;
+    for (int p = 0; p < nslot; p++) {
+        t[p].info = d_info + 4 * p;
+        if (p >= nprob) continue;   // empty slot: n = 0, skipped by every kernel
         t[p].A = A[p];
         t[p].n = n[p];
         t[p].lda = lda[p];
         t[p].Dinv = pool + off;
         off += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE;
-        t[p].info = (int *)(pool + off);
-        off += 4;
         t[p].z = pool + off;
-        off += (size_t)n[p] + 4;
+        off += (size_t)n[p] + 8;
         t[p].rhs = rhs[p];
         t[p].x = x[p];
     }
-    if (pl->best < 0) {
-        if (g_autotune) autotune(pl, t);
-        else pl->best = 0;
+    if (tu->best < 0) {
+        if (g_autotune) autotune(pl, tu, t);
+        else tu->best = 0;
     }
-    irwls_dev_h2d(pl->d_tab, t, (size_t)nprob * sizeof(BatchProb));
+    irwls_dev_h2d(pl->d_tab, t, (size_t)nslot * sizeof(BatchProb));
     {
         double fl = 0.0;
         for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
         TimerScope ts(IRWLS_T_SYRK, fl, 0.0);
-        IRWLS_HIP(hipGraphLaunch(graph_for(pl, pl->best), stream()));
+        IRWLS_HIP(hipGraphLaunch(graph_for(pl, tu->best), stream()));
     }
-    int rc = 0;
-    for (int p = 0; p < nprob; p++) {
-        int v = 0;
-        irwls_dev_d2h(&v, t[p].info, sizeof(int));
-        if (info) info[p] = v;
-    }
+    int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));
+    irwls_dev_d2h(hinfo, d_info, (size_t)nslot * 4 * sizeof(int));
+    if (info)
+        for (int p = 0; p < nprob; p++) info[p] = hinfo[4 * p];
+    free(hinfo);
     free(t);
-    return rc;
+    return 0;
 }
 
 void irwls_dev_batched_autotune(int enable) { g_autotune = enable != 0; }
 
 int irwls_dev_batched_choice(int nprob, int nmax, double *ms3) {
+    (void)nprob;
     const int ncap = (nmax + 255) / 256 * 256;
-    for (int i = 0; i < g_nplans; i++)
-        if (g_plans[i].nprob == nprob && g_plans[i].ncap == ncap) {
+    for (int i = 0; i < g_ntunes; i++)
+        if (g_tunes[i].ncap == ncap) {
             if (ms3)
-                for (int e = 0; e < 3; e++) ms3[e] = g_plans[i].ms[e];
-            return g_plans[i].best >= 0 ? kShapes[g_plans[i].best] : 0;
+                for (int e = 0; e < 3; e++) ms3[e] = g_tunes[i].ms[e];
+            return g_tunes[i].best >= 0 ? kShapes[g_tunes[i].best] : 0;
         }
     return 0;
 }

@@ -2,6 +2,7 @@
 #include "common.h"
 
 #include <string.h>
+#include <time.h>
 
 namespace irwls {
 
@@ -177,6 +178,33 @@ static const size_t kStage = 16u << 20;
 static const size_t kStageMax = 1u << 20;   // larger copies go directly and wait
 static thread_local char *g_stage = nullptr;
 static thread_local size_t g_stage_head = 0;
+// Worker threads of a batched run wait for their stream through an event
+// created with hipEventBlockingSync: the thread sleeps until the device
+// signals instead of spinning, so many waiting workers leave the host cores
+// to the ones with host work.  They also read back through a pinned buffer
+// of their own (kBounce bytes) rather than the runtime's pageable path.
+static thread_local hipEvent_t t_sync_event = nullptr;
+static thread_local char *t_bounce = nullptr;
+static const size_t kBounce = (size_t)1 << 20;
+
+static thread_local double t_wait_s = 0.0;
+
+static double now_s() {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void wait_stream() {
+    const double t0 = now_s();
+    if (t_sync_event) {
+        IRWLS_HIP(hipEventRecord(t_sync_event, stream()));
+        IRWLS_HIP(hipEventSynchronize(t_sync_event));
+    } else {
+        IRWLS_HIP(hipStreamSynchronize(stream()));
+    }
+    t_wait_s += now_s() - t0;
+}
 
 static void stage_reset() { g_stage_head = 0; }
 
@@ -185,12 +213,12 @@ void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
     if (!g_stage) IRWLS_HIP(hipHostMalloc((void **)&g_stage, kStage, hipHostMallocDefault));
     if (bytes > kStageMax) {
         IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
-        IRWLS_HIP(hipStreamSynchronize(stream()));
+        wait_stream();
         stage_reset();
         return;
     }
     if (g_stage_head + bytes > kStage) {
-        IRWLS_HIP(hipStreamSynchronize(stream()));
+        wait_stream();
         stage_reset();
     }
     char *slot = g_stage + g_stage_head;
@@ -201,8 +229,14 @@ void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
 
 void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
-    IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
-    IRWLS_HIP(hipStreamSynchronize(stream()));
+    if (t_bounce && bytes <= kBounce) {
+        IRWLS_HIP(hipMemcpyAsync(t_bounce, src, bytes, hipMemcpyDeviceToHost, stream()));
+        wait_stream();
+        memcpy(dst, t_bounce, bytes);
+    } else {
+        IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+        wait_stream();
+    }
     stage_reset();
 }
 
@@ -223,7 +257,7 @@ void irwls_dev_memset0(void *dst, size_t bytes) {
 }
 
 void irwls_dev_sync(void) {
-    IRWLS_HIP(hipStreamSynchronize(stream()));
+    wait_stream();
     stage_reset();
 }
 
@@ -267,10 +301,18 @@ void irwls_dev_release_thread_buffers(void) {
     irwls_dev_release_mlp_buffers();
 }
 
+double irwls_dev_wait_seconds(int reset) {
+    const double w = t_wait_s;
+    if (reset) t_wait_s = 0.0;
+    return w;
+}
+
 void irwls_dev_thread_begin(void) {
     irwls_dev_init();
     if (t_stream) return;
     IRWLS_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+    IRWLS_HIP(hipEventCreateWithFlags(&t_sync_event, hipEventBlockingSync | hipEventDisableTiming));
+    IRWLS_HIP(hipHostMalloc((void **)&t_bounce, kBounce, hipHostMallocDefault));
 }
 
 void irwls_dev_thread_end(void) {
@@ -285,6 +327,10 @@ void irwls_dev_thread_end(void) {
     if (g_stage) IRWLS_HIP(hipHostFree(g_stage));
     g_stage = nullptr;
     g_stage_head = 0;
+    IRWLS_HIP(hipHostFree(t_bounce));
+    t_bounce = nullptr;
+    IRWLS_HIP(hipEventDestroy(t_sync_event));
+    t_sync_event = nullptr;
     IRWLS_HIP(hipStreamDestroy(t_stream));
     t_stream = nullptr;
 }

@@ -218,16 +218,27 @@ __global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
     double acc[NARROW];
 #pragma unroll
     for (int c = 0; c < NARROW; c++) acc[c] = 0.0;
-    for (int64_t p = p0; p < p1; p++) {
-        const int f = col[p];
-        if (f >= d) continue;
-        const unsigned m = (unsigned)cmask[f];
-        if (m == 0u) continue;
-        const double v = val[p];
-        const double *xr = XcT + (size_t)f * ldc;
+    // four nonzeros per step: their columns, values and masks load together
+    for (int64_t p = p0; p < p1; p += 4) {
+        int f[4];
+        double v[4];
+        unsigned m[4];
 #pragma unroll
-        for (int c = 0; c < NARROW; c++)
-            if ((m >> c) & 1u) acc[c] = fma(v, xr[c], acc[c]);
+        for (int u = 0; u < 4; u++) {
+            const bool ok = p + u < p1;
+            f[u] = ok ? col[p + u] : d;
+            v[u] = ok ? val[p + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) m[u] = f[u] < d ? (unsigned)cmask[f[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (m[u] == 0u) continue;
+            const double *xr = XcT + (size_t)f[u] * ldc;
+#pragma unroll
+            for (int c = 0; c < NARROW; c++)
+                if ((m[u] >> c) & 1u) acc[c] = fma(v[u], xr[c], acc[c]);
+        }
     }
     const double nr = rnrm ? rnrm[src] : 0.0;
 #pragma unroll
@@ -250,12 +261,13 @@ __global__ void kmv_finish_kernel(const double *__restrict__ part, int n, int ns
 // XcT is d x ldc; cmask[(j / 64) * d + f] has bit j % 64 set when column j's
 // feature f is nonzero.  One wave per row, lanes over 64 columns per chunk.
 // The row's nonzeros and their features' masks are fetched 64 at a time with
-// one coalesced load each and broadcast lane by lane from registers; a
-// nonzero whose feature no column of the chunk has is skipped by the whole
-// wave, and otherwise only the lanes whose column has it load XcT.  The
-// products are summed in nonzero order, one fused multiply-add each; the
-// skipped products are exact zeros, so the sum is that of the full
-// dense-column dot product, bit for bit.
+// one coalesced load each; the nonzeros whose feature some column of the
+// chunk has (a ballot over the masks) are then visited in order, four at a
+// time: the lanes whose column has the feature load their XcT entries (four
+// independent loads in flight), the others take 0, and the four products
+// are added in nonzero order, one fused multiply-add each.  The skipped
+// products are exact zeros, so the sum is that of the full dense-column dot
+// product, bit for bit.
 __device__ __forceinline__ double csr_row_dot(const int32_t *__restrict__ col, const double *__restrict__ val,
                                               int64_t p0, int64_t p1, const double *__restrict__ XcT, int ldc,
                                               int d, const unsigned long long *__restrict__ cmask, int j) {
@@ -271,14 +283,36 @@ __device__ __forceinline__ double csr_row_dot(const int32_t *__restrict__ col, c
             vl = val[b + lane];
             if (fl < d) ml = cmask[fl];
         }
-        for (int q = 0; q < cnt; q++) {
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)ml, q);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(ml >> 32), q);
-            if ((lo | hi) == 0u) continue;
-            const unsigned mine = lane < 32 ? (lo >> lane) : (hi >> (lane - 32));
-            const int f = __builtin_amdgcn_readlane(fl, q);
-            const double v = rdl(vl, q);
-            if (mine & 1u) dot = fma(v, XcT[(size_t)f * ldc + j], dot);
+        unsigned long long act = __ballot(ml != 0ull);
+        while (act) {
+            int qs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (act) {
+                    qs[u] = __builtin_ctzll(act);
+                    act &= act - 1;
+                } else {
+                    qs[u] = -1;
+                }
+            }
+            double x[4], v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                x[u] = 0.0;
+                v[u] = 0.0;
+                if (qs[u] >= 0) {
+                    const int q = qs[u];
+                    const unsigned lo = __builtin_amdgcn_readlane((unsigned)ml, q);
+                    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(ml >> 32), q);
+                    const unsigned bit = lane < 32 ? (lo >> lane) : (hi >> (lane - 32));
+                    const int f = __builtin_amdgcn_readlane(fl, q);
+                    v[u] = rdl(vl, q);
+                    if (bit & 1u) x[u] = XcT[(size_t)f * ldc + j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (qs[u] >= 0) dot = fma(v[u], x[u], dot);
         }
     }
     return dot;

@@ -54,24 +54,41 @@ __global__ __launch_bounds__(256) void gemv_gather_kernel(
 }
 
 // y = A x (trans 0) or A^T x (trans 1) for a column-major m x n matrix A.
-__global__ __launch_bounds__(256) void gemv_cm_kernel(const double *__restrict__ A, int m, int n, int lda, int trans,
-                                                      const double *__restrict__ x, double *__restrict__ y) {
-    if (!trans) {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= m) return;
-        double acc = 0.0;
-        for (int j = 0; j < n; j++) acc += A[(size_t)j * lda + i] * x[j];
-        y[i] = acc;
-    } else {
-        const int lane = threadIdx.x & 63;
-        const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-        if (j >= n) return;
-        const double *col = A + (size_t)j * lda;
-        double acc = 0.0;
-        for (int i = lane; i < m; i += 64) acc += col[i] * x[i];
-        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) y[j] = acc;
-    }
+// trans 0: lanes over 64 rows; the columns are cut into nsl slices, one per
+// wave (four per work-group along y), each summed in column order into
+// part[slice * m + i]; gemv_finish adds the slices in slice order.
+__global__ __launch_bounds__(256) void gemv_n_kernel(const double *__restrict__ A, int m, int n, int lda,
+                                                     const double *__restrict__ x, int per,
+                                                     double *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    const int slice = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int j0 = slice * per, j1 = min(n, j0 + per);
+    if (i >= m) return;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int j = j0; j < j1; j++) acc = fma(A[(size_t)j * lda + i], x[j], acc);
+    part[(size_t)slice * m + i] = acc;
+}
+
+__global__ void gemv_finish(const double *__restrict__ part, int m, int nsl, double *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int k = 0; k < nsl; k++) s += part[(size_t)k * m + i];
+    y[i] = s;
+}
+
+__global__ __launch_bounds__(256) void gemv_t_kernel(const double *__restrict__ A, int m, int n, int lda,
+                                                     const double *__restrict__ x, double *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    const double *col = A + (size_t)j * lda;
+    double acc = 0.0;
+    for (int i = lane; i < m; i += 64) acc += col[i] * x[i];
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) y[j] = acc;
 }
 
 }  // namespace
@@ -81,10 +98,19 @@ extern "C" {
 void irwls_dev_gemv(const double *A, int m, int n, int lda, int trans, const double *x, double *y) {
     if (m <= 0 || n <= 0) return;
     TimerScope ts(IRWLS_T_GEMV, 2.0 * m * (double)n, 8.0 * m * (double)n);
-    if (!trans)
-        hipLaunchKernelGGL(gemv_cm_kernel, dim3((m + 255) / 256), dim3(256), 0, stream(), A, m, n, lda, 0, x, y);
-    else
-        hipLaunchKernelGGL(gemv_cm_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), A, m, n, lda, 1, x, y);
+    if (!trans) {
+        // at most 32 columns per wave
+        const int groups = n <= 128 ? 1 : min(16, (n + 127) / 128);
+        const int nsl = 4 * groups;
+        const int per = (n + nsl - 1) / nsl;
+        double *part = (double *)scratch(0, (size_t)nsl * m * sizeof(double));
+        hipLaunchKernelGGL(gemv_n_kernel, dim3((m + 63) / 64, groups), dim3(256), 0, stream(), A, m, n, lda, x, per,
+                           part);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(gemv_finish, dim3((m + 255) / 256), dim3(256), 0, stream(), part, m, nsl, y);
+    } else {
+        hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), A, m, n, lda, x, y);
+    }
     IRWLS_LAUNCH_CHECK();
 }
 

@@ -119,8 +119,10 @@ static void *worker(void *arg) {
     t_sched = s;
     t_slot = j->slot;
     t_rng = &j->rd;
+    (void)irwls_dev_wait_seconds(1);
     j->out = trainFULL(j->ds, j->props);
     irwls_gpu_last_stats(&j->stats);
+    j->stats.t_wait = irwls_dev_wait_seconds(1);
     t_sched = NULL;
     t_rng = NULL;
     irwls_mixed_release();
@@ -176,6 +178,7 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
         tot.pcg_iters += jobs[p].stats.pcg_iters;
         tot.fp64_fallbacks += jobs[p].stats.fp64_fallbacks;
         tot.t_solve += jobs[p].stats.t_solve;
+        tot.t_wait += jobs[p].stats.t_wait;
     }
     tot.batch_rounds = s.rounds;
     tot.t_total = irwls_now() - t0;

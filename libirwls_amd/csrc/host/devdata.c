@@ -328,31 +328,62 @@ void irwls_ws_reserve(irwls_ws *ws, int cap) {
     ws->cap = (int)c;
     ws->Kw = (double *)irwls_dev_alloc(c * c * sizeof(double));
     ws->H = (double *)irwls_dev_alloc((c + 2) * (c + 1) * sizeof(double));
-    ws->rhs = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
     ws->esub = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
-    ws->w = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
     ws->G13 = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
-    ws->ys = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
-    ws->as = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
-    ws->rs = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
-    ws->S1 = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
-    ws->S3 = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
-    ws->cols = (int32_t *)irwls_dev_alloc((c + 1) * sizeof(int32_t));
+    const size_t ni = (c + 2) / 2, nd = c + 1;   /* c + 1 int32 or double entries, in doubles */
+    ws->span_len[0] = ni + 3 * nd;
+    ws->span_len[1] = ni + nd;
+    ws->span_len[2] = 2 * ni + 2 * nd;
+    ws->span_off[0] = 0;
+    ws->span_off[1] = ws->span_len[0];
+    ws->span_off[2] = ws->span_len[0] + ws->span_len[1];
+    const size_t tot = ws->span_off[2] + ws->span_len[2];
+    ws->pk = (double *)irwls_dev_alloc(tot * sizeof(double));
+    ws->hpk = (double *)irwls_xcalloc(tot, sizeof(double));
+    double *d = ws->pk, *h = ws->hpk;
+    size_t o = 0;
+    ws->S1 = (int32_t *)(d + o);
+    ws->hS1 = (int32_t *)(h + o);
+    o += ni;
+    ws->ys = d + o;
+    ws->hys = h + o;
+    o += nd;
+    ws->as = d + o;
+    ws->has = h + o;
+    o += nd;
+    ws->rhs = d + o;
+    ws->hrhs = h + o;
+    o += nd;
+    ws->cols = (int32_t *)(d + o);
+    ws->hcols = (int32_t *)(h + o);
+    o += ni;
+    ws->w = d + o;
+    ws->hw = h + o;
+    o += nd;
+    ws->S1g = (int32_t *)(d + o);
+    ws->hS1g = (int32_t *)(h + o);
+    o += ni;
+    ws->S3 = (int32_t *)(d + o);
+    ws->hS3 = (int32_t *)(h + o);
+    o += ni;
+    ws->wg = d + o;
+    ws->hwg = h + o;
+    o += nd;
+    ws->rs = d + o;
+    ws->hrs = h + o;
+}
+
+void irwls_ws_upload(irwls_ws *ws, int span) {
+    irwls_dev_h2d(ws->pk + ws->span_off[span], ws->hpk + ws->span_off[span], ws->span_len[span] * sizeof(double));
 }
 
 void irwls_ws_free(irwls_ws *ws) {
     if (ws->cap == 0) return;
     irwls_dev_free(ws->Kw);
     irwls_dev_free(ws->H);
-    irwls_dev_free(ws->rhs);
     irwls_dev_free(ws->esub);
-    irwls_dev_free(ws->w);
     irwls_dev_free(ws->G13);
-    irwls_dev_free(ws->ys);
-    irwls_dev_free(ws->as);
-    irwls_dev_free(ws->rs);
-    irwls_dev_free(ws->S1);
-    irwls_dev_free(ws->S3);
-    irwls_dev_free(ws->cols);
+    irwls_dev_free(ws->pk);
+    free(ws->hpk);
     memset(ws, 0, sizeof(*ws));
 }

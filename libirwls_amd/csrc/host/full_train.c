@@ -52,8 +52,7 @@ static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x, const in
         if (!done && N - 1 >= 128) g_stats.fp64_fallbacks++;
     }
     if (!done) {
-        irwls_dev_h2d(ws->rhs, et, (size_t)N * sizeof(double));
-        /* failure code ignored, as the reference; in a batched run the solve
+        /* the right-hand side went up with the KKT span; failure code ignored, as the reference; in a batched run the solve
            joins the other problems' pending solves */
         if (!irwls_batch_solve(ws->H, N, N + 1, ws->rhs, ws->w)) (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);
         irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
@@ -108,9 +107,11 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
             et[i] = 1.0 - G13[i] - GIN[S1[i]];
         }
         et[nS1] = -G13[nS1] - GIN[l];
-        irwls_dev_h2d(ws->S1, S1, (size_t)N * sizeof(int32_t));
-        irwls_dev_h2d(ws->ys, hbuf, (size_t)N * sizeof(double));
-        irwls_dev_h2d(ws->as, hbuf2, (size_t)N * sizeof(double));
+        memcpy(ws->hS1, S1, (size_t)N * sizeof(int32_t));
+        memcpy(ws->hys, hbuf, (size_t)N * sizeof(double));
+        memcpy(ws->has, hbuf2, (size_t)N * sizeof(double));
+        memcpy(ws->hrhs, et, (size_t)N * sizeof(double));
+        irwls_ws_upload(ws, 0);
         irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
         memset(bAux, 0, (size_t)N * sizeof(double));
         kkt_solve(ws, N, et, bAux, S1, hbuf, gen);
@@ -142,8 +143,9 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
                 nc++;
             }
         }
-        irwls_dev_h2d(ws->cols, cols, (size_t)nc * sizeof(int32_t));
-        irwls_dev_h2d(ws->w, hbuf, (size_t)nc * sizeof(double));
+        memcpy(ws->hcols, cols, (size_t)nc * sizeof(int32_t));
+        memcpy(ws->hw, hbuf, (size_t)nc * sizeof(double));
+        irwls_ws_upload(ws, 1);
         irwls_dev_gemv_gather(ws->Kw, l, NULL, l, ws->cols, nc, ws->w, 1.0, -1.0, -(bNew[l] - beta[l]), NULL,
                               ws->esub);
         irwls_dev_d2h(e, ws->esub, (size_t)l * sizeof(double));
@@ -187,11 +189,13 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
             }
             for (int i = 0; i < nS1; i++) hbuf2[i] = y[S1[i]];
             if (nS1 > 0) {
-                irwls_dev_h2d(ws->S1, S1, (size_t)nS1 * sizeof(int32_t));
-                irwls_dev_h2d(ws->S3, S3, (size_t)nS3 * sizeof(int32_t));
-                irwls_dev_h2d(ws->w, hbuf, (size_t)nS3 * sizeof(double));
-                irwls_dev_h2d(ws->rs, hbuf2, (size_t)nS1 * sizeof(double));
-                irwls_dev_gemv_gather(ws->Kw, l, ws->S1, nS1, ws->S3, nS3, ws->w, 0.0, 1.0, 0.0, ws->rs, ws->G13);
+                memcpy(ws->hS1g, S1, (size_t)nS1 * sizeof(int32_t));
+                memcpy(ws->hS3, S3, (size_t)nS3 * sizeof(int32_t));
+                memcpy(ws->hwg, hbuf, (size_t)nS3 * sizeof(double));
+                memcpy(ws->hrs, hbuf2, (size_t)nS1 * sizeof(double));
+                irwls_ws_upload(ws, 2);
+                irwls_dev_gemv_gather(ws->Kw, l, ws->S1g, nS1, ws->S3, nS3, ws->wg, 0.0, 1.0, 0.0, ws->rs,
+                                      ws->G13);
                 irwls_dev_d2h(G13, ws->G13, (size_t)nS1 * sizeof(double));
             }
             G13[nS1] = g;
@@ -215,11 +219,13 @@ double *subIRWLS(svm_dataset dataset, properties props, double *GIN, double *e, 
     const double t_start = irwls_now();
     irwls_dev_init();
     memset(&g_stats, 0, sizeof(g_stats));
+    const double wait0 = irwls_dev_wait_seconds(0);
     irwls_devdata *dd = irwls_devdata_acquire(&dataset, 0, -1, 0);
     irwls_ws ws;
     memset(&ws, 0, sizeof(ws));
     double *b = irwls_sub_irwls_dev(&ws, dd, NULL, dataset.l, dataset.y, props, GIN, e, beta);
     g_stats.t_total = irwls_now() - t_start;
+    g_stats.t_wait = irwls_dev_wait_seconds(0) - wait0;
     irwls_ws_free(&ws);
     irwls_devdata_release(dd);
     return b;
@@ -228,6 +234,7 @@ double *subIRWLS(svm_dataset dataset, properties props, double *GIN, double *e, 
 double *trainFULL(svm_dataset dataset, properties props) {
     const double t_start = irwls_now();
     memset(&g_stats, 0, sizeof(g_stats));
+    const double wait0 = irwls_dev_wait_seconds(0);
     if (props.verbose == 1) printf("\n");
     irwls_dev_init();
     const int n = dataset.l, W = props.MaxSize;
@@ -409,6 +416,7 @@ double *trainFULL(svm_dataset dataset, properties props) {
     free(wv);
     if (props.verbose == 1) printf("\n");
     g_stats.t_total = irwls_now() - t_start;
+    g_stats.t_wait = irwls_dev_wait_seconds(0) - wait0;
     return bNew;   /* the last iterate, as the reference returns (src/full-train.c:666) */
 }
 

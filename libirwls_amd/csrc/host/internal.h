@@ -75,12 +75,23 @@ double irwls_now(void);
 /* Device work buffers of the full trainer's working-set IRWLS. */
 typedef struct irwls_ws {
     int cap;                   /* capacity in working-set rows */
-    double *Kw, *H, *rhs, *esub, *w, *G13, *ys, *as, *rs;
-    int32_t *S1, *S3, *cols;
+    double *Kw, *H, *esub, *G13;
+    /* The small per-iteration vectors live in three contiguous device spans,
+       each uploaded with one copy from a host mirror of the same layout
+       (irwls_ws_upload): span 0, the KKT system (S1, ys, as, rhs); span 1,
+       the error update (cols, w); span 2, G13 (S1g, S3, wg, rs).  w also
+       receives the dense solve's solution. */
+    double *pk, *hpk;
+    size_t span_off[3], span_len[3];      /* in doubles */
+    int32_t *S1, *cols, *S1g, *S3;
+    double *ys, *as, *rhs, *w, *wg, *rs;
+    int32_t *hS1, *hcols, *hS1g, *hS3;
+    double *hys, *has, *hrhs, *hw, *hwg, *hrs;
 } irwls_ws;
 
 void irwls_ws_reserve(irwls_ws *ws, int cap);
 void irwls_ws_free(irwls_ws *ws);
+void irwls_ws_upload(irwls_ws *ws, int span);
 
 /* Inner IRWLS on a working set whose rows are `wrows` (device view).  Same
    contract as subIRWLS (src/full-train.c:96-362): y = labels of the working

@@ -31,7 +31,7 @@ class irwls_gpu_stats(C.Structure):
                 ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double),
                 ("t_solve", C.c_double), ("factorizations", C.c_int), ("pcg_iters", C.c_int),
                 ("fp64_fallbacks", C.c_int), ("learned_steps", C.c_int), ("mlp_loss0", C.c_double),
-                ("mlp_loss", C.c_double), ("batch_rounds", C.c_int)]
+                ("mlp_loss", C.c_double), ("batch_rounds", C.c_int), ("t_wait", C.c_double)]
 
 
 class irwls_rows(C.Structure):

@@ -82,14 +82,15 @@ def test_random_centroids_same_stream():
     assert list(a) == list(b)
 
 
-def test_gemm_f64_all_transposes():
+@pytest.mark.parametrize("m,n,k,beta", [(130, 70, 45, -0.5), (200, 59, 700, -0.5), (37, 5, 1000, 0.0)])
+def test_gemm_f64_all_transposes(m, n, k, beta):
+    """Tile path (small k) and the k-split path (few tiles, long k)."""
     import ctypes as C
     from gpu_util import Dev
     L = lib.load()
     L.irwls_dev_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int,
                                  C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int]
     rng = np.random.default_rng(35)
-    m, n, k = 130, 70, 45
     for ta in (0, 1):
         for tb in (0, 1):
             A = rng.standard_normal((k, m) if ta else (m, k))
@@ -97,8 +98,30 @@ def test_gemm_f64_all_transposes():
             Cm = rng.standard_normal((m, n))
             Ad, Bd = Dev(np.asfortranarray(A).ravel(order="F")), Dev(np.asfortranarray(B).ravel(order="F"))
             Cd = Dev(np.asfortranarray(Cm).ravel(order="F"))
-            L.irwls_dev_gemm(ta, tb, m, n, k, 1.5, Ad.p, A.shape[0], Bd.p, B.shape[0], -0.5, Cd.p, m)
+            L.irwls_dev_gemm(ta, tb, m, n, k, 1.5, Ad.p, A.shape[0], Bd.p, B.shape[0], beta, Cd.p, m)
             opA = A.T if ta else A
             opB = B.T if tb else B
-            want = 1.5 * opA @ opB - 0.5 * Cm
-            np.testing.assert_allclose(Cd.get().reshape((m, n), order="F"), want, rtol=1e-12, atol=1e-12)
+            want = 1.5 * opA @ opB + beta * Cm
+            np.testing.assert_allclose(Cd.get().reshape((m, n), order="F"), want, rtol=1e-12,
+                                       atol=1e-13 * np.sqrt(k) * np.abs(want).max())
+
+
+@pytest.mark.parametrize("m,n", [(5, 3), (300, 100), (1000, 1000), (70, 2100)])
+def test_gemv_both_orientations(m, n):
+    """y = A x and y = A^T x for a column-major A (ld > m), the untransposed
+    product in column slices (more than 128 columns) and in one pass."""
+    import ctypes as C
+    from gpu_util import Dev
+    L = lib.load()
+    L.irwls_dev_gemv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(m + n)
+    lda = m + 3
+    A = rng.standard_normal((lda, n))
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    x, xt = rng.standard_normal(n), rng.standard_normal(m)
+    xd, xtd = Dev(x), Dev(xt)
+    yd, ytd = Dev(np.zeros(m)), Dev(np.zeros(n))
+    L.irwls_dev_gemv(Ad.p, m, n, lda, 0, xd.p, yd.p)
+    L.irwls_dev_gemv(Ad.p, m, n, lda, 1, xtd.p, ytd.p)
+    np.testing.assert_allclose(yd.get(), A[:m] @ x, rtol=1e-12, atol=1e-13 * np.sqrt(n))
+    np.testing.assert_allclose(ytd.get(), A[:m].T @ xt, rtol=1e-12, atol=1e-13 * np.sqrt(m))
