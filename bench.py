@@ -146,6 +146,9 @@ def roofline(r, steps):
     t = r["tm"][r["dom"]]
     avg_ms = t["ms"] / max(t["launches"], 1)
     dom = r["dom"]
+    if avg_ms <= 0.0:
+        return {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
+                "kernel": dom, "note": "no timed launches of the dominant kernel"}
     if dom in MFMA_KINDS:
         peak = PEAK_BF16_TFLOPS if dom == "bf16" else PEAK_FP64_TFLOPS
         ach = (t["flops"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e12

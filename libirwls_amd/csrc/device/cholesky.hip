@@ -919,7 +919,7 @@ int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const 
     {
         double fl = 0.0;
         for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
-        TimerScope ts(IRWLS_T_SYRK, fl, 0.0);
+        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true);   // one batch runs at a time
         IRWLS_HIP(hipGraphLaunch(graph_for(pl, tu->best), stream()));
     }
     int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));

@@ -42,7 +42,9 @@ struct TimerScope {
     int kind;
     double flops, bytes;
     bool on;
-    TimerScope(int kind, double flops, double bytes);
+    // any_thread: also on worker threads (for a launch that only one thread
+    // at a time can make, such as a batched solve)
+    TimerScope(int kind, double flops, double bytes, bool any_thread = false);
     ~TimerScope();
 };
 

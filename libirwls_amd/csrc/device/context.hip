@@ -43,8 +43,8 @@ static void resolve_pool() {
     g_evn = 0;
 }
 
-TimerScope::TimerScope(int k, double f, double b)
-    : kind(k), flops(f), bytes(b), on(g_timing && !t_stream && ((g_timing_mask >> k) & 1u)) {
+TimerScope::TimerScope(int k, double f, double b, bool any_thread)
+    : kind(k), flops(f), bytes(b), on(g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)) {
     if (!on) return;
     if (g_evn == kPool) resolve_pool();
     IRWLS_HIP(hipEventRecord(g_evs[g_evn][0], stream()));

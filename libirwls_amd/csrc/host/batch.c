@@ -8,9 +8,14 @@
  * assembly and vector updates overlap on the device.  A problem that
  * reaches its KKT solve (src/full-train.c:192-199) finishes its stream's
  * work, queues the solve and waits; when every problem still running is
- * waiting, one batched factorization and solve runs for all of them
+ * waiting, or when the earliest waiting one has waited a set time
+ * (IRWLS_BATCH_WAIT_US, default 1000 microseconds) and no batch is running,
+ * one batched factorization and solve runs for the waiting ones
  * (irwls_dev_posv_aug_batched: one replay of a captured launch sequence,
- * tile shape chosen by its autotuner), and they continue.
+ * tile shape chosen by its autotuner), and they continue; problems that
+ * queue while it runs join the next batch.  The time limit keeps a problem
+ * busy with a longer step (an outer iteration's kernel products) from
+ * holding back the others.
  * The per-problem arithmetic is that of the standalone solve, so each
  * problem's result is bit-identical to its standalone trainFULL run.
  *
@@ -24,6 +29,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "internal.h"
 #include "full-train.h"
@@ -32,7 +38,10 @@ typedef struct {
     pthread_mutex_t mu;
     pthread_cond_t cv;
     int nprob, active, npend, rounds;
-    int *pending;
+    int running;          /* a batch is being solved (lock released meanwhile) */
+    double wait_s;        /* longest wait before a partial batch runs */
+    int *pending;         /* 0 idle, 1 queued, 2 in the running batch */
+    double *t_queue;      /* when each queued problem queued */
     double **A;
     int *n, *lda;
     const double **rhs;
@@ -63,27 +72,35 @@ int irwls_rand(void) {
     return rand();
 }
 
-/* Runs the pending solves (called with the lock held). */
+/* Runs the queued solves.  Called with the lock held and no batch running;
+   releases the lock during the device work and returns with it held. */
 static void run_batch(batch_sched *s) {
-    int k = 0;
-    double **A = (double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
-    int *n = (int *)irwls_xmalloc((size_t)s->npend * sizeof(int));
-    int *lda = (int *)irwls_xmalloc((size_t)s->npend * sizeof(int));
-    const double **rhs = (const double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
-    double **x = (double **)irwls_xmalloc((size_t)s->npend * sizeof(double *));
-    for (int i = 0; i < s->nprob; i++) {
-        if (!s->pending[i]) continue;
-        A[k] = s->A[i];
-        n[k] = s->n[i];
-        lda[k] = s->lda[i];
-        rhs[k] = s->rhs[i];
-        x[k] = s->x[i];
-        k++;
+    const int k = s->npend;
+    double **A = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
+    int *n = (int *)irwls_xmalloc((size_t)k * sizeof(int));
+    int *lda = (int *)irwls_xmalloc((size_t)k * sizeof(int));
+    const double **rhs = (const double **)irwls_xmalloc((size_t)k * sizeof(double *));
+    double **x = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
+    int m = 0;
+    for (int i = 0; i < s->nprob && m < k; i++) {
+        if (s->pending[i] != 1) continue;
+        A[m] = s->A[i];
+        n[m] = s->n[i];
+        lda[m] = s->lda[i];
+        rhs[m] = s->rhs[i];
+        x[m] = s->x[i];
+        s->pending[i] = 2;
+        m++;
     }
-    irwls_dev_posv_aug_batched(k, A, n, lda, rhs, x, NULL);
-    irwls_dev_sync();
-    for (int i = 0; i < s->nprob; i++) s->pending[i] = 0;
     s->npend = 0;
+    s->running = 1;
+    pthread_mutex_unlock(&s->mu);
+    irwls_dev_posv_aug_batched(m, A, n, lda, rhs, x, NULL);
+    irwls_dev_sync();
+    pthread_mutex_lock(&s->mu);
+    for (int i = 0; i < s->nprob; i++)
+        if (s->pending[i] == 2) s->pending[i] = 0;
+    s->running = 0;
     s->rounds++;
     free(A);
     free(n);
@@ -91,6 +108,14 @@ static void run_batch(batch_sched *s) {
     free(rhs);
     free(x);
     pthread_cond_broadcast(&s->cv);
+}
+
+/* Earliest queue time of the queued problems (s->npend > 0). */
+static double first_queued(const batch_sched *s) {
+    double t = 1e300;
+    for (int i = 0; i < s->nprob; i++)
+        if (s->pending[i] == 1 && s->t_queue[i] < t) t = s->t_queue[i];
+    return t;
 }
 
 int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x) {
@@ -105,9 +130,30 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
     s->rhs[me] = d_rhs;
     s->x[me] = d_x;
     s->pending[me] = 1;
+    s->t_queue[me] = irwls_now();
     s->npend++;
-    if (s->npend == s->active) run_batch(s);
-    while (s->pending[me]) pthread_cond_wait(&s->cv, &s->mu);
+    while (s->pending[me] != 0) {
+        if (!s->running && s->npend > 0) {
+            const double due = first_queued(s) + s->wait_s;
+            if (s->npend == s->active || irwls_now() >= due) {
+                run_batch(s);
+                continue;
+            }
+            struct timespec ts;
+            clock_gettime(CLOCK_REALTIME, &ts);
+            const double left = due - irwls_now();
+            const long ns = (long)((left > 0.0 ? left : 0.0) * 1e9) + 1000;
+            ts.tv_sec += ns / 1000000000L;
+            ts.tv_nsec += ns % 1000000000L;
+            if (ts.tv_nsec >= 1000000000L) {
+                ts.tv_sec++;
+                ts.tv_nsec -= 1000000000L;
+            }
+            pthread_cond_timedwait(&s->cv, &s->mu, &ts);
+        } else {
+            pthread_cond_wait(&s->cv, &s->mu);
+        }
+    }
     pthread_mutex_unlock(&s->mu);
     return 1;
 }
@@ -129,7 +175,7 @@ static void *worker(void *arg) {
     irwls_dev_thread_end();
     pthread_mutex_lock(&s->mu);
     s->active--;
-    if (s->npend > 0 && s->npend == s->active) run_batch(s);
+    pthread_cond_broadcast(&s->cv);   /* the waiting ones may now be all that is left */
     pthread_mutex_unlock(&s->mu);
     return NULL;
 }
@@ -145,6 +191,11 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
     s.nprob = nprob;
     s.active = nprob;
     s.pending = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
+    s.t_queue = (double *)irwls_xcalloc((size_t)nprob, sizeof(double));
+    {
+        const char *w = getenv("IRWLS_BATCH_WAIT_US");
+        s.wait_s = (w && *w) ? atof(w) * 1e-6 : 1e-3;
+    }
     s.A = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
     s.n = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
     s.lda = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
@@ -186,6 +237,7 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
     pthread_cond_destroy(&s.cv);
     pthread_mutex_destroy(&s.mu);
     free(s.pending);
+    free(s.t_queue);
     free(s.A);
     free(s.n);
     free(s.lda);
