@@ -142,7 +142,38 @@ def run_mode(lib, L, ds, props, precision, warmup, steps):
     return dict(times=times, value=float(np.mean(times)), beta=beta, stats=stats, tm=tm, tw=tw, dom=dom)
 
 
-def roofline(r, steps):
+def measured_traffic(workload, mode, kind):
+    """HBM bytes per launch of a kernel kind, from the committed summary of
+    the two counter runs (FETCH_SIZE, WRITE_SIZE) for this workload and mode,
+    calibrated against the copy probe; None when there is no such summary."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_%s_%s_%s.md" % (kind, mode, workload))
+    try:
+        with open(path) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def copy_probe():
+    """The calibration run for the traffic counters: five launches of the
+    library's plain copy kernel over 512 MiB (each reads and writes that)."""
+    from libirwls_amd import lib
+    L = lib.load()
+    L.irwls_dev_init()
+    nbytes = 512 << 20
+    src, dst = L.irwls_dev_alloc(nbytes), L.irwls_dev_alloc(nbytes)
+    for _ in range(5):
+        L.irwls_dev_copy_kernel(dst, src, nbytes)
+    L.irwls_dev_sync()
+    L.irwls_dev_free(src)
+    L.irwls_dev_free(dst)
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}))
+
+
+def roofline(r, steps, workload=None, mode=None):
     t = r["tm"][r["dom"]]
     avg_ms = t["ms"] / max(t["launches"], 1)
     dom = r["dom"]
@@ -158,6 +189,8 @@ def roofline(r, steps):
         ach = (t["bytes"] / max(t["launches"], 1)) / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
                 "traffic": None}
+    if workload is not None:
+        roof["traffic"] = measured_traffic(workload, mode, dom)
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_ms
     roof["launches_per_step"] = t["launches"] / steps
@@ -243,7 +276,7 @@ def run_budgeted(args, wl, lib, abi):
                      "irwlspar_iters": v["par_stats"]["irwlspar_iters"]} for k, v in runs.items()},
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
         "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps),
+        "roofline": roofline(r, args.steps, args.workload, main_mode),
         "cpu_baseline": cpu,
     }
     if "plain" in runs and "learned" in runs:
@@ -333,7 +366,7 @@ def run_batched(args, wl, lib, abi):
         "runs": {k: {"s": v["value"], "mean_test_accuracy": float(v["acc"].mean())} for k, v in runs.items()},
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
         "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps),
+        "roofline": roofline(r, args.steps, args.workload, main_mode),
         "cpu_baseline": None,
     }
     if r["stats"]:
@@ -381,10 +414,15 @@ def main():
     ap.add_argument("--mode", default="both", choices=["plain", "mixed", "learned", "batched", "both"],
                     help="plain FP64; mixed precision (full training) or learned proposal (budgeted); "
                          "both: value = the mixed / learned trainer, plain reported beside it")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the traffic-counter calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-outer", type=int, default=None,
                     help="outer iterations in the CPU sample (0 = skip, -1 = the whole run; "
                          "default: the workload's own setting)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"
               % args.gpus, file=sys.stderr)
@@ -454,7 +492,7 @@ def main():
         "runs": {k: summary(v) for k, v in runs.items()},
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
         "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps),
+        "roofline": roofline(r, args.steps, args.workload, main_mode),
         "cpu_baseline": cpu,
     }
     if "plain" in runs and "mixed" in runs:

@@ -32,6 +32,9 @@ void  irwls_dev_h2d(void *d_dst, const void *src, size_t bytes);   /* synchronou
 void  irwls_dev_d2h(void *dst, const void *d_src, size_t bytes);   /* synchronous */
 void  irwls_dev_d2d(void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
 void  irwls_dev_memset0(void *d_dst, size_t bytes);                /* fill kernel */
+/* Plain copy kernel (16-byte aligned pointers, bytes a multiple of 16): its
+   traffic is known, which calibrates the hardware traffic counters. */
+void  irwls_dev_copy_kernel(void *d_dst, const void *d_src, size_t bytes);
 void  irwls_dev_sync(void);
 void  irwls_dev_trim(void);                 /* release scratch and solver workspace */
 

@@ -86,6 +86,16 @@ void *scratch(int slot, size_t bytes) {
 
 using namespace irwls;
 
+// Plain device-to-device copy, 16 bytes per thread and step: the kernel of
+// known traffic (reads = writes = the copied bytes) against which the
+// hardware's fetch and write counters are calibrated.
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void fill_zero_kernel(unsigned char *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -253,6 +263,20 @@ void irwls_dev_memset0(void *dst, size_t bytes) {
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream(),
                        (unsigned char *)dst, bytes);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_copy_kernel(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    if (((uintptr_t)dst & 15) || ((uintptr_t)src & 15) || (bytes & 15)) {
+        fprintf(stderr, "libirwls: irwls_dev_copy_kernel needs 16-byte aligned pointers and size\n");
+        exit(2);
+    }
+    const size_t n16 = bytes / 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream(), (const uint4 *)src,
+                       (uint4 *)dst, n16);
     IRWLS_LAUNCH_CHECK();
 }
 
