@@ -412,11 +412,16 @@ __global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restr
     const int wr = wave >> 1, wc = wave & 1;
     const int ql = threadIdx.x >> 4, cl = (threadIdx.x & 15) * 8;
     double pa[8], pb[8];
-    auto load = [&](int q) {
+    // source row of this thread's row of the chunk at q (-1 past the end);
+    // fetched a chunk before the row itself, so the two loads do not chain
+    auto src_of = [&](int q) -> int {
         const int qq = q + ql;
-        const bool ok = qq < q1;
-        const int src = ok ? (act ? act[qq] : qq) : 0;
-        const double *row = Phi + (size_t)src * B;
+        return qq < q1 ? (act ? act[qq] : qq) : -1;
+    };
+    auto load = [&](int q, int src) {
+        const int qq = q + ql;
+        const bool ok = src >= 0;
+        const double *row = Phi + (size_t)(ok ? src : 0) * B;
         const double w = (ok && sw) ? sw[qq] : 1.0;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -444,13 +449,18 @@ __global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restr
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
         for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
-    load(q0);
+    load(q0, src_of(q0));
+    int src_next = src_of(q0 + WK);
     store(0);
     __syncthreads();
     int buf = 0;
     for (int q = q0; q < q1; q += WK) {
         const bool more = q + WK < q1;
-        if (more) load(q + WK);
+        if (more) {
+            const int src = src_next;
+            src_next = src_of(q + 2 * WK);
+            load(q + WK, src);
+        }
         double (*A)[WLP] = Sa[buf];
         double (*Bm)[WLP] = diag ? Sa[buf] : Sb[buf];
 #pragma unroll

@@ -246,6 +246,84 @@ __global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
         if (c < ncols) K[(size_t)i * ldk + c] = kfun(kernel, gamma, acc[c], nr, cnrm ? cnrm[c] : 0.0);
 }
 
+// The same product for a block of 256 consecutive rows (no row selection):
+// the block's nonzeros are one contiguous range, brought into LDS in chunks
+// of BCH with coalesced loads, and each thread walks its row's part of each
+// chunk in order.  The per-thread walk of the kernel above reads its row
+// straight from memory, so every wave-wide load touches 64 cache lines for
+// 12 useful bytes each; here the nonzeros cross HBM and L2 once.
+constexpr int BCH = 2048;
+
+__global__ __launch_bounds__(256) void kmat_csr_block_narrow_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ rnrm, int nrows, const double *__restrict__ XcT, int ldc, int d,
+    const unsigned long long *__restrict__ cmask, const double *__restrict__ cnrm, int ncols, int kernel,
+    double gamma, double *__restrict__ K, int ldk) {
+    __shared__ int s_col[BCH];
+    __shared__ double s_val[BCH];
+    const int r0 = blockIdx.x * 256;
+    const int i = r0 + threadIdx.x;
+    const int rend = min(nrows, r0 + 256);
+    const int64_t P0 = rowptr[r0], P1 = rowptr[rend];
+    const bool live = i < nrows;
+    const int64_t p0 = live ? rowptr[i] : 0, p1 = live ? rowptr[i + 1] : 0;
+    double acc[NARROW];
+#pragma unroll
+    for (int c = 0; c < NARROW; c++) acc[c] = 0.0;
+    for (int64_t cb = P0; cb < P1; cb += BCH) {
+        const int clen = (int)min((int64_t)BCH, P1 - cb);
+        __syncthreads();   // the previous chunk has been read
+        {
+            // all of this thread's loads in flight before the first store
+            int cv[BCH / 256];
+            double vv[BCH / 256];
+#pragma unroll
+            for (int u = 0; u < BCH / 256; u++) {
+                const int k = threadIdx.x + 256 * u;
+                cv[u] = k < clen ? col[cb + k] : 0;
+                vv[u] = k < clen ? val[cb + k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < BCH / 256; u++) {
+                const int k = threadIdx.x + 256 * u;
+                if (k < clen) {
+                    s_col[k] = cv[u];
+                    s_val[k] = vv[u];
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t b = max(p0, cb), e = min(p1, cb + clen);
+        // four nonzeros per step, their masks fetched together
+        for (int64_t p = b; p < e; p += 4) {
+            int f[4];
+            double v[4];
+            unsigned m[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool ok = p + u < e;
+                f[u] = ok ? s_col[p + u - cb] : d;
+                v[u] = ok ? s_val[p + u - cb] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) m[u] = f[u] < d ? (unsigned)cmask[f[u]] : 0u;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (m[u] == 0u) continue;
+                const double *xr = XcT + (size_t)f[u] * ldc;
+#pragma unroll
+                for (int c = 0; c < NARROW; c++)
+                    if ((m[u] >> c) & 1u) acc[c] = fma(v[u], xr[c], acc[c]);
+            }
+        }
+    }
+    if (!live) return;
+    const double nr = rnrm ? rnrm[i] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NARROW; c++)
+        if (c < ncols) K[(size_t)i * ldk + c] = kfun(kernel, gamma, acc[c], nr, cnrm ? cnrm[c] : 0.0);
+}
+
 // Deterministic split reduction + output epilogue.
 __global__ void kmv_finish_kernel(const double *__restrict__ part, int n, int nsplit, double a, double b,
                                   double cst, const double *__restrict__ rsc, double *__restrict__ out) {
@@ -541,8 +619,14 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         cols_T(c, d, &XcT, &cn, &cm);
         const double nz = r->nnz_per_row * r->n;
         TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        hipLaunchKernelGGL(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->rowptr,
-                           r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
+        if (r->idx)
+            hipLaunchKernelGGL(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->rowptr,
+                               r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K,
+                               ldk);
+        else
+            hipLaunchKernelGGL(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(),
+                               r->rowptr, r->col, r->val, r->nrm, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma,
+                               K, ldk);
         IRWLS_LAUNCH_CHECK();
     } else {
         double *XcT, *cn;

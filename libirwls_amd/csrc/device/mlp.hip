@@ -280,13 +280,54 @@ __global__ __launch_bounds__(256) void mlp_infer_bf16(const uint16_t *__restrict
 // centroids (KNC row), |x_c|^2 and the step.
 constexpr int FMAXP = 1024;
 
+// Kernel values of each candidate to the current centroids (its KNC row,
+// s entries), summarised by one wave per candidate with coalesced reads:
+// max, min, sum, sum of squares and the counts above 0.1, 0.5 and 0.9, each
+// lane over every 64th entry and the lanes combined in a fixed butterfly.
+__global__ __launch_bounds__(256) void knc_stats_kernel(int P, int s, const double *__restrict__ KNC, int ldk,
+                                                        float *__restrict__ st) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= P) return;
+    float kmax = 0.f, kmin = (s > 0) ? 1e30f : 0.f, ksum = 0.f, ksq = 0.f, c1 = 0.f, c5 = 0.f, c9 = 0.f;
+    for (int e = lane; e < s; e += 64) {
+        const float k = (float)KNC[(size_t)c * ldk + e];
+        kmax = fmaxf(kmax, k);
+        kmin = fminf(kmin, k);
+        ksum += k;
+        ksq += k * k;
+        c1 += (k > 0.1f) ? 1.f : 0.f;
+        c5 += (k > 0.5f) ? 1.f : 0.f;
+        c9 += (k > 0.9f) ? 1.f : 0.f;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        kmax = fmaxf(kmax, __shfl_xor(kmax, off));
+        kmin = fminf(kmin, __shfl_xor(kmin, off));
+        ksum += __shfl_xor(ksum, off);
+        ksq += __shfl_xor(ksq, off);
+        c1 += __shfl_xor(c1, off);
+        c5 += __shfl_xor(c5, off);
+        c9 += __shfl_xor(c9, off);
+    }
+    if (lane == 0) {
+        float *o = st + (size_t)c * 8;
+        o[0] = kmax;
+        o[1] = kmin;
+        o[2] = ksum;
+        o[3] = ksq;
+        o[4] = c1;
+        o[5] = c5;
+        o[6] = c9;
+    }
+}
+
 __global__ __launch_bounds__(256) void sgma_features_kernel(int P, int s, int B, const double *__restrict__ KNC,
                                                             int ldk, const double *__restrict__ eta,
                                                             const double *__restrict__ gsub,
                                                             const double *__restrict__ gsub2,
                                                             const double *__restrict__ ycand,
                                                             const double *__restrict__ qvcand, float pos_frac,
-                                                            float *__restrict__ F) {
+                                                            const float *__restrict__ kst, float *__restrict__ F) {
     __shared__ float est[FMAXP], leta[FMAXP], lg[FMAXP];
     __shared__ float red[6][256];
     const int t = threadIdx.x;
@@ -328,17 +369,8 @@ __global__ __launch_bounds__(256) void sgma_features_kernel(int P, int s, int B,
     const float emax = red[2][0], emin = red[3][0];
     const float mleta = red[4][0] * inv, mlg = red[5][0] * inv;
     for (int c = t; c < P; c += 256) {
-        float kmax = 0.f, kmin = (s > 0) ? 1e30f : 0.f, ksum = 0.f, ksq = 0.f, c1 = 0.f, c5 = 0.f, c9 = 0.f;
-        for (int e = 0; e < s; e++) {
-            const float k = (float)KNC[(size_t)c * ldk + e];
-            kmax = fmaxf(kmax, k);
-            kmin = fminf(kmin, k);
-            ksum += k;
-            ksq += k * k;
-            c1 += (k > 0.1f) ? 1.f : 0.f;
-            c5 += (k > 0.5f) ? 1.f : 0.f;
-            c9 += (k > 0.9f) ? 1.f : 0.f;
-        }
+        const float *ks = kst + (size_t)c * 8;
+        const float kmax = ks[0], kmin = ks[1], ksum = ks[2], ksq = ks[3], c1 = ks[4], c5 = ks[5], c9 = ks[6];
         const float is = 1.0f / (float)(s > 0 ? s : 1);
         const float y = (float)ycand[c];
         const float rel = est[c] - mean;
@@ -453,8 +485,11 @@ void irwls_dev_sgma_features(int P, int s, int B, const double *KNC, int ldk, co
         fprintf(stderr, "libirwls: irwls_dev_sgma_features: pool of %d candidates (at most %d)\n", P, FMAXP);
         exit(2);
     }
+    float *kst = (float *)scratch(2, (size_t)P * 8 * sizeof(float));
+    hipLaunchKernelGGL(knc_stats_kernel, dim3((P + 3) / 4), dim3(256), 0, stream(), P, s, KNC, ldk, kst);
+    IRWLS_LAUNCH_CHECK();
     hipLaunchKernelGGL(sgma_features_kernel, dim3(1), dim3(256), 0, stream(), P, s, B, KNC, ldk, eta, gsub, gsub2,
-                       ycand, qvcand, pos_frac, F);
+                       ycand, qvcand, pos_frac, kst, F);
     IRWLS_LAUNCH_CHECK();
 }
 

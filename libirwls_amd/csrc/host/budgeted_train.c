@@ -108,12 +108,14 @@ int *SGMA(svm_dataset dataset, properties props) {
     int *cent = (int *)irwls_xmalloc((size_t)(B > 0 ? B : 1) * sizeof(int));
     int32_t *cidx = (int32_t *)irwls_xmalloc((size_t)cmax * sizeof(int32_t));
     int32_t *cidx2 = (int32_t *)irwls_xmalloc((size_t)cmax * sizeof(int32_t));
-    double *eta = (double *)irwls_xmalloc((size_t)cmax * sizeof(double));
-    double *q = (double *)irwls_xmalloc((size_t)cmax * sizeof(double));
+    /* eta and q side by side (host and device), read back with one copy */
+    double *eq = (double *)irwls_xcalloc(2 * (size_t)cmax, sizeof(double));
+    double *eta = eq, *q = eq + cmax;
     float *score = (float *)irwls_xmalloc((size_t)cmax * sizeof(float));
     char *taken = (char *)irwls_xmalloc((size_t)cmax);
-    double *L2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
-    double *IL2 = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
+    /* L2 and IL2 likewise */
+    double *lv = (double *)irwls_xcalloc(2 * ((size_t)(B > 0 ? B : 1) + 1), sizeof(double));
+    double *L2 = lv, *IL2 = lv + (B > 0 ? B : 1) + 1;
     double *row = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     const size_t Bz = (size_t)(B > 0 ? B : 1);
     const int kmax = ncand > keep ? ncand : keep;   /* exact-scored candidates per step */
@@ -124,10 +126,10 @@ int *SGMA(svm_dataset dataset, properties props) {
     double *d_KNC = (double *)irwls_dev_alloc((size_t)cmax * Bz * sizeof(double)); /* cmax x B, row-major */
     double *d_W = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));          /* B x cmax, column-major */
     double *d_Z = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));
-    double *d_eta = (double *)irwls_dev_alloc((size_t)cmax * sizeof(double));
-    double *d_q = (double *)irwls_dev_alloc((size_t)cmax * sizeof(double));
-    double *d_v1 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
-    double *d_v2 = (double *)irwls_dev_alloc((Bz + 1) * sizeof(double));
+    double *d_eq = (double *)irwls_dev_alloc(2 * (size_t)cmax * sizeof(double));
+    double *d_eta = d_eq, *d_q = d_eq + cmax;
+    double *d_lv = (double *)irwls_dev_alloc(2 * (Bz + 1) * sizeof(double));
+    double *d_v1 = d_lv, *d_v2 = d_lv + Bz + 1;
     double *d_tmp = (double *)irwls_dev_alloc(64 * sizeof(double));
     int32_t *d_cidx = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
     int32_t *d_cent = (int32_t *)irwls_dev_alloc(Bz * sizeof(int32_t));
@@ -151,8 +153,7 @@ int *SGMA(svm_dataset dataset, properties props) {
                 irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
                 score_exact(dd, props, n, B, s, ncand, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
                             d_q);
-                irwls_dev_d2h(eta, d_eta, (size_t)ncand * sizeof(double));
-                irwls_dev_d2h(q, d_q, (size_t)ncand * sizeof(double));
+                irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
                 const int best = best_gain(ncand, q, eta, &value);
                 cent[s] = cidx[best];
                 /* the chosen candidate's column is the new KSC column */
@@ -202,8 +203,7 @@ int *SGMA(svm_dataset dataset, properties props) {
                     irwls_dev_h2d(d_cidx, cidx2, (size_t)keep * sizeof(int32_t));
                     score_exact(dd, props, n, B, s, keep, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
                                 d_q);
-                    irwls_dev_d2h(eta, d_eta, (size_t)keep * sizeof(double));
-                    irwls_dev_d2h(q, d_q, (size_t)keep * sizeof(double));
+                    irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
                     const int best = best_gain(keep, q, eta, &value);
                     cent[s] = cidx2[best];
                     irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
@@ -249,8 +249,7 @@ int *SGMA(svm_dataset dataset, properties props) {
             }
             irwls_dev_gemv(d_Li, s, s, B, 0, knc, d_v1);        /* L2 = L^-1 k */
             irwls_dev_gemv(d_Li, s, s, B, 1, d_v1, d_v2);       /* IL2 = L2^T L^-1 */
-            irwls_dev_d2h(L2, d_v1, (size_t)s * sizeof(double));
-            irwls_dev_d2h(IL2, d_v2, (size_t)s * sizeof(double));
+            irwls_dev_d2h(lv, d_lv, (Bz + 1 + (size_t)s) * sizeof(double));
             double L3 = kcc + 0.00001;
             for (int i = 0; i < s; i++) L3 = L3 - (L2[i] * L2[i]);
             L3 = sqrt(L3);
@@ -272,10 +271,8 @@ int *SGMA(svm_dataset dataset, properties props) {
     irwls_dev_free(d_KNC);
     irwls_dev_free(d_W);
     irwls_dev_free(d_Z);
-    irwls_dev_free(d_eta);
-    irwls_dev_free(d_q);
-    irwls_dev_free(d_v1);
-    irwls_dev_free(d_v2);
+    irwls_dev_free(d_eq);
+    irwls_dev_free(d_lv);
     irwls_dev_free(d_tmp);
     irwls_dev_free(d_cidx);
     irwls_dev_free(d_cent);
@@ -286,10 +283,8 @@ int *SGMA(svm_dataset dataset, properties props) {
     free(cidx2);
     free(score);
     free(taken);
-    free(eta);
-    free(q);
-    free(L2);
-    free(IL2);
+    free(eq);
+    free(lv);
     free(row);
     g_bstats.t_total = irwls_now() - t_start;
     irwls_stats_set(&g_bstats);
