@@ -55,6 +55,7 @@ enum irwls_timer_kind {
     IRWLS_T_PCG,          /* mixed-precision refinement (preconditioned CG) */
     IRWLS_T_BF16,         /* BF16 hi/lo kernel tiles (mixed precision) */
     IRWLS_T_MLP,          /* proposal network training / inference */
+    IRWLS_T_GAIN16,       /* SGMA gain of at most 16 candidates (HBM-bound pass over KSC) */
     IRWLS_T_COUNT
 };
 void irwls_dev_timing(int enable);          /* enable/disable and reset */

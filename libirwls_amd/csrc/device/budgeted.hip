@@ -646,7 +646,8 @@ void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const doubl
     const int rt = (n + gain_tile_rows(ncand) - 1) / gain_tile_rows(ncand), ct = (ncand + T - 1) / T;
     double *part = (double *)scratch(10, (size_t)rt * ncand * sizeof(double));
     {
-        TimerScope ts(IRWLS_T_SGMA, 2.0 * n * (double)s * ncand, 8.0 * (double)n * s + 8.0 * (double)n * ncand);
+        TimerScope ts(ncand <= 16 ? IRWLS_T_GAIN16 : IRWLS_T_SGMA, 2.0 * n * (double)s * ncand,
+                      8.0 * (double)n * s + 8.0 * (double)n * ncand);
         if (ncand <= 16)
             hipLaunchKernelGGL(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz,
                                KSM, ldm, part);

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libirwls.so")
 
 # timer kinds (enum irwls_timer_kind in include/irwls_device.h)
 TIMER_KINDS = ["kmv", "kmat", "potf2", "trsm", "syrk", "potrs", "kkt", "gemv", "sgma", "csr",
-               "syrkd", "matvec", "pcg", "bf16"]
+               "syrkd", "matvec", "pcg", "bf16", "mlp", "gain16"]
 
 
 class irwls_gpu_options(C.Structure):
