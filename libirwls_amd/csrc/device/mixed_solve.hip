@@ -18,45 +18,43 @@ using namespace irwls;
 namespace {
 
 // out_c(i) = sum_{k in range(i)} M[i*ld + off + k] v_c(k), c = 0, 1, with
-// range [0, n) (mode 0), [0, i] (mode 1) or [i, n) (mode 2).  One wave per i.
-// When dot_with is set, work-group partials of sum_i dot_with_c(i) out_c(i)
-// go to part[2*blockIdx.x + c].
+// range [0, n) (mode 0), [0, i] (mode 1) or [i, n) (mode 2).  One
+// work-group per i: its four waves take contiguous quarters of the range
+// (loads unrolled four deep), added in wave order.  When dot_with is set,
+// dot_with_c(i) out_c(i) goes to part[2*i + c].
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void coldot2(const T *__restrict__ M, int ld, size_t off, int n,
                                                const double *__restrict__ v, double *__restrict__ out,
                                                const double *__restrict__ dot_with, double *__restrict__ part) {
     __shared__ double wp[4][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wave;
+    const int i = blockIdx.x;
+    const int lo = (MODE == 2) ? i : 0;
+    const int hi = (MODE == 1) ? i + 1 : n;
+    const int len = hi - lo;
+    const int qa = lo + (int)(((long long)len * wave) / 4), qb = lo + (int)(((long long)len * (wave + 1)) / 4);
+    const T *col = M + (size_t)i * ld + off;
     double s0 = 0.0, s1 = 0.0;
-    if (i < n) {
-        const int lo = (MODE == 2) ? i : 0;
-        const int hi = (MODE == 1) ? i + 1 : n;
-        const T *col = M + (size_t)i * ld + off;
-        for (int k = lo + lane; k < hi; k += 64) {
-            const double m = (double)col[k];
-            s0 = fma(m, v[k], s0);
-            s1 = fma(m, v[n + k], s1);
-        }
-        for (int o = 32; o >= 1; o >>= 1) {
-            s0 += __shfl_xor(s0, o);
-            s1 += __shfl_xor(s1, o);
-        }
-        if (lane == 0) {
-            out[i] = s0;
-            out[n + i] = s1;
-        }
+#pragma unroll 4
+    for (int k = qa + lane; k < qb; k += 64) {
+        const double m = (double)col[k];
+        s0 = fma(m, v[k], s0);
+        s1 = fma(m, v[n + k], s1);
     }
-    if (dot_with) {
-        if (lane == 0) {
-            wp[wave][0] = (i < n) ? dot_with[i] * s0 : 0.0;
-            wp[wave][1] = (i < n) ? dot_with[n + i] * s1 : 0.0;
-        }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            const int c = threadIdx.x;
-            part[2 * blockIdx.x + c] = (wp[0][c] + wp[1][c]) + (wp[2][c] + wp[3][c]);
-        }
+    for (int o = 32; o >= 1; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        wp[wave][0] = s0;
+        wp[wave][1] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int c = threadIdx.x;
+        const double t = (wp[0][c] + wp[1][c]) + (wp[2][c] + wp[3][c]);
+        out[(size_t)c * n + i] = t;
+        if (dot_with) part[2 * i + c] = dot_with[(size_t)c * n + i] * t;
     }
 }
 
@@ -217,7 +215,7 @@ void pcg_reserve(int n) {
     double **ps[] = {&g_pcg.part_a, &g_pcg.part_b, &g_pcg.part_c};
     for (double **v : ps) {
         if (*v) irwls_dev_free(*v);
-        *v = (double *)irwls_dev_alloc((size_t)2 * ((n + 3) / 4 + 1) * sizeof(double));
+        *v = (double *)irwls_dev_alloc((size_t)2 * (n + 1) * sizeof(double));
     }
     if (!g_pcg.rz) g_pcg.rz = (double *)irwls_dev_alloc(4 * sizeof(double));
     g_pcg.cap = n;
@@ -238,7 +236,7 @@ struct Precond {
 int precond(const Precond &M, int n, const double *A, int lda, const double *r, double *w, double *z,
             double *part) {
     const int nf = M.map ? M.nf : n;
-    const int g = (nf + 3) / 4;
+    const int g = nf;   // one work-group, and one partial pair, per entry
     if (!M.map) {
         hipLaunchKernelGGL((coldot2<float, 1>), dim3(g), dim3(256), 0, stream(), M.F, M.ldf, (size_t)nf, nf, r, w,
                            (const double *)nullptr, (double *)nullptr);
@@ -274,7 +272,7 @@ int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, 
     pcg_reserve(n > nf ? n : nf);
     const Precond M = {F, ldf, ZT, ldz, nf, map, inv};
     const size_t vb = (size_t)2 * n * sizeof(double);
-    const int gcol = (n + 3) / 4;           // work-groups of the one-wave-per-entry kernels
+    const int gcol = n;                     // work-groups (and partial pairs) of the column products
     const int gvec = (n + 255) / 256;       // work-groups of the elementwise kernels
     double bn[2] = {0.0, 0.0};
     {
