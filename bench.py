@@ -52,8 +52,9 @@ WORKLOADS = {
                                         seed=1, cpu_outer=1),
     # budgeted workloads (SGMA + IRWLSpar): CSR, d=2000, 1% density, values U[0,1], labels from an RBF
     # expansion over 200 anchors; budget 1000, 59 candidates, gamma 0.5, C 10
+    # learned proposal: gains on a fixed subset of 65536 of the 10^6 rows (proposal_rows)
     "sparse_rbf_n1000000_d2000": dict(kind="budgeted", n=1000000, n_test=100000, d=2000, density=0.01, gamma=0.5,
-                                      C=10.0, B=1000, cand=59, seed=3, cpu_sgma=3),
+                                      C=10.0, B=1000, cand=59, seed=3, cpu_sgma=3, proposal_rows=65536),
     "sparse_rbf_n100000_d2000": dict(kind="budgeted", n=100000, n_test=20000, d=2000, density=0.01, gamma=0.5,
                                      C=10.0, B=500, cand=59, seed=3, cpu_sgma=3),
     # batched workload: 12 one-vs-rest tasks of a 12-class mixture (n log-uniform in [1000, 30000], d in
@@ -233,7 +234,9 @@ def run_budgeted(args, wl, lib, abi):
     props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], size=wl["B"])
     pp = abi.make_predict_properties(labels=0)
     modes = {"plain": dict(sgma_candidates=wl["cand"], proposal=0),
-             "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"])}
+             "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"],
+                             proposal_rows=args.proposal_rows if args.proposal_rows is not None
+                             else wl.get("proposal_rows", 0))}
     runs = {}
     sel = ["plain", "learned"] if args.mode == "both" else [{"plain": "plain"}.get(args.mode, "learned")]
     for mode in sel:
@@ -265,7 +268,7 @@ def run_budgeted(args, wl, lib, abi):
             log("bench: %s step %d: %.3f s (SGMA %.3f s) %s %s" % (mode, i, times[-1], t_sgma[-1], sst, pst))
         tm = lib.timers()
         lib.timing(False)
-        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0)
+        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
         m = lib.budgeted_model(props, ds, cent, beta)
         acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
         if dom is None:
@@ -283,6 +286,7 @@ def run_budgeted(args, wl, lib, abi):
         "ms_per_step": r["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+bf16-mlp" if main_mode == "learned" else "f64", "data": "synthetic",
         "config": {"workload": args.workload, "mode": main_mode, "n": n, "d": wl["d"], "B": wl["B"],
+                   "proposal_rows": modes["learned"]["proposal_rows"] if main_mode == "learned" else None,
                    "candidates": wl["cand"], "gamma": wl["gamma"], "C": wl["C"], "n_test": wl["n_test"],
                    "seed": wl["seed"]},
         "test_accuracy": r["acc"],
@@ -430,6 +434,9 @@ def main():
     ap.add_argument("--mode", default="both", choices=["plain", "mixed", "learned", "batched", "both"],
                     help="plain FP64; mixed precision (full training) or learned proposal (budgeted); "
                          "both: value = the mixed / learned trainer, plain reported beside it")
+    ap.add_argument("--proposal-rows", type=int, default=None,
+                    help="budgeted, learned mode: rows on which the shortlist's gains are computed "
+                         "(0 = all rows; default: the workload's own setting)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the traffic-counter calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-outer", type=int, default=None,

@@ -24,6 +24,8 @@ typedef struct irwls_gpu_options {
     double refine_tol;    /* mixed precision: relative residual target (default 1e-10) */
     int proposal_pool;    /* learned proposal: candidates drawn per step (0: 4 x sgma_candidates, at most 1024) */
     int proposal_warmup;  /* learned proposal: plain SGMA steps that train the network (0: B/10 in [20, 100]) */
+    int proposal_rows;    /* learned proposal: rows on which the shortlist's gains are computed (0: all n, the
+                             exact gains; m < n: a fixed random subset of m rows, an estimate) */
 } irwls_gpu_options;
 
 /* Defaults, overridable by IRWLS_PRECISION=fp64|mixed, IRWLS_SGMA_CAND=<n>,

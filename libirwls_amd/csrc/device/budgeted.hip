@@ -410,38 +410,42 @@ __global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restr
     const int q1 = min(nact, q0 + chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int ql = threadIdx.x >> 4, cl = (threadIdx.x & 15) * 8;
-    double pa[8], pb[8];
-    // source row of this thread's row of the chunk at q (-1 past the end);
-    // fetched a chunk before the row itself, so the two loads do not chain
-    auto src_of = [&](int q) -> int {
-        const int qq = q + ql;
-        return qq < q1 ? (act ? act[qq] : qq) : -1;
-    };
-    auto load = [&](int q, int src) {
-        const int qq = q + ql;
-        const bool ok = src >= 0;
-        const double *row = Phi + (size_t)(ok ? src : 0) * B;
-        const double w = (ok && sw) ? sw[qq] : 1.0;
+    // staging: thread t carries centroid column cl = t % 128 of the tile
+    // side and the chunk's rows kb + 2u (u < 8), kb = t / 128; a wave's load
+    // of one row is 64 consecutive doubles, and its LDS stores (row cl of
+    // [centroid][k], stride 17) fall in distinct banks
+    const int cl = threadIdx.x & 127, kb = threadIdx.x >> 7;
+    double pa[8], pb[8], pw[8];
+    int srcs[8];
+    // source rows of this thread's chunk rows at q (-1 past the end); fetched
+    // a chunk before the rows themselves, so the two loads do not chain
+    auto src_of = [&](int q, int (&sr)[8]) {
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int ca = rI + cl + u, cb = rJ + cl + u;
-            double va = (ok && ca < B) ? row[ca] : 0.0;
-            double vb = (ok && !diag && cb < B) ? row[cb] : 0.0;
-            if (sw) {
-                va *= w;
-                vb *= w;
-            }
-            pa[u] = va;
-            pb[u] = vb;
+            const int qq = q + kb + 2 * u;
+            sr[u] = qq < q1 ? (act ? act[qq] : qq) : -1;
+        }
+    };
+    auto load = [&](int q, const int (&sr)[8]) {
+        // raw values only: the weight is applied at the store, so nothing
+        // waits for these loads before the chunk in LDS has been multiplied
+        const int ca = rI + cl, cb = rJ + cl;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int qq = q + kb + 2 * u;
+            const bool ok = sr[u] >= 0;
+            const double *row = Phi + (size_t)(ok ? sr[u] : 0) * B;
+            pw[u] = (ok && sw) ? sw[qq] : 1.0;
+            pa[u] = (ok && ca < B) ? row[ca] : 0.0;
+            pb[u] = (ok && !diag && cb < B) ? row[cb] : 0.0;
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) Sa[buf][cl + u][ql] = pa[u];
+        for (int u = 0; u < 8; u++) Sa[buf][cl][kb + 2 * u] = sw ? pa[u] * pw[u] : pa[u];
         if (!diag) {
 #pragma unroll
-            for (int u = 0; u < 8; u++) Sb[buf][cl + u][ql] = pb[u];
+            for (int u = 0; u < 8; u++) Sb[buf][cl][kb + 2 * u] = sw ? pb[u] * pw[u] : pb[u];
         }
     };
     v4d acc[4][4];
@@ -449,17 +453,17 @@ __global__ __launch_bounds__(256) void wgram_kernel(int B, const double *__restr
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
         for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
-    load(q0, src_of(q0));
-    int src_next = src_of(q0 + WK);
+    src_of(q0, srcs);
+    load(q0, srcs);
+    src_of(q0 + WK, srcs);
     store(0);
     __syncthreads();
     int buf = 0;
     for (int q = q0; q < q1; q += WK) {
         const bool more = q + WK < q1;
         if (more) {
-            const int src = src_next;
-            src_next = src_of(q + 2 * WK);
-            load(q + WK, src);
+            load(q + WK, srcs);
+            src_of(q + 2 * WK, srcs);
         }
         double (*A)[WLP] = Sa[buf];
         double (*Bm)[WLP] = diag ? Sa[buf] : Sb[buf];
@@ -695,14 +699,18 @@ void irwls_dev_wgram(int B, const double *Phi, const int32_t *act, const double 
                      double *K1, int ldk) {
     const int tb = (B + WT - 1) / WT;
     const int ntiles = tb * (tb + 1) / 2;
-    // about two work-groups per CU, at least 256 rows per split, partials
-    // under 1 GiB; a multiple of 8 splits when there are 8 or more
-    int nsplit = (512 + ntiles - 1) / ntiles;
+    // about four rounds of two work-groups per CU (a short last round idles
+    // less of the device), at least 256 rows per split, partials under 1 GiB;
+    // a multiple of 8 splits when there are 8 or more
+    int nsplit = (4 * 512 + ntiles - 1) / ntiles;
     if (nsplit > nact / 256) nsplit = nact / 256;
     const size_t BB = (size_t)B * B;
     while (nsplit > 1 && (size_t)nsplit * BB * sizeof(double) > ((size_t)1 << 30)) nsplit--;
     if (nsplit < 1) nsplit = 1;
-    if (nsplit >= 8) nsplit &= ~7;
+    if (nsplit >= 8) {
+        nsplit = (nsplit + 7) & ~7;
+        while (nsplit > 8 && (size_t)nsplit * BB * sizeof(double) > ((size_t)1 << 30)) nsplit -= 8;
+    }
     int chunk = (nact + nsplit - 1) / nsplit;
     chunk = (chunk + WK - 1) / WK * WK;
     if (chunk < WK) chunk = WK;

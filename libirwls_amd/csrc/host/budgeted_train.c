@@ -53,10 +53,11 @@ static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b
    (n x nc row-major), KNC(c, e) = k(cand_c, centroid_e) (ld B),
    Z = L^-T L^-1 KNC^T (ld B), eta_c = 1 - KNC_c . Z_c and
    q_c = |KSM(:, c) - KSC Z_c|^2 (src/budgeted-train.c:176-203). */
-static void score_exact(irwls_devdata *dd, properties props, int n, int B, int s, int nc, const int32_t *d_c,
-                        const int32_t *d_cent, const double *d_Li, const double *d_KSC, double *d_KSM, double *d_KNC,
-                        double *d_W, double *d_Z, double *d_eta, double *d_q) {
-    irwls_rows all = irwls_devdata_rows(dd, NULL, n);
+static void score_exact(irwls_devdata *dd, properties props, const int32_t *d_rows, int n, int B, int s, int nc,
+                        const int32_t *d_c, const int32_t *d_cent, const double *d_Li, const double *d_KSC,
+                        double *d_KSM, double *d_KNC, double *d_W, double *d_Z, double *d_eta, double *d_q) {
+    /* rows: all n rows (d_rows NULL) or the n rows listed; KSC has leading dimension n */
+    irwls_rows all = irwls_devdata_rows(dd, d_rows, n);
     irwls_rows cr = irwls_devdata_rows(dd, d_c, nc);
     irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
     irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, nc);
@@ -65,6 +66,11 @@ static void score_exact(irwls_devdata *dd, properties props, int n, int B, int s
     irwls_dev_gemm(1, 0, s, nc, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
     irwls_dev_sgma_eta(s, nc, d_KNC, B, d_Z, B, d_eta);
     irwls_dev_sgma_gain(n, s, nc, d_KSC, d_Z, B, d_KSM, nc, d_q);
+}
+
+static int cmp_i32(const void *a, const void *b) {
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
 }
 
 /* argmax of q/eta with the first maximum winning (src/budgeted-train.c:208-216) */
@@ -119,10 +125,15 @@ int *SGMA(svm_dataset dataset, properties props) {
     double *row = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     const size_t Bz = (size_t)(B > 0 ? B : 1);
     const int kmax = ncand > keep ? ncand : keep;   /* exact-scored candidates per step */
-    double *d_KSC = (double *)irwls_dev_alloc((size_t)n * Bz * sizeof(double));   /* n x B, column-major */
+    /* learned proposal with proposal_rows = m < n: every gain of the run (the
+       warm-up's candidates and the shortlists) on a fixed random subset of m
+       rows (in ascending order) instead of all n, with KSC kept on those rows
+       only (d_KSCe, m x B); the full KSC and KSM are then not needed */
+    const int me = (learned && opt->proposal_rows > 0 && opt->proposal_rows < n) ? opt->proposal_rows : 0;
+    double *d_KSC = me ? NULL : (double *)irwls_dev_alloc((size_t)n * Bz * sizeof(double));   /* n x B, column-major */
     double *d_L = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));            /* B x B, column-major */
     double *d_Li = (double *)irwls_dev_alloc(Bz * Bz * sizeof(double));
-    double *d_KSM = (double *)irwls_dev_alloc((size_t)n * (kmax > 0 ? kmax : 1) * sizeof(double)); /* row-major */
+    double *d_KSM = me ? NULL : (double *)irwls_dev_alloc((size_t)n * (kmax > 0 ? kmax : 1) * sizeof(double));
     double *d_KNC = (double *)irwls_dev_alloc((size_t)cmax * Bz * sizeof(double)); /* cmax x B, row-major */
     double *d_W = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));          /* B x cmax, column-major */
     double *d_Z = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));
@@ -136,6 +147,27 @@ int *SGMA(svm_dataset dataset, properties props) {
     int32_t *d_pair = (int32_t *)irwls_dev_alloc(2 * sizeof(int32_t));
     irwls_rows all = irwls_devdata_rows(dd, NULL, n);
     irwls_proposal *lp = learned ? irwls_proposal_begin(n, B, cmax, warm * ncand) : NULL;
+    int32_t *d_sube = NULL;
+    double *d_KSCe = NULL, *d_KSMe = NULL;
+    if (me) {
+        int32_t *perm = (int32_t *)irwls_xmalloc((size_t)n * sizeof(int32_t));
+        for (int i = 0; i < n; i++) perm[i] = i;
+        uint64_t st = 0xD1B54A32D192ED03ULL;
+        for (int i = 0; i < me; i++) {
+            st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+            const int j = i + (int)((st >> 11) % (uint64_t)(n - i));
+            const int32_t t = perm[i];
+            perm[i] = perm[j];
+            perm[j] = t;
+        }
+        qsort(perm, (size_t)me, sizeof(int32_t), cmp_i32);
+        d_sube = (int32_t *)irwls_dev_alloc((size_t)me * sizeof(int32_t));
+        irwls_dev_h2d(d_sube, perm, (size_t)me * sizeof(int32_t));
+        free(perm);
+        d_KSCe = (double *)irwls_dev_alloc((size_t)me * Bz * sizeof(double));
+        d_KSMe = (double *)irwls_dev_alloc((size_t)me * (kmax > 0 ? kmax : 1) * sizeof(double));
+    }
+    int full_col = 1;   /* the step's centroid has its full KSC column */
     const double *knc = NULL;       /* device row of k(new centroid, previous centroids) */
     int npos = 0;
 
@@ -151,13 +183,22 @@ int *SGMA(svm_dataset dataset, properties props) {
                     cidx[i] = ix;
                 }
                 irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
-                score_exact(dd, props, n, B, s, ncand, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
-                            d_q);
+                if (me)
+                    score_exact(dd, props, d_sube, me, B, s, ncand, d_cidx, d_cent, d_Li, d_KSCe, d_KSMe, d_KNC, d_W,
+                                d_Z, d_eta, d_q);
+                else
+                    score_exact(dd, props, NULL, n, B, s, ncand, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W,
+                                d_Z, d_eta, d_q);
                 irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
                 const int best = best_gain(ncand, q, eta, &value);
                 cent[s] = cidx[best];
                 /* the chosen candidate's column is the new KSC column */
-                irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
+                if (me) {
+                    irwls_dev_gather_col(d_KSMe, ncand, best, me, d_KSCe + (size_t)s * me);
+                    full_col = 0;
+                } else {
+                    irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
+                }
                 knc = d_KNC + (size_t)best * B;
                 if (lp) {
                     /* warm-up: log the candidates' features with their exact gains */
@@ -201,19 +242,35 @@ int *SGMA(svm_dataset dataset, properties props) {
                 }
                 if (keep > 0) {
                     irwls_dev_h2d(d_cidx, cidx2, (size_t)keep * sizeof(int32_t));
-                    score_exact(dd, props, n, B, s, keep, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W, d_Z, d_eta,
-                                d_q);
+                    if (me)
+                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx, d_cent, d_Li, d_KSCe, d_KSMe, d_KNC,
+                                    d_W, d_Z, d_eta, d_q);
+                    else
+                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W,
+                                    d_Z, d_eta, d_q);
                     irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
                     const int best = best_gain(keep, q, eta, &value);
                     cent[s] = cidx2[best];
-                    irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
+                    if (me) {
+                        irwls_dev_gather_col(d_KSMe, keep, best, me, d_KSCe + (size_t)s * me);
+                        full_col = 0;
+                    } else {
+                        irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
+                    }
                     knc = d_KNC + (size_t)best * B;
                 } else {
                     const int32_t c1 = cent[s];
                     irwls_dev_h2d(d_pair, &c1, sizeof(int32_t));
                     irwls_rows cc = irwls_devdata_rows(dd, d_pair, 1);
-                    irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n,
-                                            1);
+                    if (me) {
+                        irwls_rows re = irwls_devdata_rows(dd, d_sube, me);
+                        irwls_dev_kernel_matrix(&re, &cc, dd->d, props.kernelType, props.Kgamma,
+                                                d_KSCe + (size_t)s * me, 1);
+                        full_col = 0;
+                    } else {
+                        irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma,
+                                                d_KSC + (size_t)s * n, 1);
+                    }
                 }
                 g_bstats.learned_steps++;
             }
@@ -231,9 +288,21 @@ int *SGMA(svm_dataset dataset, properties props) {
         irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
         if (s <= 1) {
             irwls_rows cc = irwls_devdata_rows(dd, d_cent + s, 1);
-            irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n, 1);
+            if (me) {
+                irwls_rows re = irwls_devdata_rows(dd, d_sube, me);
+                irwls_dev_kernel_matrix(&re, &cc, dd->d, props.kernelType, props.Kgamma, d_KSCe + (size_t)s * me, 1);
+                full_col = 0;
+            } else {
+                irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma, d_KSC + (size_t)s * n, 1);
+            }
         }
-        if (lp) irwls_proposal_add_centroid(lp, s, d_KSC + (size_t)s * n);
+        if (full_col) {
+            if (lp) irwls_proposal_add_centroid(lp, s, d_KSC + (size_t)s * n);
+            if (me) irwls_dev_gather_idx(d_KSC + (size_t)s * n, d_sube, me, d_KSCe + (size_t)s * me);
+        } else {
+            irwls_proposal_add_centroid_rows(lp, dd, props, s, d_cent + s);
+        }
+        full_col = 1;
         /* grow L and L^-1 of K_C + jitter by one row (src/budgeted-train.c:247-279) */
         const double kcc = kernel_pair(dd, d_pair, cent[s], cent[s], props, d_tmp);
         if (s == 0) {
@@ -272,6 +341,11 @@ int *SGMA(svm_dataset dataset, properties props) {
     irwls_dev_free(d_W);
     irwls_dev_free(d_Z);
     irwls_dev_free(d_eq);
+    if (me) {
+        irwls_dev_free(d_sube);
+        irwls_dev_free(d_KSCe);
+        irwls_dev_free(d_KSMe);
+    }
     irwls_dev_free(d_lv);
     irwls_dev_free(d_tmp);
     irwls_dev_free(d_cidx);

@@ -46,6 +46,7 @@ static void opts_defaults(void) {
     g_opts.refine_tol = 1e-10;
     g_opts.proposal_pool = 0;
     g_opts.proposal_warmup = 0;
+    g_opts.proposal_rows = 0;
     const char *p = getenv("IRWLS_PRECISION");
     if (p && strcasecmp(p, "mixed") == 0) g_opts.precision = 1;
     const char *c = getenv("IRWLS_SGMA_CAND");

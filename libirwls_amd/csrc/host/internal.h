@@ -113,6 +113,8 @@ irwls_proposal *irwls_proposal_begin(int n, int B, int pmax, int log_cap);
 void irwls_proposal_end(irwls_proposal *lp);
 int irwls_proposal_trained(const irwls_proposal *lp);
 void irwls_proposal_add_centroid(irwls_proposal *lp, int s, const double *d_col);
+void irwls_proposal_add_centroid_rows(irwls_proposal *lp, irwls_devdata *dd, properties props, int s,
+                                      const int32_t *d_c);
 void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_dataset *ds, properties props,
                              int s, int P, const int32_t *d_c, const int32_t *cidx, const double *d_KNC,
                              const double *d_Z, const double *d_eta, float pos_frac);

@@ -114,6 +114,15 @@ void irwls_proposal_add_centroid(irwls_proposal *lp, int s, const double *d_col)
     irwls_dev_gather_idx(d_col, lp->d_sub, lp->m, lp->d_KSCs + (size_t)s * lp->m);
 }
 
+/* KSC column s on the subsample, computed from the centroid (device index
+   d_c) when the full column is not at hand. */
+void irwls_proposal_add_centroid_rows(irwls_proposal *lp, irwls_devdata *dd, properties props, int s,
+                                      const int32_t *d_c) {
+    irwls_rows sub = irwls_devdata_rows(dd, lp->d_sub, lp->m);
+    irwls_rows cc = irwls_devdata_rows(dd, d_c, 1);
+    irwls_dev_kernel_matrix(&sub, &cc, dd->d, props.kernelType, props.Kgamma, lp->d_KSCs + (size_t)s * lp->m, 1);
+}
+
 /* Features of a pool of P candidates (device rows d_c, host indices cidx)
    at step s, given their KNC rows (ld B), Z (ld B) and eta (device). */
 void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_dataset *ds, properties props,

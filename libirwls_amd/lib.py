@@ -23,7 +23,7 @@ TIMER_KINDS = ["kmv", "kmat", "potf2", "trsm", "syrk", "potrs", "kkt", "gemv", "
 class irwls_gpu_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("sgma_candidates", C.c_int), ("proposal", C.c_int),
                 ("proposal_keep", C.c_int), ("refine_tol", C.c_double), ("proposal_pool", C.c_int),
-                ("proposal_warmup", C.c_int)]
+                ("proposal_warmup", C.c_int), ("proposal_rows", C.c_int)]
 
 
 class irwls_gpu_stats(C.Structure):
