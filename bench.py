@@ -172,6 +172,66 @@ def measured_traffic(workload, mode, kind):
         return None
 
 
+def kernel_matrix_probe(reps=10):
+    """MFMA utilisation of the kernel-matrix products at the mixed-precision
+    workload's shape: K(X, X_W) w for n = 200000 rows, a working set of
+    W = 4000 columns, d = 256, RBF; FP64 MFMA tiles and BF16 hi/lo x3 tiles,
+    HIP-event timed over `reps` launches each.  Flops are 2 n W d counted
+    once; the BF16 path issues three products for each."""
+    from libirwls_amd import lib
+    L = lib.load()
+    L.irwls_dev_init()
+    vp, pb = C.c_void_p, C.POINTER(lib.irwls_rows_bf16)
+    L.irwls_dev_bf16_split.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp]
+    L.irwls_dev_kernel_matvec_bf16.argtypes = [pb, pb, C.c_int, C.c_double, vp, C.c_double, C.c_double,
+                                               C.c_double, vp, vp]
+    n, W, d, g = 200000, 4000, 256, 1.0 / 256
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((n, d)) * 0.5
+
+    def dev(a):
+        a = np.ascontiguousarray(a)
+        ptr = L.irwls_dev_alloc(max(a.nbytes, 8))
+        L.irwls_dev_h2d(ptr, a.ctypes.data, a.nbytes)
+        return ptr
+    Xd, idx, w, out = dev(X), dev(np.arange(W, dtype=np.int32) * (n // W)), dev(rng.standard_normal(W)), \
+        dev(np.zeros(n))
+    nrm = L.irwls_dev_alloc(n * 8)
+    L.irwls_dev_row_norms(Xd, d, n, d, nrm)
+    r, c = lib.irwls_rows(), lib.irwls_rows()
+    r.X, r.ldx, r.nrm, r.n = Xd, d, nrm, n
+    c.X, c.ldx, c.nrm, c.idx, c.n = Xd, d, nrm, idx, W
+    ldb = (d + 31) // 32 * 32
+    Xh, Xl, n32 = L.irwls_dev_alloc(n * ldb * 2), L.irwls_dev_alloc(n * ldb * 2), L.irwls_dev_alloc(n * 4)
+    L.irwls_dev_bf16_split(Xd, d, n, d, Xh, Xl, ldb, n32)
+    rb, cb = lib.irwls_rows_bf16(), lib.irwls_rows_bf16()
+    rb.Xh, rb.Xl, rb.ld, rb.nrm, rb.idx, rb.n = Xh, Xl, ldb, n32, None, n
+    cb.Xh, cb.Xl, cb.ld, cb.nrm, cb.idx, cb.n = Xh, Xl, ldb, n32, idx, W
+    flops = 2.0 * n * W * d
+    res = {}
+    for name, call in [("fp64", lambda: L.irwls_dev_kernel_matvec(C.byref(r), C.byref(c), d, 1, g, w, 0.0, 1.0, 0.0,
+                                                                   None, out)),
+                       ("bf16x3", lambda: L.irwls_dev_kernel_matvec_bf16(C.byref(rb), C.byref(cb), 1, g, w, 0.0,
+                                                                         1.0, 0.0, None, out))]:
+        call()
+        L.irwls_dev_sync()
+        lib.timing(True, kinds=["kmv", "bf16"])
+        for _ in range(reps):
+            call()
+        L.irwls_dev_sync()
+        t = lib.timers()["kmv" if name == "fp64" else "bf16"]
+        lib.timing(False)
+        ms = t["ms"] / max(t["launches"], 1)
+        tf = flops / (ms * 1e-3) / 1e12
+        peak = PEAK_FP64_TFLOPS if name == "fp64" else PEAK_BF16_TFLOPS
+        hw = tf * (3.0 if name == "bf16x3" else 1.0)
+        res[name] = {"ms_per_launch": ms, "algorithmic_tflops": tf, "peak_tflops": peak,
+                     "mfma_utilisation": hw / peak}
+        log("kernel-matrix probe %s: %.3f ms per launch, %.1f TFLOP/s algorithmic, MFMA utilisation %.3f"
+            % (name, ms, tf, hw / peak))
+    print(json.dumps({"kernel_matrix_probe": {"n": n, "W": W, "d": d, "launches": reps, **res}}))
+
+
 def copy_probe():
     """The calibration run for the traffic counters: five launches of the
     library's plain copy kernel over 512 MiB (each reads and writes that)."""
@@ -437,6 +497,8 @@ def main():
     ap.add_argument("--proposal-rows", type=int, default=None,
                     help="budgeted, learned mode: rows on which the shortlist's gains are computed "
                          "(0 = all rows; default: the workload's own setting)")
+    ap.add_argument("--kernel-matrix-probe", action="store_true",
+                    help="only time the kernel-matrix products (FP64 and BF16 x3 MFMA) at the mixed workload's shape")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the traffic-counter calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-outer", type=int, default=None,
@@ -445,6 +507,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.kernel_matrix_probe:
+        kernel_matrix_probe()
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus %d)"

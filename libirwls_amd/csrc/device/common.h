@@ -51,4 +51,21 @@ struct TimerScope {
 // Scratch device buffer owned by the context, grown on demand.
 void *scratch(int slot, size_t bytes);
 
+// Compute units of the current device.
+int device_cus();
+
+// Column splits of a kernel-block x vector launch over rt row tiles and ct
+// column tiles, for a kernel of which `per_cu` work-groups fit on a compute
+// unit: the fewest splits (powers of two) that give each resident
+// work-group at least 16 row-tile x split items over the launch, while each
+// split keeps at least two column tiles.  Depends on the shape only, so a
+// given shape always sums in the same order.
+inline void column_splits(int rt, int ct, int per_cu, int *tps, int *nsplit) {
+    const long slots = (long)device_cus() * per_cu;
+    int ns = 1;
+    while ((long)rt * ns < 16 * slots && ct >= 4 * ns) ns *= 2;
+    *tps = (ct + ns - 1) / ns;
+    *nsplit = (ct + *tps - 1) / *tps;
+}
+
 }  // namespace irwls

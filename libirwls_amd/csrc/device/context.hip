@@ -60,6 +60,17 @@ TimerScope::~TimerScope() {
     g_bytes[kind] += bytes;
 }
 
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        IRWLS_HIP(hipGetDevice(&dev));
+        IRWLS_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        cus = n > 0 ? n : 1;
+    }
+    return cus;
+}
+
 // ---- scratch slots -------------------------------------------------------
 static const int kSlots = 16;
 static thread_local void *g_scr[kSlots];

@@ -9,11 +9,14 @@
 // precision).  The squared norms are taken from the same representation,
 // sum(hi^2 + 2 hi lo), in FP64 and stored in FP32, so |x|^2 + |x|^2 - 2 x.x
 // cancels to the accumulation error.  Distances are clamped at zero and the
-// exponential is evaluated in FP32; the matvec epilogue sums w_j k_ij in FP64.
+// exponential is evaluated in FP32; the matvec epilogue adds w_j k_ij in FP32
+// groups of four columns into FP64 row sums.
 //
-// Tiles: 128 x 128 per work-group, four waves of 64 x 64 (4 x 4 MFMA
-// blocks of 16 x 16), K in steps of 32 staged through LDS with the next
-// step's loads in flight during the current step's MFMAs.  MFMA operand
+// Materialised blocks: 128 x 128 tiles per work-group, four waves of 64 x 64
+// (4 x 4 MFMA blocks of 16 x 16), K in steps of 32 staged through LDS with
+// the next step's loads in flight during the current step's MFMAs.  The
+// matvec uses 256 x 256 tiles of eight waves (described at its kernel),
+// which halves the operand traffic per product.  MFMA operand
 // layout (K = 32): A lane l = row l%16, k = 8(l/16)..8(l/16)+7; B likewise
 // with columns; accumulator register r of lane l = C[4(l/16) + r][l%16].
 #include "common.h"
@@ -123,85 +126,211 @@ __device__ __forceinline__ void tile_dots_bf16(v4f (&acc)[4][4], uint16_t (*As)[
     }
 }
 
+// ---- kernel block x vector: 256 x 256 tiles --------------------------------
+// Eight waves per work-group, each a 128 (columns, MFMA A side) x 64 (rows,
+// B side) block, so a lane keeps four fixed rows and its per-row sums stay in
+// registers.  Both sides of a k step (hi and lo planes, 64 bytes per row)
+// are staged in double-buffered LDS without padding: the four 16-byte
+// chunks of row r are stored in the order q ^ (((r >> 1) ^ (r >> 3)) & 3),
+// so the 16 rows of a fragment read fall on distinct banks.  The next k
+// step's global loads (across tile boundaries) are in flight during the
+// current step's MFMAs.
+constexpr int MT = 256;
+
+__device__ __forceinline__ int chunk_swz(int row, int q) { return q ^ (((row >> 1) ^ (row >> 3)) & 3); }
+
+// Thread t's share of one k step of a 256-row side: row t/2, k half t%2 of
+// both planes (zeros past the last row).
+__device__ __forceinline__ void load_step(uint4 (&h)[2], uint4 (&l)[2], const Bf16Side &s, int r0, int k0) {
+    const int row = threadIdx.x >> 1, half = threadIdx.x & 1;
+    const int g = r0 + row;
+    if (g < s.n) {
+        const size_t src = (size_t)(s.idx ? s.idx[g] : g) * s.ld + k0 + half * 16;
+        const uint4 *ph = reinterpret_cast<const uint4 *>(s.Xh + src);
+        const uint4 *pl = reinterpret_cast<const uint4 *>(s.Xl + src);
+        h[0] = ph[0];
+        h[1] = ph[1];
+        l[0] = pl[0];
+        l[1] = pl[1];
+    } else {
+        h[0] = h[1] = l[0] = l[1] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// S: hi plane [MT][32] followed by the lo plane.
+__device__ __forceinline__ void store_step(uint16_t *S, const uint4 (&h)[2], const uint4 (&l)[2]) {
+    const int row = threadIdx.x >> 1, half = threadIdx.x & 1;
+    uint16_t *rh = S + row * 32, *rl = S + (MT + row) * 32;
+    *reinterpret_cast<uint4 *>(rh + 8 * chunk_swz(row, 2 * half)) = h[0];
+    *reinterpret_cast<uint4 *>(rh + 8 * chunk_swz(row, 2 * half + 1)) = h[1];
+    *reinterpret_cast<uint4 *>(rl + 8 * chunk_swz(row, 2 * half)) = l[0];
+    *reinterpret_cast<uint4 *>(rl + 8 * chunk_swz(row, 2 * half + 1)) = l[1];
+}
+
+__device__ __forceinline__ v8bf frag_swz(const uint16_t *P, int row0) {
+    const int lane = threadIdx.x & 63;
+    const int row = row0 + (lane & 15);
+    const uint4 v = *reinterpret_cast<const uint4 *>(P + row * 32 + 8 * chunk_swz(row, lane >> 4));
+    return __builtin_bit_cast(v8bf, v);
+}
+
 __device__ __forceinline__ float side_nrm(const Bf16Side &s, int i) {
     return s.nrm[s.idx ? s.idx[i] : i];
 }
 
-// Kernel block x vector; grid (row tiles, column splits), split s covers
-// column tiles [s*tps, (s+1)*tps).  Partial sums to part[s*n + i] when
+// grid (row tiles, column splits), split s covers column tiles
+// [s*tps, (s+1)*tps).  ng2 = -gamma log2(e): k = 2^(ng2 max(d, 0)), one
+// v_exp_f32.  Each lane forms w_j k_ij in FP32, sums groups of four columns
+// in FP32 and adds the groups in FP64 (at n = 200000, 4000 columns, d = 256
+// the error against FP64 kernel values was rms 7.1e-6 against 6.2e-6 for
+// FP64 accumulation of every term, and 1.2e-5 for FP32 sums over whole
+// tiles).  Partial sums to part[s*n + i] when
 // nsplit > 1, else the output epilogue out = a out + rs (b sum) + cst.
-__global__ __launch_bounds__(256) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs, int kp, int kernel, float gamma,
+__global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs, int kp, int kernel, float ng2,
                                                        const double *__restrict__ w, int tps, int nsplit,
                                                        double *__restrict__ part, double a, double b, double cst,
                                                        const double *__restrict__ rsc, double *__restrict__ out) {
-    __shared__ __align__(16) uint16_t As[2][BT][LROW];
-    __shared__ __align__(16) uint16_t Bs[2][BT][LROW];
-    __shared__ double red[2][BT];
+    __shared__ __align__(16) uint16_t L[2][2][2 * MT * 32];   // [buffer][A columns, B rows][hi plane, lo plane]
+    __shared__ __align__(16) float wn[3][2][MT];              // [tile % 3][w, column norm][column]
+    __shared__ double red[2][MT];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int r0 = blockIdx.x * BT;
-    const int ntiles = (cs.n + BT - 1) / BT;
+    const int wa = wave >> 2, wb = wave & 3;
+    const int r0 = blockIdx.x * MT;
+    const int ntiles = (cs.n + MT - 1) / MT;
     const int ct_begin = blockIdx.y * tps;
     const int ct_end = (ct_begin + tps < ntiles) ? ct_begin + tps : ntiles;
-    double rsum[4][4];
-    float nr[4][4];
+    const int nks = kp / 32;
+    const int total = (ct_end - ct_begin) * nks;   // (tile, k step) pairs of this work-group
+    float nr[4];
+    double rsum[4];
 #pragma unroll
-    for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            rsum[mb][r] = 0.0;
-            const int gi = r0 + wr * 64 + mb * 16 + 4 * (lane >> 4) + r;
-            nr[mb][r] = (gi < rs.n) ? side_nrm(rs, gi) : 0.0f;
+    for (int nb = 0; nb < 4; nb++) {
+        const int gi = r0 + wb * 64 + nb * 16 + (lane & 15);
+        nr[nb] = (gi < rs.n) ? side_nrm(rs, gi) : 0.0f;
+        rsum[nb] = 0.0;
+    }
+    // w and the column norms of tile t; three buffers, so the one written is
+    // never the one a slower wave may still read in the previous epilogue
+    auto stage_cols = [&](int t) {
+        if (threadIdx.x < MT) {
+            const int gj = (ct_begin + t) * MT + threadIdx.x;
+            const bool in = gj < cs.n;
+            wn[t % 3][0][threadIdx.x] = in ? (float)w[gj] : 0.0f;
+            wn[t % 3][1][threadIdx.x] = in ? side_nrm(cs, gj) : 0.0f;
         }
-    for (int ct = ct_begin; ct < ct_end; ct++) {
-        const int c0 = ct * BT;
-        v4f acc[4][4];
-        tile_dots_bf16(acc, As, Bs, rs, cs, r0, c0, kp);
+    };
+    if (total > 0) {
+        uint4 ah[2], al[2], bh[2], bl[2];
+        load_step(ah, al, cs, ct_begin * MT, 0);
+        load_step(bh, bl, rs, r0, 0);
+        stage_cols(0);
+        store_step(L[0][0], ah, al);
+        store_step(L[0][1], bh, bl);
+        __syncthreads();
+        v4f acc[8][4];
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) {
-            const int gj = c0 + wc * 64 + nb * 16 + (lane & 15);
-            if (gj < cs.n) {
-                const double wj = w[gj];
-                const float ncj = side_nrm(cs, gj);
+        for (int mb = 0; mb < 8; mb++)
 #pragma unroll
-                for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        rsum[mb][r] = fma(wj, (double)kval(kernel, gamma, acc[mb][nb][r], nr[mb][r], ncj),
-                                          rsum[mb][r]);
+            for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4f){0.f, 0.f, 0.f, 0.f};
+        int t = 0, s = 0;
+        for (int it = 0; it < total; it++) {
+            const int cur = it & 1;
+            const bool more = it + 1 < total;
+            int tn = t, sn = s + 1;
+            if (sn == nks) {
+                tn = t + 1;
+                sn = 0;
             }
+            if (more) {
+                load_step(ah, al, cs, (ct_begin + tn) * MT, sn * 32);
+                load_step(bh, bl, rs, r0, sn * 32);
+            }
+            const uint16_t *PA = L[cur][0], *PB = L[cur][1];
+            v8bf fbh[4], fbl[4];
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++) {
+                fbh[nb] = frag_swz(PB, wb * 64 + nb * 16);
+                fbl[nb] = frag_swz(PB + MT * 32, wb * 64 + nb * 16);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 8; mb++) {
+                const v8bf fah = frag_swz(PA, wa * 128 + mb * 16);
+                const v8bf fal = frag_swz(PA + MT * 32, wa * 128 + mb * 16);
+#pragma unroll
+                for (int nb = 0; nb < 4; nb++) {
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah, fbh[nb], acc[mb][nb], 0, 0, 0);
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah, fbl[nb], acc[mb][nb], 0, 0, 0);
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal, fbh[nb], acc[mb][nb], 0, 0, 0);
+                }
+            }
+            if (more) {
+                // the other buffer was last read in the previous step, before its barrier
+                store_step(L[cur ^ 1][0], ah, al);
+                store_step(L[cur ^ 1][1], bh, bl);
+                if (sn == 0) stage_cols(tn);
+            }
+            __syncthreads();
+            if (s == nks - 1) {
+                // epilogue of tile t: acc[mb][nb][r] is the dot product of column
+                // wa*128 + mb*16 + 4 (l/16) + r and row wb*64 + nb*16 + l%16
+                const int wbuf = t % 3;
+#pragma unroll
+                for (int mb = 0; mb < 8; mb++) {
+                    const int cl = wa * 128 + mb * 16 + 4 * (lane >> 4);
+                    const float4 wv = *reinterpret_cast<const float4 *>(&wn[wbuf][0][cl]);
+                    const float4 nv = *reinterpret_cast<const float4 *>(&wn[wbuf][1][cl]);
+                    const float w4[4] = {wv.x, wv.y, wv.z, wv.w}, n4[4] = {nv.x, nv.y, nv.z, nv.w};
+#pragma unroll
+                    for (int nb = 0; nb < 4; nb++) {
+                        float p4 = 0.0f;   // four columns in FP32, then into the FP64 row sum
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            float k;
+                            if (kernel == 0) {
+                                k = acc[mb][nb][r];
+                            } else {
+                                float dd = fmaf(-2.0f, acc[mb][nb][r], nr[nb] + n4[r]);
+                                dd = dd > 0.0f ? dd : 0.0f;
+                                k = __builtin_amdgcn_exp2f(ng2 * dd);
+                            }
+                            p4 = fmaf(w4[r], k, p4);
+                        }
+                        rsum[nb] += (double)p4;
+                    }
+                }
+#pragma unroll
+                for (int mb = 0; mb < 8; mb++)
+#pragma unroll
+                    for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4f){0.f, 0.f, 0.f, 0.f};
+            }
+            t = tn;
+            s = sn;
         }
     }
-    // the 16 lanes with the same lane/16 share rows
+    // the lanes l, l^16, l^32, l^48 hold the same rows
 #pragma unroll
-    for (int mb = 0; mb < 4; mb++)
+    for (int nb = 0; nb < 4; nb++) {
+        double v = rsum[nb];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        rsum[nb] = v;
+    }
+    if (lane < 16) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            double v = rsum[mb][r];
-            v += __shfl_xor(v, 1);
-            v += __shfl_xor(v, 2);
-            v += __shfl_xor(v, 4);
-            v += __shfl_xor(v, 8);
-            rsum[mb][r] = v;
-        }
-    if ((lane & 15) == 0) {
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) red[wc][wr * 64 + mb * 16 + 4 * (lane >> 4) + r] = rsum[mb][r];
+        for (int nb = 0; nb < 4; nb++) red[wa][wb * 64 + nb * 16 + lane] = rsum[nb];
     }
     __syncthreads();
-    if (threadIdx.x < BT) {
+    if (threadIdx.x < MT) {
         const int i = threadIdx.x;
         const int gi = r0 + i;
         if (gi < rs.n) {
-            const double s = red[0][i] + red[1][i];
+            const double sv = red[0][i] + red[1][i];
             if (nsplit > 1) {
-                part[(size_t)blockIdx.y * rs.n + gi] = s;
+                part[(size_t)blockIdx.y * rs.n + gi] = sv;
             } else {
                 const double sc = rsc ? rsc[gi] : 1.0;
-                out[gi] = (a == 0.0 ? 0.0 : a * out[gi]) + sc * (b * s) + cst;
+                out[gi] = (a == 0.0 ? 0.0 : a * out[gi]) + sc * (b * sv) + cst;
             }
         }
     }
@@ -320,19 +449,18 @@ void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
         irwls_dev_kernel_matvec(&rr, &none, 0, kernel, gamma, w, a, b, cst, rs, out);
         return;
     }
-    const int rt = (r->n + BT - 1) / BT;
-    const int ct = (c->n + BT - 1) / BT;
-    int nsplit = 1;
-    while (rt * nsplit < 768 && nsplit * 2 <= ct) nsplit *= 2;
-    const int tps = (ct + nsplit - 1) / nsplit;
-    nsplit = (ct + tps - 1) / tps;
+    const int rt = (r->n + MT - 1) / MT;
+    const int ct = (c->n + MT - 1) / MT;
+    int tps, nsplit;
+    column_splits(rt, ct, 1, &tps, &nsplit);
     double *part = nullptr;
     if (nsplit > 1) part = (double *)scratch(14, (size_t)nsplit * r->n * sizeof(double));
     const double flops = 2.0 * r->n * (double)c->n * r->ld;
     const double bytes = 4.0 * ((double)r->n * r->ld + (double)c->n * r->ld) + 8.0 * 2.0 * r->n;
     TimerScope ts(IRWLS_T_BF16, flops, bytes);
-    hipLaunchKernelGGL(kmv_bf16_kernel, dim3(rt, nsplit), dim3(256), 0, stream(), side_of(r), side_of(c), r->ld,
-                       kernel, (float)gamma, w, tps, nsplit, part, a, b, cst, rs, out);
+    const float ng2 = (float)(-gamma * 1.4426950408889634);   // -gamma log2(e)
+    hipLaunchKernelGGL(kmv_bf16_kernel, dim3(rt, nsplit), dim3(512), 0, stream(), side_of(r), side_of(c), r->ld,
+                       kernel, ng2, w, tps, nsplit, part, a, b, cst, rs, out);
     IRWLS_LAUNCH_CHECK();
     if (nsplit > 1) {
         hipLaunchKernelGGL(kmv_finish_bf16, dim3((r->n + 255) / 256), dim3(256), 0, stream(), part, r->n, nsplit, a,

@@ -1,8 +1,9 @@
 // Kernel-matrix tiles for gfx950.
 //
 // Dense rows: k(x_i, z_j) from the norm expansion |x|^2 + |z|^2 - 2 x.z with
-// the x.z block computed by FP64 MFMA (v_mfma_f64_16x16x4f64) on 64x64 tiles
-// staged through LDS, and the exp / weight / row-reduction epilogue fused.
+// the x.z block computed by FP64 MFMA (v_mfma_f64_16x16x4f64) on 128x128
+// tiles staged through LDS with the next k chunk's loads in flight, and the
+// exp / weight / row-reduction epilogue fused.
 // CSR rows: lanes run over columns of a transposed dense column block, each
 // row's (col, val) pairs are broadcast; same epilogues.
 //
@@ -15,12 +16,9 @@ using namespace irwls;
 
 namespace {
 
-constexpr int TM = 64;       // tile rows
-constexpr int TN = 64;       // tile cols
+constexpr int TB = 128;      // tile rows / columns per work-group
 constexpr int BK = 16;       // k chunk staged in LDS
 constexpr int LDP = BK + 1;  // padded LDS row (doubles)
-
-enum { MODE_MATVEC = 0, MODE_MATRIX = 1 };
 
 struct DenseSide {
     const double *X;
@@ -35,63 +33,80 @@ __device__ __forceinline__ double kfun(int kernel, double gamma, double dot, dou
     return exp(-gamma * ((nr + nc) - 2.0 * dot));
 }
 
-// Loads a TM x BK chunk of rows [r0, r0+TM) x [k0, k0+BK) into LDS (zero padded).
-__device__ __forceinline__ void load_chunk(double (*S)[LDP], const DenseSide &s, int r0, int k0, int d) {
-    const int t = threadIdx.x;          // 256 threads: 64 rows x 4 quads
-    const int row = t >> 2;
-    const int kq = (t & 3) * 4;
-    const int gr = r0 + row;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    if (gr < s.n) {
-        const int src = s.idx ? s.idx[gr] : gr;
-        const double *p = s.X + (size_t)src * s.ldx + k0 + kq;
-        const int kk = k0 + kq;
-        if (kk + 3 < d) {
-            v0 = p[0]; v1 = p[1]; v2 = p[2]; v3 = p[3];
+// One k chunk of a 128-row tile in registers: thread t holds row t/2,
+// features k0 + 8 (t%2) .. +7 (zero beyond d and past the last row).
+__device__ __forceinline__ void load_rows(double (&v)[8], const DenseSide &s, int r0, int k0, int d) {
+    const int row = threadIdx.x >> 1, kh = k0 + (threadIdx.x & 1) * 8, g = r0 + row;
+    if (g < s.n && kh < d) {
+        const double *p = s.X + (size_t)(s.idx ? s.idx[g] : g) * s.ldx + kh;
+        if (kh + 8 <= d) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = p[q];
         } else {
-            if (kk < d) v0 = p[0];
-            if (kk + 1 < d) v1 = p[1];
-            if (kk + 2 < d) v2 = p[2];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (kh + q < d) ? p[q] : 0.0;
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = 0.0;
     }
-    S[row][kq] = v0;
-    S[row][kq + 1] = v1;
-    S[row][kq + 2] = v2;
-    S[row][kq + 3] = v3;
 }
 
-// One 64x64 tile of dot products, accumulated in acc[mb][nb] (v4d each):
-// lane l of wave (wr, wc) owns rows wr*32 + mb*16 + l/16 + 4*r and column
-// wc*32 + nb*16 + l%16 of the tile.
-__device__ __forceinline__ void tile_dots(v4d acc[2][2], double (*As)[TM][LDP], double (*Bs)[TN][LDP],
-                                          const DenseSide &rs, const DenseSide &cs,
-                                          int r0, int c0, int d) {
+__device__ __forceinline__ void store_rows(double (*S)[LDP], const double (&v)[8]) {
+    const int row = threadIdx.x >> 1, kh = (threadIdx.x & 1) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; q++) S[row][kh + q] = v[q];
+}
+
+// One 128 x 128 tile of dot products on FP64 MFMA: acc[mb][nb][r] = a_i . b_j
+// for i = a0 + 64 wa + 16 mb + l/16 + 4 r of side A and j = b0 + 64 wb +
+// 16 nb + l%16 of side B (wave 2 wa + wb, lane l; the accumulator layout of
+// v_mfma_f64_16x16x4f64).  k runs in chunks of 16 through double-buffered LDS;
+// the next chunk's global loads are issued before the current chunk's MFMAs
+// and written to LDS after them, so their latency is hidden.
+__device__ __forceinline__ void tile_dots(v4d (&acc)[4][4], double (*As)[TB][LDP], double (*Bs)[TB][LDP],
+                                          const DenseSide &sa, const DenseSide &sb, int a0, int b0, int d) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    for (int mb = 0; mb < 2; mb++)
-        for (int nb = 0; nb < 2; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int wa = wave >> 1, wb = wave & 1;
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
     const int nchunk = (d + BK - 1) / BK;
-    load_chunk(As[0], rs, r0, 0, d);
-    load_chunk(Bs[0], cs, c0, 0, d);
+    double va[8], vb[8];
+    load_rows(va, sa, a0, 0, d);
+    load_rows(vb, sb, b0, 0, d);
+    __syncthreads();   // the previous tile's last chunk has been read
+    store_rows(As[0], va);
+    store_rows(Bs[0], vb);
     __syncthreads();
     for (int ch = 0; ch < nchunk; ch++) {
         const int cur = ch & 1;
-        if (ch + 1 < nchunk) {
-            load_chunk(As[cur ^ 1], rs, r0, (ch + 1) * BK, d);
-            load_chunk(Bs[cur ^ 1], cs, c0, (ch + 1) * BK, d);
+        const bool more = ch + 1 < nchunk;
+        if (more) {
+            load_rows(va, sa, a0, (ch + 1) * BK, d);
+            load_rows(vb, sb, b0, (ch + 1) * BK, d);
         }
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ks++) {
             const int kk = ks * 4 + (lane >> 4);
-            double a0 = As[cur][wr * 32 + (lane & 15)][kk];
-            double a1 = As[cur][wr * 32 + 16 + (lane & 15)][kk];
-            double b0 = Bs[cur][wc * 32 + (lane & 15)][kk];
-            double b1 = Bs[cur][wc * 32 + 16 + (lane & 15)][kk];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double fa[4], fb[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                fa[q] = As[cur][wa * 64 + q * 16 + (lane & 15)][kk];
+                fb[q] = Bs[cur][wb * 64 + q * 16 + (lane & 15)][kk];
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 4; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mb], fb[nb], acc[mb][nb], 0, 0, 0);
+        }
+        if (more) {
+            // the other buffer was last read in the previous chunk, before its barrier
+            store_rows(As[cur ^ 1], va);
+            store_rows(Bs[cur ^ 1], vb);
         }
         __syncthreads();
     }
@@ -101,65 +116,76 @@ __device__ __forceinline__ double row_nrm(const DenseSide &s, int i) {
     return s.nrm ? s.nrm[s.idx ? s.idx[i] : i] : 0.0;
 }
 
-// Fused kernel-block x vector.  grid = (row tiles, column splits).  Split s
-// covers column tiles [s*tps, (s+1)*tps).  Writes partial sums part[s*n + i]
-// when nsplit > 1, else applies the output epilogue directly.
-__global__ __launch_bounds__(256) void kmv_dense_kernel(
+// Fused kernel-block x vector.  grid = (row tiles, column splits); split s
+// covers column tiles [s*tps, (s+1)*tps).  The columns are the MFMA A side,
+// so each lane keeps four fixed rows (r0 + 64 wb + 16 nb + l%16) and sums
+// w_j k_ij over its columns in registers; the lanes sharing a row are
+// reduced once at the end.  Writes partial sums part[s*n + i] when
+// nsplit > 1, else applies the output epilogue directly.
+__global__ __launch_bounds__(256, 2) void kmv_dense_kernel(
     DenseSide rs, DenseSide cs, int d, int kernel, double gamma, const double *__restrict__ w,
     int tps, int nsplit, double *__restrict__ part,
     double a, double b, double cst, const double *__restrict__ rsc, double *__restrict__ out) {
-    __shared__ double As[2][TM][LDP];
-    __shared__ double Bs[2][TN][LDP];
-    __shared__ double red[2][TM];
+    __shared__ double As[2][TB][LDP];
+    __shared__ double Bs[2][TB][LDP];
+    __shared__ __align__(16) double wn[2][2][TB];   // [tile parity][w, column norm][permuted column]
+    __shared__ double red[2][TB];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int r0 = blockIdx.x * TM;
+    const int wa = wave >> 1, wb = wave & 1;
+    const int r0 = blockIdx.x * TB;
     const int ct_begin = blockIdx.y * tps;
-    int ct_end = ct_begin + tps;
-    const int ntiles = (cs.n + TN - 1) / TN;
-    if (ct_end > ntiles) ct_end = ntiles;
-
-    // per-lane row partial sums: rows wr*32 + mb*16 + lane/16 + 4*r
-    double rsum[2][4];
-    double nr[2][4];
-    for (int mb = 0; mb < 2; mb++)
-        for (int r = 0; r < 4; r++) {
-            rsum[mb][r] = 0.0;
-            const int gi = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
-            nr[mb][r] = (gi < rs.n) ? row_nrm(rs, gi) : 0.0;
-        }
+    const int ntiles = (cs.n + TB - 1) / TB;
+    const int ct_end = ct_begin + tps < ntiles ? ct_begin + tps : ntiles;
+    double nr[4], rsum[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int gi = r0 + wb * 64 + nb * 16 + (lane & 15);
+        nr[nb] = gi < rs.n ? row_nrm(rs, gi) : 0.0;
+        rsum[nb] = 0.0;
+    }
     for (int ct = ct_begin; ct < ct_end; ct++) {
-        const int c0 = ct * TN;
-        v4d acc[2][2];
-        tile_dots(acc, As, Bs, rs, cs, r0, c0, d);
-        for (int nb = 0; nb < 2; nb++) {
-            const int gj = c0 + wc * 32 + nb * 16 + (lane & 15);
-            if (gj < cs.n) {
-                const double wj = w[gj];
-                const double ncj = row_nrm(cs, gj);
-                for (int mb = 0; mb < 2; mb++)
-                    for (int r = 0; r < 4; r++)
-                        rsum[mb][r] += wj * kfun(kernel, gamma, acc[mb][nb][r], nr[mb][r], ncj);
-            }
+        const int c0 = ct * TB, par = (ct - ct_begin) & 1;
+        if (threadIdx.x < TB) {
+            // column j of the tile is kept at 16 (j/16) + 4 (j%4) + (j%16)/4, so the lane with l/16 = q
+            // finds its columns q + 4 r (r = 0..3) of each 16-column block as one contiguous run; the
+            // parity buffer written here was last read two tiles ago, before this tile's barriers
+            const int j = threadIdx.x, gj = c0 + j;
+            const int pos = (j & ~15) + 4 * (j & 3) + ((j & 15) >> 2);
+            wn[par][0][pos] = gj < cs.n ? w[gj] : 0.0;
+            wn[par][1][pos] = gj < cs.n ? row_nrm(cs, gj) : 0.0;
+        }
+        v4d acc[4][4];
+        tile_dots(acc, As, Bs, cs, rs, c0, r0, d);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const int p0 = wa * 64 + mb * 16 + 4 * (lane >> 4);
+            const double2 w01 = *reinterpret_cast<const double2 *>(&wn[par][0][p0]);
+            const double2 w23 = *reinterpret_cast<const double2 *>(&wn[par][0][p0 + 2]);
+            const double2 n01 = *reinterpret_cast<const double2 *>(&wn[par][1][p0]);
+            const double2 n23 = *reinterpret_cast<const double2 *>(&wn[par][1][p0 + 2]);
+            const double wv[4] = {w01.x, w01.y, w23.x, w23.y};
+            const double nv[4] = {n01.x, n01.y, n23.x, n23.y};
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) rsum[nb] += wv[r] * kfun(kernel, gamma, acc[mb][nb][r], nr[nb], nv[r]);
         }
     }
-    // reduce over the 16 lanes that share rows (lane % 16 differs)
-    for (int mb = 0; mb < 2; mb++)
-        for (int r = 0; r < 4; r++) {
-            double v = rsum[mb][r];
-            v += __shfl_xor(v, 1);
-            v += __shfl_xor(v, 2);
-            v += __shfl_xor(v, 4);
-            v += __shfl_xor(v, 8);
-            rsum[mb][r] = v;
-        }
-    if ((lane & 15) == 0) {
-        for (int mb = 0; mb < 2; mb++)
-            for (int r = 0; r < 4; r++) red[wc][wr * 32 + mb * 16 + (lane >> 4) + 4 * r] = rsum[mb][r];
+    // the lanes l, l^16, l^32, l^48 hold the same rows
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        double v = rsum[nb];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        rsum[nb] = v;
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) red[wa][wb * 64 + nb * 16 + lane] = rsum[nb];
     }
     __syncthreads();
-    if (threadIdx.x < TM) {
+    if (threadIdx.x < TB) {
         const int i = threadIdx.x;
         const int gi = r0 + i;
         if (gi < rs.n) {
@@ -174,25 +200,29 @@ __global__ __launch_bounds__(256) void kmv_dense_kernel(
     }
 }
 
-// Materialised kernel block K[i*ldk + j].
-__global__ __launch_bounds__(256) void kmat_dense_kernel(DenseSide rs, DenseSide cs, int d, int kernel,
-                                                         double gamma, double *__restrict__ K, int ldk) {
-    __shared__ double As[2][TM][LDP];
-    __shared__ double Bs[2][TN][LDP];
+// Materialised kernel block K[i*ldk + j]; rows on the MFMA A side, so each
+// register of a lane stores 16 consecutive columns across the lanes.
+__global__ __launch_bounds__(256, 2) void kmat_dense_kernel(DenseSide rs, DenseSide cs, int d, int kernel,
+                                                            double gamma, double *__restrict__ K, int ldk) {
+    __shared__ double As[2][TB][LDP];
+    __shared__ double Bs[2][TB][LDP];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int r0 = blockIdx.x * TM;
-    const int c0 = blockIdx.y * TN;
-    v4d acc[2][2];
+    const int wa = wave >> 1, wb = wave & 1;
+    const int r0 = blockIdx.x * TB;
+    const int c0 = blockIdx.y * TB;
+    v4d acc[4][4];
     tile_dots(acc, As, Bs, rs, cs, r0, c0, d);
-    for (int nb = 0; nb < 2; nb++) {
-        const int gj = c0 + wc * 32 + nb * 16 + (lane & 15);
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int gj = c0 + wb * 64 + nb * 16 + (lane & 15);
         if (gj >= cs.n) continue;
         const double ncj = row_nrm(cs, gj);
-        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gi = r0 + wr * 32 + mb * 16 + (lane >> 4) + 4 * r;
+                const int gi = r0 + wa * 64 + mb * 16 + (lane >> 4) + 4 * r;
                 if (gi < rs.n)
                     K[(size_t)gi * ldk + gj] = kfun(kernel, gamma, acc[mb][nb][r], row_nrm(rs, gi), ncj);
             }
@@ -569,13 +599,10 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
     if (r->X) {
         DenseSide rsd = dense_side(r);
         DenseSide csd = dense_cols(c, d, 6, 7);
-        const int rt = (r->n + TM - 1) / TM;
-        const int ct = (c->n + TN - 1) / TN;
-        // enough work groups to fill 256 CUs a few times over
-        int nsplit = 1;
-        while (rt * nsplit < 1024 && nsplit * 2 <= ct) nsplit *= 2;
-        const int tps = (ct + nsplit - 1) / nsplit;
-        nsplit = (ct + tps - 1) / tps;
+        const int rt = (r->n + TB - 1) / TB;
+        const int ct = (c->n + TB - 1) / TB;
+        int tps, nsplit;
+        column_splits(rt, ct, 2, &tps, &nsplit);
         double *part = nullptr;
         if (nsplit > 1) part = (double *)scratch(8, (size_t)nsplit * r->n * sizeof(double));
         const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + 2.0 * r->n);
@@ -610,7 +637,7 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         DenseSide csd = dense_cols(c, d, 6, 7);
         const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + (double)r->n * c->n);
         TimerScope ts(IRWLS_T_KMAT, flops, bytes);
-        hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TM - 1) / TM, (c->n + TN - 1) / TN), dim3(256), 0,
+        hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TB - 1) / TB, (c->n + TB - 1) / TB), dim3(256), 0,
                            stream(), rsd, csd, d, kernel, gamma, K, ldk);
         IRWLS_LAUNCH_CHECK();
     } else if (c->n <= NARROW) {

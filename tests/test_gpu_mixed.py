@@ -215,11 +215,15 @@ def test_bf16_kernel_matrix_error_bound(n, m, d):
     assert np.all(np.abs(diag - 1.0) <= 1e-5)
 
 
-@pytest.mark.parametrize("n,m,d", [(5000, 700, 128), (300, 3000, 64)])
-def test_bf16_kernel_matvec(n, m, d):
+@pytest.mark.parametrize("n,m,d,kernel", [(5000, 700, 128, 1), (300, 3000, 64, 1), (2000, 600, 100, 1),
+                                          (2000, 600, 100, 0)])
+def test_bf16_kernel_matvec(n, m, d, kernel):
     """out = a out + rs (b K w) + cst with the BF16 block, against FP64
-    numpy; also covers the split-column path (many column tiles)."""
-    rng = np.random.default_rng(700 + n)
+    numpy; also covers the split-column path (many column tiles), ragged
+    row and column tiles and zero-padded features (d = 100 in rows of 128).
+    RBF: |error| <= 1e-4 sum |w| (k <= 1).  Linear: the dot products are off
+    by at most about 2^-15 |x||z|, so |error_i| <= 2^-14 |x_i| sum_j |w_j||z_j|."""
+    rng = np.random.default_rng(700 + n + d + kernel)
     X = rng.standard_normal((n, d))
     Z = rng.standard_normal((m, d))
     w = rng.standard_normal(m)
@@ -233,8 +237,13 @@ def test_bf16_kernel_matvec(n, m, d):
     r, keep_r = _bf16_rows(Xd, n, d)
     c, keep_c = _bf16_rows(Zd, m, d)
     wd, rsd, outd = Dev(w), Dev(rs), Dev(out0)
-    L.irwls_dev_kernel_matvec_bf16(C.byref(r), C.byref(c), 1, g, wd.p, 0.5, -1.0, 0.25, rsd.p, outd.p)
-    want = 0.5 * out0 + rs * (-(rbf(X, Z, g) @ w)) + 0.25
-    err = np.abs(outd.get() - want).max()
-    print("matvec max abs err", err, "scale", np.abs(w).sum())
-    assert err <= 1e-4 * np.abs(w).sum()
+    L.irwls_dev_kernel_matvec_bf16(C.byref(r), C.byref(c), kernel, g, wd.p, 0.5, -1.0, 0.25, rsd.p, outd.p)
+    K = rbf(X, Z, g) if kernel == 1 else X @ Z.T
+    want = 0.5 * out0 + rs * (-(K @ w)) + 0.25
+    err = np.abs(outd.get() - want)
+    if kernel == 1:
+        bound = 1e-4 * np.abs(w).sum()
+    else:
+        bound = 2.0 ** -14 * np.linalg.norm(X, axis=1) * (np.abs(w) * np.linalg.norm(Z, axis=1)).sum()
+    print("matvec max abs err", err.max(), "max err / bound", (err / bound).max())
+    assert np.all(err <= bound)
