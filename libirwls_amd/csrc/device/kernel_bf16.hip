@@ -9,8 +9,7 @@
 // precision).  The squared norms are taken from the same representation,
 // sum(hi^2 + 2 hi lo), in FP64 and stored in FP32, so |x|^2 + |x|^2 - 2 x.x
 // cancels to the accumulation error.  Distances are clamped at zero and the
-// exponential is evaluated in FP32; the matvec epilogue adds w_j k_ij in FP32
-// groups of four columns into FP64 row sums.
+// exponential is evaluated in FP32; the matvec epilogue sums w_j k_ij in FP64.
 //
 // Materialised blocks: 128 x 128 tiles per work-group, four waves of 64 x 64
 // (4 x 4 MFMA blocks of 16 x 16), K in steps of 32 staged through LDS with
@@ -39,11 +38,18 @@ struct Bf16Side {
     int n;
 };
 
-__device__ __forceinline__ float kval(int kernel, float gamma, float dot, float nr, float nc) {
-    if (kernel == 0) return dot;
-    float d = (nr + nc) - 2.0f * dot;
+// Kernel value from a dot product and the two squared norms: the dot itself
+// (linear) or 2^(ng2 max(|x|^2 + |z|^2 - 2 x.z, 0)) with ng2 = -gamma log2(e)
+// (RBF, one v_exp_f32).  The materialised blocks and the matvec both use this
+// function on bit-identical dot products, so the working-set block the
+// sub-problem solves with and the error updates see the same kernel values:
+// the training loop relies on that (a matvec whose exponential was rounded
+// differently from the blocks', by about 1e-7 relatively, took 2.4 times the
+// iterations on a batched workload with C up to 100).
+__device__ __forceinline__ float kvalue(bool lin, float ng2, float dot, float nr, float nc) {
+    float d = fmaf(-2.0f, dot, nr + nc);
     d = d > 0.0f ? d : 0.0f;
-    return expf(-gamma * d);
+    return lin ? dot : __builtin_amdgcn_exp2f(ng2 * d);
 }
 
 // Prefetch registers of one k step: thread t holds row t/2, k half (t%2)*16
@@ -179,19 +185,21 @@ __device__ __forceinline__ float side_nrm(const Bf16Side &s, int i) {
 }
 
 // grid (row tiles, column splits), split s covers column tiles
-// [s*tps, (s+1)*tps).  ng2 = -gamma log2(e): k = 2^(ng2 max(d, 0)), one
-// v_exp_f32.  Each lane forms w_j k_ij in FP32, sums groups of four columns
-// in FP32 and adds the groups in FP64 (at n = 200000, 4000 columns, d = 256
-// the error against FP64 kernel values was rms 7.1e-6 against 6.2e-6 for
-// FP64 accumulation of every term, and 1.2e-5 for FP32 sums over whole
-// tiles).  Partial sums to part[s*n + i] when
-// nsplit > 1, else the output epilogue out = a out + rs (b sum) + cst.
+// [s*tps, (s+1)*tps).  Kernel values as kvalue(); each lane adds w_j k_ij to
+// its row sums in FP64 (FP64 w times the FP32 kernel value), so the training
+// loop's error updates e -= K dbeta add up exactly to e0 - K beta for the
+// fixed kernel values.  The three products of a k step are issued in the
+// materialised blocks' order (hi.hi, lo of the rows . hi of the columns,
+// hi of the rows . lo of the columns) with the operands' roles swapped,
+// which gives the same FP32 dot products.  Partial sums to part[s*n + i]
+// when nsplit > 1, else the output epilogue out = a out + rs (b sum) + cst.
 __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs, int kp, int kernel, float ng2,
                                                        const double *__restrict__ w, int tps, int nsplit,
                                                        double *__restrict__ part, double a, double b, double cst,
                                                        const double *__restrict__ rsc, double *__restrict__ out) {
     __shared__ __align__(16) uint16_t L[2][2][2 * MT * 32];   // [buffer][A columns, B rows][hi plane, lo plane]
-    __shared__ __align__(16) float wn[3][2][MT];              // [tile % 3][w, column norm][column]
+    __shared__ __align__(16) double wt[3][MT];                // [tile % 3][column]: w
+    __shared__ __align__(16) float nt[3][MT];                 // [tile % 3][column]: column norm
     __shared__ double red[2][MT];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -202,6 +210,7 @@ __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs,
     const int ct_end = (ct_begin + tps < ntiles) ? ct_begin + tps : ntiles;
     const int nks = kp / 32;
     const int total = (ct_end - ct_begin) * nks;   // (tile, k step) pairs of this work-group
+    const bool lin = kernel == 0;                  // linear kernel: k = x.z (a select, no branch)
     float nr[4];
     double rsum[4];
 #pragma unroll
@@ -216,8 +225,8 @@ __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs,
         if (threadIdx.x < MT) {
             const int gj = (ct_begin + t) * MT + threadIdx.x;
             const bool in = gj < cs.n;
-            wn[t % 3][0][threadIdx.x] = in ? (float)w[gj] : 0.0f;
-            wn[t % 3][1][threadIdx.x] = in ? side_nrm(cs, gj) : 0.0f;
+            wt[t % 3][threadIdx.x] = in ? w[gj] : 0.0;
+            nt[t % 3][threadIdx.x] = in ? side_nrm(cs, gj) : 0.0f;
         }
     };
     if (total > 0) {
@@ -260,8 +269,8 @@ __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs,
 #pragma unroll
                 for (int nb = 0; nb < 4; nb++) {
                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah, fbh[nb], acc[mb][nb], 0, 0, 0);
-                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah, fbl[nb], acc[mb][nb], 0, 0, 0);
                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal, fbh[nb], acc[mb][nb], 0, 0, 0);
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah, fbl[nb], acc[mb][nb], 0, 0, 0);
                 }
             }
             if (more) {
@@ -277,27 +286,16 @@ __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs,
                 const int wbuf = t % 3;
 #pragma unroll
                 for (int mb = 0; mb < 8; mb++) {
-                    const int cl = wa * 128 + mb * 16 + 4 * (lane >> 4);
-                    const float4 wv = *reinterpret_cast<const float4 *>(&wn[wbuf][0][cl]);
-                    const float4 nv = *reinterpret_cast<const float4 *>(&wn[wbuf][1][cl]);
-                    const float w4[4] = {wv.x, wv.y, wv.z, wv.w}, n4[4] = {nv.x, nv.y, nv.z, nv.w};
 #pragma unroll
-                    for (int nb = 0; nb < 4; nb++) {
-                        float p4 = 0.0f;   // four columns in FP32, then into the FP64 row sum
+                    for (int r = 0; r < 4; r++) {
+                        const int cl = wa * 128 + mb * 16 + 4 * (lane >> 4) + r;
+                        const double wj = wt[wbuf][cl];
+                        const float nc = nt[wbuf][cl];
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            float k;
-                            if (kernel == 0) {
-                                k = acc[mb][nb][r];
-                            } else {
-                                float dd = fmaf(-2.0f, acc[mb][nb][r], nr[nb] + n4[r]);
-                                dd = dd > 0.0f ? dd : 0.0f;
-                                k = __builtin_amdgcn_exp2f(ng2 * dd);
-                            }
-                            p4 = fmaf(w4[r], k, p4);
-                        }
-                        rsum[nb] += (double)p4;
+                        for (int nb = 0; nb < 4; nb++)
+                            rsum[nb] = fma(wj, (double)kvalue(lin, ng2, acc[mb][nb][r], nr[nb], nc), rsum[nb]);
                     }
+                    __builtin_amdgcn_sched_barrier(0);   // keep each block's loads and temporaries local
                 }
 #pragma unroll
                 for (int mb = 0; mb < 8; mb++)
@@ -337,7 +335,7 @@ __global__ __launch_bounds__(512) void kmv_bf16_kernel(Bf16Side rs, Bf16Side cs,
 }
 
 // Materialised kernel block K[i*ldk + j] (FP64 storage of the FP32 values).
-__global__ __launch_bounds__(256) void kmat_bf16_kernel(Bf16Side rs, Bf16Side cs, int kp, int kernel, float gamma,
+__global__ __launch_bounds__(256) void kmat_bf16_kernel(Bf16Side rs, Bf16Side cs, int kp, int kernel, float ng2,
                                                         double *__restrict__ K, int ldk) {
     __shared__ __align__(16) uint16_t As[2][BT][LROW];
     __shared__ __align__(16) uint16_t Bs[2][BT][LROW];
@@ -357,7 +355,8 @@ __global__ __launch_bounds__(256) void kmat_bf16_kernel(Bf16Side rs, Bf16Side cs
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int gi = r0 + wr * 64 + mb * 16 + 4 * (lane >> 4) + r;
-                if (gi < rs.n) K[(size_t)gi * ldk + gj] = kval(kernel, gamma, acc[mb][nb][r], side_nrm(rs, gi), ncj);
+                if (gi < rs.n)
+                    K[(size_t)gi * ldk + gj] = kvalue(kernel == 0, ng2, acc[mb][nb][r], side_nrm(rs, gi), ncj);
             }
     }
 }
@@ -392,17 +391,28 @@ __global__ void split_bf16_kernel(const double *__restrict__ X, int ldx, int n, 
     Xl[(size_t)i * ldb + k] = lb;
 }
 
-__global__ void split_norms_kernel(const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, int ldb, int n,
-                                   float *__restrict__ nrm) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per row: lane l sums features l, l + 64, ... in FP64, then the
+// wave's lanes are combined in a fixed butterfly order.
+__global__ __launch_bounds__(256) void split_norms_kernel(const uint16_t *__restrict__ Xh,
+                                                          const uint16_t *__restrict__ Xl, int ldb, int n,
+                                                          float *__restrict__ nrm) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (i >= n) return;
     double s = 0.0;
-    for (int k = 0; k < ldb; k++) {
+    for (int k = lane; k < ldb; k += 64) {
         const double h = (double)(float)__builtin_bit_cast(__bf16, Xh[(size_t)i * ldb + k]);
         const double l = (double)(float)__builtin_bit_cast(__bf16, Xl[(size_t)i * ldb + k]);
         s += h * h + 2.0 * h * l;
     }
-    nrm[i] = (float)s;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) nrm[i] = (float)s;
+}
+
+// ng2 of kvalue() for a kernel width gamma (one rounding, shared by every launch).
+float exp2_scale(double gamma) {
+    return (float)(-gamma * 1.4426950408889634);
 }
 
 Bf16Side side_of(const irwls_rows_bf16 *r) {
@@ -431,7 +441,7 @@ void irwls_dev_bf16_split(const double *X, int ldx, int n, int d, uint16_t *Xh, 
     hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream(), X, ldx, n, d,
                        Xh, Xl, ldb);
     IRWLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(split_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), Xh, Xl, ldb, n, nrm);
+    hipLaunchKernelGGL(split_norms_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), Xh, Xl, ldb, n, nrm);
     IRWLS_LAUNCH_CHECK();
 }
 
@@ -458,9 +468,8 @@ void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
     const double flops = 2.0 * r->n * (double)c->n * r->ld;
     const double bytes = 4.0 * ((double)r->n * r->ld + (double)c->n * r->ld) + 8.0 * 2.0 * r->n;
     TimerScope ts(IRWLS_T_BF16, flops, bytes);
-    const float ng2 = (float)(-gamma * 1.4426950408889634);   // -gamma log2(e)
     hipLaunchKernelGGL(kmv_bf16_kernel, dim3(rt, nsplit), dim3(512), 0, stream(), side_of(r), side_of(c), r->ld,
-                       kernel, ng2, w, tps, nsplit, part, a, b, cst, rs, out);
+                       kernel, exp2_scale(gamma), w, tps, nsplit, part, a, b, cst, rs, out);
     IRWLS_LAUNCH_CHECK();
     if (nsplit > 1) {
         hipLaunchKernelGGL(kmv_finish_bf16, dim3((r->n + 255) / 256), dim3(256), 0, stream(), part, r->n, nsplit, a,
@@ -480,7 +489,7 @@ void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
     const double bytes = 4.0 * ((double)r->n * r->ld + (double)c->n * r->ld) + 8.0 * (double)r->n * c->n;
     TimerScope ts(IRWLS_T_BF16, flops, bytes);
     hipLaunchKernelGGL(kmat_bf16_kernel, dim3((r->n + BT - 1) / BT, (c->n + BT - 1) / BT), dim3(256), 0, stream(),
-                       side_of(r), side_of(c), r->ld, kernel, (float)gamma, K, ldk);
+                       side_of(r), side_of(c), r->ld, kernel, exp2_scale(gamma), K, ldk);
     IRWLS_LAUNCH_CHECK();
 }
 

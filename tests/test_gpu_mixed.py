@@ -247,3 +247,41 @@ def test_bf16_kernel_matvec(n, m, d, kernel):
         bound = 2.0 ** -14 * np.linalg.norm(X, axis=1) * (np.abs(w) * np.linalg.norm(Z, axis=1)).sum()
     print("matvec max abs err", err.max(), "max err / bound", (err / bound).max())
     assert np.all(err <= bound)
+
+
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_bf16_matvec_uses_the_blocks_kernel_values(kernel):
+    """The BF16 matvec (256 x 256 tiles, columns on the MFMA A side) and the
+    materialised BF16 block (128 x 128 tiles, rows on the A side) produce the
+    same FP32 kernel values, so K w from the matvec equals the FP64 sum of
+    w_j K_ij over the block up to FP64 summation order.  Training relies on
+    it: the working-set block and the error updates must see one kernel.
+    Rows and columns are index selections of one data set (as in training),
+    with ragged tiles and zero-padded features."""
+    rng = np.random.default_rng(900 + kernel)
+    N, d = 3000, 100
+    X = rng.standard_normal((N, d)) * 0.7
+    ridx = rng.permutation(N)[:1100].astype(np.int32)
+    cidx = rng.permutation(N)[:700].astype(np.int32)
+    w = rng.standard_normal(700) * np.where(rng.random(700) < 0.1, 100.0, 1.0)
+    g = 0.5 / d
+    L = lib.load()
+    L.irwls_dev_kernel_matrix_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int]
+    L.irwls_dev_kernel_matvec_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_double,
+                                               C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+    Xd = Dev(X)
+    base, keep = _bf16_rows(Xd, N, d)
+    rd, cd = Dev(ridx), Dev(cidx)
+    r, c = lib.irwls_rows_bf16(), lib.irwls_rows_bf16()
+    for s, ix, n in ((r, rd, 1100), (c, cd, 700)):
+        s.Xh, s.Xl, s.ld, s.nrm, s.idx, s.n = base.Xh, base.Xl, base.ld, base.nrm, ix.p, n
+    Kd = Dev(np.zeros(1100 * 700))
+    L.irwls_dev_kernel_matrix_bf16(C.byref(r), C.byref(c), kernel, g, Kd.p, 700)
+    K = Kd.get().reshape(1100, 700)
+    wd, od = Dev(w), Dev(np.zeros(1100))
+    L.irwls_dev_kernel_matvec_bf16(C.byref(r), C.byref(c), kernel, g, wd.p, 0.0, 1.0, 0.0, None, od.p)
+    got, want = od.get(), K @ w
+    scale = np.abs(K) @ np.abs(w)
+    rel = np.abs(got - want) / scale
+    print("kernel", kernel, "max |matvec - block w| / sum |w k|: %.3g" % rel.max())
+    assert rel.max() < 1e-13
