@@ -74,6 +74,16 @@ svm_dataset irwls_dataset_from_dense(const double *X, const double *y, int n, in
 svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, const double *val,
                                    const double *y, int n);
 
+/* Dataset from a dense row-major n x d matrix with the conversion of the
+   reference's Python extension (numpy2dataset, and numpy2datasetWithAverage
+   when with_average is nonzero; python-package/pythonmodule.c:210-330): only
+   nonzero values stored, sparse = 1 as soon as one value is nonzero, and,
+   with with_average, the extension's two mean rows exactly as it builds them
+   (the first lists indices 1..d and an index-0 entry, the second is empty,
+   their norms are sums of the mean values).  Backs the Python 3 module
+   libirwls_amd.LIBIRWLS.  Free with freeDataset. */
+svm_dataset irwls_dataset_from_numpy(const double *X, const double *y, int n, int d, int with_average);
+
 /* Batched full training: nprob independent problems (datasets[p],
    props[p]); out[p] receives trainFULL's result for problem p (l_p + 1
    doubles, malloc'd, the caller frees it).  Each problem equals its

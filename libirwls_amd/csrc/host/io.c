@@ -13,6 +13,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <ctype.h>
+#include <math.h>
 #include <errno.h>
 #include <stddef.h>
 #include <stdio.h>
@@ -196,6 +197,86 @@ svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, cons
     svm_dataset ds = assemble(rp, ix, val, y, n, 1);
     free(rp);
     free(ix);
+    return ds;
+}
+
+/* Dataset from a dense row-major matrix with the conversion of the
+   reference's Python extension (numpy2dataset and numpy2datasetWithAverage,
+   python-package/pythonmodule.c:210-330), which differs from the readers':
+   only nonzero values are stored (index k+1), |x|^2 is the sum of their
+   pow(x, 2), maxdim is the column count and `sparse` is 1 as soon as any value
+   is nonzero.  with_average (its budgeted trainer's conversion) appends rows
+   l (y = +1) and l+1 (y = -1) holding the class means, accumulated in row
+   order as x/count over the rows labelled exactly 1.0, resp. all others,
+   with their "norms" stored as the plain sums of the mean values.  The
+   extension writes the first mean row's terminator one slot too far, onto
+   the second row's first entry (pythonmodule.c:293-294): the first mean row
+   then lists indices 1..d followed by an (index 0, value 0) entry, and the
+   second mean row is empty.  Both are reproduced, since the budgeted
+   trainer's first two centroids are these rows. */
+svm_dataset irwls_dataset_from_numpy(const double *X, const double *y, int n, int d, int with_average) {
+    svm_dataset ds;
+    memset(&ds, 0, sizeof(ds));
+    const int extra = with_average ? 2 : 0;
+    ds.l = n;
+    ds.maxdim = d;
+    ds.y = (double *)irwls_xcalloc((size_t)n + extra, sizeof(double));
+    ds.quadratic_value = (double *)irwls_xcalloc((size_t)n + extra, sizeof(double));
+    ds.x = (svm_sample **)irwls_xcalloc((size_t)n + extra, sizeof(svm_sample *));
+    int64_t nz = 0;
+    int pos = 0, neg = 0;
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < d; k++)
+            if (X[(size_t)i * d + k] != 0.0) nz++;
+        ds.y[i] = y ? y[i] : 0.0;
+        if (ds.y[i] == 1.0)
+            pos++;
+        else
+            neg++;
+    }
+    ds.sparse = nz > 0;
+    const int64_t elements = nz + n;   /* the rows' entries and terminators */
+    ds.features = (svm_sample *)irwls_xcalloc((size_t)elements + (with_average ? 2 * ((size_t)d + 1) : 0),
+                                              sizeof(svm_sample));
+    svm_sample *mp = NULL, *mn = NULL;
+    if (with_average) {
+        mp = ds.features + elements;
+        mn = mp + d + 1;
+        for (int k = 0; k < d; k++) {
+            mp[k].index = k + 1;
+            mn[k].index = k + 1;
+        }
+        mn[0].index = -1;   /* the misplaced terminator: mp[d] stays (0, 0.0), mn is empty */
+        mn[d].index = -1;
+        ds.y[n] = 1.0;
+        ds.y[n + 1] = -1.0;
+        ds.x[n] = mp;
+        ds.x[n + 1] = mn;
+    }
+    int64_t j = 0;
+    for (int i = 0; i < n; i++) {
+        ds.x[i] = &ds.features[j];
+        for (int k = 0; k < d; k++) {
+            const double v = X[(size_t)i * d + k];
+            if (v == 0.0) continue;
+            ds.quadratic_value[i] += pow(v, 2.0);
+            ds.features[j].index = k + 1;
+            ds.features[j].value = v;
+            j++;
+            if (with_average) {
+                if (ds.y[i] == 1.0)
+                    mp[k].value += v / pos;
+                else
+                    mn[k].value += v / neg;
+            }
+        }
+        ds.features[j++].index = -1;
+    }
+    if (with_average)
+        for (int k = 0; k < d; k++) {
+            ds.quadratic_value[n] += mp[k].value;
+            ds.quadratic_value[n + 1] += mn[k].value;
+        }
     return ds;
 }
 

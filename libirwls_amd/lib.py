@@ -89,6 +89,7 @@ def load():
         "writeOutput": (None, [C.c_char_p, dp, C.c_int]),
         "irwls_dataset_from_dense": (svm_dataset, [dp, dp, C.c_int, C.c_int]),
         "irwls_dataset_from_csr": (svm_dataset, [P(C.c_longlong), ip, dp, dp, C.c_int]),
+        "irwls_dataset_from_numpy": (svm_dataset, [dp, dp, C.c_int, C.c_int, C.c_int]),
         "irwls_gpu_get_options": (None, [P(irwls_gpu_options)]),
         "irwls_gpu_set_options": (None, [P(irwls_gpu_options)]),
         "irwls_gpu_pin_dataset": (None, [svm_dataset, C.c_int]),
@@ -165,6 +166,15 @@ class Dataset:
         y = np.ascontiguousarray(y, dtype=np.float64)
         n, d = X.shape
         return cls(load().irwls_dataset_from_dense(_dptr(X), _dptr(y), n, d))
+
+    @classmethod
+    def numpy(cls, X, y, with_average=False):
+        """The conversion of the reference's Python extension (zeros dropped,
+        sparse = 1, optional mean rows exactly as it builds them)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        n, d = X.shape
+        return cls(load().irwls_dataset_from_numpy(_dptr(X), _dptr(y), n, d, 1 if with_average else 0))
 
     @classmethod
     def csr(cls, rowptr, col, val, y):
