@@ -23,6 +23,9 @@ class orc_stats(C.Structure):
 
 def build():
     subprocess.check_call(["make", "-s", "-B", "-C", HERE, "liboracle.so"])
+    # the reference's reader / model-file source, where the reference tree is present
+    if os.path.isdir("/root/reference/src"):
+        subprocess.check_call(["make", "-s", "-B", "-C", HERE, "_ref/libref_io.so"])
 
 
 _lib = None
