@@ -84,6 +84,16 @@ svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, cons
    libirwls_amd.LIBIRWLS.  Free with freeDataset. */
 svm_dataset irwls_dataset_from_numpy(const double *X, const double *y, int n, int d, int with_average);
 
+/* Binary dataset cache (no counterpart in the reference; a parsed text
+   dataset saved once reloads without parsing).  irwls_dataset_save writes
+   the dataset's rows (with the two class-mean rows when with_means is
+   nonzero, as the labelled readers build them) and returns 0, or -1 when
+   the file cannot be written.  irwls_dataset_load returns the dataset as
+   it was saved; a file that is not such a cache is reported as "Wrong file
+   format" with exit(2), as the text readers do.  Free with freeDataset. */
+int irwls_dataset_save(const svm_dataset *ds, int with_means, const char *path);
+svm_dataset irwls_dataset_load(const char *path);
+
 /* Batched full training: nprob independent problems (datasets[p],
    props[p]); out[p] receives trainFULL's result for problem p (l_p + 1
    doubles, malloc'd, the caller frees it).  Each problem equals its

@@ -13,6 +13,8 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <ctype.h>
+#include <pthread.h>
+#include <unistd.h>
 #include <math.h>
 #include <errno.h>
 #include <stddef.h>
@@ -295,118 +297,254 @@ static FILE *open_or_die(const char *filename) {
     return f;
 }
 
-/* Reads one line of any length; returns NULL at end of file. */
-static char *read_line(FILE *f, char **buf, size_t *cap) {
-    size_t len = 0;
-    if (!*buf) {
-        *cap = 1 << 16;
-        *buf = (char *)irwls_xmalloc(*cap);
-    }
-    for (;;) {
-        if (!fgets(*buf + len, (int)(*cap - len), f)) return len ? *buf : NULL;
-        len += strlen(*buf + len);
-        if (len > 0 && (*buf)[len - 1] == '\n') return *buf;
-        if (len + 1 < *cap) return *buf;   /* last line without newline */
-        *cap *= 2;
-        *buf = (char *)realloc(*buf, *cap);
-        if (!*buf) {
-            fprintf(stderr, "libirwls: out of host memory\n");
-            exit(2);
-        }
-    }
-}
-
 static int blank_line(const char *s) {
     for (; *s; s++)
         if (!isspace((unsigned char)*s)) return 0;
     return 1;
 }
 
-/* libsvm rows "[label] idx:val idx:val ..." */
-static svm_dataset read_libsvm(const char *filename, int labelled) {
-    FILE *f = open_or_die(filename);
-    rowbuf b;
-    rb_init(&b);
-    char *line = NULL;
-    size_t cap = 0;
-    while (read_line(f, &line, &cap)) {
-        if (blank_line(line)) continue;
-        char *p = line;
-        double label = 0.0;
-        if (labelled) {
-            while (*p == ' ' || *p == '\t') p++;
-            char *end;
-            label = strtod(p, &end);
-            if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
-            p = end;
-        }
-        int last = -1;
-        for (;;) {
-            while (*p == ' ' || *p == '\t') p++;
-            if (*p == '\0' || *p == '\n' || *p == '\r') break;
-            char *end;
-            errno = 0;
-            long ix = strtol(p, &end, 10);
-            if (end == p || *end != ':' || errno != 0 || ix <= last) bad_format();
-            p = end + 1;
-            double v = strtod(p, &end);
-            if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
-            p = end;
-            rb_entry(&b, (int)ix, v);
-            last = (int)ix;
-        }
-        rb_endrow(&b, label);
+/* One libsvm row "[label] idx:val idx:val ..." (the line without its
+   newline) appended to b. */
+static void parse_libsvm_line(char *line, int labelled, rowbuf *b) {
+    char *p = line;
+    double label = 0.0;
+    if (labelled) {
+        while (*p == ' ' || *p == '\t') p++;
+        char *end;
+        label = strtod(p, &end);
+        if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+        p = end;
     }
-    free(line);
+    int last = -1;
+    for (;;) {
+        while (*p == ' ' || *p == '\t') p++;
+        if (*p == '\0' || *p == '\n' || *p == '\r') break;
+        char *end;
+        errno = 0;
+        long ix = strtol(p, &end, 10);
+        if (end == p || *end != ':' || errno != 0 || ix <= last) bad_format();
+        p = end + 1;
+        double v = strtod(p, &end);
+        if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+        p = end;
+        rb_entry(b, (int)ix, v);
+        last = (int)ix;
+    }
+    rb_endrow(b, label);
+}
+
+/* One CSV row "[label]<sep>v1<sep>v2..."; nonzero values are stored with
+   their 1-based column position as index. */
+static void parse_csv_line(char *line, const char *delim, int labelled, rowbuf *b) {
+    char *save = NULL;
+    char *tok = strtok_r(line, delim, &save);
+    double label = 0.0;
+    int column = 0;
+    if (labelled) {
+        if (!tok) bad_format();
+        char *end;
+        label = strtod(tok, &end);
+        if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+        tok = strtok_r(NULL, delim, &save);
+    }
+    for (; tok; tok = strtok_r(NULL, delim, &save)) {
+        column++;
+        if (blank_line(tok)) break;
+        char *end;
+        double v = strtod(tok, &end);
+        if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
+        if (v != 0.0) rb_entry(b, column, v);
+    }
+    rb_endrow(b, label);
+}
+
+/* A run of whole lines [begin, end) of the file image, parsed by one thread
+   into a row store of its own. */
+typedef struct {
+    char *begin, *end;
+    int csv, labelled;
+    const char *delim;
+    rowbuf b;
+} parse_chunk;
+
+static void *parse_lines(void *arg) {
+    parse_chunk *c = (parse_chunk *)arg;
+    char *p = c->begin;
+    while (p < c->end) {
+        char *nl = (char *)memchr(p, '\n', (size_t)(c->end - p));
+        char *eol = nl ? nl : c->end;
+        *eol = '\0';   /* the image is NUL-terminated past its last byte */
+        if (!blank_line(p)) {
+            if (c->csv)
+                parse_csv_line(p, c->delim, c->labelled, &c->b);
+            else
+                parse_libsvm_line(p, c->labelled, &c->b);
+        }
+        p = eol + 1;
+    }
+    return NULL;
+}
+
+/* Reads a whole text dataset: the file image is split at line boundaries
+   into chunks of at least 256 KiB, parsed on up to 16 threads (one row store
+   each), and the stores are joined in file order, so the result equals a
+   single sequential pass line for line. */
+static svm_dataset read_text(const char *filename, int csv, const char *sep, int labelled) {
+    FILE *f = open_or_die(filename);
+    if (fseek(f, 0, SEEK_END) != 0) bad_format();
+    const long sz = ftell(f);
+    if (sz < 0) bad_format();
+    rewind(f);
+    char *img = (char *)irwls_xmalloc((size_t)sz + 1);
+    if ((long)fread(img, 1, (size_t)sz, f) != sz) {
+        fprintf(stderr, "Error reading data file\n");
+        exit(2);
+    }
+    img[sz] = '\0';
     fclose(f);
-    svm_dataset ds = assemble(b.rowptr, b.idx, b.val, labelled ? b.y : NULL, b.n, labelled);
-    rb_free(&b);
+    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    int nt = (int)(sz / (256L << 10));
+    if (nt > 16) nt = 16;
+    if (nt > ncpu) nt = (int)ncpu;
+    if (nt < 1) nt = 1;
+    parse_chunk *ch = (parse_chunk *)irwls_xcalloc((size_t)nt, sizeof(parse_chunk));
+    char *pos = img;
+    for (int t = 0; t < nt; t++) {
+        char *end = img + sz;
+        if (t + 1 < nt) {
+            char *cut = img + (size_t)((double)sz * (t + 1) / nt);
+            if (cut < pos) cut = pos;
+            char *nl = (char *)memchr(cut, '\n', (size_t)(img + sz - cut));
+            end = nl ? nl + 1 : img + sz;
+        }
+        ch[t].begin = pos;
+        ch[t].end = end;
+        ch[t].csv = csv;
+        ch[t].labelled = labelled;
+        ch[t].delim = (sep && *sep) ? sep : ",";
+        rb_init(&ch[t].b);
+        pos = end;
+    }
+    pthread_t *th = (pthread_t *)irwls_xcalloc((size_t)nt, sizeof(pthread_t));
+    for (int t = 1; t < nt; t++)
+        if (pthread_create(&th[t], NULL, parse_lines, &ch[t]) != 0) {
+            fprintf(stderr, "libirwls: cannot start a reader thread\n");
+            exit(2);
+        }
+    parse_lines(&ch[0]);
+    for (int t = 1; t < nt; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(img);
+    /* join the row stores in file order */
+    int n = 0;
+    int64_t nnz = 0;
+    for (int t = 0; t < nt; t++) {
+        n += ch[t].b.n;
+        nnz += ch[t].b.nnz;
+    }
+    int64_t *rowptr = (int64_t *)irwls_xmalloc((size_t)(n + 1) * sizeof(int64_t));
+    int32_t *idx = (int32_t *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+    double *val = (double *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+    double *y = (double *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int r = 0;
+    int64_t z = 0;
+    rowptr[0] = 0;
+    for (int t = 0; t < nt; t++) {
+        const rowbuf *b = &ch[t].b;
+        memcpy(idx + z, b->idx, (size_t)b->nnz * sizeof(int32_t));
+        memcpy(val + z, b->val, (size_t)b->nnz * sizeof(double));
+        memcpy(y + r, b->y, (size_t)b->n * sizeof(double));
+        for (int i = 0; i < b->n; i++) rowptr[r + i + 1] = z + b->rowptr[i + 1];
+        r += b->n;
+        z += b->nnz;
+        rb_free(&ch[t].b);
+    }
+    free(ch);
+    svm_dataset ds = assemble(rowptr, idx, val, labelled ? y : NULL, n, labelled);
+    free(rowptr);
+    free(idx);
+    free(val);
+    free(y);
     return ds;
 }
 
-/* CSV rows "[label]<sep>v1<sep>v2..."; nonzero values are stored with their
-   1-based column position as index. */
+static svm_dataset read_libsvm(const char *filename, int labelled) { return read_text(filename, 0, NULL, labelled); }
 static svm_dataset read_csv(const char *filename, const char *sep, int labelled) {
-    FILE *f = open_or_die(filename);
-    rowbuf b;
-    rb_init(&b);
-    char *line = NULL;
-    size_t cap = 0;
-    const char *delim = (sep && *sep) ? sep : ",";
-    while (read_line(f, &line, &cap)) {
-        if (blank_line(line)) continue;
-        char *save = NULL;
-        char *tok = strtok_r(line, delim, &save);
-        double label = 0.0;
-        int column = 0;
-        if (labelled) {
-            if (!tok) bad_format();
-            char *end;
-            label = strtod(tok, &end);
-            if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
-            tok = strtok_r(NULL, delim, &save);
-        }
-        for (; tok; tok = strtok_r(NULL, delim, &save)) {
-            column++;
-            if (blank_line(tok)) break;
-            char *end;
-            double v = strtod(tok, &end);
-            if (end == tok || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();
-            if (v != 0.0) rb_entry(&b, column, v);
-        }
-        rb_endrow(&b, label);
-    }
-    free(line);
-    fclose(f);
-    svm_dataset ds = assemble(b.rowptr, b.idx, b.val, labelled ? b.y : NULL, b.n, labelled);
-    rb_free(&b);
-    return ds;
+    return read_text(filename, 1, sep, labelled);
 }
 
 svm_dataset readTrainFile(char filename[]) { return read_libsvm(filename, 1); }
 svm_dataset readTrainFileCSV(char filename[], char *separator) { return read_csv(filename, separator, 1); }
 svm_dataset readUnlabeledFile(char filename[]) { return read_libsvm(filename, 0); }
 svm_dataset readUnlabeledFileCSV(char filename[], char *separator) { return read_csv(filename, separator, 0); }
+
+/* ---- binary dataset cache ---------------------------------------------------
+   "IRWLSDS1", then int32 l, sparse, maxdim, rows (l, or l + 2 with the
+   class-mean rows), int64 entry count, then y[rows] (zeros when the dataset
+   has no labels), quadratic_value[rows] and the entries (16-byte index/value
+   pairs, each row ended by index -1) in row order.  Row k starts after the
+   k-th terminator, as in the model file. */
+static const char kDsMagic[8] = {'I', 'R', 'W', 'L', 'S', 'D', 'S', '1'};
+
+int irwls_dataset_save(const svm_dataset *ds, int with_means, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    const int32_t head[4] = {ds->l, ds->sparse, ds->maxdim, ds->l + (with_means ? 2 : 0)};
+    const int rows = head[3];
+    int64_t ne = 0;
+    for (int i = 0; i < rows; i++) {
+        const svm_sample *s = ds->x[i];
+        while (s->index != -1) s++;
+        ne += (s - ds->x[i]) + 1;
+    }
+    int ok = fwrite(kDsMagic, 1, 8, f) == 8 && fwrite(head, sizeof(int32_t), 4, f) == 4 &&
+             fwrite(&ne, sizeof(int64_t), 1, f) == 1;
+    double *y = (double *)irwls_xcalloc((size_t)(rows > 0 ? rows : 1), sizeof(double));
+    if (ds->y) memcpy(y, ds->y, (size_t)rows * sizeof(double));
+    ok = ok && fwrite(y, sizeof(double), (size_t)rows, f) == (size_t)rows;
+    ok = ok && fwrite(ds->quadratic_value, sizeof(double), (size_t)rows, f) == (size_t)rows;
+    free(y);
+    for (int i = 0; ok && i < rows; i++) {
+        const svm_sample *s = ds->x[i];
+        while (s->index != -1) s++;
+        const size_t k = (size_t)(s - ds->x[i]) + 1;
+        ok = fwrite(ds->x[i], sizeof(svm_sample), k, f) == k;
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : -1;
+}
+
+svm_dataset irwls_dataset_load(const char *path) {
+    FILE *f = open_or_die(path);
+    char magic[8];
+    int32_t head[4];
+    int64_t ne = 0;
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kDsMagic, 8) != 0 || fread(head, sizeof(int32_t), 4, f) != 4 ||
+        fread(&ne, sizeof(int64_t), 1, f) != 1 || head[0] < 0 || head[3] < head[0] || ne < head[3])
+        bad_format();
+    svm_dataset ds;
+    memset(&ds, 0, sizeof(ds));
+    ds.l = head[0];
+    ds.sparse = head[1];
+    ds.maxdim = head[2];
+    const int rows = head[3];
+    const size_t rr = (size_t)(rows > 0 ? rows : 1);
+    ds.y = (double *)irwls_xmalloc(rr * sizeof(double));
+    ds.quadratic_value = (double *)irwls_xmalloc(rr * sizeof(double));
+    ds.x = (svm_sample **)irwls_xcalloc(rr, sizeof(svm_sample *));
+    ds.features = (svm_sample *)irwls_xmalloc((size_t)(ne > 0 ? ne : 1) * sizeof(svm_sample));
+    if (fread(ds.y, sizeof(double), (size_t)rows, f) != (size_t)rows ||
+        fread(ds.quadratic_value, sizeof(double), (size_t)rows, f) != (size_t)rows ||
+        fread(ds.features, sizeof(svm_sample), (size_t)ne, f) != (size_t)ne)
+        bad_format();
+    fclose(f);
+    int row = 0;
+    if (rows > 0) ds.x[row++] = ds.features;
+    for (int64_t k = 0; k < ne; k++)
+        if (ds.features[k].index == -1 && row < rows && k + 1 < ne) ds.x[row++] = &ds.features[k + 1];
+    if (row != rows || ds.features[ne - 1].index != -1) bad_format();
+    return ds;
+}
 
 /* ---- model file ------------------------------------------------------------ */
 

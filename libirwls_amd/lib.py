@@ -90,6 +90,8 @@ def load():
         "irwls_dataset_from_dense": (svm_dataset, [dp, dp, C.c_int, C.c_int]),
         "irwls_dataset_from_csr": (svm_dataset, [P(C.c_longlong), ip, dp, dp, C.c_int]),
         "irwls_dataset_from_numpy": (svm_dataset, [dp, dp, C.c_int, C.c_int, C.c_int]),
+        "irwls_dataset_save": (C.c_int, [P(svm_dataset), C.c_int, C.c_char_p]),
+        "irwls_dataset_load": (svm_dataset, [C.c_char_p]),
         "irwls_gpu_get_options": (None, [P(irwls_gpu_options)]),
         "irwls_gpu_set_options": (None, [P(irwls_gpu_options)]),
         "irwls_gpu_pin_dataset": (None, [svm_dataset, C.c_int]),
@@ -195,6 +197,15 @@ class Dataset:
         else:
             ds = L.readUnlabeledFileCSV(p, sep) if csv else L.readUnlabeledFile(p)
         return cls(ds)
+
+    @classmethod
+    def load(cls, path):
+        """A dataset saved with save() (the library's binary cache)."""
+        return cls(load().irwls_dataset_load(os.fspath(path).encode()))
+
+    def save(self, path, with_means=True):
+        if load().irwls_dataset_save(C.byref(self.ds), 1 if with_means else 0, os.fspath(path).encode()) != 0:
+            raise OSError("cannot write %s" % path)
 
     def arrays(self, with_means=True):
         return dataset_arrays(self.ds, with_means)

@@ -131,6 +131,49 @@ def _compare_features(mine, ref, rows):
         assert a[key] == b[key], key
 
 
+@pytest.mark.parametrize("fmt", ["libsvm", "csv"])
+def test_large_file_read_in_parallel_chunks(tmp_path, fmt):
+    """Files of a few MB are parsed in chunks on several threads; the
+    joined result equals the reference's single pass."""
+    rng = np.random.default_rng(17)
+    path = str(tmp_path / ("big." + fmt))
+    if fmt == "libsvm":
+        _libsvm(path, 5000, 60, rng)
+    else:
+        _csv(path, 5000, 80, rng, ",")
+    assert os.path.getsize(path) > 1 << 20   # at least four chunks of 256 KiB
+    mine = lib.Dataset.read(path, csv=(fmt == "csv"))
+    ref = (refio.lib().readTrainFileCSV(path.encode(), b",") if fmt == "csv"
+           else refio.lib().readTrainFile(path.encode()))
+    _compare(mine.ds, ref, ref.l + 2)
+    refio.lib().freeDataset(ref)
+
+
+@pytest.mark.parametrize("labelled", [True, False])
+def test_binary_cache_round_trip(tmp_path, labelled):
+    """A parsed dataset saved to the binary cache loads back equal to the
+    reference reader's result, class-mean rows included; a file that is not
+    a cache is rejected the way the text readers reject bad input."""
+    rng = np.random.default_rng(18)
+    path, cache = str(tmp_path / "d.libsvm"), tmp_path / "d.irwlsds"
+    _libsvm(path, 300, 40, rng, labelled=labelled)
+    lib.Dataset.read(path, labelled=labelled).save(cache, with_means=labelled)
+    back = lib.Dataset.load(cache)
+    if labelled:
+        ref = refio.lib().readTrainFile(path.encode())
+        _compare(back.ds, ref, ref.l + 2)
+    else:
+        ref = refio.lib().readUnlabeledFile(path.encode())
+        _compare_features(back.ds, ref, ref.l)
+    refio.lib().freeDataset(ref)
+    import subprocess
+    import sys
+    code = "from libirwls_amd import lib; lib.Dataset.load(%r)" % path
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 2 and "Wrong file format" in r.stderr
+
+
 def test_model_file_both_ways(tmp_path):
     """A model built here, written by this library and by the reference,
     gives the same bytes; each side reads the other's file back."""
