@@ -102,6 +102,15 @@ svm_dataset irwls_dataset_load(const char *path);
    in the reference. */
 int trainFULLBatched(const svm_dataset *datasets, const properties *props, int nprob, double **out);
 
+/* Decision values of nmodels models on one test set (no counterpart in the
+   reference; one-vs-rest bundles and grids of models): out[g * l + i] is
+   what softTest(dataset, models[g], ...) returns for row i, bit for bit.
+   The test rows are uploaded once and the models' launches are queued
+   without waiting in between; out holds l * nmodels doubles (caller's
+   memory).  With props.Labels == 1 each model's accuracy is printed as
+   softTest prints it.  Returns 0. */
+int softTestMany(svm_dataset dataset, const model *models, int nmodels, predictProperties props, double *out);
+
 /* Release the device context (initMemory/freeMemory do this for callers of
    the reference interface). */
 void irwls_gpu_shutdown(void);

@@ -9,6 +9,19 @@
 #include "internal.h"
 #include "LIBIRWLS-predict.h"
 
+/* The support vectors of a model viewed as a dataset. */
+static svm_dataset sv_view(const model *m) {
+    svm_dataset sv;
+    memset(&sv, 0, sizeof(sv));
+    sv.l = m->nSVs;
+    sv.sparse = m->sparse;
+    sv.maxdim = m->maxdim;
+    sv.x = m->x;
+    sv.quadratic_value = m->quadratic_value;
+    sv.features = m->features;
+    return sv;
+}
+
 static double *decision_values(svm_dataset dataset, model m) {
     const int n = dataset.l;
     double *out = (double *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
@@ -18,15 +31,7 @@ static double *decision_values(svm_dataset dataset, model m) {
         return out;
     }
     irwls_dev_init();
-    /* model support vectors viewed as a dataset */
-    svm_dataset sv;
-    memset(&sv, 0, sizeof(sv));
-    sv.l = m.nSVs;
-    sv.sparse = m.sparse;
-    sv.maxdim = m.maxdim;
-    sv.x = m.x;
-    sv.quadratic_value = m.quadratic_value;
-    sv.features = m.features;
+    svm_dataset sv = sv_view(&m);
     const int bt = irwls_dataset_base(&dataset, n), bm = irwls_dataset_base(&sv, m.nSVs);
     const int base = (bt < bm) ? bt : bm;
     const int mt = irwls_dataset_maxindex(&dataset, n), mm = irwls_dataset_maxindex(&sv, m.nSVs);
@@ -59,6 +64,61 @@ static void print_accuracy(const svm_dataset *ds, const double *pred) {
         if (pred[i] <= 0 && ds->y[i] <= 0) hits++;
     }
     printf("Accuracy: %f\n", hits / (double)ds->l);
+}
+
+int softTestMany(svm_dataset dataset, const model *models, int nmodels, predictProperties props, double *out) {
+    const int n = dataset.l;
+    if (n <= 0 || nmodels <= 0) return 0;
+    irwls_dev_init();
+    const int bt = irwls_dataset_base(&dataset, n), mt = irwls_dataset_maxindex(&dataset, n);
+    double *d_out = (double *)irwls_dev_alloc((size_t)n * nmodels * sizeof(double));
+    /* kept until the single read-back, so no release waits on a queued launch */
+    irwls_devdata **dt = (irwls_devdata **)irwls_xcalloc((size_t)nmodels, sizeof(irwls_devdata *));
+    irwls_devdata **dm = (irwls_devdata **)irwls_xcalloc((size_t)nmodels, sizeof(irwls_devdata *));
+    double **d_w = (double **)irwls_xcalloc((size_t)nmodels, sizeof(double *));
+    for (int g = 0; g < nmodels; g++) {
+        const model *m = &models[g];
+        double *og = d_out + (size_t)g * n;
+        if (m->nSVs <= 0) {
+            irwls_rows r, none;
+            memset(&r, 0, sizeof(r));
+            memset(&none, 0, sizeof(none));
+            r.n = n;
+            irwls_dev_kernel_matvec(&r, &none, 0, m->kernelType, m->Kgamma, NULL, 0.0, 1.0, m->bias, NULL, og);
+            continue;
+        }
+        /* the same column layout softTest uses for this model (the device
+           copy of the test rows is shared between models that agree on it) */
+        svm_dataset sv = sv_view(m);
+        const int bm = irwls_dataset_base(&sv, sv.l), mm = irwls_dataset_maxindex(&sv, sv.l);
+        const int base = (bt < bm) ? bt : bm;
+        const int cols = ((mt > mm) ? mt : mm) + 1 - base;
+        dt[g] = irwls_devdata_acquire(&dataset, 0, base, cols);
+        dm[g] = irwls_devdata_acquire(&sv, 0, base, cols);
+        const int d = (dt[g]->d > dm[g]->d) ? dt[g]->d : dm[g]->d;
+        irwls_rows r = irwls_devdata_rows(dt[g], NULL, n);
+        irwls_rows c = irwls_devdata_rows(dm[g], NULL, m->nSVs);
+        r.nrm = dt[g]->qv;
+        c.nrm = dm[g]->qv;
+        if (r.X) r.ldx = dt[g]->d;
+        if (c.X) c.ldx = dm[g]->d;
+        d_w[g] = (double *)irwls_dev_alloc((size_t)m->nSVs * sizeof(double));
+        irwls_dev_h2d(d_w[g], m->weights, (size_t)m->nSVs * sizeof(double));
+        irwls_dev_kernel_matvec(&r, &c, d, m->kernelType, m->Kgamma, d_w[g], 0.0, 1.0, m->bias, NULL, og);
+    }
+    irwls_dev_d2h(out, d_out, (size_t)n * nmodels * sizeof(double));
+    for (int g = 0; g < nmodels; g++) {
+        if (d_w[g]) irwls_dev_free(d_w[g]);
+        if (dm[g]) irwls_devdata_release(dm[g]);
+        if (dt[g]) irwls_devdata_release(dt[g]);
+    }
+    free(d_w);
+    free(dm);
+    free(dt);
+    irwls_dev_free(d_out);
+    if (props.Labels == 1)
+        for (int g = 0; g < nmodels; g++) print_accuracy(&dataset, out + (size_t)g * n);
+    return 0;
 }
 
 double *softTest(svm_dataset dataset, model mymodel, predictProperties props) {

@@ -69,6 +69,7 @@ def load():
         "calculateBudgetedModel": (model, [properties, svm_dataset, ip, dp]),
         "test": (dp, [svm_dataset, model, predictProperties]),
         "softTest": (dp, [svm_dataset, model, predictProperties]),
+        "softTestMany": (C.c_int, [svm_dataset, P(model), C.c_int, predictProperties, dp]),
         "kernelFunction": (C.c_double, [svm_dataset, C.c_int, C.c_int, properties]),
         "kernelTest": (C.c_double, [svm_dataset, C.c_int, model, C.c_int]),
         "initMemory": (None, [C.c_int, C.c_int]),
@@ -311,6 +312,22 @@ def budgeted_model(props, ds, centroids, beta):
 
 def soft_test(ds, m, pprops):
     return take_doubles(load().softTest(ds.ds, m.m, pprops), ds.ds.l)
+
+
+def soft_test_many(ds, models, pprops):
+    """Decision values of several models on one dataset (softTestMany):
+    array (len(models), rows), row g equal to soft_test(ds, models[g])."""
+    arr = (model * len(models))(*[m.m for m in models])
+    out = np.zeros((len(models), ds.ds.l))
+    load().softTestMany(ds.ds, arr, len(models), pprops, _dptr(out))
+    return out
+
+
+def one_vs_rest(ds, models, pprops, classes=None):
+    """Class of each row from a one-vs-rest bundle: the model with the largest
+    decision value (classes[g] for model g, or g)."""
+    g = np.argmax(soft_test_many(ds, models, pprops), axis=0)
+    return g if classes is None else np.asarray(classes)[g]
 
 
 def hard_test(ds, m, pprops):
