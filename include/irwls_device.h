@@ -88,7 +88,8 @@ void irwls_dev_row_norms(const double *d_X, int ldx, int n, int d, double *d_nrm
 
 /* out[i] = a*out[i] + rs[i]*(b * sum_j w[j] k(r_i, c_j)) + c
    (rs NULL means 1).  Deterministic: fixed summation order for a given
-   (rows, cols) shape. */
+   (rows, cols) shape.  With the linear kernel the block is not formed:
+   u = sum_j w[j] c_j over the features, then out from r_i . u. */
 void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d,
                              int kernel, double gamma, const double *d_w,
                              double a, double b, double cst, const double *d_rs,

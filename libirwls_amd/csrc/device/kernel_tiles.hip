@@ -526,6 +526,72 @@ __global__ void axpb_kernel(int n, double a, double cst, double *__restrict__ ou
     if (i < n) out[i] = (a == 0.0 ? 0.0 : a * out[i]) + cst;
 }
 
+// ---- linear kernel: K w = X u with u = sum_j w_j z_j -------------------------
+// For k(x, z) = x.z the kernel block never needs forming: two passes over the
+// rows' and the columns' features replace rows x cols x d flop by
+// (rows + cols) x d.  Fixed summation orders throughout.
+constexpr int PRIMAL_SPLITS = 64;   // column ranges summed separately, then in range order
+
+// part[s*d + k] = sum over the columns j of range s of w_j z_j[k] (thread per feature).
+__global__ void primal_part_dense(DenseSide cs, int d, const double *__restrict__ w, int per,
+                                  double *__restrict__ part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k >= d) return;
+    const int j0 = s * per;
+    const int j1 = (j0 + per < cs.n) ? j0 + per : cs.n;
+    double acc = 0.0;
+    for (int j = j0; j < j1; j++) acc = fma(w[j], cs.X[(size_t)(cs.idx ? cs.idx[j] : j) * cs.ldx + k], acc);
+    part[(size_t)s * d + k] = acc;
+}
+
+__global__ void primal_sum(const double *__restrict__ part, int nsplit, int d, double *__restrict__ u) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    double s = 0.0;
+    for (int p = 0; p < nsplit; p++) s += part[(size_t)p * d + k];
+    u[k] = s;
+}
+
+__device__ __forceinline__ void primal_store(int i, int gi, double s, double a, double b, double cst,
+                                             const double *__restrict__ rsc, double *__restrict__ out) {
+    const double sc = rsc ? rsc[i] : 1.0;
+    out[gi] = (a == 0.0 ? 0.0 : a * out[gi]) + sc * (b * s) + cst;
+}
+
+// out_i = a out_i + rs_i (b x_i.u) + cst, one wave per dense row.
+__global__ __launch_bounds__(256) void primal_apply_dense(DenseSide rsd, int d, const double *__restrict__ u, double a,
+                                                          double b, double cst, const double *__restrict__ rsc,
+                                                          double *__restrict__ out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= rsd.n) return;
+    const double *x = rsd.X + (size_t)(rsd.idx ? rsd.idx[i] : i) * rsd.ldx;
+    double s = 0.0;
+    for (int k = lane; k < d; k += 64) s = fma(x[k], u[k], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) primal_store(i, i, s, a, b, cst, rsc, out);
+}
+
+// The same for CSR rows: their nonzeros against u.
+__global__ __launch_bounds__(256) void primal_apply_csr(const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ val,
+                                                        const int32_t *__restrict__ ridx, int n,
+                                                        const double *__restrict__ u, double a, double b, double cst,
+                                                        const double *__restrict__ rsc, double *__restrict__ out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const int row = ridx ? ridx[i] : i;
+    double s = 0.0;
+    for (int64_t p = rowptr[row] + lane; p < rowptr[row + 1]; p += 64) s = fma(val[p], u[col[p]], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) primal_store(i, i, s, a, b, cst, rsc, out);
+}
+
 DenseSide dense_side(const irwls_rows *r) {
     DenseSide s;
     s.X = r->X;
@@ -592,6 +658,28 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
     if (r->n <= 0) return;
     if (c->n <= 0) {
         hipLaunchKernelGGL(axpb_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->n, a, cst, out);
+        IRWLS_LAUNCH_CHECK();
+        return;
+    }
+    if (kernel == 0 && d > 0) {
+        // linear kernel: u = sum_j w_j z_j, then one dot product per row
+        DenseSide csd = dense_cols(c, d, 6, 7);
+        const int ns = c->n < PRIMAL_SPLITS ? c->n : PRIMAL_SPLITS;
+        const int per = (c->n + ns - 1) / ns;
+        double *part = (double *)scratch(3, (size_t)(ns + 1) * d * sizeof(double));
+        double *u = part + (size_t)ns * d;
+        const double nz = r->X ? (double)r->n * d : r->nnz_per_row * r->n;
+        TimerScope ts(IRWLS_T_KMV, 2.0 * ((double)c->n * d + nz), 8.0 * ((double)c->n * d + nz + 2.0 * r->n));
+        hipLaunchKernelGGL(primal_part_dense, dim3((d + 255) / 256, ns), dim3(256), 0, stream(), csd, d, w, per, part);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(primal_sum, dim3((d + 255) / 256), dim3(256), 0, stream(), part, ns, d, u);
+        IRWLS_LAUNCH_CHECK();
+        if (r->X)
+            hipLaunchKernelGGL(primal_apply_dense, dim3((r->n + 3) / 4), dim3(256), 0, stream(), dense_side(r), d, u,
+                               a, b, cst, rs, out);
+        else
+            hipLaunchKernelGGL(primal_apply_csr, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
+                               r->val, r->idx, r->n, u, a, b, cst, rs, out);
         IRWLS_LAUNCH_CHECK();
         return;
     }
