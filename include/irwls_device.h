@@ -196,6 +196,13 @@ void irwls_dev_release_mlp_buffers(void);
    on SPD systems of that batch's sizes.  Returns 0. */
 int  irwls_dev_posv_aug_batched(int nprob, double *const *d_A, const int *n, const int *lda,
                                 const double *const *d_rhs, double *const *d_x, int *info);
+/* The same on round slot `round` (0..3, irwls_dev_posv_aug_batched uses 0):
+   each slot has its own captured sequences and device buffers, so calls
+   with different slots may run at the same time from different host threads
+   (each on its own stream, irwls_dev_thread_begin); calls with one slot must
+   not overlap. */
+int  irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *d_A, const int *n, const int *lda,
+                                     const double *const *d_rhs, double *const *d_x, int *info);
 void irwls_dev_batched_autotune(int enable);
 /* Rows per work-group (in 64-row tiles) chosen for batches whose largest
    size rounds up to the same multiple of 256 as nmax, 0 if none yet; ms3

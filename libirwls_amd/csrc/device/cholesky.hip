@@ -25,6 +25,8 @@
 // boundary on the library stream.
 #include "common.h"
 
+#include <mutex>
+
 #include <utility>
 
 using namespace irwls;
@@ -710,6 +712,7 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStr
 // per capacity, on the first batch that has it.
 struct BatchPlan {
     int nslot = 0, ncap = 0;
+    int round = -1;           // the round slot whose calls use this plan
     BatchProb *d_tab = nullptr;
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
 };
@@ -719,24 +722,34 @@ struct BatchTune {
     double ms[3] = {0, 0, 0};
 };
 constexpr int kShapes[3] = {1, 2, 4};
-constexpr int kMaxPlans = 64;
+constexpr int kRounds = 4;     // round slots: batches that may run at once, one host thread each
+constexpr int kMaxPlans = 256;
 constexpr int kMaxTunes = 64;
 BatchPlan g_plans[kMaxPlans];
 int g_nplans = 0;
 BatchTune g_tunes[kMaxTunes];
 int g_ntunes = 0;
 int g_autotune = 1;
+// the plan and tuning tables are shared by the round slots; a plan and the
+// buffer pool of a slot are used only by the thread running that slot
+std::mutex g_bmu;
 
 int slots_for(int nprob) { return (nprob + 15) / 16 * 16; }
 
-BatchPlan *plan_for(int nslot, int ncap) {
+// Called with g_bmu held.
+BatchPlan *plan_for(int nslot, int ncap, int round) {
     for (int i = 0; i < g_nplans; i++)
-        if (g_plans[i].nslot == nslot && g_plans[i].ncap == ncap) return &g_plans[i];
-    BatchPlan *pl;
+        if (g_plans[i].nslot == nslot && g_plans[i].ncap == ncap && g_plans[i].round == round) return &g_plans[i];
+    BatchPlan *pl = nullptr;
     if (g_nplans < kMaxPlans) {
         pl = &g_plans[g_nplans++];
-    } else {   // evict the oldest
-        pl = &g_plans[0];
+    } else {   // evict the oldest plan of the same round slot (no other thread uses it)
+        for (int i = 0; i < kMaxPlans && !pl; i++)
+            if (g_plans[i].round == round) pl = &g_plans[i];
+        if (!pl) {
+            fprintf(stderr, "libirwls: batched solve: no plan entry free for round slot %d\n", round);
+            exit(2);
+        }
         for (int e = 0; e < 3; e++)
             if (pl->exec[e]) IRWLS_HIP(hipGraphExecDestroy(pl->exec[e]));
         irwls_dev_free(pl->d_tab);
@@ -744,10 +757,12 @@ BatchPlan *plan_for(int nslot, int ncap) {
     }
     pl->nslot = nslot;
     pl->ncap = ncap;
+    pl->round = round;
     pl->d_tab = (BatchProb *)irwls_dev_alloc((size_t)nslot * sizeof(BatchProb));
     return pl;
 }
 
+// Called with g_bmu held.
 BatchTune *tune_for(int ncap) {
     for (int i = 0; i < g_ntunes; i++)
         if (g_tunes[i].ncap == ncap) return &g_tunes[i];
@@ -760,9 +775,11 @@ BatchTune *tune_for(int ncap) {
 hipGraphExec_t graph_for(BatchPlan *pl, int e) {
     if (pl->exec[e]) return pl->exec[e];
     hipGraph_t g;
+    capture_lock();
     IRWLS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
     batched_sequence(pl->d_tab, pl->nslot, pl->ncap, kShapes[e], stream());
     IRWLS_HIP(hipStreamEndCapture(stream(), &g));
+    capture_unlock();
     IRWLS_HIP(hipGraphInstantiate(&pl->exec[e], g, nullptr, nullptr, 0));
     IRWLS_HIP(hipGraphDestroy(g));
     return pl->exec[e];
@@ -870,7 +887,17 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
 
 int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const int *lda, const double *const *rhs,
                                double *const *x, int *info) {
+    return irwls_dev_posv_aug_batched_slot(0, nprob, A, n, lda, rhs, x, info);
+}
+
+int irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *A, const int *n, const int *lda,
+                                    const double *const *rhs, double *const *x, int *info) {
     if (nprob <= 0) return 0;
+    if (round < 0 || round >= kRounds) {
+        fprintf(stderr, "libirwls: irwls_dev_posv_aug_batched_slot: round slot %d outside [0, %d)\n", round,
+                kRounds);
+        exit(2);
+    }
     int nmax = 0;
     for (int p = 0; p < nprob; p++) {
         if (lda[p] < n[p] + 1) {
@@ -882,12 +909,19 @@ int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const 
     }
     const int ncap = (nmax + 255) / 256 * 256;
     const int nslot = slots_for(nprob);
-    BatchPlan *pl = plan_for(nslot, ncap);
-    BatchTune *tu = tune_for(ncap);
+    BatchPlan *pl;
+    BatchTune *tu;
+    {
+        std::lock_guard<std::mutex> lk(g_bmu);
+        pl = plan_for(nslot, ncap, round);
+        tu = tune_for(ncap);
+    }
     // per-problem info words (together, read back with one copy), tile
-    // inverses and forward-sweep vectors
-    static double *pool = nullptr;
-    static size_t pool_sz = 0;
+    // inverses and forward-sweep vectors; one pool per round slot
+    static double *pools[kRounds] = {};
+    static size_t pool_szs[kRounds] = {};
+    double *&pool = pools[round];
+    size_t &pool_sz = pool_szs[round];
     size_t need = (size_t)nslot * 2 + 8;   // 4 info ints per slot
     for (int p = 0; p < nprob; p++) need += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE + (size_t)n[p] + 8;
     if (pool_sz < need) {
@@ -911,16 +945,22 @@ int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const 
         t[p].rhs = rhs[p];
         t[p].x = x[p];
     }
-    if (tu->best < 0) {
-        if (g_autotune) autotune(pl, tu, t);
-        else tu->best = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_bmu);   // one tuning per capacity, whichever slot gets there first
+        if (tu->best < 0) {
+            if (g_autotune) autotune(pl, tu, t);
+            else tu->best = 0;
+        }
     }
     irwls_dev_h2d(pl->d_tab, t, (size_t)nslot * sizeof(BatchProb));
     {
+        // capture (first use of this class on this slot) before the timer's
+        // first event is recorded on this stream
+        hipGraphExec_t ex = graph_for(pl, tu->best);
         double fl = 0.0;
         for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
-        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true);   // one batch runs at a time
-        IRWLS_HIP(hipGraphLaunch(graph_for(pl, tu->best), stream()));
+        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true);   // batches of different round slots may overlap
+        IRWLS_HIP(hipGraphLaunch(ex, stream()));
     }
     int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));
     irwls_dev_d2h(hinfo, d_info, (size_t)nslot * 4 * sizeof(int));

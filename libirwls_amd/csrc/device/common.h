@@ -42,6 +42,7 @@ struct TimerScope {
     int kind;
     double flops, bytes;
     bool on;
+    int idx;   // event pair reserved for this launch
     // any_thread: also on worker threads (for a launch that only one thread
     // at a time can make, such as a batched solve)
     TimerScope(int kind, double flops, double bytes, bool any_thread = false);
@@ -53,6 +54,11 @@ void *scratch(int slot, size_t bytes);
 
 // Compute units of the current device.
 int device_cus();
+
+// Held around a stream capture: no timer event is recorded or read meanwhile
+// (an event last recorded on a capturing stream cannot be used).
+void capture_lock();
+void capture_unlock();
 
 // Column splits of a kernel-block x vector launch over rt row tiles and ct
 // column tiles, for a kernel of which `per_cu` work-groups fit on a compute

@@ -2,6 +2,8 @@
 #include "common.h"
 
 #include <string.h>
+
+#include <mutex>
 #include <time.h>
 
 namespace irwls {
@@ -29,36 +31,65 @@ static double g_ms[IRWLS_T_COUNT], g_flops[IRWLS_T_COUNT], g_bytes[IRWLS_T_COUNT
 static const int kPool = 4096;
 static hipEvent_t g_evs[kPool][2];
 static int g_evkind[kPool];
+static bool g_evdone[kPool];
 static int g_evn = 0;
 static bool g_pool_ready = false;
+// Timed launches of several host threads (batched solves of different round
+// slots) may overlap: each reserves its event pair under this lock.
+static std::mutex g_tmu;
 
-static void resolve_pool() {
-    if (g_evn == 0) return;
-    IRWLS_HIP(hipEventSynchronize(g_evs[g_evn - 1][1]));
+// Reads the elapsed times of the recorded pairs; with g_tmu held, and only
+// when every reserved pair has been recorded.  One device-wide wait instead
+// of a wait per event: the streams the pairs were recorded on may have been
+// destroyed since (worker threads end theirs).
+static bool resolve_pool_locked() {
+    if (g_evn == 0) return true;
+    for (int i = 0; i < g_evn; i++)
+        if (!g_evdone[i]) return false;
+    IRWLS_HIP(hipDeviceSynchronize());
     for (int i = 0; i < g_evn; i++) {
         float ms = 0.f;
         IRWLS_HIP(hipEventElapsedTime(&ms, g_evs[i][0], g_evs[i][1]));
         g_ms[g_evkind[i]] += ms;
     }
     g_evn = 0;
+    return true;
+}
+
+static void resolve_pool() {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    resolve_pool_locked();
 }
 
 TimerScope::TimerScope(int k, double f, double b, bool any_thread)
-    : kind(k), flops(f), bytes(b), on(g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)) {
+    : kind(k), flops(f), bytes(b), on(g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)),
+      idx(-1) {
     if (!on) return;
-    if (g_evn == kPool) resolve_pool();
-    IRWLS_HIP(hipEventRecord(g_evs[g_evn][0], stream()));
+    std::lock_guard<std::mutex> lk(g_tmu);
+    if (g_evn == kPool && !resolve_pool_locked()) {
+        on = false;   // pool full of pairs still in flight: this launch goes untimed
+        return;
+    }
+    idx = g_evn++;
+    g_evdone[idx] = false;
+    IRWLS_HIP(hipEventRecord(g_evs[idx][0], stream()));
 }
 
 TimerScope::~TimerScope() {
     if (!on) return;
-    IRWLS_HIP(hipEventRecord(g_evs[g_evn][1], stream()));
-    g_evkind[g_evn] = kind;
-    g_evn++;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    IRWLS_HIP(hipEventRecord(g_evs[idx][1], stream()));
+    g_evkind[idx] = kind;
+    g_evdone[idx] = true;
     g_launch[kind] += 1;
     g_flops[kind] += flops;
     g_bytes[kind] += bytes;
 }
+
+// Graph capture excludes every use of the timer events: HIP refuses an
+// event operation while the stream the event was last recorded on captures.
+void capture_lock() { g_tmu.lock(); }
+void capture_unlock() { g_tmu.unlock(); }
 
 int device_cus() {
     static int cus = 0;

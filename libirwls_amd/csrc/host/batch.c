@@ -7,15 +7,16 @@
  * (irwls_dev_thread_begin), so the problems' kernel-block products, KKT
  * assembly and vector updates overlap on the device.  A problem that
  * reaches its KKT solve (src/full-train.c:192-199) finishes its stream's
- * work, queues the solve and waits; when every problem still running is
- * waiting, or when the earliest waiting one has waited a set time
- * (IRWLS_BATCH_WAIT_US, default 1000 microseconds) and no batch is running,
- * one batched factorization and solve runs for the waiting ones
- * (irwls_dev_posv_aug_batched: one replay of a captured launch sequence,
- * tile shape chosen by its autotuner), and they continue; problems that
- * queue while it runs join the next batch.  The time limit keeps a problem
- * busy with a longer step (an outer iteration's kernel products) from
- * holding back the others.
+ * work, queues the solve and waits; when every problem that is not already
+ * in a running batch is waiting, or when the earliest waiting one has waited
+ * a set time (IRWLS_BATCH_WAIT_US, default 1000 microseconds), one batched
+ * factorization and solve runs for the waiting ones
+ * (irwls_dev_posv_aug_batched_slot: one replay of a captured launch
+ * sequence, tile shape chosen by its autotuner), and they continue.  Up to
+ * BATCH_SLOTS batches run at once, each on the stream of the waiting problem
+ * whose thread started it, so a problem that queues while a batch runs need
+ * not wait for that batch.  The time limit keeps a problem busy with a longer
+ * step (an outer iteration's kernel products) from holding back the others.
  * The per-problem arithmetic is that of the standalone solve, so each
  * problem's result is bit-identical to its standalone trainFULL run.
  *
@@ -34,13 +35,16 @@
 #include "internal.h"
 #include "full-train.h"
 
+#define BATCH_SLOTS 4     /* batches in flight at once (round slots of the device solver) */
+
 typedef struct {
     pthread_mutex_t mu;
     pthread_cond_t cv;
     int nprob, active, npend, rounds;
-    int running;          /* a batch is being solved (lock released meanwhile) */
+    int inflight;         /* problems in running batches (lock released meanwhile) */
+    unsigned busy;        /* round slots in use */
     double wait_s;        /* longest wait before a partial batch runs */
-    int *pending;         /* 0 idle, 1 queued, 2 in the running batch */
+    int *pending;         /* 0 idle, 1 queued, 2 in a running batch */
     double *t_queue;      /* when each queued problem queued */
     double **A;
     int *n, *lda;
@@ -72,10 +76,18 @@ int irwls_rand(void) {
     return rand();
 }
 
-/* Runs the queued solves.  Called with the lock held and no batch running;
-   releases the lock during the device work and returns with it held. */
-static void run_batch(batch_sched *s) {
+/* A free round slot, or -1. */
+static int free_slot(const batch_sched *s) {
+    for (int k = 0; k < BATCH_SLOTS; k++)
+        if (!((s->busy >> k) & 1u)) return k;
+    return -1;
+}
+
+/* Runs the queued solves on round slot `slot`.  Called with the lock held;
+   releases it during the device work and returns with it held. */
+static void run_batch(batch_sched *s, int slot) {
     const int k = s->npend;
+    int *ids = (int *)irwls_xmalloc((size_t)k * sizeof(int));
     double **A = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
     int *n = (int *)irwls_xmalloc((size_t)k * sizeof(int));
     int *lda = (int *)irwls_xmalloc((size_t)k * sizeof(int));
@@ -84,6 +96,7 @@ static void run_batch(batch_sched *s) {
     int m = 0;
     for (int i = 0; i < s->nprob && m < k; i++) {
         if (s->pending[i] != 1) continue;
+        ids[m] = i;
         A[m] = s->A[i];
         n[m] = s->n[i];
         lda[m] = s->lda[i];
@@ -93,15 +106,17 @@ static void run_batch(batch_sched *s) {
         m++;
     }
     s->npend = 0;
-    s->running = 1;
+    s->inflight += m;
+    s->busy |= 1u << slot;
     pthread_mutex_unlock(&s->mu);
-    irwls_dev_posv_aug_batched(m, A, n, lda, rhs, x, NULL);
+    irwls_dev_posv_aug_batched_slot(slot, m, A, n, lda, rhs, x, NULL);
     irwls_dev_sync();
     pthread_mutex_lock(&s->mu);
-    for (int i = 0; i < s->nprob; i++)
-        if (s->pending[i] == 2) s->pending[i] = 0;
-    s->running = 0;
+    for (int j = 0; j < m; j++) s->pending[ids[j]] = 0;
+    s->inflight -= m;
+    s->busy &= ~(1u << slot);
     s->rounds++;
+    free(ids);
     free(A);
     free(n);
     free(lda);
@@ -133,10 +148,12 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
     s->t_queue[me] = irwls_now();
     s->npend++;
     while (s->pending[me] != 0) {
-        if (!s->running && s->npend > 0) {
+        /* a thread whose own problem is queued starts the next batch, on a free round slot */
+        const int slot = (s->pending[me] == 1) ? free_slot(s) : -1;
+        if (slot >= 0) {
             const double due = first_queued(s) + s->wait_s;
-            if (s->npend == s->active || irwls_now() >= due) {
-                run_batch(s);
+            if (s->npend == s->active - s->inflight || irwls_now() >= due) {
+                run_batch(s, slot);
                 continue;
             }
             struct timespec ts;

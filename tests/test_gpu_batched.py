@@ -70,9 +70,14 @@ def test_posv_aug_batched_equals_single_and_replays():
     assert choice in (1, 2, 4)
 
 
-def test_train_full_batched_equals_standalone():
+@pytest.mark.parametrize("wait_us", [None, "50"])
+def test_train_full_batched_equals_standalone(wait_us, monkeypatch):
     """Six problems of uneven size (two datasets, three C/gamma points):
-    every batched result equals its standalone trainFULL run."""
+    every batched result equals its standalone trainFULL run.  With a 50 us
+    trigger, batches start as soon as a problem has waited that long, so
+    several run at once on different round slots."""
+    if wait_us is not None:
+        monkeypatch.setenv("IRWLS_BATCH_WAIT_US", wait_us)
     tasks = []
     for seed, (n, d) in enumerate([(900, 8), (1600, 16)]):
         X, y = synth.two_gaussians(n, d, seed=900 + seed)
