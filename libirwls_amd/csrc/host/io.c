@@ -542,7 +542,7 @@ svm_dataset irwls_dataset_load(const char *path) {
     if (rows > 0) ds.x[row++] = ds.features;
     for (int64_t k = 0; k < ne; k++)
         if (ds.features[k].index == -1 && row < rows && k + 1 < ne) ds.x[row++] = &ds.features[k + 1];
-    if (row != rows || ds.features[ne - 1].index != -1) bad_format();
+    if (row != rows || (ne > 0 && ds.features[ne - 1].index != -1)) bad_format();
     return ds;
 }
 

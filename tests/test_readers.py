@@ -174,6 +174,15 @@ def test_binary_cache_round_trip(tmp_path, labelled):
     assert r.returncode == 2 and "Wrong file format" in r.stderr
 
 
+def test_binary_cache_of_an_empty_dataset(tmp_path):
+    """An empty unlabelled dataset round-trips through the cache."""
+    path, cache = tmp_path / "empty.libsvm", tmp_path / "empty.irwlsds"
+    path.write_text("")
+    lib.Dataset.read(str(path), labelled=False).save(cache, with_means=False)
+    back = lib.Dataset.load(cache)
+    assert back.ds.l == 0
+
+
 def test_model_file_both_ways(tmp_path):
     """A model built here, written by this library and by the reference,
     gives the same bytes; each side reads the other's file back."""
