@@ -44,136 +44,253 @@ int *info_buf() {
 }
 
 // ------------------------------------------------- diagonal tile factor ---
-// The 64 x 64 tile is held in registers: wave c0 owns the columns
-// c = c0 + 4q (q = 0..15) and lane i row i, both of the tile (a[q]) and of
-// the inverse of its factor (x[q], starting from the identity).  Pivot j is
-// one barrier: the wave that owns column j publishes it through LDS (rows > j,
-// zeros above) with the pivot value; every wave then takes r = 1/sqrt(a_jj)
-// and updates its elements
-//     a_ic -= (w_i r^2) w_c      (c > j: trailing matrix, l_i l_c)
-//     x_ic -= (w_i r^2) x_jc     (c <= j: rows below of the inverse)
-// where w_c comes from lane c of the published column and x_jc from lane j
-// of the wave's own inverse columns, both by v_readlane (wave-uniform
-// operands, no LDS traffic); row j of the inverse becomes x_j r and column
-// j of the tile w r (LAPACK potf2 also scales by the reciprocal of the
-// square root).  Rows i <= j read w_i = 0, so their updates are no-ops and
-// need no branch.  The broadcast buffers alternate with the pivot parity, so
-// a buffer is only rewritten after every thread has passed the barrier that
-// follows its last read.  Rows and columns at or beyond kb hold the
-// identity.  The pivot index is a template parameter so that every register
-// index is static.
-struct PivotBufs {
-    double w[2][NB];     // column j of the trailing matrix, zero for rows <= j
-    double piv[2][2];    // a_jj
-};
+// The 64 x 64 tile is factored and inverted in LDS, in four stages of 16
+// columns.  S[i][c] (c <= i) holds the tile on entry and the factor L on
+// exit; the inverse X = L^-1 is built beside it, X(i, c) for i > c at
+// S[c][i] (the upper triangle) and X(i, i) at S[i][64].  Block s is rows and
+// columns 16s..16s+15.  Stage s:
+//   1. wave 0 factors and inverts the diagonal block (s, s) in registers:
+//      lane i' row i' (the four 16-lane groups alike), register c column c;
+//      pivot j takes r = 1/sqrt(a_jj) and updates
+//          a_ic -= (w_i r^2) w_c   (c > j),   x_ic -= (w_i r^2) x_jc   (c <= j)
+//      with w = column j and the values of other rows taken by DPP row
+//      broadcasts; row j of the inverse becomes x_j r and column j of the
+//      block w r (LAPACK potf2 also scales by the reciprocal of the square
+//      root).  One wave and no LDS traffic: the chain is bound by the wave's
+//      instruction issue, so the broadcasts are single DPP moves;
+//   2. with FP64 MFMA, one 16 x 16 block per wave: the panel blocks
+//      L_ts = A_ts X_ss^T (t > s) and the inverse blocks left of the
+//      diagonal, X_su = -X_ss sum_{u <= v < s} L_sv X_vu (u < s); the wave
+//      with panel s+1 also applies its update to block (s+1, s+1);
+//   3. waves 1-3 apply the other trailing updates A_tt' -= L_ts L_t's^T
+//      (s < t' <= t) while wave 0 goes on to stage s+1 (block (s+1, s+1) is
+//      all it reads).
+// A barrier follows phases 1 and 2.  Rows and columns at or beyond kb hold
+// the identity.  A pivot that is not positive ends the factorization as
+// LAPACK potrf does: its Schur value stays on the diagonal, the column below
+// it is left unscaled, row j of the inverse is divided by the Schur value
+// (as triangular solves with the factor as it stands do), the inverse blocks
+// of its block row are still formed, and the rows after it are left
+// unfinished (their inverse rows are stored as zero).
+typedef double (*TileLds)[NB + 1];
 
-__device__ __forceinline__ int tile_class() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+// Lane l broadcasts lane c of its 16-lane row to the row (DPP row_newbcast).
+template <int c>
+__device__ __forceinline__ double row_bcast(double v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + c, 0xf, 0xf, true);
+}
 
+template <int j, int... C>
+__device__ __forceinline__ void blk_updates(double (&d)[16], double (&x)[16], double m,
+                                            std::integer_sequence<int, C...>) {
+    ((C > j ? (void)(d[C] = fma(-m, row_bcast<C>(d[j]), d[C])) : (void)0), ...);
+    ((C <= j ? (void)(x[C] = fma(-m, row_bcast<j>(x[C]), x[C])) : (void)0), ...);
+}
+
+// Pivot j of the diagonal block, without branches: once a pivot has failed
+// every later one is a no-op (r = 1, m = 0).  Row j of the inverse is scaled
+// at the end (rs): the updates read it unscaled.
 template <int j>
-__device__ __forceinline__ bool pivot_step(double (&a)[16], double (&x)[16], PivotBufs &B, int c0, int kb, int &fail,
-                                           double &wfail) {
-    if (j >= kb) return false;
-    const int i = threadIdx.x & 63;
-    constexpr int p = j & 1, cj = j & 3, qj = j >> 2;
-    if (c0 == cj) {   // wave-uniform
-        const double v = a[qj];
-        B.w[p][i] = (i > j) ? v : 0.0;
-        if (i == j) B.piv[p][0] = v;
-    }
-    __syncthreads();
-    const double wv = B.w[p][i];
-    const double wjj = B.piv[p][0];
-    if (!(wjj > 0.0)) {   // uniform
-        fail = j;
-        wfail = wjj;
-        return false;
-    }
-    const double r = rsqrt(wjj);
+__device__ __forceinline__ void blk_pivot(double (&d)[16], double (&x)[16], int i, int &fail, double &wfail,
+                                          double &rs) {
+    const double wjj = row_bcast<j>(d[j]);
+    const bool ok = (fail < 0) && (wjj > 0.0);
+    wfail = (fail < 0 && !ok) ? wjj : wfail;
+    fail = (fail < 0 && !ok) ? j : fail;
+    const double rq = rsqrt(wjj);
+    const double r = ok ? rq : 1.0;
+    const double wv = (ok && i > j) ? d[j] : 0.0;
     const double li = wv * r;   // l_ij, zero for i <= j
     const double m = li * r;
-#pragma unroll
-    for (int q = qj; q < 16; q++) a[q] = fma(-m, rdl(wv, c0 + 4 * q), a[q]);
-#pragma unroll
-    for (int q = 0; q <= qj; q++) x[q] = fma(-m, rdl(x[q], j), x[q]);
-    if (i == j) {
-#pragma unroll
-        for (int q = 0; q <= qj; q++) x[q] = x[q] * r;
-    }
-    const double nv = (i == j) ? wjj * r : li;
-    if (c0 == cj && i >= j) a[qj] = nv;
-    return true;
+    blk_updates<j>(d, x, m, std::make_integer_sequence<int, 16>{});
+    rs = (i == j) ? r : rs;
+    if (ok && i >= j) d[j] = (i == j) ? wjj * r : li;
 }
 
 template <int... J>
-__device__ __forceinline__ void pivot_chain(double (&a)[16], double (&x)[16], PivotBufs &B, int c0, int kb, int &fail,
-                                            double &wfail, std::integer_sequence<int, J...>) {
-    (pivot_step<J>(a, x, B, c0, kb, fail, wfail) && ...);
+__device__ __forceinline__ void blk_chain(double (&d)[16], double (&x)[16], int i, int &fail, double &wfail,
+                                          double &rs, std::integer_sequence<int, J...>) {
+    (blk_pivot<J>(d, x, i, fail, wfail, rs), ...);
 }
 
-// Returns the failing pivot or -1.  After a failure at pivot j the Schur
-// value stays at a_jj, column j below it is left unscaled, row j of the
-// inverse is divided by the Schur value (as the triangular solves with the
-// factor as it stands do) and the rows below it are left unfinished.
-__device__ __forceinline__ int tile_factor(double (&a)[16], double (&x)[16], PivotBufs &B, int kb) {
-    const int c0 = tile_class();
-    int fail = -1;
-    double wfail = 0.0;
-    pivot_chain(a, x, B, c0, kb, fail, wfail, std::make_integer_sequence<int, NB>{});
-    const int i = threadIdx.x & 63;
-    if (fail >= 0 && i == fail) {
+// X(i, c) of the inverse held in S (zero above the diagonal)
+__device__ __forceinline__ double xinv(const TileLds S, int i, int c) {
+    return (i > c) ? S[c][i] : (i == c ? S[i][NB] : 0.0);
+}
+
+__device__ __forceinline__ v4d mma_f64(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Phase 1 (wave 0): returns the failing pivot inside block s or -1.  The
+// four 16-lane rows of the wave hold the same block.
+__device__ __forceinline__ int blk_diag(TileLds S, int s) {
+    const int lane = threadIdx.x & 63, i = lane & 15, R = 16 * s + i;
+    double d[16], x[16];
 #pragma unroll
-        for (int q = 0; q < 16; q++)
-            if (c0 + 4 * q <= fail) x[q] = x[q] / wfail;
+    for (int c = 0; c < 16; c++) {
+        d[c] = (c <= i) ? S[R][16 * s + c] : 0.0;
+        x[c] = (c == i) ? 1.0 : 0.0;
+    }
+    int fail = -1;
+    double wfail = 0.0, rs = 1.0;
+    blk_chain(d, x, i, fail, wfail, rs, std::make_integer_sequence<int, 16>{});
+    fail = __builtin_amdgcn_readfirstlane(fail);
+    if (lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            x[c] *= rs;
+            if (fail >= 0 && i == fail && c <= fail) x[c] = x[c] / wfail;
+            if (c <= i) S[R][16 * s + c] = d[c];
+            if (c < i) S[16 * s + c][R] = x[c];
+            if (c == i) S[R][NB] = x[c];
+        }
     }
     return fail;
 }
 
-__device__ __forceinline__ void tile_identity_inverse(double (&x)[16]) {
-    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
+// MFMA operands of a 16 x 16 x 16 product: step q, lane l holds A(l%16, 4q + l/16)
+// and B(4q + l/16, l%16); register r of the result holds C(l/16 + 4r, l%16).
+// Panel: L_ts = A_ts X_ss^T, in place.
+__device__ __forceinline__ void blk_panel(TileLds S, int s, int t) {
+    const int lane = threadIdx.x & 63, rr = lane & 15, kq = lane >> 4;
+    double a[4], b[4];
 #pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = (c0 + 4 * q == i) ? 1.0 : 0.0;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * q + kq;
+        a[q] = S[16 * t + rr][16 * s + k];
+        b[q] = xinv(S, 16 * s + rr, 16 * s + k);
+    }
+    v4d acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mma_f64(a[q], b[q], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) S[16 * t + kq + 4 * r][16 * s + rr] = acc[r];
+}
+
+// Trailing block: A_tt' -= L_ts L_t's^T.
+__device__ __forceinline__ void blk_update(TileLds S, int s, int t, int tp) {
+    const int lane = threadIdx.x & 63, rr = lane & 15, kq = lane >> 4;
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * q + kq;
+        a[q] = S[16 * t + rr][16 * s + k];
+        b[q] = S[16 * tp + rr][16 * s + k];
+    }
+    v4d acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mma_f64(a[q], b[q], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) S[16 * t + kq + 4 * r][16 * tp + rr] -= acc[r];
+}
+
+// Inverse block: X_su = -X_ss sum_{u <= v < s} L_sv X_vu (u < s).  The sum's
+// result registers are the B operands of the second product as they stand.
+__device__ __forceinline__ void blk_inverse(TileLds S, int s, int u) {
+    const int lane = threadIdx.x & 63, rr = lane & 15, kq = lane >> 4;
+    v4d tsum = {0, 0, 0, 0};
+    for (int v = u; v < s; v++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = 4 * q + kq;
+            a[q] = S[16 * s + rr][16 * v + k];
+            b[q] = xinv(S, 16 * v + k, 16 * u + rr);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) tsum = mma_f64(a[q], b[q], tsum);
+    }
+    double a[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = xinv(S, 16 * s + rr, 16 * s + 4 * q + kq);
+    v4d acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc = mma_f64(a[q], tsum[q], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) S[16 * u + rr][16 * s + kq + 4 * r] = -acc[r];
+}
+
+__device__ __forceinline__ int tile_class() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Factor and invert the tile in S (kb x kb used); returns the failing pivot
+// or -1.  Called by all 256 threads; *sfail is an LDS word.
+__device__ __forceinline__ int tile_factor(TileLds S, int kb, int *sfail) {
+    const int wave = tile_class();
+    for (int e = threadIdx.x; e < TILE; e += 256) {
+        const int i = e >> 6, c = e & 63;
+        if (i >= kb && c <= i) S[i][c] = (i == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int ns = (kb + 15) >> 4;
+    for (int s = 0; s < ns; s++) {
+        if (wave == 0) {
+            const int f = blk_diag(S, s);
+            if ((threadIdx.x & 63) == 0) *sfail = (f >= 0) ? 16 * s + f : -1;
+        }
+        __syncthreads();
+        const int fail = *sfail;
+        const int npan = (fail >= 0) ? 0 : ns - 1 - s;
+        for (int it = wave; it < npan + s; it += 4) {
+            if (it < npan) {
+                blk_panel(S, s, s + 1 + it);
+                if (it == 0) blk_update(S, s, s + 1, s + 1);
+            } else {
+                blk_inverse(S, s, it - npan);
+            }
+        }
+        __syncthreads();
+        if (fail >= 0) return fail;
+        if (wave > 0) {
+            int it = 0;
+            for (int t = s + 2; t < ns; t++)
+                for (int tp = s + 1; tp <= t; tp++, it++)
+                    if (it % 3 == wave - 1) blk_update(S, s, t, tp);
+        }
+    }
+    __syncthreads();
+    return -1;
 }
 
 // Write the factored tile (lower kb x kb part) back to A(k0.., k0..), the
 // inverse of the factor to Dinv (column-major: Dinv[c*64 + i] = Linv(i, c))
 // and its transpose to DinvT (DinvT[c*64 + i] = Linv(c, i)); both are zero
-// outside the lower kb x kb triangle.  T is the storage type of the matrix.
+// outside the lower kb x kb triangle and in the rows after a failed pivot.
+// T is the storage type of the matrix.
 template <typename T>
-__device__ __forceinline__ void tile_store(const double (&a)[16], const double (&x)[16], T *A, int lda, int k0,
-                                           int kb, T *Dinv, T *DinvT) {
-    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int c = c0 + 4 * q;
+__device__ __forceinline__ void tile_store(const TileLds S, T *A, int lda, int k0, int kb, int fail, T *Dinv,
+                                           T *DinvT) {
+    const int last = (fail < 0) ? NB : fail + 1;   // inverse rows kept
+    for (int e = threadIdx.x; e < TILE; e += 256) {   // lanes along columns of A and Dinv
+        const int c = e >> 6, i = e & 63;
         const bool low = (i < kb && c <= i);
-        if (low) A[(size_t)(k0 + c) * lda + k0 + i] = (T)a[q];
-        const T v = low ? (T)x[q] : (T)0;
-        Dinv[c * NB + i] = v;
-        DinvT[i * NB + c] = v;
+        if (low) A[(size_t)(k0 + c) * lda + k0 + i] = (T)S[i][c];
+        Dinv[c * NB + i] = (low && i < last) ? (T)xinv(S, i, c) : (T)0;
+    }
+    for (int e = threadIdx.x; e < TILE; e += 256) {   // and along columns of DinvT
+        const int i = e >> 6, c = e & 63;
+        DinvT[i * NB + c] = (i < kb && c <= i && i < last) ? (T)xinv(S, i, c) : (T)0;
     }
 }
 
 template <typename T>
 __device__ __forceinline__ void diag_first_body(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
-                                                T *__restrict__ Dinv, PivotBufs &B) {
-    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
-    double a[16], x[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int c = c0 + 4 * q;
-        a[q] = (i < kb && c <= i) ? (double)A[(size_t)c * lda + i] : (c == i ? 1.0 : 0.0);
+                                                T *__restrict__ Dinv, TileLds S, int *sfail) {
+    for (int e = threadIdx.x; e < TILE; e += 256) {
+        const int c = e >> 6, i = e & 63;
+        if (i < kb && c <= i) S[i][c] = (double)A[(size_t)c * lda + i];
     }
-    tile_identity_inverse(x);
-    const int fail = tile_factor(a, x, B, kb);
-    tile_store<T>(a, x, A, lda, 0, kb, Dinv, Dinv + TILE);
+    const int fail = tile_factor(S, kb, sfail);
+    tile_store<T>(S, A, lda, 0, kb, fail, Dinv, Dinv + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void diag_first_kernel(T *__restrict__ A, int lda, int kb, int *__restrict__ info,
                                                          T *__restrict__ Dinv) {
-    __shared__ PivotBufs B;
-    diag_first_body<T>(A, lda, kb, info, Dinv, B);
+    __shared__ double S[NB][NB + 1];
+    __shared__ int sfail;
+    diag_first_body<T>(A, lda, kb, info, Dinv, S, &sfail);
 }
 
 // ------------------------------------------------------------ MFMA tiles ---
@@ -276,7 +393,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(T *__restrict__ A, int n, int
 template <typename T>
 __device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
                                                int *__restrict__ info, T *__restrict__ Dnext, int tj, int ti,
-                                               double (*S)[NB + 1], PivotBufs &B) {
+                                               TileLds S, int *sfail) {
     const int t0 = k0 + kb;
     if (t0 + tj * NB >= n || t0 + ti * NB >= nrows) return;
     // stop only for a failure in blocks <= k (pivot index < t0); work-group
@@ -310,17 +427,8 @@ __device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nro
             }
     }
     if (!diag) return;
-    __syncthreads();
-    const int i = threadIdx.x & 63, c0 = threadIdx.x >> 6;
-    double a[16], x[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int c = c0 + 4 * q;
-        a[q] = (i < kb2 && c <= i) ? S[i][c] : (c == i ? 1.0 : 0.0);
-    }
-    tile_identity_inverse(x);
-    const int fail = tile_factor(a, x, B, kb2);
-    tile_store<T>(a, x, A, lda, t0, kb2, Dnext, Dnext + TILE);
+    const int fail = tile_factor(S, kb2, sfail);   // its first barrier also orders the stores to S above
+    tile_store<T>(S, A, lda, t0, kb2, fail, Dnext, Dnext + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
@@ -328,8 +436,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
                                                         int *__restrict__ info, T *__restrict__ Dnext) {
     __shared__ double S[NB][NB + 1];
-    __shared__ PivotBufs B;
-    syrk_next_body<T>(A, n, nrows, lda, k0, kb, info, Dnext, blockIdx.x, blockIdx.y, S, B);
+    __shared__ int sfail;
+    syrk_next_body<T>(A, n, nrows, lda, k0, kb, info, Dnext, blockIdx.x, blockIdx.y, S, &sfail);
 }
 
 // -------------------------------------------------------- triangular sweeps ---
@@ -615,10 +723,11 @@ __global__ void b_prepare(const BatchProb *__restrict__ tab) {
 }
 
 __global__ __launch_bounds__(256) void b_diag_first(const BatchProb *__restrict__ tab) {
-    __shared__ PivotBufs B;
+    __shared__ double S[NB][NB + 1];
+    __shared__ int sfail;
     const BatchProb p = tab[blockIdx.z];
     if (p.n <= 0) return;
-    diag_first_body<double>(p.A, p.lda, p.n < NB ? p.n : NB, p.info, p.Dinv, B);
+    diag_first_body<double>(p.A, p.lda, p.n < NB ? p.n : NB, p.info, p.Dinv, S, &sfail);
 }
 
 template <int R>
@@ -636,13 +745,13 @@ __global__ __launch_bounds__(256) void b_trsm(const BatchProb *__restrict__ tab,
 template <int R>
 __global__ __launch_bounds__(256) void b_syrk(const BatchProb *__restrict__ tab, int kbk) {
     __shared__ double S[NB][NB + 1];
-    __shared__ PivotBufs B;
+    __shared__ int sfail;
     const BatchProb p = tab[blockIdx.z];
     if (kbk + 1 >= nblk_of(p.n)) return;
     const int k0 = kbk * NB;
     for (int r = 0; r < R; r++) {
         syrk_next_body<double>(p.A, p.n, p.n + 1, p.lda, k0, NB, p.info, p.Dinv + (size_t)(kbk + 1) * 2 * TILE,
-                               blockIdx.x, blockIdx.y * R + r, S, B);
+                               blockIdx.x, blockIdx.y * R + r, S, &sfail);
         __syncthreads();
     }
 }
