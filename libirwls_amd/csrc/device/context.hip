@@ -5,6 +5,8 @@
 
 #include <mutex>
 #include <time.h>
+#include <unordered_map>
+#include <vector>
 
 namespace irwls {
 
@@ -102,6 +104,88 @@ int device_cus() {
     return cus;
 }
 
+// ---- device memory cache -------------------------------------------------
+// hipFree waits for the whole device, so with many worker threads (batched
+// training) every free stalls its thread on every other thread's work.
+// Freed blocks go back to per-size free lists instead.  The caller has
+// synchronized the stream that used the block, so any thread may take it
+// again at once.  Sizes are rounded up to 2^k x {1.25, 1.5, 1.75, 2} (at least
+// 4 KiB).  The lists go back to the driver only when an allocation fails and
+// when the library is trimmed or shut down.  Blocks above kCacheMax are
+// returned at once (a buffer that grows in steps would otherwise keep every
+// step).
+static std::mutex g_cmu;
+static std::unordered_map<void *, size_t> g_csize;            // every cached-allocator block: its size
+static std::unordered_map<size_t, std::vector<void *>> g_cfree;
+static const size_t kCacheMax = (size_t)1 << 30;
+
+static size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    size_t p = 4096;
+    while (p * 2 < b) p *= 2;   // p < b <= 2p
+    for (size_t q = 5; q < 8; q++)
+        if (p / 4 * q >= b) return p / 4 * q;
+    return 2 * p;
+}
+
+static void cache_release_locked() {
+    for (auto &kv : g_cfree)
+        for (void *q : kv.second) {
+            g_csize.erase(q);
+            IRWLS_HIP(hipFree(q));
+        }
+    g_cfree.clear();
+}
+
+static void *cached_malloc(size_t bytes, size_t *got) {
+    const size_t c = size_class(bytes);
+    *got = c;
+    {
+        std::lock_guard<std::mutex> g(g_cmu);
+        auto it = g_cfree.find(c);
+        if (it != g_cfree.end() && !it->second.empty()) {
+            void *q = it->second.back();
+            it->second.pop_back();
+            return q;
+        }
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, c);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> g(g_cmu);
+        cache_release_locked();
+        e = hipMalloc(&q, c);
+    }
+    IRWLS_HIP(e);
+    std::lock_guard<std::mutex> g(g_cmu);
+    g_csize[q] = c;
+    return q;
+}
+
+static void cached_free(void *q) {
+    if (!q) return;
+    {
+        std::lock_guard<std::mutex> g(g_cmu);
+        auto it = g_csize.find(q);
+        if (it == g_csize.end()) {
+            fprintf(stderr, "libirwls: freeing device memory the library did not allocate\n");
+            exit(2);
+        }
+        if (it->second <= kCacheMax) {
+            g_cfree[it->second].push_back(q);
+            return;
+        }
+        g_csize.erase(it);
+    }
+    IRWLS_HIP(hipFree(q));
+}
+
+static void cache_release() {
+    std::lock_guard<std::mutex> g(g_cmu);
+    cache_release_locked();
+}
+
 // ---- scratch slots -------------------------------------------------------
 static const int kSlots = 16;
 static thread_local void *g_scr[kSlots];
@@ -115,13 +199,49 @@ void *scratch(int slot, size_t bytes) {
     if (g_scr_sz[slot] < bytes) {
         if (g_scr[slot]) {
             IRWLS_HIP(hipStreamSynchronize(stream()));
-            IRWLS_HIP(hipFree(g_scr[slot]));
+            cached_free(g_scr[slot]);
         }
-        size_t sz = bytes + (bytes >> 2) + 256;
-        IRWLS_HIP(hipMalloc(&g_scr[slot], sz));
-        g_scr_sz[slot] = sz;
+        g_scr[slot] = cached_malloc(bytes + (bytes >> 2) + 256, &g_scr_sz[slot]);
     }
     return g_scr[slot];
+}
+
+static void scratch_release() {
+    for (int s = 0; s < kSlots; s++) {
+        cached_free(g_scr[s]);
+        g_scr[s] = nullptr;
+        g_scr_sz[s] = 0;
+    }
+}
+
+// ---- pinned host buffers and worker streams ------------------------------
+// Worker threads of a batched run take their pinned staging buffers and
+// their stream from pools and give them back when they finish: creating a
+// stream and freeing pinned memory both cost milliseconds, and hipHostFree
+// also waits for the device.
+static std::mutex g_pmu;
+static std::unordered_map<size_t, std::vector<void *>> g_pinned;
+static std::vector<hipStream_t> g_streams;
+
+static void *pinned_get(size_t bytes) {
+    {
+        std::lock_guard<std::mutex> g(g_pmu);
+        auto it = g_pinned.find(bytes);
+        if (it != g_pinned.end() && !it->second.empty()) {
+            void *q = it->second.back();
+            it->second.pop_back();
+            return q;
+        }
+    }
+    void *q = nullptr;
+    IRWLS_HIP(hipHostMalloc(&q, bytes, hipHostMallocDefault));
+    return q;
+}
+
+static void pinned_put(void *q, size_t bytes) {
+    if (!q) return;
+    std::lock_guard<std::mutex> g(g_pmu);
+    g_pinned[bytes].push_back(q);
 }
 
 }  // namespace irwls
@@ -194,11 +314,8 @@ int irwls_dev_init(void) {
 void irwls_dev_shutdown(void) {
     if (!g_init) return;
     IRWLS_HIP(hipStreamSynchronize(g_stream));
-    for (int s = 0; s < kSlots; s++) {
-        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
-        g_scr[s] = nullptr;
-        g_scr_sz[s] = 0;
-    }
+    scratch_release();
+    cache_release();
     resolve_pool();
     IRWLS_HIP(hipStreamDestroy(g_stream));
     g_stream = nullptr;
@@ -207,9 +324,9 @@ void irwls_dev_shutdown(void) {
 
 void *irwls_dev_alloc(size_t bytes) {
     irwls_dev_init();
-    void *p = nullptr;
     if (bytes == 0) bytes = 8;
-    IRWLS_HIP(hipMalloc(&p, bytes));
+    size_t got = 0;
+    void *p = cached_malloc(bytes, &got);
     irwls_dev_memset0(p, bytes);
     return p;
 }
@@ -217,7 +334,7 @@ void *irwls_dev_alloc(size_t bytes) {
 void irwls_dev_free(void *p) {
     if (!p) return;
     IRWLS_HIP(hipStreamSynchronize(stream()));
-    IRWLS_HIP(hipFree(p));
+    cached_free(p);
 }
 
 // Small host-to-device copies go through a pinned staging ring: the caller's
@@ -262,7 +379,7 @@ static void stage_reset() { g_stage_head = 0; }
 
 void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
-    if (!g_stage) IRWLS_HIP(hipHostMalloc((void **)&g_stage, kStage, hipHostMallocDefault));
+    if (!g_stage) g_stage = (char *)pinned_get(kStage);
     if (bytes > kStageMax) {
         IRWLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
         wait_stream();
@@ -330,11 +447,8 @@ void irwls_dev_sync(void) {
 void irwls_dev_trim(void) {
     if (!g_init) return;
     IRWLS_HIP(hipStreamSynchronize(g_stream));
-    for (int s = 0; s < kSlots; s++) {
-        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
-        g_scr[s] = nullptr;
-        g_scr_sz[s] = 0;
-    }
+    scratch_release();
+    cache_release();
 }
 
 void irwls_dev_timing_mask(unsigned mask) { g_timing_mask = mask; }
@@ -376,28 +490,32 @@ double irwls_dev_wait_seconds(int reset) {
 void irwls_dev_thread_begin(void) {
     irwls_dev_init();
     if (t_stream) return;
-    IRWLS_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
+    {
+        std::lock_guard<std::mutex> g(g_pmu);
+        if (!g_streams.empty()) {
+            t_stream = g_streams.back();
+            g_streams.pop_back();
+        }
+    }
+    if (!t_stream) IRWLS_HIP(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking));
     IRWLS_HIP(hipEventCreateWithFlags(&t_sync_event, hipEventBlockingSync | hipEventDisableTiming));
-    IRWLS_HIP(hipHostMalloc((void **)&t_bounce, kBounce, hipHostMallocDefault));
+    t_bounce = (char *)pinned_get(kBounce);
 }
 
 void irwls_dev_thread_end(void) {
     if (!t_stream) return;
     IRWLS_HIP(hipStreamSynchronize(t_stream));
     irwls_dev_release_thread_buffers();
-    for (int s = 0; s < kSlots; s++) {
-        if (g_scr[s]) IRWLS_HIP(hipFree(g_scr[s]));
-        g_scr[s] = nullptr;
-        g_scr_sz[s] = 0;
-    }
-    if (g_stage) IRWLS_HIP(hipHostFree(g_stage));
+    scratch_release();
+    pinned_put(g_stage, kStage);
     g_stage = nullptr;
     g_stage_head = 0;
-    IRWLS_HIP(hipHostFree(t_bounce));
+    pinned_put(t_bounce, kBounce);
     t_bounce = nullptr;
     IRWLS_HIP(hipEventDestroy(t_sync_event));
     t_sync_event = nullptr;
-    IRWLS_HIP(hipStreamDestroy(t_stream));
+    std::lock_guard<std::mutex> g(g_pmu);
+    g_streams.push_back(t_stream);
     t_stream = nullptr;
 }
 
