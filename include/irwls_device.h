@@ -27,7 +27,7 @@ extern "C" {
 int   irwls_dev_init(void);                 /* idempotent; returns 0 */
 void  irwls_dev_shutdown(void);
 void *irwls_dev_alloc(size_t bytes);        /* zero-filled */
-void  irwls_dev_free(void *d_ptr);
+void  irwls_dev_free(void *d_ptr);          /* waits for the stream, keeps the block for reuse */
 void  irwls_dev_h2d(void *d_dst, const void *src, size_t bytes);   /* synchronous */
 void  irwls_dev_d2h(void *dst, const void *d_src, size_t bytes);   /* synchronous */
 void  irwls_dev_d2d(void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
@@ -36,7 +36,7 @@ void  irwls_dev_memset0(void *d_dst, size_t bytes);                /* fill kerne
    traffic is known, which calibrates the hardware traffic counters. */
 void  irwls_dev_copy_kernel(void *d_dst, const void *d_src, size_t bytes);
 void  irwls_dev_sync(void);
-void  irwls_dev_trim(void);                 /* release scratch and solver workspace */
+void  irwls_dev_trim(void);                 /* release scratch and the cached free device blocks */
 
 /* ---- per-kernel timing (HIP events on the library stream) -------------- */
 enum irwls_timer_kind {

@@ -108,6 +108,7 @@ def load():
         "irwls_dev_h2d": (None, [vp, vp, C.c_size_t]),
         "irwls_dev_d2h": (None, [vp, vp, C.c_size_t]),
         "irwls_dev_sync": (None, []),
+        "irwls_dev_trim": (None, []),
         "irwls_dev_copy_kernel": (None, [vp, vp, C.c_size_t]),
         "irwls_dev_potrf": (C.c_int, [vp, C.c_int, C.c_int]),
         "irwls_dev_potrs": (None, [vp, C.c_int, C.c_int, vp]),
