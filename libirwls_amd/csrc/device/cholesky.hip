@@ -74,6 +74,8 @@ int *info_buf() {
 // unfinished (their inverse rows are stored as zero).
 typedef double (*TileLds)[NB + 1];
 
+__device__ __forceinline__ int tile_class() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Lane l broadcasts lane c of its 16-lane row to the row (DPP row_newbcast).
 template <int c>
 __device__ __forceinline__ double row_bcast(double v) {
@@ -81,18 +83,15 @@ __device__ __forceinline__ double row_bcast(double v) {
 }
 
 template <int j, int... C>
-__device__ __forceinline__ void blk_updates(double (&d)[16], double (&x)[16], double m,
-                                            std::integer_sequence<int, C...>) {
+__device__ __forceinline__ void blk_updates(double (&d)[16], double m, std::integer_sequence<int, C...>) {
     ((C > j ? (void)(d[C] = fma(-m, row_bcast<C>(d[j]), d[C])) : (void)0), ...);
-    ((C <= j ? (void)(x[C] = fma(-m, row_bcast<j>(x[C]), x[C])) : (void)0), ...);
 }
 
 // Pivot j of the diagonal block, without branches: once a pivot has failed
-// every later one is a no-op (r = 1, m = 0).  Row j of the inverse is scaled
-// at the end (rs): the updates read it unscaled.
+// every later one is a no-op (r = 1, m = 0).  rs keeps r of the lane's own
+// row: the diagonal of the block's inverse.
 template <int j>
-__device__ __forceinline__ void blk_pivot(double (&d)[16], double (&x)[16], int i, int &fail, double &wfail,
-                                          double &rs) {
+__device__ __forceinline__ void blk_pivot(double (&d)[16], int i, int &fail, double &wfail, double &rs) {
     const double wjj = row_bcast<j>(d[j]);
     const bool ok = (fail < 0) && (wjj > 0.0);
     wfail = (fail < 0 && !ok) ? wjj : wfail;
@@ -102,15 +101,30 @@ __device__ __forceinline__ void blk_pivot(double (&d)[16], double (&x)[16], int 
     const double wv = (ok && i > j) ? d[j] : 0.0;
     const double li = wv * r;   // l_ij, zero for i <= j
     const double m = li * r;
-    blk_updates<j>(d, x, m, std::make_integer_sequence<int, 16>{});
+    blk_updates<j>(d, m, std::make_integer_sequence<int, 16>{});
     rs = (i == j) ? r : rs;
     if (ok && i >= j) d[j] = (i == j) ? wjj * r : li;
 }
 
 template <int... J>
-__device__ __forceinline__ void blk_chain(double (&d)[16], double (&x)[16], int i, int &fail, double &wfail,
-                                          double &rs, std::integer_sequence<int, J...>) {
-    (blk_pivot<J>(d, x, i, fail, wfail, rs), ...);
+__device__ __forceinline__ void blk_chain(double (&d)[16], int i, int &fail, double &wfail, double &rs,
+                                          std::integer_sequence<int, J...>) {
+    (blk_pivot<J>(d, i, fail, wfail, rs), ...);
+}
+
+// Step k of the forward substitution for one column of the block's inverse:
+// lane k holds x_k = acc_k X(k, k); rows below take acc_i -= D(i, k) x_k.
+template <int k>
+__device__ __forceinline__ void inv_step(const double (&dl)[16], double xd, double &acc, double &x, int i) {
+    const double xk = row_bcast<k>(acc * xd);
+    x = (i == k) ? xk : x;
+    acc = fma(-dl[k], xk, acc);
+}
+
+template <int... K>
+__device__ __forceinline__ void inv_chain(const double (&dl)[16], double xd, double &acc, double &x, int i,
+                                          std::integer_sequence<int, K...>) {
+    (inv_step<K>(dl, xd, acc, x, i), ...);
 }
 
 // X(i, c) of the inverse held in S (zero above the diagonal)
@@ -122,31 +136,41 @@ __device__ __forceinline__ v4d mma_f64(double a, double b, v4d c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Phase 1 (wave 0): returns the failing pivot inside block s or -1.  The
-// four 16-lane rows of the wave hold the same block.
+// Phase 1 (wave 0): factors block s in place and stores the diagonal of its
+// inverse (1/sqrt of each pivot, 1/Schur value for a failed one) in column
+// 64; returns the failing pivot inside block s or -1.  The four 16-lane rows
+// of the wave hold the same block.
 __device__ __forceinline__ int blk_diag(TileLds S, int s) {
     const int lane = threadIdx.x & 63, i = lane & 15, R = 16 * s + i;
-    double d[16], x[16];
+    double d[16];
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        d[c] = (c <= i) ? S[R][16 * s + c] : 0.0;
-        x[c] = (c == i) ? 1.0 : 0.0;
-    }
+    for (int c = 0; c < 16; c++) d[c] = (c <= i) ? S[R][16 * s + c] : 0.0;
     int fail = -1;
     double wfail = 0.0, rs = 1.0;
-    blk_chain(d, x, i, fail, wfail, rs, std::make_integer_sequence<int, 16>{});
+    blk_chain(d, i, fail, wfail, rs, std::make_integer_sequence<int, 16>{});
     fail = __builtin_amdgcn_readfirstlane(fail);
+    if (fail >= 0 && i == fail) rs = 1.0 / wfail;
     if (lane < 16) {
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
-            x[c] *= rs;
-            if (fail >= 0 && i == fail && c <= fail) x[c] = x[c] / wfail;
+        for (int c = 0; c < 16; c++)
             if (c <= i) S[R][16 * s + c] = d[c];
-            if (c < i) S[16 * s + c][R] = x[c];
-            if (c == i) S[R][NB] = x[c];
-        }
+        S[R][NB] = rs;
     }
     return fail;
+}
+
+// Phase 1b (all waves): the rest of the block's inverse, X_ss = D_ss^-1, by
+// forward substitution; wave w takes columns 4w..4w+3, one 16-lane row per
+// column, lane position = row.
+__device__ __forceinline__ void blk_inv_diag(TileLds S, int s) {
+    const int lane = threadIdx.x & 63, i = lane & 15, q = 4 * tile_class() + (lane >> 4), R = 16 * s + i;
+    double dl[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) dl[k] = (k < i) ? S[R][16 * s + k] : 0.0;
+    const double xd = S[R][NB];
+    double acc = (i == q) ? 1.0 : 0.0, x = 0.0;
+    inv_chain(dl, xd, acc, x, i, std::make_integer_sequence<int, 16>{});
+    if (i > q) S[16 * s + q][R] = x;
 }
 
 // MFMA operands of a 16 x 16 x 16 product: step q, lane l holds A(l%16, 4q + l/16)
@@ -211,7 +235,6 @@ __device__ __forceinline__ void blk_inverse(TileLds S, int s, int u) {
     for (int r = 0; r < 4; r++) S[16 * u + rr][16 * s + kq + 4 * r] = -acc[r];
 }
 
-__device__ __forceinline__ int tile_class() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Factor and invert the tile in S (kb x kb used); returns the failing pivot
 // or -1.  Called by all 256 threads; *sfail is an LDS word.
@@ -228,6 +251,8 @@ __device__ __forceinline__ int tile_factor(TileLds S, int kb, int *sfail) {
             const int f = blk_diag(S, s);
             if ((threadIdx.x & 63) == 0) *sfail = (f >= 0) ? 16 * s + f : -1;
         }
+        __syncthreads();
+        blk_inv_diag(S, s);
         __syncthreads();
         const int fail = *sfail;
         const int npan = (fail >= 0) ? 0 : ns - 1 - s;
