@@ -236,9 +236,32 @@ __device__ __forceinline__ void blk_inverse(TileLds S, int s, int u) {
 }
 
 
+// Rows 16s..16s+15 of the factor (to A) and of its inverse (to Dinv and its
+// transpose DinvT, zero entries included), by threads t, t + nt, ... of the
+// work-group.  A block row is final once its stage has passed phase 2.
+template <typename T>
+__device__ __forceinline__ void store_block_row(const TileLds S, int s, int kb, T *A, int lda, int k0, T *Dinv,
+                                                T *DinvT, int t, int nt) {
+    for (int e = t; e < 16 * NB; e += nt) {   // 16 rows of each column of A and Dinv
+        const int c = e >> 4, R = 16 * s + (e & 15);
+        const bool low = (R < kb && c <= R);
+        if (low) A[(size_t)(k0 + c) * lda + k0 + R] = (T)S[R][c];
+        Dinv[c * NB + R] = low ? (T)xinv(S, R, c) : (T)0;
+    }
+    for (int e = t; e < 16 * NB; e += nt) {   // and along the rows of DinvT
+        const int R = 16 * s + (e >> 6), c = e & 63;
+        DinvT[R * NB + c] = (R < kb && c <= R) ? (T)xinv(S, R, c) : (T)0;
+    }
+}
+
 // Factor and invert the tile in S (kb x kb used); returns the failing pivot
-// or -1.  Called by all 256 threads; *sfail is an LDS word.
-__device__ __forceinline__ int tile_factor(TileLds S, int kb, int *sfail) {
+// or -1.  Called by all 256 threads; *sfail is an LDS word.  Without a
+// failure the factor and its inverse are stored as they become final (waves
+// 1-3 store block row s while wave 0 factors block s+1); after a failure the
+// caller stores the tile with tile_store.
+template <typename T>
+__device__ __forceinline__ int tile_factor(TileLds S, int kb, int *sfail, T *A, int lda, int k0, T *Dinv,
+                                           T *DinvT) {
     const int wave = tile_class();
     for (int e = threadIdx.x; e < TILE; e += 256) {
         const int i = e >> 6, c = e & 63;
@@ -271,9 +294,11 @@ __device__ __forceinline__ int tile_factor(TileLds S, int kb, int *sfail) {
             for (int t = s + 2; t < ns; t++)
                 for (int tp = s + 1; tp <= t; tp++, it++)
                     if (it % 3 == wave - 1) blk_update(S, s, t, tp);
+            store_block_row<T>(S, s, kb, A, lda, k0, Dinv, DinvT, threadIdx.x - 64, 192);
         }
     }
-    __syncthreads();
+    for (int s = ns; s < 4; s++)   // rows beyond the tile: zero rows of the inverse
+        store_block_row<T>(S, s, kb, A, lda, k0, Dinv, DinvT, threadIdx.x, 256);
     return -1;
 }
 
@@ -305,8 +330,8 @@ __device__ __forceinline__ void diag_first_body(T *__restrict__ A, int lda, int 
         const int c = e >> 6, i = e & 63;
         if (i < kb && c <= i) S[i][c] = (double)A[(size_t)c * lda + i];
     }
-    const int fail = tile_factor(S, kb, sfail);
-    tile_store<T>(S, A, lda, 0, kb, fail, Dinv, Dinv + TILE);
+    const int fail = tile_factor<T>(S, kb, sfail, A, lda, 0, Dinv, Dinv + TILE);
+    if (fail >= 0) tile_store<T>(S, A, lda, 0, kb, fail, Dinv, Dinv + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = fail + 1;
 }
 
@@ -452,8 +477,9 @@ __device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nro
             }
     }
     if (!diag) return;
-    const int fail = tile_factor(S, kb2, sfail);   // its first barrier also orders the stores to S above
-    tile_store<T>(S, A, lda, t0, kb2, fail, Dnext, Dnext + TILE);
+    // tile_factor's first barrier also orders the stores to S above
+    const int fail = tile_factor<T>(S, kb2, sfail, A, lda, t0, Dnext, Dnext + TILE);
+    if (fail >= 0) tile_store<T>(S, A, lda, t0, kb2, fail, Dnext, Dnext + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
 }
 
