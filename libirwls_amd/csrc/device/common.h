@@ -52,6 +52,14 @@ struct TimerScope {
 // Scratch device buffer owned by the context, grown on demand.
 void *scratch(int slot, size_t bytes);
 
+// Small device-to-host read that does not drain the stream: the copy is
+// queued into a pinned buffer of the calling thread (at most kReadbackMax
+// bytes) and marked by an event; readback_wait blocks until it has landed
+// and returns the buffer.  One read in flight per thread.
+constexpr size_t kReadbackMax = (size_t)64 << 10;
+void readback_start(const void *src, size_t bytes);
+const void *readback_wait();
+
 // Compute units of the current device.
 int device_cus();
 

@@ -357,6 +357,8 @@ static thread_local char *t_bounce = nullptr;
 static const size_t kBounce = (size_t)1 << 20;
 
 static thread_local double t_wait_s = 0.0;
+static thread_local void *t_rb = nullptr;
+static thread_local hipEvent_t t_rb_event = nullptr;
 
 static double now_s() {
     struct timespec ts;
@@ -408,6 +410,43 @@ void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
     }
     stage_reset();
 }
+
+}  // extern "C"
+
+namespace irwls {
+
+void readback_start(const void *src, size_t bytes) {
+    if (bytes > kReadbackMax) {
+        fprintf(stderr, "libirwls: readback of %zu bytes exceeds %zu\n", bytes, kReadbackMax);
+        exit(2);
+    }
+    if (!t_rb) {
+        t_rb = pinned_get(kReadbackMax);
+        const unsigned fl = hipEventDisableTiming | (t_stream ? hipEventBlockingSync : 0u);
+        IRWLS_HIP(hipEventCreateWithFlags(&t_rb_event, fl));
+    }
+    IRWLS_HIP(hipMemcpyAsync(t_rb, src, bytes, hipMemcpyDeviceToHost, stream()));
+    IRWLS_HIP(hipEventRecord(t_rb_event, stream()));
+}
+
+const void *readback_wait() {
+    const double t0 = now_s();
+    IRWLS_HIP(hipEventSynchronize(t_rb_event));
+    t_wait_s += now_s() - t0;
+    return t_rb;
+}
+
+static void readback_release() {
+    if (!t_rb) return;
+    pinned_put(t_rb, kReadbackMax);
+    t_rb = nullptr;
+    IRWLS_HIP(hipEventDestroy(t_rb_event));
+    t_rb_event = nullptr;
+}
+
+}  // namespace irwls
+
+extern "C" {
 
 void irwls_dev_d2d(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
@@ -507,6 +546,7 @@ void irwls_dev_thread_end(void) {
     IRWLS_HIP(hipStreamSynchronize(t_stream));
     irwls_dev_release_thread_buffers();
     scratch_release();
+    readback_release();
     pinned_put(g_stage, kStage);
     g_stage = nullptr;
     g_stage_head = 0;

@@ -291,41 +291,62 @@ int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, 
     hipLaunchKernelGGL(update_p, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_c, np, g_pcg.rz, 1, g_pcg.rz,
                        g_pcg.z, g_pcg.p);
     IRWLS_LAUNCH_CHECK();
-    double *rr_host = (double *)malloc((size_t)2 * gvec * sizeof(double));
+    const size_t rrb = (size_t)2 * gvec * sizeof(double);
+    // relative residuals from the partials read back for one iteration;
+    // returns 1 when converged, -1 when diverged (or NaN), else 0
+    auto judge = [&](const double *rr_host) {
+        double rr[2] = {0.0, 0.0};
+        for (int g = 0; g < gvec; g++) {
+            rr[0] += rr_host[2 * g];
+            rr[1] += rr_host[2 * g + 1];
+        }
+        for (int c = 0; c < 2; c++) relres[c] = (bn[c] > 0.0) ? sqrt(rr[c] / bn[c]) : 0.0;
+        if (relres[0] <= tol && relres[1] <= tol) return 1;
+        if (!(relres[0] < 1e30 && relres[1] < 1e30)) return -1;
+        return 0;
+    };
     int it = 0, done = 0;
+    bool ap_queued = false;   // the product A p of this iteration was queued by the one before
     relres[0] = relres[1] = 1.0;
-    for (it = 1; it <= maxit; it++) {
-        double *rz_cur = g_pcg.rz + 2 * ((it - 1) & 1);
-        double *rz_next = g_pcg.rz + 2 * (it & 1);
+    auto queue_ap = [&]() {
         hipLaunchKernelGGL((coldot2<double, 0>), dim3(gcol), dim3(256), 0, stream(), A, lda, (size_t)0, n, g_pcg.p,
                            g_pcg.q, g_pcg.p, g_pcg.part_a);
         IRWLS_LAUNCH_CHECK();
+    };
+    for (it = 1; it <= maxit; it++) {
+        double *rz_cur = g_pcg.rz + 2 * ((it - 1) & 1);
+        double *rz_next = g_pcg.rz + 2 * (it & 1);
+        if (!ap_queued) queue_ap();
+        ap_queued = false;
         hipLaunchKernelGGL(update_xr, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_a, gcol, rz_cur, g_pcg.p,
                            g_pcg.q, X, g_pcg.r, g_pcg.part_b);
         IRWLS_LAUNCH_CHECK();
-        // the residual is read back (a host synchronisation) every other
-        // iteration and at the last one; extra iterations after convergence
-        // only shrink it further
-        if ((it & 1) == 0 || it == maxit) {
-            irwls_dev_d2h(rr_host, g_pcg.part_b, (size_t)2 * gvec * sizeof(double));
-            double rr[2] = {0.0, 0.0};
-            for (int g = 0; g < gvec; g++) {
-                rr[0] += rr_host[2 * g];
-                rr[1] += rr_host[2 * g + 1];
-            }
-            for (int c = 0; c < 2; c++) relres[c] = (bn[c] > 0.0) ? sqrt(rr[c] / bn[c]) : 0.0;
-            if (relres[0] <= tol && relres[1] <= tol) {
-                done = 1;
-                break;
-            }
-            if (!(relres[0] < 1e30 && relres[1] < 1e30)) break;   // diverged or NaN
+        if (it == maxit) {
+            readback_start(g_pcg.part_b, rrb);
+            done = judge((const double *)readback_wait()) > 0;
+            break;
         }
+        // The residual of every other iteration is read back without
+        // draining the stream: the preconditioner, the direction update and
+        // the next product A p (none of which touches X) are queued before
+        // the host waits for it, so a converged solve stops at the same
+        // iteration, with the same X, as one that waits at once.
+        const bool check = (it & 1) == 0;
+        if (check) readback_start(g_pcg.part_b, rrb);
         np = precond(M, n, A, lda, g_pcg.r, g_pcg.w, g_pcg.z, g_pcg.part_c);
         hipLaunchKernelGGL(update_p, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_c, np, rz_cur, 0, rz_next,
                            g_pcg.z, g_pcg.p);
         IRWLS_LAUNCH_CHECK();
+        if (check) {
+            queue_ap();
+            ap_queued = true;
+            const int v = judge((const double *)readback_wait());
+            if (v != 0) {
+                done = v > 0;
+                break;
+            }
+        }
     }
-    free(rr_host);
     return done ? it : -it;
 }
 
