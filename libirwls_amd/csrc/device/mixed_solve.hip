@@ -142,6 +142,32 @@ __global__ __launch_bounds__(256) void update_p(int n, const double *__restrict_
     }
 }
 
+// Work-group partials of |b_c|^2 (c = 0, 1) to part[2 * blockIdx.x + c],
+// summed like the residual partials of update_xr.
+__global__ __launch_bounds__(256) void sumsq2_parts(int n, const double *__restrict__ b, double *__restrict__ part) {
+    __shared__ double wsum[2][4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (i < n) {
+        s0 = b[i] * b[i];
+        s1 = b[n + i] * b[n + i];
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wsum[0][wave] = s0;
+        wsum[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int c = threadIdx.x;
+        part[2 * blockIdx.x + c] = (wsum[c][0] + wsum[c][1]) + (wsum[c][2] + wsum[c][3]);
+    }
+}
+
 // Preconditioner of a system whose index set differs from the factored
 // one.  inv[f] = row of the system at factor position f (or -1), map[i] =
 // factor position of row i (or -1).  r is scattered into the factor's index
@@ -274,31 +300,27 @@ int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, 
     const size_t vb = (size_t)2 * n * sizeof(double);
     const int gcol = n;                     // work-groups (and partial pairs) of the column products
     const int gvec = (n + 255) / 256;       // work-groups of the elementwise kernels
-    double bn[2] = {0.0, 0.0};
-    {
-        double *hb = (double *)malloc(vb);
-        irwls_dev_d2h(hb, B, vb);
-        for (int i = 0; i < n; i++) {
-            bn[0] += hb[i] * hb[i];
-            bn[1] += hb[n + i] * hb[n + i];
-        }
-        free(hb);
-    }
     TimerScope ts(IRWLS_T_PCG, 0.0, 0.0);
+    // partials of |B|^2 sit after the residual partials in part_b (2 (n + 1)
+    // entries >= 4 gvec) and travel with every residual read
+    hipLaunchKernelGGL(sumsq2_parts, dim3(gvec), dim3(256), 0, stream(), n, B, g_pcg.part_b + 2 * gvec);
+    IRWLS_LAUNCH_CHECK();
     irwls_dev_memset0(X, vb);
     irwls_dev_d2d(g_pcg.r, B, vb);
     int np = precond(M, n, A, lda, g_pcg.r, g_pcg.w, g_pcg.z, g_pcg.part_c);
     hipLaunchKernelGGL(update_p, dim3(gvec), dim3(256), 0, stream(), n, g_pcg.part_c, np, g_pcg.rz, 1, g_pcg.rz,
                        g_pcg.z, g_pcg.p);
     IRWLS_LAUNCH_CHECK();
-    const size_t rrb = (size_t)2 * gvec * sizeof(double);
+    const size_t rrb = (size_t)4 * gvec * sizeof(double);
     // relative residuals from the partials read back for one iteration;
     // returns 1 when converged, -1 when diverged (or NaN), else 0
     auto judge = [&](const double *rr_host) {
-        double rr[2] = {0.0, 0.0};
+        double rr[2] = {0.0, 0.0}, bn[2] = {0.0, 0.0};
         for (int g = 0; g < gvec; g++) {
             rr[0] += rr_host[2 * g];
             rr[1] += rr_host[2 * g + 1];
+            bn[0] += rr_host[2 * gvec + 2 * g];
+            bn[1] += rr_host[2 * gvec + 2 * g + 1];
         }
         for (int c = 0; c < 2; c++) relres[c] = (bn[c] > 0.0) ? sqrt(rr[c] / bn[c]) : 0.0;
         if (relres[0] <= tol && relres[1] <= tol) return 1;
