@@ -41,18 +41,34 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd(const float *__restrict__ P, 
     }
     __syncthreads();
     const int j = t & (NH - 1), sh = (t >> 7) * (SB / 2);   // unit j, samples sh..sh+31
-    // layer 1
-    for (int s = sh; s < sh + SB / 2; s++) {
-        float a = P[OB1 + j];
-        for (int i = 0; i < NIN; i++) a = fmaf(xs[s][i], P[OW1 + i * NH + j], a);
-        h1[s][j] = a > 0.0f ? a : 0.0f;
+    // layer 1 (each weight loaded once for the thread's 32 samples; every
+    // sample's sum still runs over i in order)
+    float acc[SB / 2];
+    {
+        const float b = P[OB1 + j];
+#pragma unroll
+        for (int u = 0; u < SB / 2; u++) acc[u] = b;
+        for (int i = 0; i < NIN; i++) {
+            const float w = P[OW1 + i * NH + j];
+#pragma unroll
+            for (int u = 0; u < SB / 2; u++) acc[u] = fmaf(xs[sh + u][i], w, acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < SB / 2; u++) h1[sh + u][j] = acc[u] > 0.0f ? acc[u] : 0.0f;
     }
     __syncthreads();
     // layer 2
-    for (int s = sh; s < sh + SB / 2; s++) {
-        float a = P[OB2 + j];
-        for (int i = 0; i < NH; i++) a = fmaf(h1[s][i], P[OW2 + i * NH + j], a);
-        h2[s][j] = a > 0.0f ? a : 0.0f;
+    {
+        const float b = P[OB2 + j];
+#pragma unroll
+        for (int u = 0; u < SB / 2; u++) acc[u] = b;
+        for (int i = 0; i < NH; i++) {
+            const float w = P[OW2 + i * NH + j];
+#pragma unroll
+            for (int u = 0; u < SB / 2; u++) acc[u] = fmaf(h1[sh + u][i], w, acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < SB / 2; u++) h2[sh + u][j] = acc[u] > 0.0f ? acc[u] : 0.0f;
     }
     __syncthreads();
     // output and loss
@@ -97,11 +113,15 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd(const float *__restrict__ P, 
         g[OB2 + t] = a;
     }
     // dh1 (into h2, no longer needed): sum_j dh2[s][j] W2[i][j], masked by h1 > 0
-    for (int s = sh; s < sh + SB / 2; s++) {
-        float a = 0.0f;
-        for (int jj = 0; jj < NH; jj++) a = fmaf(dh[s][jj], P[OW2 + j * NH + jj], a);
-        h2[s][j] = (h1[s][j] > 0.0f) ? a : 0.0f;
+#pragma unroll
+    for (int u = 0; u < SB / 2; u++) acc[u] = 0.0f;
+    for (int jj = 0; jj < NH; jj++) {
+        const float w = P[OW2 + j * NH + jj];
+#pragma unroll
+        for (int u = 0; u < SB / 2; u++) acc[u] = fmaf(dh[sh + u][jj], w, acc[u]);
     }
+#pragma unroll
+    for (int u = 0; u < SB / 2; u++) h2[sh + u][j] = (h1[sh + u][j] > 0.0f) ? acc[u] : 0.0f;
     __syncthreads();
     // W1, b1
     for (int e = t; e < NIN * NH; e += 256) {
