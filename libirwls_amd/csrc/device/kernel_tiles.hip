@@ -636,7 +636,9 @@ void cols_T(const irwls_rows *c, int d, double **XcT, double **cn, unsigned long
         hipLaunchKernelGGL(cols_T_from_dense, dim3((c->n + 255) / 256, d), dim3(256), 0, stream(), c->X,
                            c->ldx, c->idx, c->nrm, c->n, d, *XcT, *cn, *mask);
     } else {
-        irwls_dev_memset0(*XcT, (size_t)c->n * d * sizeof(double));
+        // only the entries of features a column has are written (and their
+        // mask bits set); the CSR row kernels read an entry only when its
+        // bit is set, so the rest of the block needs no clearing
         hipLaunchKernelGGL(cols_from_csr, dim3(c->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val,
                            c->idx, c->nrm, c->n, d, 1, *XcT, *cn, *mask);
     }
