@@ -510,6 +510,65 @@ __global__ void cols_from_csr(const int64_t *__restrict__ rowptr, const int32_t 
     if (threadIdx.x == 0) cn[j] = nrm ? nrm[src] : 0.0;
 }
 
+// The same transposed block and masks in one launch, one work-group per
+// 64-column chunk: lane j of wave 0 reads column j's row range, a wave
+// prefix sum places the chunk's nonzeros, and all 256 threads scatter them
+// (the column of nonzero e by binary search over the offsets) while the
+// mask is built in LDS and then written whole.  Only entries whose mask bit
+// is set are written; the CSR row kernels read no others.  d <= kMaskLds.
+constexpr int kMaskLds = 4096;
+__global__ __launch_bounds__(256) void cols_T_csr_chunk(const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ val,
+                                                        const int32_t *__restrict__ idx,
+                                                        const double *__restrict__ nrm, int nc, int d,
+                                                        double *__restrict__ out, double *__restrict__ cn,
+                                                        unsigned long long *__restrict__ mask) {
+    __shared__ unsigned long long m[kMaskLds];
+    __shared__ int64_t beg[64];
+    __shared__ int off[65];
+    const int t = threadIdx.x;
+    const int j0 = blockIdx.x * 64;
+    const int w = (nc - j0 < 64) ? nc - j0 : 64;
+    for (int f = t; f < d; f += 256) m[f] = 0ull;
+    if (t < 64) {
+        int len = 0;
+        int64_t b = 0;
+        if (t < w) {
+            const int src = idx ? idx[j0 + t] : j0 + t;
+            b = rowptr[src];
+            len = (int)(rowptr[src + 1] - b);
+            cn[j0 + t] = nrm ? nrm[src] : 0.0;
+        }
+        beg[t] = b;
+        int x = len;   // inclusive prefix sum over the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (t >= o) x += y;
+        }
+        off[t + 1] = x;
+        if (t == 0) off[0] = 0;
+    }
+    __syncthreads();
+    const int total = off[64];
+    for (int e = t; e < total; e += 256) {
+        int lo = 0, hi = 63;   // the last column whose range starts at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t p = beg[lo] + (e - off[lo]);
+        const int f = col[p];
+        if (f < d) {
+            out[(size_t)f * nc + j0 + lo] = val[p];
+            atomicOr(&m[f], 1ull << lo);
+        }
+    }
+    __syncthreads();
+    for (int f = t; f < d; f += 256) mask[(size_t)blockIdx.x * d + f] = m[f];
+}
+
 __global__ void row_norms_kernel(const double *__restrict__ X, int ldx, int n, int d, double *__restrict__ nrm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -631,6 +690,12 @@ void cols_T(const irwls_rows *c, int d, double **XcT, double **cn, unsigned long
     *cn = (double *)scratch(5, (size_t)c->n * sizeof(double) + 8);
     *mask = (unsigned long long *)scratch(15, nchunk * d * sizeof(unsigned long long) + 8);
     if (c->n == 0) return;
+    if (!c->X && d <= kMaskLds) {
+        hipLaunchKernelGGL(cols_T_csr_chunk, dim3((unsigned)nchunk), dim3(256), 0, stream(), c->rowptr, c->col,
+                           c->val, c->idx, c->nrm, c->n, d, *XcT, *cn, *mask);
+        IRWLS_LAUNCH_CHECK();
+        return;
+    }
     irwls_dev_memset0(*mask, nchunk * d * sizeof(unsigned long long));
     if (c->X) {
         hipLaunchKernelGGL(cols_T_from_dense, dim3((c->n + 255) / 256, d), dim3(256), 0, stream(), c->X,
