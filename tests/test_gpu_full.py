@@ -145,6 +145,30 @@ def test_potrf_potrs_spd(n):
     np.testing.assert_allclose(bd.get(), np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,f", [(100, 37), (64, 16), (130, 70), (200, 127), (40, 0)])
+def test_potrf_pivot_failing_inside_a_tile(n, f):
+    """A matrix whose pivot f has Schur value -1 (LAPACK potrf semantics):
+    info = f + 1, the leading f x f factor is the Cholesky factor of A's
+    leading block, row f holds l_f = L^-1 a_f and the failing diagonal keeps
+    the Schur value.  f falls inside a 64-row tile and inside one of its
+    16-column stages (or on a stage or tile boundary)."""
+    rng = np.random.default_rng(300 + n + f)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    lead = A[:f, :f]
+    A[f, f] = (A[f, :f] @ np.linalg.solve(lead, A[:f, f]) if f else 0.0) - 1.0
+    L = lib.load()
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    info = L.irwls_dev_potrf(Ad.p, n, n)
+    assert info == f + 1
+    F = Ad.get().reshape((n, n), order="F")
+    if f:
+        Ll = np.linalg.cholesky(lead)
+        np.testing.assert_allclose(np.tril(F[:f, :f]), Ll, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(F[f, :f], np.linalg.solve(Ll, A[:f, f]), rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(F[f, f], -1.0, rtol=1e-8)
+
+
 @pytest.mark.parametrize("n", [2, 50, 129, 257])
 def test_kkt_failed_last_pivot(n):
     """Indefinite KKT [A y; y^T 0]: the factorization stops at the last pivot
