@@ -95,6 +95,25 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d,
                              double a, double b, double cst, const double *d_rs,
                              double *d_out);
 
+/* irwls_dev_kernel_matvec without any device allocation.  It has no
+   counterpart in the reference: it is for callers that cannot allocate
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same kernels and summation order as irwls_dev_kernel_matvec,
+   so the same result bit for bit; the column splits' partial sums live in a
+   fixed device array of IRWLS_NOALLOC_SCRATCH_BYTES per GPU (held until the
+   process ends) instead of the library's scratch pool.  Columns must be
+   dense (c->X set), and rows too for the RBF kernel.  Not timed.  Needs
+   irwls_dev_init first.  All calls on one GPU share the array, so calls
+   that overlap on one GPU (several host threads or streams) race.
+   Returns 0, or -1 with a message on stderr when the inputs are CSR or the
+   partial sums would exceed the array (nsplit x r->n doubles for RBF,
+   (min(c->n, 64) + 1) x d for the linear kernel). */
+#define IRWLS_NOALLOC_SCRATCH_BYTES (16 << 20)
+int  irwls_dev_kernel_matvec_noalloc(const irwls_rows *r, const irwls_rows *c, int d,
+                                     int kernel, double gamma, const double *d_w,
+                                     double a, double b, double cst, const double *d_rs,
+                                     double *d_out);
+
 /* K[i*ldk + j] = k(r_i, c_j) for all i < r->n, j < c->n. */
 void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d,
                              int kernel, double gamma, double *d_K, int ldk);

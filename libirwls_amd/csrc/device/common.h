@@ -37,7 +37,8 @@ __device__ __forceinline__ double rdl(double v, int lane) {
 // The library stream: every kernel and copy of the library is issued here.
 hipStream_t stream();
 
-// Timing brackets around launches of one kind (no-ops unless enabled).
+// Timing brackets around launches of one kind (no-ops unless enabled; a
+// negative kind is never timed).
 struct TimerScope {
     int kind;
     double flops, bytes;
@@ -51,6 +52,23 @@ struct TimerScope {
 
 // Scratch device buffer owned by the context, grown on demand.
 void *scratch(int slot, size_t bytes);
+
+// Kernel-block x vector product over dense columns (c->X set, r->n and
+// c->n > 0) with a caller-given partial-sum buffer of kmv_part_bytes bytes
+// (0: none needed); timed = false records no timer events.
+size_t kmv_dense_part_bytes(int rn, int cn);
+size_t kmv_primal_part_bytes(int cn, int d);
+inline size_t kmv_part_bytes(const irwls_rows *r, const irwls_rows *c, int d, int kernel) {
+    return (kernel == 0 && d > 0) ? kmv_primal_part_bytes(c->n, d) : kmv_dense_part_bytes(r->n, c->n);
+}
+void kmv_dense_cols_run(const irwls_rows *r, const irwls_rows *c, int d, int kernel, double gamma, const double *w,
+                        double a, double b, double cst, const double *rs, double *out, double *part, bool timed);
+void kmv_empty_cols(int n, double a, double cst, double *out);
+// True once irwls_dev_init has run.
+bool dev_initialised();
+// Looks up the current device's address of the fixed partial-sum array of
+// irwls_dev_kernel_matvec_noalloc (called by irwls_dev_init).
+void noalloc_setup();
 
 // Small device-to-host read that does not drain the stream: the copy is
 // queued into a pinned buffer of the calling thread (at most kReadbackMax

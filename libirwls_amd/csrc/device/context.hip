@@ -22,6 +22,8 @@ hipStream_t stream() {
     return g_stream;
 }
 
+bool dev_initialised() { return g_init; }
+
 // ---- timers --------------------------------------------------------------
 // Each timed launch records an event pair from a pool without synchronising;
 // pairs are resolved (elapsed time read) when the pool fills or the host asks
@@ -64,7 +66,8 @@ static void resolve_pool() {
 }
 
 TimerScope::TimerScope(int k, double f, double b, bool any_thread)
-    : kind(k), flops(f), bytes(b), on(g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)),
+    : kind(k), flops(f), bytes(b),
+      on(k >= 0 && g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)),
       idx(-1) {
     if (!on) return;
     std::lock_guard<std::mutex> lk(g_tmu);
@@ -300,6 +303,7 @@ int irwls_dev_init(void) {
         g_pool_ready = true;
     }
     g_init = true;
+    noalloc_setup();
     // first allocation and first launch (code-object load) inside the guard too
     void *probe = nullptr;
     IRWLS_HIP(hipMalloc(&probe, 256));

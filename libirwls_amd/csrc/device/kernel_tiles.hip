@@ -712,6 +712,76 @@ void cols_T(const irwls_rows *c, int d, double **XcT, double **cn, unsigned long
 
 }  // namespace
 
+namespace irwls {
+
+void kmv_empty_cols(int n, double a, double cst, double *out) {
+    hipLaunchKernelGGL(axpb_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), n, a, cst, out);
+    IRWLS_LAUNCH_CHECK();
+}
+
+// Column splits of the dense kernel-block x vector product for rows x cols.
+static void kmv_dense_shape(int rn, int cn, int *rt, int *tps, int *nsplit) {
+    *rt = (rn + TB - 1) / TB;
+    column_splits(*rt, (cn + TB - 1) / TB, 2, tps, nsplit);
+}
+
+size_t kmv_dense_part_bytes(int rn, int cn) {
+    int rt, tps, nsplit;
+    kmv_dense_shape(rn, cn, &rt, &tps, &nsplit);
+    return nsplit > 1 ? (size_t)nsplit * rn * sizeof(double) : 0;
+}
+
+size_t kmv_primal_part_bytes(int cn, int d) {
+    const int ns = cn < PRIMAL_SPLITS ? cn : PRIMAL_SPLITS;
+    return (size_t)(ns + 1) * d * sizeof(double);
+}
+
+// Dense rows x dense columns (r->n, c->n > 0, c->X set).  part holds
+// kmv_dense_part_bytes(r->n, c->n) bytes (RBF) or kmv_primal_part_bytes(c->n,
+// d) (linear kernel, rows dense or CSR).  timed = false records no timer
+// events (for callers that may be capturing the stream).
+void kmv_dense_cols_run(const irwls_rows *r, const irwls_rows *c, int d, int kernel, double gamma, const double *w,
+                        double a, double b, double cst, const double *rs, double *out, double *part, bool timed) {
+    const DenseSide csd = dense_side(c);
+    if (kernel == 0 && d > 0) {
+        const int ns = csd.n < PRIMAL_SPLITS ? csd.n : PRIMAL_SPLITS;
+        const int per = (csd.n + ns - 1) / ns;
+        double *u = part + (size_t)ns * d;
+        const double nz = r->X ? (double)r->n * d : r->nnz_per_row * r->n;
+        TimerScope ts(timed ? IRWLS_T_KMV : -1, 2.0 * ((double)csd.n * d + nz),
+                      8.0 * ((double)csd.n * d + nz + 2.0 * r->n));
+        hipLaunchKernelGGL(primal_part_dense, dim3((d + 255) / 256, ns), dim3(256), 0, stream(), csd, d, w, per,
+                           part);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(primal_sum, dim3((d + 255) / 256), dim3(256), 0, stream(), part, ns, d, u);
+        IRWLS_LAUNCH_CHECK();
+        if (r->X)
+            hipLaunchKernelGGL(primal_apply_dense, dim3((r->n + 3) / 4), dim3(256), 0, stream(), dense_side(r), d,
+                               u, a, b, cst, rs, out);
+        else
+            hipLaunchKernelGGL(primal_apply_csr, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
+                               r->val, r->idx, r->n, u, a, b, cst, rs, out);
+        IRWLS_LAUNCH_CHECK();
+        return;
+    }
+    const DenseSide rsd = dense_side(r);
+    int rt, tps, nsplit;
+    kmv_dense_shape(rsd.n, csd.n, &rt, &tps, &nsplit);
+    const double flops = 2.0 * rsd.n * (double)csd.n * d;
+    const double bytes = 8.0 * ((double)rsd.n * d + (double)csd.n * d + 2.0 * rsd.n);
+    TimerScope ts(timed ? IRWLS_T_KMV : -1, flops, bytes);
+    hipLaunchKernelGGL(kmv_dense_kernel, dim3(rt, nsplit), dim3(256), 0, stream(), rsd, csd, d, kernel, gamma, w,
+                       tps, nsplit, part, a, b, cst, rs, out);
+    IRWLS_LAUNCH_CHECK();
+    if (nsplit > 1) {
+        hipLaunchKernelGGL(kmv_finish_kernel, dim3((rsd.n + 255) / 256), dim3(256), 0, stream(), part, rsd.n,
+                           nsplit, a, b, cst, rs, out);
+        IRWLS_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace irwls
+
 extern "C" {
 
 void irwls_dev_row_norms(const double *X, int ldx, int n, int d, double *nrm) {
@@ -724,52 +794,23 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
                              const double *w, double a, double b, double cst, const double *rs, double *out) {
     if (r->n <= 0) return;
     if (c->n <= 0) {
-        hipLaunchKernelGGL(axpb_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->n, a, cst, out);
-        IRWLS_LAUNCH_CHECK();
+        kmv_empty_cols(r->n, a, cst, out);
         return;
     }
-    if (kernel == 0 && d > 0) {
-        // linear kernel: u = sum_j w_j z_j, then one dot product per row
-        DenseSide csd = dense_cols(c, d, 6, 7);
-        const int ns = c->n < PRIMAL_SPLITS ? c->n : PRIMAL_SPLITS;
-        const int per = (c->n + ns - 1) / ns;
-        double *part = (double *)scratch(3, (size_t)(ns + 1) * d * sizeof(double));
-        double *u = part + (size_t)ns * d;
-        const double nz = r->X ? (double)r->n * d : r->nnz_per_row * r->n;
-        TimerScope ts(IRWLS_T_KMV, 2.0 * ((double)c->n * d + nz), 8.0 * ((double)c->n * d + nz + 2.0 * r->n));
-        hipLaunchKernelGGL(primal_part_dense, dim3((d + 255) / 256, ns), dim3(256), 0, stream(), csd, d, w, per, part);
-        IRWLS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(primal_sum, dim3((d + 255) / 256), dim3(256), 0, stream(), part, ns, d, u);
-        IRWLS_LAUNCH_CHECK();
-        if (r->X)
-            hipLaunchKernelGGL(primal_apply_dense, dim3((r->n + 3) / 4), dim3(256), 0, stream(), dense_side(r), d, u,
-                               a, b, cst, rs, out);
-        else
-            hipLaunchKernelGGL(primal_apply_csr, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
-                               r->val, r->idx, r->n, u, a, b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
-        return;
-    }
-    const double flops = 2.0 * r->n * (double)c->n * d;
-    if (r->X) {
-        DenseSide rsd = dense_side(r);
-        DenseSide csd = dense_cols(c, d, 6, 7);
-        const int rt = (r->n + TB - 1) / TB;
-        const int ct = (c->n + TB - 1) / TB;
-        int tps, nsplit;
-        column_splits(rt, ct, 2, &tps, &nsplit);
-        double *part = nullptr;
-        if (nsplit > 1) part = (double *)scratch(8, (size_t)nsplit * r->n * sizeof(double));
-        const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + 2.0 * r->n);
-        TimerScope ts(IRWLS_T_KMV, flops, bytes);
-        hipLaunchKernelGGL(kmv_dense_kernel, dim3(rt, nsplit), dim3(256), 0, stream(), rsd, csd, d, kernel,
-                           gamma, w, tps, nsplit, part, a, b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
-        if (nsplit > 1) {
-            hipLaunchKernelGGL(kmv_finish_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), part, r->n,
-                               nsplit, a, b, cst, rs, out);
-            IRWLS_LAUNCH_CHECK();
+    if ((kernel == 0 && d > 0) || r->X) {
+        // dense columns (CSR columns scattered into a dense block first); the
+        // linear kernel forms u = sum_j w_j z_j, then one dot product per row
+        irwls_rows cd = *c;
+        if (!c->X) {
+            const DenseSide csd = dense_cols(c, d, 6, 7);
+            cd.X = csd.X;
+            cd.ldx = csd.ldx;
+            cd.idx = nullptr;
+            cd.nrm = csd.nrm;
         }
+        const size_t pb = kmv_part_bytes(r, &cd, d, kernel);
+        double *part = pb ? (double *)scratch(kernel == 0 && d > 0 ? 3 : 8, pb) : nullptr;
+        kmv_dense_cols_run(r, &cd, d, kernel, gamma, w, a, b, cst, rs, out, part, true);
     } else {
         double *XcT, *cn;
         unsigned long long *cm;
