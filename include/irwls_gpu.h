@@ -26,6 +26,8 @@ typedef struct irwls_gpu_options {
     int proposal_warmup;  /* learned proposal: plain SGMA steps that train the network (0: B/10 in [20, 100]) */
     int proposal_rows;    /* learned proposal: rows on which the shortlist's gains are computed (0: all n, the
                              exact gains; m < n: a fixed random subset of m rows, an estimate) */
+    int par_max_iters;    /* IRWLSpar: stop after this many iterations (0: the reference's rules only; a
+                             diagnostic cap to compare iterates) */
 } irwls_gpu_options;
 
 /* Defaults, overridable by IRWLS_PRECISION=fp64|mixed, IRWLS_SGMA_CAND=<n>,
@@ -35,7 +37,10 @@ void irwls_gpu_set_options(const irwls_gpu_options *opts);
 
 /* Keep a dataset resident in device memory so later trainFULL / SGMA /
    IRWLSpar / test calls on it skip the host-to-device upload.  The dataset is
-   identified by its `features` pointer; it must not change while pinned. */
+   identified by its features, x, y and quadratic_value pointers.  Labels,
+   norms and the mean rows may be changed in place (one-vs-rest relabelling):
+   every call compares their fingerprints and uploads what changed.  Feature
+   values of the other rows must not change while pinned.  freeDataset unpins. */
 void irwls_gpu_pin_dataset(svm_dataset dataset, int with_means);
 void irwls_gpu_unpin_dataset(svm_dataset dataset);
 
@@ -63,6 +68,7 @@ typedef struct irwls_gpu_stats {
     int batch_rounds;      /* trainFULLBatched: batched solve rounds */
     double t_wait;         /* seconds the host waited for the device (read-backs, synchronisations;
                               trainFULLBatched: summed over the workers, batched solves excluded) */
+    int par_m_decays;      /* IRWLSpar: times its weight cap M was divided by 10 */
 } irwls_gpu_stats;
 void irwls_gpu_last_stats(irwls_gpu_stats *out);
 

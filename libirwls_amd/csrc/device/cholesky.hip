@@ -922,11 +922,19 @@ BatchPlan *plan_for(int nslot, int ncap, int round) {
     return pl;
 }
 
-// Called with g_bmu held.
+// Called with g_bmu held.  Entries are never reset or reused (another
+// round slot may hold one); past kMaxTunes capacity classes the fixed shape
+// entry (one 64-row tile per work-group, not tuned) serves the rest.
+BatchTune g_untuned;
 BatchTune *tune_for(int ncap) {
     for (int i = 0; i < g_ntunes; i++)
         if (g_tunes[i].ncap == ncap) return &g_tunes[i];
-    BatchTune *t = g_ntunes < kMaxTunes ? &g_tunes[g_ntunes++] : &g_tunes[0];
+    if (g_ntunes == kMaxTunes) {
+        g_untuned.ncap = -1;
+        g_untuned.best = 0;
+        return &g_untuned;
+    }
+    BatchTune *t = &g_tunes[g_ntunes++];
     *t = BatchTune();
     t->ncap = ncap;
     return t;
@@ -1105,18 +1113,20 @@ int irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *A, cons
         t[p].rhs = rhs[p];
         t[p].x = x[p];
     }
+    int best;
     {
         std::lock_guard<std::mutex> lk(g_bmu);   // one tuning per capacity, whichever slot gets there first
         if (tu->best < 0) {
             if (g_autotune) autotune(pl, tu, t);
             else tu->best = 0;
         }
+        best = tu->best;
     }
     irwls_dev_h2d(pl->d_tab, t, (size_t)nslot * sizeof(BatchProb));
     {
         // capture (first use of this class on this slot) before the timer's
         // first event is recorded on this stream
-        hipGraphExec_t ex = graph_for(pl, tu->best);
+        hipGraphExec_t ex = graph_for(pl, best);
         double fl = 0.0;
         for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
         TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true);   // batches of different round slots may overlap

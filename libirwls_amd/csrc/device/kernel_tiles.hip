@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
             v[u] = ok ? val[p + u] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) m[u] = f[u] < d ? (unsigned)cmask[f[u]] : 0u;
+        for (int u = 0; u < 4; u++) m[u] = (unsigned)f[u] < (unsigned)d ? (unsigned)cmask[f[u]] : 0u;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             if (m[u] == 0u) continue;
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void kmat_csr_block_narrow_kernel(
                 v[u] = ok ? s_val[p + u - cb] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) m[u] = f[u] < d ? (unsigned)cmask[f[u]] : 0u;
+            for (int u = 0; u < 4; u++) m[u] = (unsigned)f[u] < (unsigned)d ? (unsigned)cmask[f[u]] : 0u;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (m[u] == 0u) continue;
@@ -389,7 +389,7 @@ __device__ __forceinline__ double csr_row_dot(const int32_t *__restrict__ col, c
         if (lane < cnt) {
             fl = col[b + lane];
             vl = val[b + lane];
-            if (fl < d) ml = cmask[fl];
+            if ((unsigned)fl < (unsigned)d) ml = cmask[fl];
         }
         unsigned long long act = __ballot(ml != 0ull);
         while (act) {
@@ -499,7 +499,7 @@ __global__ void cols_from_csr(const int64_t *__restrict__ rowptr, const int32_t 
     const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
     for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         const int f = col[p];
-        if (f < d) {
+        if ((unsigned)f < (unsigned)d) {
             if (trans) {
                 out[(size_t)f * nc + j] = val[p];
                 if (mask) atomicOr(&mask[(size_t)(j >> 6) * d + f], 1ull << (j & 63));
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(256) void cols_T_csr_chunk(const int64_t *__restric
         }
         const int64_t p = beg[lo] + (e - off[lo]);
         const int f = col[p];
-        if (f < d) {
+        if ((unsigned)f < (unsigned)d) {
             out[(size_t)f * nc + j0 + lo] = val[p];
             atomicOr(&m[f], 1ull << lo);
         }
@@ -714,6 +714,35 @@ void cols_T(const irwls_rows *c, int d, double **XcT, double **cn, unsigned long
 
 namespace irwls {
 
+// The CSR row kernels read the columns from a transposed dense block of
+// cols x d doubles plus per-chunk feature masks.  For high-dimensional data
+// (libsvm text sets with d around 10^6) that block is built and used
+// kCsrColBudget bytes of columns at a time, 64 columns at least.
+constexpr size_t kCsrColBudget = (size_t)1 << 30;
+
+static int csr_col_chunk(int d) {
+    const size_t per = (size_t)(d > 0 ? d : 1) * sizeof(double);
+    size_t c = kCsrColBudget / per / 64 * 64;
+    if (c < 64) c = 64;
+    return c > (size_t)INT32_MAX ? INT32_MAX : (int)c;
+}
+
+// Columns [j0, j0 + cn) of a column side, as a side of its own.
+static irwls_rows cols_slice(const irwls_rows *c, int j0, int cn) {
+    irwls_rows s = *c;
+    s.n = cn;
+    if (c->idx) {
+        s.idx = c->idx + j0;
+    } else if (c->X) {
+        s.X = c->X + (size_t)j0 * c->ldx;
+        s.nrm = c->nrm + j0;
+    } else {
+        s.rowptr = c->rowptr + j0;
+        s.nrm = c->nrm + j0;
+    }
+    return s;
+}
+
 void kmv_empty_cols(int n, double a, double cst, double *out) {
     hipLaunchKernelGGL(axpb_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), n, a, cst, out);
     IRWLS_LAUNCH_CHECK();
@@ -812,15 +841,23 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
         double *part = pb ? (double *)scratch(kernel == 0 && d > 0 ? 3 : 8, pb) : nullptr;
         kmv_dense_cols_run(r, &cd, d, kernel, gamma, w, a, b, cst, rs, out, part, true);
     } else {
-        double *XcT, *cn;
-        unsigned long long *cm;
-        cols_T(c, d, &XcT, &cn, &cm);
-        // algorithmic: the rows' nonzeros (value + column), their row pointers, the column table, the output
-        const double nz = r->nnz_per_row * r->n;
-        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 24.0 * r->n);
-        hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col, r->val,
-                           r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, w, a, b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
+        // column chunks of at most csr_col_chunk(d); the first applies a and
+        // cst, the later ones add their sums
+        const int chunk = csr_col_chunk(d);
+        for (int j0 = 0; j0 < c->n; j0 += chunk) {
+            const int cnk = c->n - j0 < chunk ? c->n - j0 : chunk;
+            const irwls_rows cs = cols_slice(c, j0, cnk);
+            double *XcT, *cn;
+            unsigned long long *cm;
+            cols_T(&cs, d, &XcT, &cn, &cm);
+            // algorithmic: the rows' nonzeros (value + column), their row pointers, the column table, the output
+            const double nz = r->nnz_per_row * r->n;
+            TimerScope ts(IRWLS_T_CSR, 2.0 * nz * cnk, 12.0 * nz + 8.0 * cnk * (double)d + 24.0 * r->n);
+            hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
+                               r->val, r->nrm, r->idx, r->n, XcT, cnk, d, cm, cn, cnk, kernel, gamma, w + j0,
+                               j0 == 0 ? a : 1.0, b, j0 == 0 ? cst : 0.0, rs, out);
+            IRWLS_LAUNCH_CHECK();
+        }
     }
 }
 
@@ -836,6 +873,14 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TB - 1) / TB, (c->n + TB - 1) / TB), dim3(256), 0,
                            stream(), rsd, csd, d, kernel, gamma, K, ldk);
         IRWLS_LAUNCH_CHECK();
+    } else if (c->n > csr_col_chunk(d)) {
+        // high-dimensional CSR: the block in column chunks
+        const int chunk = csr_col_chunk(d);
+        for (int j0 = 0; j0 < c->n; j0 += chunk) {
+            const int cnk = c->n - j0 < chunk ? c->n - j0 : chunk;
+            const irwls_rows cs = cols_slice(c, j0, cnk);
+            irwls_dev_kernel_matrix(r, &cs, d, kernel, gamma, K + j0, ldk);
+        }
     } else if (c->n <= NARROW) {
         double *XcT, *cn;
         unsigned long long *cm;

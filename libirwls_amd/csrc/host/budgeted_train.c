@@ -440,7 +440,10 @@ double *IRWLSpar(svm_dataset dataset, int *indexes, properties props) {
         ++iter;
         g_bstats.irwlspar_iters = iter;
         if (props.verbose == 1) printf("Iteration %d, nSVs %d, ||deltaW||^2/||W||^2=%f\n", iter, nA, dW / nW);
-        if (iter > 10 && dW / nW > 100 * oldnorm) M = M / 10.0;
+        if (iter > 10 && dW / nW > 100 * oldnorm) {
+            M = M / 10.0;
+            g_bstats.par_m_decays++;
+        }
         oldnorm = dW / nW;
         if (dW / nW < bestDW) {
             bestDW = dW / nW;
@@ -449,6 +452,7 @@ double *IRWLSpar(svm_dataset dataset, int *indexes, properties props) {
         } else {
             since++;
         }
+        if (irwls_opts()->par_max_iters > 0 && iter >= irwls_opts()->par_max_iters) break;
     }
     irwls_dev_free(d_idx);
     irwls_dev_free(d_KC);

@@ -47,6 +47,7 @@ static void opts_defaults(void) {
     g_opts.proposal_pool = 0;
     g_opts.proposal_warmup = 0;
     g_opts.proposal_rows = 0;
+    g_opts.par_max_iters = 0;
     const char *p = getenv("IRWLS_PRECISION");
     if (p && strcasecmp(p, "mixed") == 0) g_opts.precision = 1;
     const char *c = getenv("IRWLS_SGMA_CAND");
@@ -75,6 +76,7 @@ void irwls_gpu_set_options(const irwls_gpu_options *o) {
     if (g_opts.proposal_pool < 0) g_opts.proposal_pool = 0;
     if (g_opts.proposal_pool > 1024) g_opts.proposal_pool = 1024;
     if (g_opts.proposal_warmup < 0) g_opts.proposal_warmup = 0;
+    if (g_opts.par_max_iters < 0) g_opts.par_max_iters = 0;
 }
 
 void irwls_gpu_timing(int enable) { irwls_dev_timing(enable); }
@@ -108,9 +110,50 @@ int irwls_dataset_maxindex(const svm_dataset *ds, int rows) {
     return m;
 }
 
+/* 64-bit FNV-1a over 8-byte words. */
+static uint64_t fp_words(uint64_t h, const void *p, size_t nwords) {
+    const uint64_t *w = (const uint64_t *)p;
+    for (size_t i = 0; i < nwords; i++) {
+        h ^= w[i];
+        h *= 0x100000001b3ULL;
+    }
+    return h;
+}
+
+static uint64_t fp_labels(const svm_dataset *ds, int rows) {
+    uint64_t h = 0xcbf29ce484222325ULL;
+    if (ds->y) h = fp_words(h, ds->y, (size_t)rows);
+    if (ds->quadratic_value) h = fp_words(h, ds->quadratic_value, (size_t)rows);
+    return h;
+}
+
+static uint64_t fp_means(const svm_dataset *ds, int rows) {
+    uint64_t h = 0xcbf29ce484222325ULL;
+    for (int i = ds->l; i < rows; i++) {
+        const svm_sample *s = ds->x[i];
+        size_t k = 0;
+        while (s[k].index != -1) k++;
+        h = fp_words(h, &k, 1);
+        for (size_t j = 0; j < k; j++) {
+            const uint64_t idx = (uint64_t)(uint32_t)s[j].index;
+            h = fp_words(h, &idx, 1);
+            h = fp_words(h, &s[j].value, 1);
+        }
+    }
+    return h;
+}
+
+static void upload_fail(int row, int index) {
+    fprintf(stderr, "libirwls: feature index %d of row %d is out of range\n", index, row);
+    exit(2);
+}
+
 static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, int min_cols) {
     irwls_devdata *dd = (irwls_devdata *)irwls_xcalloc(1, sizeof(irwls_devdata));
     dd->key = ds->features;
+    dd->key_x = ds->x;
+    dd->key_y = ds->y;
+    dd->key_q = ds->quadratic_value;
     dd->l = ds->l;
     dd->rows = ds->l + (with_means ? 2 : 0);
     dd->dense = (ds->sparse == 0);
@@ -134,7 +177,8 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
         for (int i = 0; i < rows; i++)
             for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
                 const int c = s->index - base;
-                if (c >= 0 && c < d) hx[(size_t)i * d + c] = s->value;
+                if (c < 0) upload_fail(i, s->index);
+                if (c < d) hx[(size_t)i * d + c] = s->value;
             }
         dd->X = (double *)irwls_dev_alloc((size_t)rows * d * sizeof(double));
         irwls_dev_h2d(dd->X, hx, (size_t)rows * d * sizeof(double));
@@ -152,6 +196,7 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
         rp[0] = 0;
         for (int i = 0; i < rows; i++) {
             for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
+                if (s->index < base) upload_fail(i, s->index);
                 cl[k] = s->index - base;
                 vl[k] = s->value;
                 k++;
@@ -170,8 +215,11 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
         dd->nnz_per_row = rows > 0 ? (double)nnz / rows : 0.0;
         dd->nrm = dd->qv;
     }
+    dd->fp_yq = fp_labels(ds, rows);
+    dd->fp_mean = fp_means(ds, rows);
     return dd;
 }
+
 
 static void destroy(irwls_devdata *dd) {
     if (dd->Xh) irwls_dev_free(dd->Xh);
@@ -187,13 +235,51 @@ static void destroy(irwls_devdata *dd) {
     free(dd);
 }
 
+/* A pinned copy is the dataset's when the arrays are the same and the
+   fingerprints of its labels, norms and mean rows still match: labels or
+   norms changed in place (one-vs-rest relabelling of a shared dataset) are
+   copied again into the same device buffers, changed mean rows (whose entry
+   count may differ) take a new upload.  Called with g_dd_mu held; the copies
+   are complete before another thread can use the entry.  Feature values
+   changed in place are not detected: unpin a dataset before editing them. */
+static irwls_devdata *pinned_match(int slot, const svm_dataset *ds) {
+    irwls_devdata *p = g_pinned[slot];
+    if (p->key != ds->features || p->key_x != ds->x || p->key_y != ds->y || p->key_q != ds->quadratic_value ||
+        p->l != ds->l || p->dense != (ds->sparse == 0))
+        return NULL;
+    if (fp_means(ds, p->rows) != p->fp_mean) {
+        const int with_means = p->rows > p->l;
+        const int base = p->base, d = p->d;
+        destroy(p);
+        p = upload(ds, with_means, base, d);
+        irwls_dev_sync();
+        p->pinned = 1;
+        g_pinned[slot] = p;
+        return p;
+    }
+    const uint64_t fy = fp_labels(ds, p->rows);
+    if (fy != p->fp_yq) {
+        const size_t rr = (size_t)(p->rows > 0 ? p->rows : 1);
+        double *hy = (double *)irwls_xcalloc(rr, sizeof(double));
+        for (int i = 0; i < p->rows; i++) hy[i] = ds->y ? ds->y[i] : 0.0;
+        irwls_dev_h2d(p->y, hy, (size_t)p->rows * sizeof(double));
+        if (ds->quadratic_value) irwls_dev_h2d(p->qv, ds->quadratic_value, (size_t)p->rows * sizeof(double));
+        irwls_dev_sync();
+        free(hy);
+        p->fp_yq = fy;
+    }
+    return p;
+}
+
 irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols) {
     irwls_dev_init();
     pthread_mutex_lock(&g_dd_mu);
     for (int i = 0; i < MAX_PINNED; i++) {
         irwls_devdata *p = g_pinned[i];
-        if (p && p->key == ds->features && p->l == ds->l && p->dense == (ds->sparse == 0) &&
-            (p->rows >= ds->l + (with_means ? 2 : 0)) && (base < 0 || p->base == base) && p->d >= min_cols) {
+        if (p && p->key == ds->features && (p->rows >= ds->l + (with_means ? 2 : 0)) &&
+            (base < 0 || p->base == base) && p->d >= min_cols) {
+            p = pinned_match(i, ds);
+            if (!p) continue;
             pthread_mutex_unlock(&g_dd_mu);
             return p;
         }
@@ -280,23 +366,29 @@ void irwls_kmat(irwls_devdata *dd, const int32_t *d_ridx, int rn, const int32_t 
 void irwls_gpu_pin_dataset(svm_dataset ds, int with_means) {
     irwls_dev_init();
     pthread_mutex_lock(&g_dd_mu);
-    for (int i = 0; i < MAX_PINNED; i++)
-        if (g_pinned[i] && g_pinned[i]->key == ds.features) {
-            pthread_mutex_unlock(&g_dd_mu);
-            return;
-        }
+    int free_slot = -1;
     for (int i = 0; i < MAX_PINNED; i++) {
-        if (!g_pinned[i]) {
-            irwls_devdata *dd = upload(&ds, with_means, -1, 0);
-            irwls_dev_sync();
-            dd->pinned = 1;
-            g_pinned[i] = dd;
-            pthread_mutex_unlock(&g_dd_mu);
-            return;
+        if (g_pinned[i] && g_pinned[i]->key == ds.features) {
+            /* the same arrays, with at least the rows asked for: keep it
+               (refreshed); otherwise this pin replaces the old entry */
+            if (g_pinned[i]->rows >= ds.l + (with_means ? 2 : 0) && pinned_match(i, &ds)) {
+                pthread_mutex_unlock(&g_dd_mu);
+                return;
+            }
+            destroy(g_pinned[i]);
+            g_pinned[i] = NULL;
         }
+        if (!g_pinned[i] && free_slot < 0) free_slot = i;
     }
-    fprintf(stderr, "libirwls: more than %d pinned datasets\n", MAX_PINNED);
-    exit(2);
+    if (free_slot < 0) {
+        fprintf(stderr, "libirwls: more than %d pinned datasets\n", MAX_PINNED);
+        exit(2);
+    }
+    irwls_devdata *dd = upload(&ds, with_means, -1, 0);
+    irwls_dev_sync();
+    dd->pinned = 1;
+    g_pinned[free_slot] = dd;
+    pthread_mutex_unlock(&g_dd_mu);
 }
 
 void irwls_gpu_unpin_dataset(svm_dataset ds) {

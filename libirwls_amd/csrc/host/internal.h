@@ -22,7 +22,14 @@
    qv always holds the stored quadratic_value (prediction uses it,
    src/kernels.c:163). */
 typedef struct irwls_devdata {
-    const svm_sample *key;     /* dataset.features, identifies the dataset */
+    /* identity: the dataset's features, x, y and quadratic_value arrays */
+    const svm_sample *key;
+    const void *key_x, *key_y, *key_q;
+    /* pinned copies: fingerprints of the host's y and quadratic_value
+       (rows [0, rows)) and of the mean rows' entries, checked on every
+       acquire so that labels or mean rows changed in place are uploaded
+       again */
+    uint64_t fp_yq, fp_mean;
     int l;                     /* labelled rows */
     int rows;                  /* uploaded rows (l, or l+2 with the mean rows) */
     int d;                     /* column count of the device layout */

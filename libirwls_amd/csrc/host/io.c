@@ -13,6 +13,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <ctype.h>
+#include <limits.h>
 #include <pthread.h>
 #include <unistd.h>
 #include <math.h>
@@ -31,6 +32,7 @@ _Static_assert(sizeof(predictProperties) == 32, "predictProperties layout");
 _Static_assert(sizeof(model) == 72, "model layout");
 
 void freeDataset(svm_dataset data) {
+    irwls_gpu_unpin_dataset(data);   /* a pinned device copy goes with the host arrays */
     free(data.y);
     free(data.quadratic_value);
     free(data.x);
@@ -195,7 +197,14 @@ svm_dataset irwls_dataset_from_csr(const long long *rowptr, const int *col, cons
     int64_t *rp = (int64_t *)irwls_xmalloc((size_t)(n + 1) * sizeof(int64_t));
     int32_t *ix = (int32_t *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
     for (int i = 0; i <= n; i++) rp[i] = (int64_t)rowptr[i];
-    for (int64_t k = 0; k < nnz; k++) ix[k] = col[k] + 1;
+    for (int64_t k = 0; k < nnz; k++) {
+        if (col[k] < 0 || col[k] >= INT_MAX - 1) {
+            fprintf(stderr, "libirwls: irwls_dataset_from_csr: column %d of entry %lld is out of range\n", col[k],
+                    (long long)k);
+            exit(2);
+        }
+        ix[k] = col[k] + 1;
+    }
     svm_dataset ds = assemble(rp, ix, val, y, n, 1);
     free(rp);
     free(ix);
@@ -322,7 +331,7 @@ static void parse_libsvm_line(char *line, int labelled, rowbuf *b) {
         char *end;
         errno = 0;
         long ix = strtol(p, &end, 10);
-        if (end == p || *end != ':' || errno != 0 || ix <= last) bad_format();
+        if (end == p || *end != ':' || errno != 0 || ix <= last || ix >= INT_MAX) bad_format();
         p = end + 1;
         double v = strtod(p, &end);
         if (end == p || !(*end == '\0' || isspace((unsigned char)*end))) bad_format();

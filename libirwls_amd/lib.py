@@ -23,7 +23,7 @@ TIMER_KINDS = ["kmv", "kmat", "potf2", "trsm", "syrk", "potrs", "kkt", "gemv", "
 class irwls_gpu_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("sgma_candidates", C.c_int), ("proposal", C.c_int),
                 ("proposal_keep", C.c_int), ("refine_tol", C.c_double), ("proposal_pool", C.c_int),
-                ("proposal_warmup", C.c_int), ("proposal_rows", C.c_int)]
+                ("proposal_warmup", C.c_int), ("proposal_rows", C.c_int), ("par_max_iters", C.c_int)]
 
 
 class irwls_gpu_stats(C.Structure):
@@ -31,7 +31,8 @@ class irwls_gpu_stats(C.Structure):
                 ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double),
                 ("t_solve", C.c_double), ("factorizations", C.c_int), ("pcg_iters", C.c_int),
                 ("fp64_fallbacks", C.c_int), ("learned_steps", C.c_int), ("mlp_loss0", C.c_double),
-                ("mlp_loss", C.c_double), ("batch_rounds", C.c_int), ("t_wait", C.c_double)]
+                ("mlp_loss", C.c_double), ("batch_rounds", C.c_int), ("t_wait", C.c_double),
+                ("par_m_decays", C.c_int)]
 
 
 class irwls_rows(C.Structure):

@@ -301,6 +301,113 @@ int *orc_rpermute(int n) {
     return p;
 }
 
+/* Inner-loop continuation rule, src/full-train.c:158, with C's precedence:
+   (iter<5) || (bounds violated && iter<1000 && since<5 && dW/nW > 1e-6). */
+int orc_inner_continue(int iter, double minb, double maxb, double C, int since, double dW, double nW) {
+    return (iter < 5) || ((minb < 0.0 || maxb > C) && iter < 1000 && since < 5 && dW / nW > 1e-6);
+}
+
+/* Re-weighting and regrouping after an inner iteration
+   (src/full-train.c:264-300), in the reference's order of tests. */
+void orc_regroup(int l, const double *e, const double *y, const double *bNew, double C, double *a, int *grp) {
+    for (int i = 0; i < l; i++) {
+        const double ey = e[i] * y[i];
+        if (ey < 0.0) a[i] = 0.0;
+        else if (ey < 1.0 / 10000) a[i] = C * 10000.0;
+        else a[i] = 1.0 * y[i] * C / e[i];
+        if (ey < 0.0 && grp[i] != 2) grp[i] = 2;
+        if (grp[i] == 1 && y[i] * bNew[i] >= 0.99 * C && y[i] * bNew[i] <= 1.01 * C) grp[i] = 3;
+        if (a[i] == 0.0 && grp[i] == 1) grp[i] = 2;
+        if (grp[i] == 2 && a[i] != 0.0) grp[i] = 1;
+    }
+}
+
+/* Initial weights and groups of a sub-problem (src/full-train.c:131-155). */
+void orc_initial_groups(int l, const double *e, const double *y, const double *beta, double C, double *a,
+                        int *grp) {
+    for (int i = 0; i < l; i++) {
+        a[i] = (e[i] * y[i] < 0) ? 0.0 : 1.0 * y[i] * C / e[i];
+        if (a[i] == 0) grp[i] = 2;
+        else if (beta[i] == y[i] * C) grp[i] = 3;
+        else grp[i] = 1;
+    }
+}
+
+/* Outer-iteration bookkeeping of trainFULL (src/full-train.c:514-533):
+   returns 1 when dW/nW < eta (the run ends); tracks the best ratio and the
+   iterations since it improved. */
+int orc_outer_track(double dW, double nW, double eta, double *bestN, int *since) {
+    const int done = (dW / nW < eta);
+    if (dW / nW < *bestN) { *bestN = dW / nW; *since = 0; }
+    else (*since)++;
+    return done;
+}
+
+/* Working-set rebuild (src/full-train.c:540-643): one direct pick per
+   (class, state) among the KKT violators, state 0 = beta 0, 1 = free,
+   2 = at the bound C; other violators and every free sample are candidates
+   SC, the rest go to SIN.  Candidates fill the working set in order when
+   fewer than W - nSW, else through rpermute (libc rand()).  Returns the
+   number of candidates; found (6 ints, [y==1][state]) receives the flags. */
+int orc_ws_rebuild(int n, const double *bNew, const double *e, const double *y, double C, int W, int *SW, int *nSWo,
+                   int *SIN, int *nSIno, int *SC, int *found) {
+    const double thr = 0.001;
+    int f[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int nSW = 0, nSIn = 0, nSC = 0;
+    for (int i = 0; i < n; i++) {
+        const double by = bNew[i] * y[i];
+        int state, viol;
+        if (by == C) { state = 2; viol = (e[i] * y[i] < -1.0 * thr); }
+        else if (bNew[i] == 0.0) { state = 0; viol = (e[i] * y[i] > thr); }
+        else if (by != 0.0 && by != C) { state = 1; viol = (fabs(e[i] * y[i]) > thr); }
+        else continue;   /* in no set, as in the reference */
+        if (viol) {
+            int cls = -1;
+            if (y[i] == -1) cls = 0;
+            else if (y[i] == 1) cls = 1;
+            if (cls >= 0 && f[cls][state] == 0) { SW[nSW++] = i; f[cls][state] = 1; }
+            else SC[nSC++] = i;
+        } else if (state == 1) {
+            SC[nSC++] = i;
+        } else {
+            SIN[nSIn++] = i;
+        }
+    }
+    if (found) for (int c = 0; c < 2; c++) for (int s2 = 0; s2 < 3; s2++) found[3 * c + s2] = f[c][s2];
+    const int ncand = nSC;
+    if (nSC < W - nSW) {
+        for (int i = 0; i < nSC; i++) SW[nSW++] = SC[i];
+    } else {
+        int *perm = orc_rpermute(nSC);
+        const int space = W - nSW;
+        for (int i = 0; i < nSC; i++) {
+            if (i < space) SW[nSW++] = SC[perm[i]];
+            else SIN[nSIn++] = SC[perm[i]];
+        }
+        free(perm);
+    }
+    *nSWo = nSW;
+    *nSIno = nSIn;
+    return ncand;
+}
+
+/* IRWLSpar loop rule (src/budgeted-train.c:399). */
+int orc_par_continue(int iter, double dW, double nW, int since) {
+    return (iter < 500) && (dW / nW > 1e-6) && (since < 5);
+}
+
+/* IRWLSpar bookkeeping after iteration `iter` (already incremented;
+   src/budgeted-train.c:479-489): M /= 10 when iter > 10 and the ratio grew
+   more than 100 times; best-ratio tracking.  Returns 1 when the iterate is
+   the new best. */
+int orc_par_track(int iter, double ratio, double *M, double *oldnorm, double *bestDW, int *since) {
+    if (iter > 10 && ratio > 100 * *oldnorm) *M = *M / 10.0;
+    *oldnorm = ratio;
+    if (ratio < *bestDW) { *bestDW = ratio; *since = 0; return 1; }
+    (*since)++;
+    return 0;
+}
+
 double *orc_subIRWLS(svm_dataset ds, properties props, double *GIN, double *e, double *beta) {
     const int l = ds.l;
     const double C = props.C;
@@ -319,16 +426,15 @@ double *orc_subIRWLS(svm_dataset ds, properties props, double *GIN, double *e, d
     double dW = 1e9, nW = 1.0, bestDW = 1e9;
 
     /* initial weights and groups (src/full-train.c:131-155) */
+    orc_initial_groups(l, e, ds.y, beta, C, a, grp);
     for (int i = 0; i < l; i++) {
-        a[i] = (e[i] * ds.y[i] < 0) ? 0.0 : 1.0 * ds.y[i] * C / e[i];
-        if (a[i] == 0) grp[i] = 2;
-        else if (beta[i] == ds.y[i] * C) { S3[nS3++] = i; grp[i] = 3; }
-        else { S1[nS1++] = i; grp[i] = 1; }
+        if (grp[i] == 3) S3[nS3++] = i;
+        else if (grp[i] == 1) S1[nS1++] = i;
     }
     S1[nS1] = l;
 
     /* loop rule of src/full-train.c:158 (C precedence) */
-    while ((iter < 5) || ((minb < 0.0 || maxb > C) && iter < 1000 && since < 5 && dW / nW > 1e-6)) {
+    while (orc_inner_continue(iter, minb, maxb, C, since, dW, nW)) {
         iter++;
         g_stats.inner_iters++;
         const int N = nS1 + 1;
@@ -383,18 +489,8 @@ double *orc_subIRWLS(svm_dataset ds, properties props, double *GIN, double *e, d
             since++;
         }
         /* re-weighting and regrouping (src/full-train.c:264-300) */
-        for (int i = 0; i < l; i++) {
-            const double ey = e[i] * ds.y[i];
-            if (ey < 0.0) a[i] = 0.0;
-            else if (ey < 1.0 / 10000) a[i] = C * 10000.0;
-            else a[i] = 1.0 * ds.y[i] * C / e[i];
-            if (ey < 0.0 && grp[i] != 2) grp[i] = 2;
-            if (grp[i] == 1 && ds.y[i] * bNew[i] >= 0.99 * C && ds.y[i] * bNew[i] <= 1.01 * C) grp[i] = 3;
-            if (a[i] == 0.0 && grp[i] == 1) grp[i] = 2;
-            if (grp[i] == 2 && a[i] != 0.0) grp[i] = 1;
-            beta[i] = bNew[i];
-        }
-        beta[l] = bNew[l];
+        orc_regroup(l, e, ds.y, bNew, C, a, grp);
+        memcpy(beta, bNew, (l + 1) * sizeof(double));
         nS1 = 0; nS3 = 0;
         for (int i = 0; i < l; i++) {
             if (grp[i] == 1) S1[nS1++] = i;
@@ -424,7 +520,7 @@ double *orc_subIRWLS(svm_dataset ds, properties props, double *GIN, double *e, d
 
 double *orc_trainFULL(svm_dataset ds, properties props) {
     const int n = ds.l, W = props.MaxSize;
-    const double thr = 0.001, C = props.C;
+    const double C = props.C;
     double t0 = now_s();
     memset(&g_stats, 0, sizeof(g_stats));
     svm_dataset sub;
@@ -443,7 +539,7 @@ double *orc_trainFULL(svm_dataset ds, properties props) {
     double *GIN = (double *)calloc(W + 1, sizeof(double));
     double *esub = (double *)calloc(W + 1, sizeof(double));
     double *bsub = (double *)calloc(W + 1, sizeof(double));
-    int nSW = 0, nSIn = 0, nSC = 0;
+    int nSW = 0, nSIn = 0;
 
     /* initial working set: every 10th sample, up to W (src/full-train.c:411-423) */
     for (int i = 0; i < n; i++) {
@@ -500,45 +596,12 @@ double *orc_trainFULL(svm_dataset ds, properties props) {
             dW += t * t;
             nW += beta[i] * beta[i];
         }
-        if (dW / nW < props.Eta) done = 1;
+        done = orc_outer_track(dW, nW, props.Eta, &bestN, &since);
         memcpy(beta, bNew, (n + 1) * sizeof(double));
-        if (dW / nW < bestN) { bestN = dW / nW; since = 0; }
-        else since++;
 
         /* working-set rebuild (src/full-train.c:540-643) */
-        int f[2][3] = {{0, 0, 0}, {0, 0, 0}};   /* [y==1][state] */
-        nSW = 0; nSIn = 0; nSC = 0;
-        for (int i = 0; i < n; i++) {
-            const double by = bNew[i] * ds.y[i];
-            int state, viol;
-            if (by == C) { state = 2; viol = (e[i] * ds.y[i] < -1.0 * thr); }
-            else if (bNew[i] == 0.0) { state = 0; viol = (e[i] * ds.y[i] > thr); }
-            else if (by != 0.0 && by != C) { state = 1; viol = (fabs(e[i] * ds.y[i]) > thr); }
-            else continue;   /* beta == 0 with y*beta == 0 is handled above */
-            if (viol) {
-                int cls = -1;
-                if (ds.y[i] == -1) cls = 0;
-                else if (ds.y[i] == 1) cls = 1;
-                if (cls >= 0 && f[cls][state] == 0) { SW[nSW++] = i; f[cls][state] = 1; }
-                else SC[nSC++] = i;
-            } else if (state == 1) {
-                SC[nSC++] = i;
-            } else {
-                SIN[nSIn++] = i;
-            }
-        }
+        orc_ws_rebuild(n, bNew, e, ds.y, C, W, SW, &nSW, SIN, &nSIn, SC, NULL);
         if (props.verbose == 1) { printf("."); fflush(stdout); }
-        if (nSC < W - nSW) {
-            for (int i = 0; i < nSC; i++) SW[nSW++] = SC[i];
-        } else {
-            int *perm = orc_rpermute(nSC);
-            int space = W - nSW;
-            for (int i = 0; i < nSC; i++) {
-                if (i < space) SW[nSW++] = SC[perm[i]];
-                else SIN[nSIn++] = SC[perm[i]];
-            }
-            free(perm);
-        }
         if (g_max_outer > 0 && g_stats.outer_iters >= g_max_outer) break;
     }
     free(e); free(beta); free(GIN); free(esub); free(bsub);
@@ -780,7 +843,7 @@ double *orc_IRWLSpar(svm_dataset ds, int *idx, properties props) {
     int *ind = (int *)calloc(n, sizeof(int));
     int iter = 0, since = 0, nA = n;
     double dW = 1e9, nW = 1.0, oldnorm = 0.0, bestDW = 1e9;
-    while (iter < 500 && dW / nW > 1e-6 && since < 5) {
+    while (orc_par_continue(iter, dW, nW, since)) {
         /* K1 = KC + Phi_A^T Phi_A, K2 = Phi_A^T (sqrt(D) y) (src/budgeted-train.c:401-416) */
         memcpy(K1, KC, (size_t)B * B * sizeof(double));
         if (nA > 0) {
@@ -831,10 +894,9 @@ double *orc_IRWLSpar(svm_dataset ds, int *idx, properties props) {
         ++iter;
         g_stats.irwlspar_iters = iter;
         if (props.verbose == 1) printf("Iteration %d, nSVs %d, ||deltaW||^2/||W||^2=%f\n", iter, nA, dW / nW);
-        if (iter > 10 && dW / nW > 100 * oldnorm) M = M / 10.0;
-        oldnorm = dW / nW;
-        if (dW / nW < bestDW) { bestDW = dW / nW; since = 0; memcpy(bBest, bNew, B * sizeof(double)); }
-        else since++;
+        const double Mold = M;
+        if (orc_par_track(iter, dW / nW, &M, &oldnorm, &bestDW, &since)) memcpy(bBest, bNew, B * sizeof(double));
+        if (M != Mold) g_stats.m_decays++;
         if (g_max_par > 0 && iter >= g_max_par) break;
     }
     free(KC); free(KSC); free(KSCA); free(Da); free(Day);

@@ -28,6 +28,7 @@ typedef struct orc_stats {
     int sgma_steps;       /* SGMA steps with candidates */
     int irwlspar_iters;   /* IRWLSpar iterations */
     double t_total;       /* seconds */
+    int m_decays;         /* IRWLSpar: times M was divided by 10 */
 } orc_stats;
 
 /* Statistics of the last run. */
@@ -69,6 +70,19 @@ void orc_linear_system(double *A, int n, const double *rhs, double *x);
 
 /* src/full-train.c:65-77 */
 int *orc_rpermute(int n);
+/* Control-flow decisions of the trainers, factored out so that tables
+   derived from the reference text can check them. */
+int orc_inner_continue(int iter, double minb, double maxb, double C, int since, double dW, double nW);  /* :158 */
+void orc_regroup(int l, const double *e, const double *y, const double *bNew, double C, double *a,
+                 int *grp);                                                                            /* :264-300 */
+void orc_initial_groups(int l, const double *e, const double *y, const double *beta, double C, double *a,
+                        int *grp);                                                                     /* :131-155 */
+int orc_outer_track(double dW, double nW, double eta, double *bestN, int *since);                     /* :514-533 */
+int orc_ws_rebuild(int n, const double *bNew, const double *e, const double *y, double C, int W, int *SW,
+                   int *nSW, int *SIN, int *nSIn, int *SC, int *found);                                /* :540-643 */
+int orc_par_continue(int iter, double dW, double nW, int since);           /* src/budgeted-train.c:399 */
+int orc_par_track(int iter, double ratio, double *M, double *oldnorm, double *bestDW, int *since); /* :479-489 */
+
 /* src/full-train.c:96-362 */
 double *orc_subIRWLS(svm_dataset ds, properties props, double *GIN, double *e, double *beta);
 /* src/full-train.c:373-668 */

@@ -18,14 +18,15 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 class orc_stats(C.Structure):
     _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("solves", C.c_int),
-                ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double)]
+                ("sgma_steps", C.c_int), ("irwlspar_iters", C.c_int), ("t_total", C.c_double),
+                ("m_decays", C.c_int)]
 
 
 def build():
     subprocess.check_call(["make", "-s", "-B", "-C", HERE, "liboracle.so"])
     # the reference's reader / model-file source, where the reference tree is present
     if os.path.isdir("/root/reference/src"):
-        subprocess.check_call(["make", "-s", "-B", "-C", HERE, "_ref/libref_io.so"])
+        subprocess.check_call(["make", "-s", "-B", "-C", HERE, "_ref/libref_io.so", "_ref/libref_predict.so"])
 
 
 _lib = None
@@ -80,6 +81,19 @@ def lib():
         L.orc_srand.argtypes = [C.c_uint]
         L.orc_last_stats.restype = P(orc_stats)
         L.orc_set_limits.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_inner_continue.restype = C.c_int
+        L.orc_inner_continue.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                         C.c_double]
+        L.orc_regroup.argtypes = [C.c_int, dp, dp, dp, C.c_double, dp, ip]
+        L.orc_initial_groups.argtypes = [C.c_int, dp, dp, dp, C.c_double, dp, ip]
+        L.orc_outer_track.restype = C.c_int
+        L.orc_outer_track.argtypes = [C.c_double, C.c_double, C.c_double, dp, ip]
+        L.orc_ws_rebuild.restype = C.c_int
+        L.orc_ws_rebuild.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_int, ip, ip, ip, ip, ip, ip]
+        L.orc_par_continue.restype = C.c_int
+        L.orc_par_continue.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_par_track.restype = C.c_int
+        L.orc_par_track.argtypes = [C.c_int, C.c_double, dp, dp, dp, ip]
         _lib = L
     return _lib
 

@@ -49,3 +49,22 @@ def test_struct_sizes():
     assert ctypes.sizeof(abi.svm_dataset) == 48
     assert ctypes.sizeof(abi.properties) == 72
     assert ctypes.sizeof(abi.model) == 72
+
+
+def test_c_caller_compiles_and_links(tmp_path):
+    """A C program written against include/*.h only (tests/c_caller/caller.c)
+    compiles with gcc and links against libirwls.so: every function it calls
+    resolves (no GPU call is made here)."""
+    import subprocess
+    exe = str(tmp_path / "caller")
+    subprocess.check_call(["gcc", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "c_caller", "caller.c"),
+                           "-L", os.path.join(ROOT, "libirwls_amd"), "-lirwls",
+                           "-Wl,-rpath," + os.path.join(ROOT, "libirwls_amd"), "-Wl,--no-undefined", "-lm",
+                           "-o", exe])
+    assert os.path.exists(exe)
+    out = subprocess.run(["nm", "-u", exe], stdout=subprocess.PIPE, text=True).stdout
+    for name in ["trainFULL", "SGMA", "IRWLSpar", "calculateFULLModel", "calculateBudgetedModel", "test",
+                 "readTrainFile", "storeModel", "readModel", "initMemory", "freeMemory",
+                 "parseTrainFULLParameters", "parseTrainParameters"]:
+        assert name in out.split(), name

@@ -70,12 +70,15 @@ def test_posv_aug_batched_equals_single_and_replays():
     assert choice in (1, 2, 4)
 
 
+@pytest.mark.parametrize("precision", [0, 3])
 @pytest.mark.parametrize("wait_us", [None, "50"])
-def test_train_full_batched_equals_standalone(wait_us, monkeypatch):
+def test_train_full_batched_equals_standalone(wait_us, precision, monkeypatch):
     """Six problems of uneven size (two datasets, three C/gamma points):
-    every batched result equals its standalone trainFULL run.  With a 50 us
-    trigger, batches start as soon as a problem has waited that long, so
-    several run at once on different round slots."""
+    every batched result equals its standalone trainFULL run, in FP64 and
+    with BF16 hi/lo kernel products (precision 3, whose BF16 copies of a
+    shared dataset the worker threads build once between them).  With a
+    50 us trigger, batches start as soon as a problem has waited that long,
+    so several run at once on different round slots."""
     if wait_us is not None:
         monkeypatch.setenv("IRWLS_BATCH_WAIT_US", wait_us)
     tasks = []
@@ -84,9 +87,18 @@ def test_train_full_batched_equals_standalone(wait_us, monkeypatch):
         ds = lib.Dataset.dense(X, y)
         for C_, g in [(1.0, 0.5 / d), (10.0, 2.0 / d), (100.0, 0.5 / d)]:
             tasks.append((ds, abi.make_properties(gamma=g, C_=C_, W=min(n // 4, 1500), eta=0.001)))
-    want = [lib.train_full(ds, p, seed=0) for ds, p in tasks]
-    got = lib.train_full_batched([t[0] for t in tasks], [t[1] for t in tasks])
-    st = lib.last_stats()
+    try:
+        lib.set_options(precision=precision)
+        want = [lib.train_full(ds, p, seed=0) for ds, p in tasks]
+        if precision:
+            for ds, _ in tasks:
+                ds.pin(with_means=False)   # one shared device copy (and its BF16 split) per dataset
+        got = lib.train_full_batched([t[0] for t in tasks], [t[1] for t in tasks])
+        st = lib.last_stats()
+    finally:
+        lib.set_options(precision=0)
+        for ds, _ in tasks:
+            ds.unpin()
     print("batched stats", st)
     assert st["batch_rounds"] > 0
     for i, (g, w) in enumerate(zip(got, want)):

@@ -125,3 +125,46 @@ def test_gemv_both_orientations(m, n):
     L.irwls_dev_gemv(Ad.p, m, n, lda, 1, xtd.p, ytd.p)
     np.testing.assert_allclose(yd.get(), A[:m] @ x, rtol=1e-12, atol=1e-13 * np.sqrt(n))
     np.testing.assert_allclose(ytd.get(), A[:m].T @ xt, rtol=1e-12, atol=1e-13 * np.sqrt(m))
+
+
+def _noisy_case(seed):
+    """A tiny one-feature problem with noisy labels (the generator of the
+    search that found IRWLSpar runs whose weight cap M decays)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 120))
+    d = int(rng.integers(1, 5))
+    X = rng.standard_normal((n, d)) * float(rng.choice([0.1, 1.0, 10.0]))
+    if rng.random() < 0.5:
+        X = np.round(X, int(rng.integers(0, 2)))
+    y = np.where(rng.random(n) < float(rng.choice([0.5, 0.8])), 1.0, -1.0)
+    kern = int(rng.choice([0, 1, 1, 1]))
+    g = float(rng.choice([0.01, 0.3, 3.0, 30.0]))
+    C_ = float(rng.choice([0.01, 1.0, 100.0, 1e4, 1e6]))
+    B = int(rng.integers(2, min(n, 30)))
+    return X, y, kern, g, C_, B
+
+
+@pytest.mark.parametrize("seed,decays", [(337, 3), (677, 1)])
+def test_irwlspar_m_decay_path(seed, decays):
+    """Runs in which M /= 10 fires after iteration 10
+    (src/budgeted-train.c:479): seed 337 (n=52, RBF gamma 30, C 1e6,
+    budget 26) decays three times in 21 iterations, seed 677 (n=50,
+    gamma 30, C 1, budget 5) once in 15.  Their iteration and decay counts
+    do not move under 1e-15 input perturbations of the oracle (weights move
+    by at most 4e-8), so the GPU run must match both counts and the weights
+    to 1e-6."""
+    X, y, kern, g, C_, B = _noisy_case(seed)
+    props = abi.make_properties(gamma=g, C_=C_, size=B, kernel=kern)
+    ods, gds = po.Dataset.dense(X, y), lib.Dataset.dense(X, y)
+    cent = po.random_centroids(ods, props, seed=seed)
+    want = po.irwls_par(ods, cent, props)
+    ost = po.stats()
+    assert ost["m_decays"] == decays
+    got = lib.irwls_par(gds, cent, props)
+    gst = lib.last_stats()
+    print("iterations oracle %d gpu %d; M decays oracle %d gpu %d"
+          % (ost["irwlspar_iters"], gst["irwlspar_iters"], ost["m_decays"], gst["par_m_decays"]))
+    assert (gst["irwlspar_iters"], gst["par_m_decays"]) == (ost["irwlspar_iters"], ost["m_decays"])
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("relative weight error %.3e" % rel)
+    assert rel < 1e-6

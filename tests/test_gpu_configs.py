@@ -1,0 +1,263 @@
+"""GPU tests on the benchmark workloads at their full sizes.
+
+- Four Gaussians n=50000, d=128, W=2000 (plain FP64): trainFULL against the
+  oracle on the same data, iteration counts, support set, weights (1e-6
+  relative) and predicted labels.
+- Four Gaussians n=200000, d=256, W=4000 (mixed precision): mixed trainFULL
+  against plain FP64 trainFULL (test accuracy within 0.1%, weight and
+  decision differences reported), and the refinement solver's true FP64
+  residual on a KKT block of this workload.
+- Sparse RBF, d=2000, 1% density (budgeted): SGMA index sequence with the
+  reference's 64 candidates and IRWLSpar against the oracle on the first
+  10^4 rows; the learned proposal at n=10^6 (network fitted, distinct
+  centroids, Nystrom residual below that of 8 random exactly-scored
+  candidates, accuracy and primal objective against plain SGMA).
+- 96 one-vs-rest x (C, gamma) problems: trainFULLBatched in the benchmark's
+  precision mode (BF16 hi/lo kernel products) equal bit for bit to
+  standalone runs in that mode, and within 0.1% accuracy of plain FP64.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import Dev, rbf
+from libirwls_amd import abi, lib, objective, synth
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _acc(f, y):
+    return float(np.mean(np.where(f >= 0, 1.0, -1.0) == y))
+
+
+def test_four_gaussians_n50000_d128_against_oracle():
+    X, y = synth.four_gaussians(50000, 128, seed=1, means_seed=1)
+    Xt, yt = synth.four_gaussians(10000, 128, seed=101, means_seed=1)
+    props = abi.make_properties(gamma=1 / 128, C_=1.0, W=2000, eta=0.001)
+    ods = po.Dataset.dense(X, y)
+    want = po.train_full(ods, props, seed=0)
+    ost = po.stats()
+    gds = lib.Dataset.dense(X, y)
+    lib.set_options(precision=0)
+    got = lib.train_full(gds, props, seed=0)
+    gst = lib.last_stats()
+    print("oracle", ost, "gpu", gst)
+    assert (gst["outer_iters"], gst["inner_iters"]) == (ost["outer_iters"], ost["inner_iters"])
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("relative weight error %.3e, max abs %.3e" % (rel, np.abs(got - want).max()))
+    assert rel < 1e-6
+    assert np.array_equal(got[:-1] != 0, want[:-1] != 0)
+    om, gm = po.full_model(props, ods, want), lib.full_model(props, gds, got)
+    pp = abi.make_predict_properties(labels=0)
+    osoft = po.soft_test(po.Dataset.dense(Xt, yt), om)
+    gsoft = lib.soft_test(lib.Dataset.dense(Xt, yt), gm, pp)
+    np.testing.assert_allclose(gsoft, osoft, rtol=1e-6, atol=1e-8)
+    assert np.array_equal(np.where(gsoft >= 0, 1, -1), np.where(osoft >= 0, 1, -1))
+    print("test accuracy %.4f" % _acc(gsoft, yt))
+
+
+def test_four_gaussians_n200000_d256_mixed_against_plain():
+    X, y = synth.four_gaussians(200000, 256, seed=1, means_seed=1)
+    Xt, yt = synth.four_gaussians(20000, 256, seed=101, means_seed=1)
+    props = abi.make_properties(gamma=1 / 256, C_=1.0, W=4000, eta=0.001)
+    ds, td = lib.Dataset.dense(X, y), lib.Dataset.dense(Xt, yt)
+    pp = abi.make_predict_properties(labels=0)
+    res = {}
+    try:
+        for name, prec in [("plain", 0), ("mixed", 1)]:
+            lib.set_options(precision=prec)
+            b = lib.train_full(ds, props, seed=0)
+            st = lib.last_stats()
+            lib.set_options(precision=0)
+            f = lib.soft_test(td, lib.full_model(props, ds, b), pp)
+            res[name] = (b, f, st)
+            print(name, "accuracy %.5f" % _acc(f, yt), st)
+    finally:
+        lib.set_options(precision=0)
+    (bp, fp, _), (bm, fm, sm) = res["plain"], res["mixed"]
+    print("mixed vs plain: weights rel %.3e, decision max %.3e rms %.3e"
+          % (np.linalg.norm(bm - bp) / np.linalg.norm(bp), np.abs(fm - fp).max(),
+             np.sqrt(np.mean((fm - fp) ** 2))))
+    assert abs(_acc(fm, yt) - _acc(fp, yt)) <= 0.001
+    assert sm["fp64_fallbacks"] == 0 and sm["pcg_iters"] > 0
+
+
+def test_refinement_residual_on_a_n200000_d256_kkt_block():
+    """The SPD block A = K_S o yy^T + diag(1/a) of a 4000-row working set of
+    the n=200000, d=256 workload (RBF gamma=1/256, C=1, weights a from
+    errors e with e*y in [1e-4, 1]): FP32 Cholesky with FP64 diagonal tiles
+    as the preconditioner, FP64 CG on two right-hand sides to a true FP64
+    relative residual of 1e-10 (measured here with numpy)."""
+    X, y = synth.four_gaussians(200000, 256, seed=1, means_seed=1)
+    rng = np.random.default_rng(5)
+    S = np.sort(rng.choice(200000, 4000, replace=False))
+    Xs, ys = X[S], y[S]
+    ey = np.exp(rng.uniform(np.log(1e-4), 0.0, len(S)))
+    a = np.minimum(1.0 / ey, 1e4)            # a = C y / e, clipped at C 1e4 (C = 1)
+    A = rbf(Xs, Xs, 1 / 256) * np.outer(ys, ys) + np.diag(1.0 / a)
+    n = len(S)
+    L = lib.load()
+    L.irwls_dev_potrf_inv_f32.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.irwls_dev_pcg_spd2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
+    Ad = Dev(np.asfortranarray(A).ravel(order="F"))
+    Fd, Zd = Dev(np.zeros(2 * n * n, np.float32)), Dev(np.zeros(n * n, np.float32))
+    assert L.irwls_dev_potrf_inv_f32(Ad.p, n, n, Fd.p, 2 * n, Zd.p, n) == 0
+    B = np.vstack([1.0 - rng.standard_normal(n) * 0.1, ys])   # a right-hand side and the border column
+    Bd, Xd = Dev(B), Dev(np.zeros((2, n)))
+    rel = (C.c_double * 2)()
+    it = L.irwls_dev_pcg_spd2(Ad.p, n, n, Bd.p, Fd.p, 2 * n, Zd.p, n, 1e-10, 200, Xd.p, rel)
+    Xs_ = Xd.get()
+    print("CG iterations", it, "reported", list(rel))
+    assert it > 0
+    for c in range(2):
+        true_rel = np.linalg.norm(B[c] - A @ Xs_[c]) / np.linalg.norm(B[c])
+        print("column %d true relative residual %.3e" % (c, true_rel))
+        assert true_rel <= 2e-10
+
+
+def _sparse_slice(n, seed=3):
+    rp, col, val, y = synth.sparse_rbf_labels(n, 2000, 0.01, gamma=0.5, seed=seed)
+    return rp, col, val, y
+
+
+def test_sparse_rbf_slice_sgma_and_irwlspar_against_oracle():
+    """The first 10^4 rows of the d=2000, 1% density generator; budget 200,
+    the reference's 64 candidates: identical SGMA index sequence; IRWLSpar's
+    first six iterates within 1e-6 relative of the oracle's (both capped at
+    six iterations); and the full IRWLSpar run inside the oracle's own
+    rounding envelope.
+
+    On this workload the reference's IRWLSpar amplifies rounding about ten
+    times per iteration after the fourth (its first pass scales the
+    right-hand side by M = 1e4 but not the Gram matrix,
+    src/budgeted-train.c:348-358, and the hinge reweighting then switches
+    rows in and out): relative input perturbations of 1e-15 change the
+    oracle's own result by 4e-13 after one iteration, 1e-9 after six, 4e-6
+    after ten and 4e-3 at the end, and its iteration count between 24 and
+    29, while its primal objective moves by 6e-5 relative.  So the final
+    iterate is compared by what that envelope keeps: the primal objective,
+    the weights to 2e-2, the iteration count to within 6."""
+    rp, col, val, y = _sparse_slice(10000)
+    ods, gds = po.Dataset.csr(rp, col, val, y), lib.Dataset.csr(rp, col, val, y)
+    props = abi.make_properties(gamma=0.5, C_=10.0, size=200)
+    lib.set_options(sgma_candidates=64, proposal=0)
+    want = po.sgma(ods, props, ncand=64, seed=1)
+    got = lib.sgma(gds, props, seed=1)
+    assert list(got) == list(want)
+    try:
+        po.set_limits(0, 0, 6)
+        lib.set_options(par_max_iters=6)
+        b6w, b6g = po.irwls_par(ods, want, props), lib.irwls_par(gds, got, props)
+    finally:
+        po.set_limits(0, 0, 0)
+        lib.set_options(par_max_iters=0)
+    rel6 = np.linalg.norm(b6g - b6w) / np.linalg.norm(b6w)
+    print("six iterations: relative weight error %.3e" % rel6)
+    assert rel6 < 1e-6
+    bw = po.irwls_par(ods, want, props)
+    ost = po.stats()
+    bg = lib.irwls_par(gds, got, props)
+    gst = lib.last_stats()
+    rel = np.linalg.norm(bg - bw) / np.linalg.norm(bw)
+    pp = abi.make_predict_properties(labels=0)
+    fg = lib.soft_test(gds, lib.budgeted_model(props, gds, got, bg), pp)
+    fw = po.soft_test(ods, po.budgeted_model(props, ods, want, bw))
+    og = objective.primal_objective(gds, got, bg, fg, 10.0, 0.5)
+    ow = objective.primal_objective(gds, want, bw, fw, 10.0, 0.5)
+    print("IRWLSpar iterations oracle %d gpu %d; relative weight difference %.3e; primal objective oracle %.10g "
+          "gpu %.10g" % (ost["irwlspar_iters"], gst["irwlspar_iters"], rel, ow, og))
+    assert abs(gst["irwlspar_iters"] - ost["irwlspar_iters"]) <= 6
+    assert rel < 2e-2
+    assert abs(og - ow) <= 5e-4 * abs(ow)
+
+
+def test_sparse_rbf_n1000000_learned_proposal():
+    """Budget 1000, 59 candidates, gamma 0.5, C 10 on 10^6 training rows.
+    The learned proposal (pool 236, 8 kept, gains on 65536 rows) against
+    8 random exactly-scored candidates per step (the same scoring budget
+    without the network) and against plain SGMA with 59: the network is
+    fitted (loss drops) and takes every step after its warm-up, the
+    centroids are distinct, its held-out Nystrom residual is below the
+    random shortlist's, and its test accuracy, primal objective and
+    residual are reported beside plain SGMA's."""
+    n, nt, B = 1000000, 20000, 1000
+    rp, col, val, y = _sparse_slice(n + nt)
+    tr, te = rp[:n + 1], rp[n:] - rp[n]
+    ds = lib.Dataset.csr(tr, col[:rp[n]], val[:rp[n]], y[:n])
+    td = lib.Dataset.csr(te, col[rp[n]:], val[rp[n]:], y[n:])
+    yt = y[n:]
+    ds.pin(with_means=True)
+    props = abi.make_properties(gamma=0.5, C_=10.0, size=B)
+    pp = abi.make_predict_properties(labels=0)
+    hold = lib.Dataset.csr(te[:2001], col[rp[n]:rp[n] + te[2000]], val[rp[n]:rp[n] + te[2000]], yt[:2000])
+    modes = {"plain59": dict(sgma_candidates=59, proposal=0),
+             "learned": dict(sgma_candidates=59, proposal=1, proposal_keep=8, proposal_pool=236,
+                             proposal_rows=65536),
+             "random8": dict(sgma_candidates=8, proposal=0)}
+    res = {}
+    try:
+        for name, opts in modes.items():
+            lib.set_options(**opts)
+            cent = lib.sgma(ds, props, seed=1)
+            st = lib.last_stats()
+            lib.set_options(sgma_candidates=64, proposal=0, proposal_rows=0)
+            beta = lib.irwls_par(ds, cent, props)
+            m = lib.budgeted_model(props, ds, cent, beta)
+            acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
+            ftr = lib.soft_test(ds, m, pp)
+            res[name] = dict(cent=cent, st=st, acc=acc,
+                             resid=objective.nystrom_residual(ds, cent, hold, 0.5),
+                             primal=objective.primal_objective(ds, cent, beta, ftr, 10.0, 0.5))
+            print(name, "accuracy %.5f held-out Nystrom residual %.6f primal %.6g" %
+                  (acc, res[name]["resid"], res[name]["primal"]),
+                  {k: st[k] for k in ("sgma_steps", "learned_steps", "mlp_loss0", "mlp_loss")})
+    finally:
+        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
+        ds.unpin()
+    le = res["learned"]
+    assert len(set(le["cent"])) == B
+    assert le["st"]["mlp_loss"] < le["st"]["mlp_loss0"]
+    assert le["st"]["learned_steps"] == B - 2 - 100
+    assert le["resid"] < res["random8"]["resid"]
+
+
+def test_ovr12_grid8_batched_equals_standalone_mixed_kernels():
+    """The 96 problems (12 one-vs-rest tasks of the 12-class mixture x 8
+    (C, gamma) points, W = min(n/4, 1500)) with BF16 hi/lo kernel products
+    and FP64 solves (precision 3): trainFULLBatched equals the standalone
+    trainFULL runs in that mode bit for bit, and every problem's test
+    accuracy is within 0.1% of the plain FP64 run's."""
+    tasks = synth.twelve_class_tasks(seed=4, n_test=2000)
+    probs = []
+    for X, y, d, n, Xt, yt in tasks:
+        ds, td = lib.Dataset.dense(X, y), lib.Dataset.dense(Xt, yt)
+        for C_ in (0.1, 1.0, 10.0, 100.0):
+            for gm in (0.5, 2.0):
+                probs.append((ds, td, yt, abi.make_properties(gamma=gm / d, C_=C_, W=min(n // 4, 1500), eta=0.001)))
+    pp = abi.make_predict_properties(labels=0)
+    for ds in {id(q[0]): q[0] for q in probs}.values():
+        ds.pin(with_means=False)        # as the benchmark: one device copy per task, shared by its 8 problems
+    try:
+        lib.set_options(precision=0)
+        plain = [lib.train_full(ds, p, seed=0) for ds, _, _, p in probs]
+        lib.set_options(precision=3)
+        solo = [lib.train_full(ds, p, seed=0) for ds, _, _, p in probs]
+        got = lib.train_full_batched([q[0] for q in probs], [q[3] for q in probs])
+        st = lib.last_stats()
+    finally:
+        lib.set_options(precision=0)
+        for ds in {id(q[0]): q[0] for q in probs}.values():
+            ds.unpin()
+    print("batched", st)
+    assert st["batch_rounds"] > 0
+    worst = 0.0
+    for i, ((ds, td, yt, p), b0, b1, b2) in enumerate(zip(probs, plain, solo, got)):
+        assert np.array_equal(b2, b1), (i, np.abs(b2 - b1).max())
+        a0 = float(np.mean(lib.hard_test(td, lib.full_model(p, ds, b0), pp) == yt))
+        a2 = float(np.mean(lib.hard_test(td, lib.full_model(p, ds, b2), pp) == yt))
+        worst = max(worst, abs(a2 - a0))
+    print("largest |accuracy difference| vs plain over 96 problems: %.5f" % worst)
+    assert worst <= 0.001
