@@ -1,24 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the IRWLS training hot path on one MI355X.
 
-A step is one full-training run (trainFULL through the C entry point) over
-the four-Gaussian workload n=50000, d=128 (RBF gamma=1/128, C=1, working
-set 2000, eta=0.001), with the dataset resident in HBM before the timed
-region.  `value` is seconds per training run (lower is better) of the
-mixed-precision trainer (BF16 hi/lo x3 kernel products, FP32 Cholesky with
-FP64 diagonal tiles and FP64 refinement); the plain FP64 trainer runs on the
-same data in the same process and its time, accuracy and the differences
-are reported beside it (--mode plain|mixed runs one of them only).
+The metric is training seconds of the mixed-precision, learned-proposal
+and batched trainers against the plain FP64 GPU trainer, on the
+benchmark's three GPU workloads, in one run:
 
-The roofline object covers the kernel with the largest share of device time
-(found in the last warmup step with every timer on); during the timed steps
-HIP events are recorded on the library's stream around that kernel's
-launches only.  The CPU baseline times the oracle (the FP64 C restatement of
-the reference, OpenMP over the host cores) on the same workload: the whole
-run by default, or its first --cpu-outer outer iterations extrapolated to the
-GPU run's outer-iteration count.
+- four_gaussians_n200000_d256 (the headline, `value`): trainFULL through
+  the C entry point, n=200000, d=256, RBF gamma=1/256, C=1, working set
+  4000, eta=0.001; mixed precision (BF16 hi/lo x3 kernel products, FP32
+  Cholesky with FP64 diagonal tiles, FP64 refinement to 1e-10) against
+  plain FP64 on the same data resident in HBM.  A step is one training run.
+- sparse_rbf_n1000000_d2000: budgeted training (SGMA + IRWLSpar, the
+  reference budgeted-train's timed window), 10^6 CSR rows, d=2000, 1%
+  density, budget 1000, 59 candidates, gamma 0.5, C 10: the learned
+  proposal (gains on 65536 rows, and with exact gains) against plain SGMA.
+- ovr12_grid8_batched: 96 problems (12 one-vs-rest tasks x 8 (C, gamma)
+  points): trainFULLBatched with BF16 hi/lo kernel products against 96
+  sequential plain FP64 trainFULL runs.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Each workload block reports both modes' seconds, the speed-up, the
+accuracy difference, a roofline object for its dominant kernel (found in
+the last warmup step with every timer on; the timed steps record HIP
+events on the library stream around that kernel's launches only) and a
+CPU baseline: the oracle (the FP64 C restatement of the reference, OpenMP
+over the host cores) on a bounded sample of the same workload,
+extrapolated and labelled as such.  The headline uses --steps/--warmup;
+the other two blocks use --side-steps/--side-warmup.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
 """
 import argparse
 import ctypes as C
@@ -32,6 +41,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 # in the mixed mode the factorization's tiles (trsm, syrk) are FP32 MFMA
@@ -54,17 +64,35 @@ WORKLOADS = {
     # expansion over 200 anchors; budget 1000, 59 candidates, gamma 0.5, C 10
     # learned proposal: gains on a fixed subset of 65536 of the 10^6 rows (proposal_rows)
     "sparse_rbf_n1000000_d2000": dict(kind="budgeted", n=1000000, n_test=100000, d=2000, density=0.01, gamma=0.5,
-                                      C=10.0, B=1000, cand=59, seed=3, cpu_sgma=3, proposal_rows=65536),
+                                      C=10.0, B=1000, cand=59, seed=3, cpu_sgma=6, cpu_par_rows=50000,
+                                      proposal_rows=65536),
     "sparse_rbf_n100000_d2000": dict(kind="budgeted", n=100000, n_test=20000, d=2000, density=0.01, gamma=0.5,
-                                     C=10.0, B=500, cand=59, seed=3, cpu_sgma=3),
+                                     C=10.0, B=500, cand=59, seed=3, cpu_sgma=6, cpu_par_rows=50000),
     # batched workload: 12 one-vs-rest tasks of a 12-class mixture (n log-uniform in [1000, 30000], d in
     # {16, 32, 64, 128}) x 8 points C in {0.1, 1, 10, 100}, gamma in {0.5/d, 2/d}; W = min(n/4, 1500)
     "ovr12_grid8_batched": dict(kind="batched", n_test=2000, seed=4, n=None, d=None),
 }
+HEADLINE = "four_gaussians_n200000_d256"
+SIDE = ["sparse_rbf_n1000000_d2000", "ovr12_grid8_batched"]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def make_data(wl):
@@ -78,72 +106,43 @@ def make_data(wl):
     return X, y, Xt, yt
 
 
-def cpu_baseline(wl, X, y, gpu_outer, max_outer):
-    """Oracle timed on the host cores for `max_outer` outer iterations."""
-    from libirwls_amd import abi
-    from oracle import pyoracle as po
-    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
-    ds = po.Dataset.dense(X, y)
-    po.set_limits(max(max_outer, 0), 0, 0)
-    try:
-        t0 = time.time()
-        po.train_full(ds, props, seed=0)
-        dt = time.time() - t0
-        st = po.stats()
-    finally:
-        po.set_limits(0, 0, 0)
-    per_outer = dt / max(st["outer_iters"], 1)
-    if max_outer < 0:
-        per_outer = dt / max(gpu_outer, 1)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
-        "value": per_outer * gpu_outer,
-        "unit": "s",
-        "cores": cores,
-        "kind": "port",
-        "sample": ("oracle trainFULL, whole run: %d outer / %d inner iterations in %.1f s"
-                   % (st["outer_iters"], st["inner_iters"], dt)) if max_outer < 0 else
-                  ("oracle trainFULL, first %d outer iterations (%d inner) in %.1f s; %.2f s/outer x %d GPU "
-                   "outer iterations (extrapolated)" % (st["outer_iters"], st["inner_iters"], dt, per_outer,
-                                                        gpu_outer)),
-    }
-
-
-def run_mode(lib, L, ds, props, precision, warmup, steps):
-    """Warmup and timed trainFULL runs in one precision mode.  The last
-    warmup step runs with every timer on and names the kernel with the
-    largest share of device time; the timed steps record HIP events around
-    that kernel's launches only."""
-    lib.set_options(precision=precision)
+def timed_steps(lib, L, fn, warmup, steps, label):
+    """W warmup calls (the last with every timer on, to name the dominant
+    kernel kind), then K timed calls with events around that kind only.
+    Returns (times, last result, timers of the timed calls, warmup timers,
+    dominant kind)."""
     dom, tw = None, None
     for i in range(warmup):
-        prof = (i == warmup - 1)
-        if prof:
+        last = (i == warmup - 1)
+        if last:
             lib.timing(True)
         t0 = time.time()
-        lib.train_full(ds, props, seed=0)
-        log("bench: precision %d warmup %d done in %.2f s" % (precision, i, time.time() - t0))
-        if prof:
+        fn()
+        log("bench: %s warmup %d done in %.2f s" % (label, i, time.time() - t0))
+        if last:
             tw = lib.timers()
             dom = dominant(tw)
             lib.timing(False)
     lib.timing(True, kinds=None if dom is None else [dom])
-    times = []
-    beta = None
+    times, res = [], None
     for i in range(steps):
         L.irwls_dev_sync()
         t0 = time.perf_counter()
-        beta = lib.train_full(ds, props, seed=0)
+        res = fn()
         L.irwls_dev_sync()
         times.append(time.perf_counter() - t0)
-        log("bench: precision %d step %d: %.3f s %s" % (precision, i, times[-1], lib.last_stats()))
-    stats = lib.last_stats()
+        log("bench: %s step %d: %.3f s" % (label, i, times[-1]))
     tm = lib.timers()
     lib.timing(False)
-    lib.set_options(precision=0)
     if dom is None:
         dom = dominant(tm)
-    return dict(times=times, value=float(np.mean(times)), beta=beta, stats=stats, tm=tm, tw=tw, dom=dom)
+    return times, res, tm, tw, dom
+
+
+def kernel_tables(r, steps):
+    return {"kernel_ms_per_step": {k: v["ms"] / steps for k, v in r["tm"].items() if v["launches"]},
+            "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items()
+                                                              if v["launches"]}}
 
 
 # timer kinds that bracket a sequence of launches (a whole CG solve, a
@@ -275,11 +274,156 @@ def roofline(r, steps, workload=None, mode=None):
     return roof
 
 
-def run_budgeted(args, wl, lib, abi):
-    """SGMA + IRWLSpar (the reference budgeted-train's timed window) with the
-    plain and the learned proposal on the same CSR data."""
-    from libirwls_amd import synth
-    log("bench: generating %s" % args.workload)
+def cpu_baseline_full(wl, X, y, gpu_outer, max_outer):
+    """Oracle trainFULL on the host cores: the whole run (max_outer < 0) or
+    its first max_outer outer iterations, extrapolated to the GPU plain
+    run's outer-iteration count."""
+    from libirwls_amd import abi
+    from oracle import pyoracle as po
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
+    ds = po.Dataset.dense(X, y)
+    po.set_limits(max(max_outer, 0), 0, 0)
+    try:
+        t0 = time.time()
+        po.train_full(ds, props, seed=0)
+        dt = time.time() - t0
+        st = po.stats()
+    finally:
+        po.set_limits(0, 0, 0)
+    per_outer = dt / max(st["outer_iters"], 1)
+    whole = max_outer < 0
+    return {
+        "value": dt if whole else per_outer * gpu_outer, "unit": "s", "cores": cores(), "kind": "port",
+        "cpu": cpu_model(),
+        "sample": ("oracle trainFULL, whole run: %d outer / %d inner iterations in %.1f s"
+                   % (st["outer_iters"], st["inner_iters"], dt)) if whole else
+                  ("oracle trainFULL, first %d outer iteration(s) (%d inner) in %.1f s; %.2f s/outer x %d GPU "
+                   "outer iterations (extrapolated)" % (st["outer_iters"], st["inner_iters"], dt, per_outer,
+                                                        gpu_outer)),
+    }
+
+
+def bench_full(args, name, steps, warmup):
+    """Plain FP64 and mixed-precision trainFULL on one dense workload, the
+    dataset pinned in HBM; then the drop-in call (host dataset converted
+    and uploaded inside the call, as a caller of the reference API makes
+    it) of the mixed trainer."""
+    from libirwls_amd import abi, lib
+    wl = WORKLOADS[name]
+    log("bench: generating %s" % name)
+    X, y, Xt, yt = make_data(wl)
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
+    L = lib.load()
+    L.irwls_dev_init()
+    ds = lib.Dataset.dense(X, y)
+    ds.pin(with_means=False)            # inputs resident in HBM before timing
+    td = lib.Dataset.dense(Xt, yt)
+    pp = abi.make_predict_properties(labels=0)
+    modes = ["plain", "mixed"] if args.mode == "both" else [args.mode]
+    runs = {}
+    for mode in modes:
+        prec = 1 if mode == "mixed" else 0
+        lib.set_options(precision=prec)
+        times, beta, tm, tw, dom = timed_steps(lib, L, lambda: lib.train_full(ds, props, seed=0), warmup, steps,
+                                               "%s %s" % (name, mode))
+        st = lib.last_stats()
+        lib.set_options(precision=0)
+        m = lib.full_model(props, ds, beta)
+        f = lib.soft_test(td, m, pp)
+        runs[mode] = dict(times=times, value=float(np.mean(times)), beta=beta, stats=st, tm=tm, tw=tw, dom=dom,
+                          decision=f, acc=float(np.mean(np.where(f >= 0, 1.0, -1.0) == yt)),
+                          nsv=int(np.count_nonzero(beta[:-1])))
+    main_mode = "mixed" if "mixed" in runs else "plain"
+    r = runs[main_mode]
+    # drop-in: the same call on an unpinned dataset (AoS conversion and upload inside the timed call)
+    ds.unpin()
+    lib.set_options(precision=1 if main_mode == "mixed" else 0)
+    drop = []
+    for i in range(2):
+        L.irwls_dev_sync()
+        t0 = time.perf_counter()
+        lib.train_full(ds, props, seed=0)
+        L.irwls_dev_sync()
+        drop.append(time.perf_counter() - t0)
+    lib.set_options(precision=0)
+    log("bench: %s drop-in (unpinned) %s: %s" % (name, main_mode, ["%.3f" % t for t in drop]))
+    cpu_outer = wl.get("cpu_outer", -1) if args.cpu_outer is None else args.cpu_outer
+    cpu = None
+    if cpu_outer != 0:
+        log("bench: cpu baseline (oracle, %s outer iterations)" % ("all" if cpu_outer < 0 else cpu_outer))
+        cpu = cpu_baseline_full(wl, X, y, runs.get("plain", r)["stats"]["outer_iters"], cpu_outer)
+
+    def summary(x):
+        return {"s": x["value"], "test_accuracy": x["acc"], "n_sv": x["nsv"], "outer_iters": x["stats"]["outer_iters"],
+                "inner_iters": x["stats"]["inner_iters"], "solve_s": x["stats"]["t_solve"],
+                "factorizations": x["stats"]["factorizations"], "pcg_iters": x["stats"]["pcg_iters"],
+                "fp64_fallbacks": x["stats"]["fp64_fallbacks"]}
+    block = {
+        "workload": name, "mode": main_mode, "s": r["value"], "steps": steps, "warmup": warmup,
+        "dtype": "bf16x3+f32+f64" if main_mode == "mixed" else "f64",
+        "config": {"n": wl["n"], "d": wl["d"], "W": wl["W"], "gamma": wl["gamma"], "C": wl["C"], "eta": wl["eta"],
+                   "n_test": wl["n_test"], "seed": wl["seed"]},
+        "test_accuracy": r["acc"],
+        "runs": {k: summary(v) for k, v in runs.items()},
+        "drop_in_s": float(np.mean(drop)),
+        "roofline": roofline(r, steps, name, main_mode),
+        "cpu_baseline": cpu,
+    }
+    block.update(kernel_tables(r, steps))
+    if "plain" in runs and "mixed" in runs:
+        p, m = runs["plain"], runs["mixed"]
+        block["plain_s"] = p["value"]
+        block["speedup_vs_plain"] = p["value"] / m["value"]
+        block["accuracy_delta"] = m["acc"] - p["acc"]
+        block["mixed_vs_plain"] = {
+            "accuracy_delta": m["acc"] - p["acc"],
+            "weight_rel_err": float(np.linalg.norm(m["beta"] - p["beta"]) / np.linalg.norm(p["beta"])),
+            "decision_max_err": float(np.abs(m["decision"] - p["decision"]).max()),
+            "decision_rms_err": float(np.sqrt(np.mean((m["decision"] - p["decision"]) ** 2))),
+        }
+    return block
+
+
+def cpu_baseline_budgeted(wl, rp, col, val, y, props, gpu_steps, gpu_iters):
+    """Oracle SGMA (the reference's 64-candidate loop at `cand` candidates)
+    up to budget size cpu_sgma, and one IRWLSpar iteration on the first
+    cpu_par_rows rows, on the host cores; extrapolated linearly to the GPU
+    run's SGMA steps and IRWLSpar iterations over all rows.  A lower bound:
+    an SGMA step's product with the basis grows with the basis size."""
+    from oracle import pyoracle as po
+    ods = po.Dataset.csr(rp, col, val, y)
+    po.set_limits(0, wl["cpu_sgma"], 1)
+    try:
+        t0 = time.time()
+        po.sgma(ods, props, ncand=wl["cand"], seed=1)
+        t1 = time.time()
+        ksteps = po.stats()["sgma_steps"]
+        m = min(wl.get("cpu_par_rows", len(y)), len(y))
+        sub = po.Dataset.csr(rp[:m + 1], col[:rp[m]], val[:rp[m]], y[:m])
+        cent = np.arange(props.size, dtype=np.int32) % m
+        t2 = time.time()
+        po.irwls_par(sub, cent, props)
+        t3 = time.time()
+    finally:
+        po.set_limits(0, 0, 0)
+    n = len(y)
+    per_step = (t1 - t0) / max(ksteps, 1)
+    per_iter = (t3 - t2) * n / m
+    return {"value": per_step * gpu_steps + per_iter * gpu_iters, "unit": "s", "cores": cores(), "kind": "port",
+            "cpu": cpu_model(),
+            "sample": "oracle SGMA %d steps in %.1f s (%.2f s/step x %d steps) and one IRWLSpar iteration on "
+                      "%d of %d rows in %.1f s (x %d rows/%d x %d iterations), extrapolated (lower bound)"
+                      % (ksteps, t1 - t0, per_step, gpu_steps, m, n, t3 - t2, n, m, gpu_iters)}
+
+
+def bench_budgeted(args, name, steps, warmup):
+    """SGMA + IRWLSpar with the plain and the learned proposal on the same
+    CSR data, and the budgeted objectives of each mode's model: the
+    held-out Nystrom residual of SGMA's basis and IRWLSpar's primal
+    objective on the training rows."""
+    from libirwls_amd import abi, lib, objective, synth
+    wl = WORKLOADS[name]
+    log("bench: generating %s" % name)
     rp, col, val, y = synth.sparse_rbf_labels(wl["n"] + wl["n_test"], wl["d"], wl["density"], gamma=wl["gamma"],
                                               seed=wl["seed"])
     n = wl["n"]
@@ -288,100 +432,122 @@ def run_budgeted(args, wl, lib, abi):
     ds = lib.Dataset.csr(tr, col[:rp[n]], val[:rp[n]], y[:n])
     td = lib.Dataset.csr(te, col[rp[n]:], val[rp[n]:], y[n:])
     yt = y[n:]
+    nh = min(2000, wl["n_test"])
+    hold = lib.Dataset.csr(te[:nh + 1], col[rp[n]:rp[n] + te[nh]], val[rp[n]:rp[n] + te[nh]], yt[:nh])
     L = lib.load()
     L.irwls_dev_init()
     ds.pin(with_means=True)
     props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], size=wl["B"])
     pp = abi.make_predict_properties(labels=0)
+    prow = args.proposal_rows if args.proposal_rows is not None else wl.get("proposal_rows", 0)
     modes = {"plain": dict(sgma_candidates=wl["cand"], proposal=0),
              "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"],
-                             proposal_rows=args.proposal_rows if args.proposal_rows is not None
-                             else wl.get("proposal_rows", 0))}
+                             proposal_rows=prow)}
+    if prow:
+        modes["learned_exact_gains"] = dict(modes["learned"], proposal_rows=0)
+    reset = dict(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
     runs = {}
-    sel = ["plain", "learned"] if args.mode == "both" else [{"plain": "plain"}.get(args.mode, "learned")]
-    for mode in sel:
-        lib.set_options(**modes[mode])
-        dom, tw = None, None
-        for i in range(args.warmup):
-            lib.timing(i == args.warmup - 1)
-            t0 = time.time()
-            c = lib.sgma(ds, props, seed=1)
-            lib.irwls_par(ds, c, props)
-            log("bench: %s warmup %d done in %.2f s" % (mode, i, time.time() - t0))
-            if i == args.warmup - 1:
-                tw = lib.timers()
-                dom = dominant(tw)
-                lib.timing(False)
-        lib.timing(True, kinds=None if dom is None else [dom])
-        times, t_sgma = [], []
-        for i in range(args.steps):
-            L.irwls_dev_sync()
+    for mode, opts in modes.items():
+        state = {}
+
+        def step():
+            lib.set_options(**opts)
             t0 = time.perf_counter()
             cent = lib.sgma(ds, props, seed=1)
-            sst = lib.last_stats()
-            t1 = time.perf_counter()
+            state["sgma_s"] = time.perf_counter() - t0
+            state["sst"] = lib.last_stats()
             beta = lib.irwls_par(ds, cent, props)
-            L.irwls_dev_sync()
-            times.append(time.perf_counter() - t0)
-            t_sgma.append(t1 - t0)
-            pst = lib.last_stats()
-            log("bench: %s step %d: %.3f s (SGMA %.3f s) %s %s" % (mode, i, times[-1], t_sgma[-1], sst, pst))
-        tm = lib.timers()
-        lib.timing(False)
-        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
+            state["pst"] = lib.last_stats()
+            return cent, beta
+        nst = steps if mode != "learned_exact_gains" else max(1, min(steps, 1))
+        times, (cent, beta), tm, tw, dom = timed_steps(lib, L, step, warmup if mode != "learned_exact_gains" else 0,
+                                                       nst, "%s %s" % (name, mode))
+        lib.set_options(**reset)
         m = lib.budgeted_model(props, ds, cent, beta)
         acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
-        if dom is None:
-            dom = dominant(tm)
-        runs[mode] = dict(value=float(np.mean(times)), sgma_s=float(np.mean(t_sgma)), acc=acc, tm=tm, tw=tw,
-                          dom=dom, sgma_stats=sst, par_stats=pst)
-    main_mode = "learned" if "learned" in runs else "plain"
+        ftr = lib.soft_test(ds, m, pp)
+        runs[mode] = dict(value=float(np.mean(times)), sgma_s=state["sgma_s"], acc=acc, tm=tm, tw=tw, dom=dom,
+                          sgma_stats=state["sst"], par_stats=state["pst"], steps=nst,
+                          resid=objective.nystrom_residual(ds, cent, hold, wl["gamma"]),
+                          primal=objective.primal_objective(ds, cent, beta, ftr, wl["C"], wl["gamma"]))
+        log("bench: %s %s: %.3f s, accuracy %.5f, held-out Nystrom residual %.6f, primal %.8g"
+            % (name, mode, runs[mode]["value"], acc, runs[mode]["resid"], runs[mode]["primal"]))
+    ds.unpin()
+    main_mode = "learned"
     r = runs[main_mode]
     cpu = None
-    if wl.get("cpu_sgma", 0) > 0 and args.cpu_outer != 0:
-        cpu = cpu_baseline_budgeted(wl, tr, col[:rp[n]], val[:rp[n]], y[:n], props, r)
-    out = {
-        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
-        "value": r["value"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": r["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64+bf16-mlp" if main_mode == "learned" else "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "mode": main_mode, "n": n, "d": wl["d"], "B": wl["B"],
-                   "proposal_rows": modes["learned"]["proposal_rows"] if main_mode == "learned" else None,
-                   "candidates": wl["cand"], "gamma": wl["gamma"], "C": wl["C"], "n_test": wl["n_test"],
-                   "seed": wl["seed"]},
+    if args.cpu_outer != 0:
+        log("bench: cpu baseline (oracle SGMA sample and IRWLSpar row sample)")
+        cpu = cpu_baseline_budgeted(wl, tr, col[:rp[n]], val[:rp[n]], y[:n], props,
+                                    runs["plain"]["sgma_stats"]["sgma_steps"],
+                                    runs["plain"]["par_stats"]["irwlspar_iters"])
+    block = {
+        "workload": name, "mode": main_mode, "s": r["value"], "steps": steps, "warmup": warmup,
+        "dtype": "f64+f32-mlp+bf16-mlp",
+        "config": {"n": n, "d": wl["d"], "B": wl["B"], "candidates": wl["cand"], "proposal_rows": prow,
+                   "gamma": wl["gamma"], "C": wl["C"], "n_test": wl["n_test"], "seed": wl["seed"],
+                   "heldout_rows_for_residual": nh},
         "test_accuracy": r["acc"],
-        "runs": {k: {"s": v["value"], "sgma_s": v["sgma_s"], "test_accuracy": v["acc"],
+        "runs": {k: {"s": v["value"], "steps": v["steps"], "sgma_s": v["sgma_s"], "test_accuracy": v["acc"],
+                     "heldout_nystrom_residual": v["resid"], "primal_objective": v["primal"],
                      "sgma_steps": v["sgma_stats"]["sgma_steps"], "learned_steps": v["sgma_stats"]["learned_steps"],
                      "mlp_loss0": v["sgma_stats"]["mlp_loss0"], "mlp_loss": v["sgma_stats"]["mlp_loss"],
                      "irwlspar_iters": v["par_stats"]["irwlspar_iters"]} for k, v in runs.items()},
-        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
-        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps, args.workload, main_mode),
+        "roofline": roofline(r, steps, name, main_mode),
         "cpu_baseline": cpu,
+        "plain_s": runs["plain"]["value"],
+        "speedup_vs_plain": runs["plain"]["value"] / r["value"],
+        "accuracy_delta": r["acc"] - runs["plain"]["acc"],
     }
-    if "plain" in runs and "learned" in runs:
-        out["speedup_vs_plain"] = runs["plain"]["value"] / runs["learned"]["value"]
-        out["learned_vs_plain"] = {"accuracy_delta": runs["learned"]["acc"] - runs["plain"]["acc"]}
-    print(json.dumps(out), flush=True)
+    block.update(kernel_tables(r, steps))
+    if "learned_exact_gains" in runs:
+        block["exact_gains_speedup_vs_plain"] = runs["plain"]["value"] / runs["learned_exact_gains"]["value"]
+        block["exact_gains_accuracy_delta"] = runs["learned_exact_gains"]["acc"] - runs["plain"]["acc"]
+    return block
 
 
-def run_batched(args, wl, lib, abi):
+def cpu_baseline_batched(probs):
+    """Oracle trainFULL on the host cores for one problem per task (the
+    (C, gamma) = (1, 0.5/d) grid point), whole runs, scaled by the 8 grid
+    points per task."""
+    from oracle import pyoracle as po
+    from libirwls_amd import abi
+    t0 = time.time()
+    k = 0
+    seen = set()
+    for pr in probs:
+        if pr["task"] in seen or pr["C"] != 1.0 or pr["gm"] != 0.5:
+            continue
+        seen.add(pr["task"])
+        po.train_full(po.Dataset.dense(pr["X"], pr["y"]), pr["props"], seed=0)
+        k += 1
+    dt = time.time() - t0
+    return {"value": dt * len(probs) / max(k, 1), "unit": "s", "cores": cores(), "kind": "port", "cpu": cpu_model(),
+            "sample": "oracle trainFULL whole runs of %d problems (one per task, C=1, gamma=0.5/d) in %.1f s, "
+                      "x %d/%d (extrapolated)" % (k, dt, len(probs), k)}
+
+
+def bench_batched(args, name, steps, warmup):
     """96 problems: sequential plain FP64 trainFULL runs against
-    trainFULLBatched with BF16 hi/lo kernel products and batched FP64 solves."""
-    from libirwls_amd import synth
-    log("bench: generating %s" % args.workload)
+    trainFULLBatched with BF16 hi/lo kernel products and batched FP64
+    solves; each task's dataset pinned once and shared by its 8 problems."""
+    from libirwls_amd import abi, lib, synth
+    wl = WORKLOADS[name]
+    log("bench: generating %s" % name)
     tasks = synth.twelve_class_tasks(seed=wl["seed"], n_test=wl["n_test"])
     L = lib.load()
     L.irwls_dev_init()
     probs = []
+    keep = []
     for t, (X, y, d, n, Xt, yt) in enumerate(tasks):
         ds = lib.Dataset.dense(X, y)
         ds.pin(with_means=False)
         td = lib.Dataset.dense(Xt, yt)
+        keep.append(ds)
         for C_ in (0.1, 1.0, 10.0, 100.0):
             for gm in (0.5, 2.0):
-                probs.append(dict(ds=ds, td=td, yt=yt, props=abi.make_properties(gamma=gm / d, C_=C_,
-                                  W=min(n // 4, 1500), eta=0.001), task=t, n=n, d=d))
+                probs.append(dict(ds=ds, td=td, yt=yt, X=X, y=y, C=C_, gm=gm, task=t, n=n, d=d,
+                                  props=abi.make_properties(gamma=gm / d, C_=C_, W=min(n // 4, 1500), eta=0.001)))
     pp = abi.make_predict_properties(labels=0)
 
     def accuracies(betas):
@@ -392,108 +558,77 @@ def run_batched(args, wl, lib, abi):
         return np.array(out)
 
     runs = {}
-    sel = ["plain", "batched"] if args.mode == "both" else (["plain"] if args.mode == "plain" else ["batched"])
-    for mode in sel:
+    for mode in ("plain", "batched"):
         lib.set_options(precision=0 if mode == "plain" else 3)
-        dom, tw = None, None
-        for i in range(args.warmup):
-            lib.timing(i == args.warmup - 1)
-            t0 = time.time()
-            if mode == "plain":
-                for pr in probs:
-                    lib.train_full(pr["ds"], pr["props"], seed=0)
-            else:
-                lib.train_full_batched([pr["ds"] for pr in probs], [pr["props"] for pr in probs])
-            log("bench: %s warmup %d done in %.2f s" % (mode, i, time.time() - t0))
-            if i == args.warmup - 1:
-                tw = lib.timers()
-                dom = dominant(tw)
-                lib.timing(False)
-        lib.timing(True, kinds=None if dom is None else [dom])
-        times = []
-        for i in range(args.steps):
-            L.irwls_dev_sync()
-            t0 = time.perf_counter()
-            if mode == "plain":
-                betas = [lib.train_full(pr["ds"], pr["props"], seed=0) for pr in probs]
-            else:
-                betas = lib.train_full_batched([pr["ds"] for pr in probs], [pr["props"] for pr in probs])
-            L.irwls_dev_sync()
-            times.append(time.perf_counter() - t0)
-            log("bench: %s step %d: %.3f s %s" % (mode, i, times[-1], lib.last_stats() if mode != "plain" else ""))
+        if mode == "plain":
+            fn = lambda: [lib.train_full(pr["ds"], pr["props"], seed=0) for pr in probs]
+        else:
+            fn = lambda: lib.train_full_batched([pr["ds"] for pr in probs], [pr["props"] for pr in probs])
+        times, betas, tm, tw, dom = timed_steps(lib, L, fn, warmup, steps, "%s %s" % (name, mode))
         st = lib.last_stats() if mode != "plain" else None
-        tm = lib.timers()
-        lib.timing(False)
         lib.set_options(precision=0)
-        if dom is None:
-            dom = dominant(tm)
         runs[mode] = dict(value=float(np.mean(times)), betas=betas, acc=accuracies(betas), tm=tm, tw=tw, dom=dom,
                           stats=st)
-    main_mode = "batched" if "batched" in runs else "plain"
-    r = runs[main_mode]
-    nmax = max(min(pr["n"] // 4, 1500) for pr in probs) + 1
-    ms = (C.c_double * 3)()
-    L.irwls_dev_batched_choice.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
-    out = {
-        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
-        "value": r["value"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": r["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3+f64" if main_mode == "batched" else "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "mode": main_mode, "problems": len(probs), "tasks": len(tasks),
-                   "n": [t[3] for t in tasks], "d": [t[2] for t in tasks], "seed": wl["seed"],
-                   "n_test_per_task": wl["n_test"]},
+    for ds in keep:
+        ds.unpin()
+    r = runs["batched"]
+    cpu = None
+    if args.cpu_outer != 0:
+        log("bench: cpu baseline (oracle, one problem per task)")
+        cpu = cpu_baseline_batched(probs)
+    dacc = r["acc"] - runs["plain"]["acc"]
+    block = {
+        "workload": name, "mode": "batched", "s": r["value"], "steps": steps, "warmup": warmup,
+        "dtype": "bf16x3+f64",
+        "config": {"problems": len(probs), "tasks": len(tasks), "n": [t[3] for t in tasks], "d": [t[2] for t in tasks],
+                   "seed": wl["seed"], "n_test_per_task": wl["n_test"]},
         "test_accuracy": float(r["acc"].mean()),
         "runs": {k: {"s": v["value"], "mean_test_accuracy": float(v["acc"].mean())} for k, v in runs.items()},
-        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
-        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps, args.workload, main_mode),
-        "cpu_baseline": None,
+        "roofline": roofline(r, steps, name, "batched"),
+        "cpu_baseline": cpu,
+        "plain_s": runs["plain"]["value"],
+        "speedup_vs_plain": runs["plain"]["value"] / r["value"],
+        "accuracy_delta": float(dacc.mean()),
+        "batched_vs_plain": {"accuracy_delta_mean": float(dacc.mean()),
+                             "accuracy_delta_max_abs": float(np.abs(dacc).max()),
+                             "within_0.1pct": int(np.sum(np.abs(dacc) <= 0.001))},
     }
     if r["stats"]:
-        out["runs"][main_mode].update({"batch_rounds": r["stats"]["batch_rounds"], "solves": r["stats"]["solves"]})
-    if "plain" in runs and "batched" in runs:
-        dacc = runs["batched"]["acc"] - runs["plain"]["acc"]
-        out["speedup_vs_plain"] = runs["plain"]["value"] / runs["batched"]["value"]
-        out["batched_vs_plain"] = {"accuracy_delta_mean": float(dacc.mean()), "accuracy_delta_max_abs":
-                                   float(np.abs(dacc).max()), "within_0.1pct": int(np.sum(np.abs(dacc) <= 0.001))}
-    print(json.dumps(out), flush=True)
+        block["runs"]["batched"].update({"batch_rounds": r["stats"]["batch_rounds"], "solves": r["stats"]["solves"]})
+    block.update(kernel_tables(r, steps))
+    return block
 
 
-def cpu_baseline_budgeted(wl, rp, col, val, y, props, r):
-    """Oracle SGMA for its first cpu_sgma steps and IRWLSpar for one
-    iteration on the host cores, extrapolated linearly to the GPU run's step
-    and iteration counts (a lower bound: later SGMA steps cost more)."""
-    from oracle import pyoracle as po
-    ods = po.Dataset.csr(rp, col, val, y)
-    k = wl["cpu_sgma"]
-    po.set_limits(0, k, 1)
-    try:
-        t0 = time.time()
-        cent = po.sgma(ods, props, ncand=wl["cand"], seed=1)
-        t1 = time.time()
-        po.irwls_par(ods, np.asarray(cent), props)
-        t2 = time.time()
-    finally:
-        po.set_limits(0, 0, 0)
-    steps = max(props.size - 2, 1)
-    per_step = (t1 - t0) / k
-    per_iter = t2 - t1
-    iters = max(r["par_stats"]["irwlspar_iters"], 1)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": per_step * steps + per_iter * iters, "unit": "s", "cores": cores, "kind": "port",
-            "sample": "oracle SGMA first %d steps in %.1f s and IRWLSpar first iteration in %.1f s, extrapolated to "
-                      "%d steps and %d iterations (lower bound)" % (k, t1 - t0, t2 - t1, steps, iters)}
+def contract_line(block, all_blocks, args):
+    """The benchmark line: `value` and `roofline` of the headline block,
+    every workload's block beside it."""
+    out = {
+        "metric": METRIC, "value": block["s"], "unit": "s", "n_gpus": 1, "steps": block["steps"],
+        "warmup": block["warmup"], "ms_per_step": block["s"] * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": block["dtype"], "data": "synthetic",
+        "config": dict({"workload": block["workload"], "mode": block["mode"]}, **block["config"]),
+        "test_accuracy": block["test_accuracy"],
+        "speedup_vs_plain": block.get("speedup_vs_plain"),
+        "accuracy_delta": block.get("accuracy_delta"),
+        "roofline": block["roofline"],
+        "cpu_baseline": block["cpu_baseline"],
+        "workloads": {b["workload"]: b for b in all_blocks},
+    }
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="four_gaussians_n50000_d128", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "learned", "batched", "both"],
-                    help="plain FP64; mixed precision (full training) or learned proposal (budgeted); "
-                         "both: value = the mixed / learned trainer, plain reported beside it")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side-steps", type=int, default=2, help="timed steps of the two other workloads")
+    ap.add_argument("--side-warmup", type=int, default=1)
+    ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS),
+                    help="all (default): the headline four_gaussians_n200000_d256 plus the budgeted and batched "
+                         "workloads; or one workload alone, as the headline")
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"],
+                    help="dense workloads: plain FP64, mixed precision, or both (value = the mixed trainer)")
     ap.add_argument("--proposal-rows", type=int, default=None,
                     help="budgeted, learned mode: rows on which the shortlist's gains are computed "
                          "(0 = all rows; default: the workload's own setting)")
@@ -502,8 +637,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the traffic-counter calibration copy (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-outer", type=int, default=None,
-                    help="outer iterations in the CPU sample (0 = skip, -1 = the whole run; "
-                         "default: the workload's own setting)")
+                    help="CPU baselines: 0 skips them; dense workloads: outer iterations in the sample "
+                         "(-1 = the whole run; default: the workload's own setting)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -516,83 +651,21 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from libirwls_amd import abi, lib
-    wl = WORKLOADS[args.workload]
-    if wl.get("kind") == "budgeted":
-        run_budgeted(args, wl, lib, abi)
-        return
-    if wl.get("kind") == "batched":
-        run_batched(args, wl, lib, abi)
-        return
-    cpu_outer = wl.get("cpu_outer", -1) if args.cpu_outer is None else args.cpu_outer
-    log("bench: generating %s" % args.workload)
-    X, y, Xt, yt = make_data(wl)
-    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], W=wl["W"], eta=wl["eta"])
-    L = lib.load()
-    L.irwls_dev_init()
-    ds = lib.Dataset.dense(X, y)
-    ds.pin(with_means=False)            # inputs resident in HBM before timing
-    td = lib.Dataset.dense(Xt, yt)
-    pp = abi.make_predict_properties(labels=0)
+    def run(name, steps, warmup):
+        kind = WORKLOADS[name].get("kind")
+        if kind == "budgeted":
+            return bench_budgeted(args, name, steps, warmup)
+        if kind == "batched":
+            return bench_batched(args, name, steps, warmup)
+        return bench_full(args, name, steps, warmup)
 
-    def evaluate(r):
-        m = lib.full_model(props, ds, r["beta"])
-        f = lib.soft_test(td, m, pp)
-        r["decision"] = f
-        r["acc"] = float(np.mean(np.where(f >= 0, 1.0, -1.0) == yt))
-        r["nsv"] = int(np.count_nonzero(r["beta"][:-1]))
-
-    runs = {}
-    for mode in (["plain", "mixed"] if args.mode == "both" else [args.mode]):
-        runs[mode] = run_mode(lib, L, ds, props, 1 if mode == "mixed" else 0, args.warmup, args.steps)
-        evaluate(runs[mode])
-    main_mode = "mixed" if "mixed" in runs else "plain"
-    r = runs[main_mode]
-
-    cpu = None
-    if cpu_outer != 0:
-        log("bench: cpu baseline (oracle, %s outer iterations)" % ("all" if cpu_outer < 0 else cpu_outer))
-        cpu = cpu_baseline(wl, X, y, runs.get("plain", r)["stats"]["outer_iters"], cpu_outer)
-
-    def summary(x):
-        return {"s": x["value"], "test_accuracy": x["acc"], "n_sv": x["nsv"], "outer_iters": x["stats"]["outer_iters"],
-                "inner_iters": x["stats"]["inner_iters"], "solve_s": x["stats"]["t_solve"],
-                "factorizations": x["stats"]["factorizations"], "pcg_iters": x["stats"]["pcg_iters"],
-                "fp64_fallbacks": x["stats"]["fp64_fallbacks"]}
-
-    out = {
-        "metric": "train s to +-0.1% acc of FP64 oracle: mixed/batched/learned vs plain GPU",
-        "value": r["value"],
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["value"] * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "bf16x3+f32+f64" if main_mode == "mixed" else "f64",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "mode": main_mode, "n": wl["n"], "d": wl["d"], "W": wl["W"],
-                   "gamma": wl["gamma"], "C": wl["C"], "eta": wl["eta"], "n_test": wl["n_test"],
-                   "seed": wl["seed"]},
-        "test_accuracy": r["acc"],
-        "runs": {k: summary(v) for k, v in runs.items()},
-        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in r["tm"].items() if v["launches"]},
-        "warmup_kernel_ms": None if r["tw"] is None else {k: v["ms"] for k, v in r["tw"].items() if v["launches"]},
-        "roofline": roofline(r, args.steps, args.workload, main_mode),
-        "cpu_baseline": cpu,
-    }
-    if "plain" in runs and "mixed" in runs:
-        p, m = runs["plain"], runs["mixed"]
-        out["speedup_vs_plain"] = p["value"] / m["value"]
-        out["mixed_vs_plain"] = {
-            "accuracy_delta": m["acc"] - p["acc"],
-            "weight_rel_err": float(np.linalg.norm(m["beta"] - p["beta"]) / np.linalg.norm(p["beta"])),
-            "decision_max_err": float(np.abs(m["decision"] - p["decision"]).max()),
-            "decision_rms_err": float(np.sqrt(np.mean((m["decision"] - p["decision"]) ** 2))),
-        }
-    print(json.dumps(out), flush=True)
+    if args.workload == "all":
+        head = run(HEADLINE, args.steps, args.warmup)
+        blocks = [head] + [run(s, args.side_steps, args.side_warmup) for s in SIDE]
+    else:
+        head = run(args.workload, args.steps, args.warmup)
+        blocks = [head]
+    print(json.dumps(contract_line(head, blocks, args)), flush=True)
 
 
 if __name__ == "__main__":

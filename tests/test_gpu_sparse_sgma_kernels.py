@@ -146,3 +146,45 @@ def test_wgram_against_numpy(B, nact, weighted):
     scale = np.abs(P).max() ** 2 * max(nact, 1) if nact else 1.0
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14 * scale)
     assert np.array_equal(got, got.T)
+
+
+@pytest.mark.parametrize("with_idx", [False, True])
+def test_csr_kernel_high_dimensional_column_chunks(with_idx):
+    """d = 3 x 10^6 features, 20 nonzeros per row: a transposed column block
+    of 150 x d doubles would take 3.6 GB, so the CSR row kernels take the
+    columns 64 at a time (1 GiB budget per block) and add the chunks' sums.
+    Block and product against scipy in FP64, with and without a column
+    selection."""
+    from scipy.sparse import csr_matrix
+    rng = np.random.default_rng(77)
+    n, d, g = 400, 3_000_000, 0.4
+    nnz_row = 20
+    col = np.concatenate([np.sort(rng.choice(d, nnz_row, replace=False)) for _ in range(n)]).astype(np.int32)
+    rp = np.arange(0, n * nnz_row + 1, nnz_row, dtype=np.int64)
+    # a few shared features so that dot products are not all zero
+    col[::7] = rng.integers(0, 50, size=len(col[::7]))
+    for i in range(n):
+        col[rp[i]:rp[i + 1]] = np.sort(np.unique(np.concatenate(
+            [col[rp[i]:rp[i + 1]], rng.choice(d, nnz_row, replace=False)]))[:nnz_row])
+    val = rng.random(len(col))
+    A = csr_matrix((val, col, rp), shape=(n, d))
+    xn = np.asarray(A.multiply(A).sum(1)).ravel()
+    cidx = rng.choice(n, 150, replace=False).astype(np.int32) if with_idx else np.arange(150, dtype=np.int32)
+    L = lib.load()
+    rows, k1 = _csr_rows(rp, col, val, xn)
+    if with_idx:
+        cols, k2 = _csr_rows(rp, col, val, xn, cidx)
+    else:
+        rp2 = rp[:151].copy()
+        cols, k2 = _csr_rows(rp2, col[:rp2[-1]], val[:rp2[-1]], xn[:150])
+    dots = (A @ A[cidx].T).toarray()
+    Kw = np.exp(-g * (xn[:, None] + xn[cidx][None, :] - 2.0 * dots))
+    Kd = Dev(np.zeros((n, 150)))
+    L.irwls_dev_kernel_matrix(C.byref(rows), C.byref(cols), d, 1, g, Kd.p, 150)
+    np.testing.assert_allclose(Kd.get(), Kw, rtol=1e-12, atol=1e-14)
+    w = rng.standard_normal(150)
+    out0 = rng.standard_normal(n)
+    rs = rng.choice([-1.0, 1.0], n)
+    wd, od, rsd = Dev(w), Dev(out0), Dev(rs)
+    L.irwls_dev_kernel_matvec(C.byref(rows), C.byref(cols), d, 1, g, wd.p, 0.5, -2.0, 0.125, rsd.p, od.p)
+    np.testing.assert_allclose(od.get(), 0.5 * out0 + rs * (-2.0 * (Kw @ w)) + 0.125, rtol=1e-11, atol=1e-12)
