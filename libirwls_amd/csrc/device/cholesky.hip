@@ -82,34 +82,90 @@ __device__ __forceinline__ double row_bcast(double v) {
     return __builtin_amdgcn_mov_dpp(v, 0x150 + c, 0xf, 0xf, true);
 }
 
-template <int j, int... C>
-__device__ __forceinline__ void blk_updates(double (&d)[16], double m, std::integer_sequence<int, C...>) {
-    ((C > j ? (void)(d[C] = fma(-m, row_bcast<C>(d[j]), d[C])) : (void)0), ...);
+// acc += bcast_c(src) * nm: one v_fmac_f64 whose first operand takes the DPP
+// row broadcast of lane c (the DP ALU's 64-bit DPP), where a separate
+// v_mov_b64_dpp would double the instructions of the chain.  The same FMA as
+// fma(nm, bcast_c(src), acc), so the same result bits.
+//
+// A DPP read needs two wait states after a VALU write of its source, and the
+// hazard recognizer does not look inside inline asm.  So every DPP read of a
+// value that an asm FMA writes goes through bcast_after_asm, which waits
+// itself; the FMAs' own DPP source (column j in pivot j) was last written by
+// pivot j-1's FMA of that column, which every pivot-j FMA follows by data
+// dependence through pivot j's head (bcast_after_asm, reciprocal square
+// root, multiplier), so the scheduler may place them freely.
+template <int c>
+__device__ __forceinline__ void fmac_bcast(double &acc, double src, double nm) {
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+v"(acc)
+        : "v"(src), "v"(nm), "n"(c));
+}
+
+// Lane c's value of v broadcast to its 16-lane row, two wait states first.
+template <int c>
+__device__ __forceinline__ double bcast_after_asm(double v) {
+    double out;
+    asm("s_nop 1\n\tv_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "=v"(out)
+        : "v"(v), "n"(c));
+    return out;
 }
 
 // Pivot j of the diagonal block, without branches: once a pivot has failed
-// every later one is a no-op (r = 1, m = 0).  rs keeps r of the lane's own
-// row: the diagonal of the block's inverse.
+// every later one is a no-op (r = 1, multiplier 0).  rs keeps r of the
+// lane's own row: the diagonal of the block's inverse.  Pivot j updates
+// column j+1 first, the column the next pivot broadcasts; the chain is bound
+// by the wave's instruction issue (about 50 FP64 and DPP instructions per
+// pivot), which the fused DPP FMAs keep short.
+struct PivotHead {
+    double wjj, r, li, nm;
+    bool ok;
+};
+
 template <int j>
-__device__ __forceinline__ void blk_pivot(double (&d)[16], int i, int &fail, double &wfail, double &rs) {
-    const double wjj = row_bcast<j>(d[j]);
-    const bool ok = (fail < 0) && (wjj > 0.0);
-    wfail = (fail < 0 && !ok) ? wjj : wfail;
-    fail = (fail < 0 && !ok) ? j : fail;
-    const double rq = rsqrt(wjj);
-    const double r = ok ? rq : 1.0;
-    const double wv = (ok && i > j) ? d[j] : 0.0;
-    const double li = wv * r;   // l_ij, zero for i <= j
-    const double m = li * r;
-    blk_updates<j>(d, m, std::make_integer_sequence<int, 16>{});
-    rs = (i == j) ? r : rs;
-    if (ok && i >= j) d[j] = (i == j) ? wjj * r : li;
+__device__ __forceinline__ PivotHead pivot_head(const double (&d)[16], int i, int &fail, double &wfail) {
+    PivotHead h;
+    h.wjj = (j == 0) ? row_bcast<j>(d[j]) : bcast_after_asm<j>(d[j]);
+    h.ok = (fail < 0) && (h.wjj > 0.0);
+    wfail = (fail < 0 && !h.ok) ? h.wjj : wfail;
+    fail = (fail < 0 && !h.ok) ? j : fail;
+    const double rq = rsqrt(h.wjj);
+    h.r = h.ok ? rq : 1.0;
+    const double wv = (h.ok && i > j) ? d[j] : 0.0;
+    h.li = wv * h.r;   // l_ij, zero for i <= j
+    h.nm = -(h.li * h.r);
+    return h;
 }
 
-template <int... J>
-__device__ __forceinline__ void blk_chain(double (&d)[16], int i, int &fail, double &wfail, double &rs,
-                                          std::integer_sequence<int, J...>) {
-    (blk_pivot<J>(d, i, fail, wfail, rs), ...);
+// pivot j's updates of columns lo..15 (lo > j)
+template <int j, int lo, int... C>
+__device__ __forceinline__ void pivot_updates(double (&d)[16], double nm, std::integer_sequence<int, C...>) {
+    ((C >= lo ? fmac_bcast<C>(d[C], d[j], nm) : (void)0), ...);
+}
+
+template <int j>
+__device__ __forceinline__ void pivot_finish(double (&d)[16], int i, const PivotHead &h, double &rs) {
+    rs = (i == j) ? h.r : rs;
+    if (h.ok && i >= j) d[j] = (i == j) ? h.wjj * h.r : h.li;
+}
+
+template <int j>
+__device__ __forceinline__ void blk_chain_from(double (&d)[16], int i, int &fail, double &wfail, double &rs,
+                                               const PivotHead &h) {
+    if constexpr (j == 15) {
+        pivot_finish<15>(d, i, h, rs);
+    } else {
+        fmac_bcast<j + 1>(d[j + 1], d[j], h.nm);   // column j+1, which the next head broadcasts
+        const PivotHead h1 = pivot_head<j + 1>(d, i, fail, wfail);
+        pivot_updates<j, j + 2>(d, h.nm, std::make_integer_sequence<int, 16>{});
+        pivot_finish<j>(d, i, h, rs);
+        blk_chain_from<j + 1>(d, i, fail, wfail, rs, h1);
+    }
+}
+
+__device__ __forceinline__ void blk_chain(double (&d)[16], int i, int &fail, double &wfail, double &rs) {
+    const PivotHead h0 = pivot_head<0>(d, i, fail, wfail);
+    blk_chain_from<0>(d, i, fail, wfail, rs, h0);
 }
 
 // Step k of the forward substitution for one column of the block's inverse:
@@ -147,7 +203,7 @@ __device__ __forceinline__ int blk_diag(TileLds S, int s) {
     for (int c = 0; c < 16; c++) d[c] = (c <= i) ? S[R][16 * s + c] : 0.0;
     int fail = -1;
     double wfail = 0.0, rs = 1.0;
-    blk_chain(d, i, fail, wfail, rs, std::make_integer_sequence<int, 16>{});
+    blk_chain(d, i, fail, wfail, rs);
     fail = __builtin_amdgcn_readfirstlane(fail);
     if (fail >= 0 && i == fail) rs = 1.0 / wfail;
     if (lane < 16) {
@@ -454,6 +510,20 @@ __device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nro
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
+    // the tile's own entries are loaded first, so their latency overlaps the
+    // panel loads and products (the panel lies left of every tile)
+    T cur[2][2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int j = rJ + wc * 32 + nb * 16 + (lane & 15);
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = rI + wr * 32 + mb * 16 + Mma<T>::row(lane, r);
+                cur[mb][nb][r] = (j < n && i < nrows && i >= j) ? A[(size_t)j * lda + i] : (T)0;
+            }
+    }
     typename Mma<T>::acc_t acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int c = 0; c < 2; c++) acc[a][c] = (typename Mma<T>::acc_t){0, 0, 0, 0};
@@ -461,16 +531,19 @@ __device__ __forceinline__ void syrk_next_body(T *__restrict__ A, int n, int nro
                  kb);
     const bool diag = (ti == 0 && tj == 0);
     const int kb2 = (n - t0 < NB) ? n - t0 : NB;
+#pragma unroll
     for (int nb = 0; nb < 2; nb++) {
         const int jl = wc * 32 + nb * 16 + (lane & 15);
         const int j = rJ + jl;
+#pragma unroll
         for (int mb = 0; mb < 2; mb++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int il = wr * 32 + mb * 16 + Mma<T>::row(lane, r);
                 const int i = rI + il;
                 const bool in_tile = diag && il < kb2;   // matrix rows of the next diagonal tile
                 if (j < n && i < nrows && i >= j) {
-                    const T v = A[(size_t)j * lda + i] - acc[mb][nb][r];
+                    const T v = cur[mb][nb][r] - acc[mb][nb][r];
                     if (in_tile) S[il][jl] = (double)v;
                     else A[(size_t)j * lda + i] = v;   // other tiles and right-hand-side rows
                 }
