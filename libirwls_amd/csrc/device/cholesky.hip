@@ -456,9 +456,8 @@ __device__ __forceinline__ void mfma_rows(typename Mma<T>::acc_t (&acc)[2][2], c
 // the factorization has failed in this block only right-hand-side rows
 // (rows >= n) are solved; after a failure in an earlier block nothing is.
 template <typename T>
-__device__ __forceinline__ void trsm_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
-                                          const int *__restrict__ info, const T *__restrict__ Dinv, int bx) {
-    const int inf = info[0];
+__device__ __forceinline__ void trsm_body_inf(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb, int inf,
+                                              const T *__restrict__ Dinv, int bx) {
     if (inf != 0 && inf <= k0) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -482,6 +481,13 @@ __device__ __forceinline__ void trsm_body(T *__restrict__ A, int n, int nrows, i
                 if (i < nrows && i >= rmin) A[(size_t)(k0 + c) * lda + i] = acc[mb][nb][r];
             }
     }
+}
+
+// info[0] as the step reads it: 0, or a failure in this or an earlier block
+template <typename T>
+__device__ __forceinline__ void trsm_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                          const int *__restrict__ info, const T *__restrict__ Dinv, int bx) {
+    trsm_body_inf<T>(A, n, nrows, lda, k0, kb, info[0], Dinv, bx);
 }
 
 template <typename T>
@@ -562,6 +568,147 @@ __global__ __launch_bounds__(256) void syrk_next_kernel(T *__restrict__ A, int n
     __shared__ double S[NB][NB + 1];
     __shared__ int sfail;
     syrk_next_body<T>(A, n, nrows, lda, k0, kb, info, Dnext, blockIdx.x, blockIdx.y, S, &sfail);
+}
+
+// ------------------------------------------------------- fused step ---
+// One launch per block column k for a single system: every work-group of the
+// trailing update solves the panel rows it needs itself, P_I = A(I, k) Linv^T
+// (the same MFMA products, in the same order, as the panel solve trsm_body
+// performs), from the panel as the earlier steps left it, so no separate
+// panel-solve launch precedes the update; the panels are solved in place by
+// one launch after the last step (panel_solve_kernel), when nothing reads
+// them any more.  Results are bit for bit those of the trsm + syrk pair.
+
+// P(rows r0 .. r0+63, 0..63) = A(rows, k0..k0+kb) Linv^T into the LDS array
+// PL (rows at or beyond rmax read as zero, as mfma_rows reads them).
+template <typename T>
+__device__ __forceinline__ void solve_rows_lds(T (*PL)[NB + 1], const T *__restrict__ A, int lda, int k0, int kb,
+                                               int r0, int rmax, const T *__restrict__ Dinv) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    typename Mma<T>::acc_t acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (typename Mma<T>::acc_t){0, 0, 0, 0};
+    mfma_rows<T>(acc, A + (size_t)k0 * lda, lda, r0 + wr * 32, rmax, Dinv, NB, wc * 32, kb, kb);
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                PL[wr * 32 + mb * 16 + Mma<T>::row(lane, r)][wc * 32 + nb * 16 + (lane & 15)] = acc[mb][nb][r];
+}
+
+// mfma_rows with both operands in LDS: acc += P(pr.., k) Q(qr.., k)^T over
+// k < K, rows of the Q side at or beyond qrows (local index) read as zero.
+template <typename T>
+__device__ __forceinline__ void mfma_rows_lds(typename Mma<T>::acc_t (&acc)[2][2], const T (*P)[NB + 1], int pr,
+                                              const T (*Q)[NB + 1], int qr, int qrows, int K) {
+    const int lane = threadIdx.x & 63;
+    const int rr = lane & 15, kq = lane >> 4;
+    T a[2][16], b[2][16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int k = 4 * s + kq;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ri = pr + h * 16 + rr, rj = qr + h * 16 + rr;
+            a[h][s] = (k < K) ? P[ri][k] : (T)0;
+            b[h][s] = (k < K && rj < qrows) ? Q[rj][k] : (T)0;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        acc[0][0] = Mma<T>::mma(a[0][s], b[0][s], acc[0][0]);
+        acc[0][1] = Mma<T>::mma(a[0][s], b[1][s], acc[0][1]);
+        acc[1][0] = Mma<T>::mma(a[1][s], b[0][s], acc[1][0]);
+        acc[1][1] = Mma<T>::mma(a[1][s], b[1][s], acc[1][1]);
+    }
+}
+
+// Trailing tile (ti, tj) of step k, with the panel solved on the fly; the
+// work-group of tile (0, 0) then factors and inverts the next diagonal tile,
+// as syrk_next_body does.  S holds that tile (diagonal work-group) or the
+// solved rows J (the others, as T).
+template <typename T>
+__global__ __launch_bounds__(256) void step_fused_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                         int *__restrict__ info, const T *__restrict__ Dinv,
+                                                         T *__restrict__ Dnext) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ T PI[NB][NB + 1];
+    __shared__ int sfail;
+    const int tj = blockIdx.x, ti = blockIdx.y;
+    const int t0 = k0 + kb;
+    if (t0 + tj * NB >= n || t0 + ti * NB >= nrows) return;
+    const int inf = info[0];
+    if (ti < tj || (inf != 0 && inf <= t0)) return;
+    const int rI = t0 + ti * NB, rJ = t0 + tj * NB;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const bool diag = (ti == 0 && tj == 0);
+    T cur[2][2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int j = rJ + wc * 32 + nb * 16 + (lane & 15);
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = rI + wr * 32 + mb * 16 + Mma<T>::row(lane, r);
+                cur[mb][nb][r] = (j < n && i < nrows && i >= j) ? A[(size_t)j * lda + i] : (T)0;
+            }
+    }
+    T(*PJ)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(&S[0][0]);
+    solve_rows_lds<T>(PI, A, lda, k0, kb, rI, nrows, Dinv);
+    if (ti != tj) solve_rows_lds<T>(PJ, A, lda, k0, kb, rJ, nrows, Dinv);
+    __syncthreads();
+    typename Mma<T>::acc_t acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2; c++) acc[a][c] = (typename Mma<T>::acc_t){0, 0, 0, 0};
+    mfma_rows_lds<T>(acc, PI, wr * 32, ti == tj ? PI : PJ, wc * 32, n - rJ, kb);
+    __syncthreads();   // every wave has read PJ (in S) before the diagonal tile is written there
+    const int kb2 = (n - t0 < NB) ? n - t0 : NB;
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int jl = wc * 32 + nb * 16 + (lane & 15);
+        const int j = rJ + jl;
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int il = wr * 32 + mb * 16 + Mma<T>::row(lane, r);
+                const int i = rI + il;
+                const bool in_tile = diag && il < kb2;   // matrix rows of the next diagonal tile
+                if (j < n && i < nrows && i >= j) {
+                    const T v = cur[mb][nb][r] - acc[mb][nb][r];
+                    if (in_tile) S[il][jl] = (double)v;
+                    else A[(size_t)j * lda + i] = v;   // other tiles and right-hand-side rows
+                }
+            }
+    }
+    if (!diag) return;
+    // tile_factor's first barrier also orders the stores to S above
+    const int fail = tile_factor<T>(S, kb2, &sfail, A, lda, t0, Dnext, Dnext + TILE);
+    if (fail >= 0) tile_store<T>(S, A, lda, t0, kb2, fail, Dnext, Dnext + TILE);
+    if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
+}
+
+// The panels of the fused steps at once, in place, after the last step:
+// block column kfirst + blockIdx.y, row tile blockIdx.x below its diagonal
+// tile, with the failure rules of trsm_body on the final info.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_solve_kernel(T *__restrict__ A, int n, int nrows, int lda,
+                                                          const int *__restrict__ info, const T *__restrict__ Dinv,
+                                                          int kfirst) {
+    const int kbk = kfirst + blockIdx.y;
+    const int k0 = kbk * NB;
+    const int kb = (n - k0 < NB) ? n - k0 : NB;
+    if (k0 + kb + (int)blockIdx.x * NB >= nrows) return;
+    // the step that solved block k saw only failures in blocks <= k
+    const int inf = info[0];
+    trsm_body_inf<T>(A, n, nrows, lda, k0, kb, (inf > k0 + kb) ? 0 : inf, Dinv + (size_t)kbk * 2 * TILE, blockIdx.x);
 }
 
 // -------------------------------------------------------- triangular sweeps ---
@@ -728,11 +875,30 @@ T *factor(T *A, int n, int nrows, int lda) {
         hipLaunchKernelGGL(diag_first_kernel<T>, dim3(1), dim3(256), 0, stream(), A, lda, kb, info, Dinv);
         IRWLS_LAUNCH_CHECK();
     }
+    // Steps with a large trailing matrix: the panel solve (trsm) and the
+    // trailing update (syrk, which factors the next diagonal tile) as two
+    // launches.  Once the trailing tiles fit in about fused_tiles
+    // work-groups, one launch per step whose work-groups solve their panel
+    // rows on the fly (step_fused_kernel): its extra products would make the
+    // early, large steps longer than the diagonal tile's chain.  The panels
+    // of the fused steps are solved in place by one launch at the end.  Both
+    // forms give the same bits.
+    const long fused_tiles = (long)device_cus() * (sizeof(T) == sizeof(double) ? 3 : 2);
+    int kf = nblk - 1;   // first fused step
+    for (int kbk = 0; kbk + 1 < nblk; kbk++) {
+        const int k0 = kbk * NB;
+        const int rest = n - k0 - NB;
+        const long tc = (rest + NB - 1) / NB, tr = (nrows - k0 - NB + NB - 1) / NB;
+        if (tc * tr - tc * (tc - 1) / 2 <= fused_tiles) {
+            kf = kbk;
+            break;
+        }
+    }
     for (int kbk = 0; kbk < nblk; kbk++) {
         const int k0 = kbk * NB;
         const int kb = (n - k0 < NB) ? (n - k0) : NB;
         const int prow = nrows - k0 - kb;
-        if (prow > 0) {
+        if (kbk < kf && prow > 0) {
             TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, (double)sizeof(T) * 2.0 * prow * kb);
             hipLaunchKernelGGL(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda,
                                k0, kb, info, Dinv + (size_t)kbk * 2 * TILE);
@@ -742,11 +908,32 @@ T *factor(T *A, int n, int nrows, int lda) {
         if (rest <= 0) break;   // kernels return at once after a failure
         const int tc = (rest + NB - 1) / NB;
         const int tr = (nrows - k0 - kb + NB - 1) / NB;
+        // algorithmic: the rank-64 update of the trailing lower triangle (and the rows below n)
         TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
                       (double)sizeof(T) * (double)rest * rest);
-        hipLaunchKernelGGL(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb, info,
-                           Dinv + (size_t)(kbk + 1) * 2 * TILE);
+        if (kbk < kf)
+            hipLaunchKernelGGL(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb,
+                               info, Dinv + (size_t)(kbk + 1) * 2 * TILE);
+        else
+            hipLaunchKernelGGL(step_fused_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb,
+                               info, Dinv + (size_t)kbk * 2 * TILE, Dinv + (size_t)(kbk + 1) * 2 * TILE);
         IRWLS_LAUNCH_CHECK();
+    }
+    {
+        const int k0 = kf * NB;
+        const int kb = (n - k0 < NB) ? n - k0 : NB;
+        const int prow = nrows - k0 - kb;   // rows below block kf's diagonal tile: the most of the fused steps
+        if (prow > 0) {
+            double fl = 0.0;
+            for (int kbk = kf; kbk < nblk; kbk++) {
+                const int q0 = kbk * NB, qb = (n - q0 < NB) ? (n - q0) : NB;
+                fl += 2.0 * (double)(nrows - q0 - qb) * qb * qb;
+            }
+            TimerScope ts(IRWLS_T_TRSM, fl, (double)sizeof(T) * 2.0 * (double)(nrows - k0) * (n - k0));
+            hipLaunchKernelGGL(panel_solve_kernel<T>, dim3((prow + NB - 1) / NB, nblk - kf), dim3(256), 0, stream(), A,
+                               n, nrows, lda, info, Dinv, kf);
+            IRWLS_LAUNCH_CHECK();
+        }
     }
     return Dinv;
 }
