@@ -319,7 +319,7 @@ def bench_full(args, name, steps, warmup):
     ds.pin(with_means=False)            # inputs resident in HBM before timing
     td = lib.Dataset.dense(Xt, yt)
     pp = abi.make_predict_properties(labels=0)
-    modes = ["plain", "mixed"] if args.mode == "both" else [args.mode]
+    modes = [args.mode] if args.mode in ("plain", "mixed") else ["plain", "mixed"]
     runs = {}
     for mode in modes:
         prec = 1 if mode == "mixed" else 0
@@ -445,6 +445,8 @@ def bench_budgeted(args, name, steps, warmup):
                              proposal_rows=prow)}
     if prow:
         modes["learned_exact_gains"] = dict(modes["learned"], proposal_rows=0)
+    if args.mode in ("plain", "learned"):
+        modes = {args.mode: modes[args.mode]}
     reset = dict(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
     runs = {}
     for mode, opts in modes.items():
@@ -473,14 +475,14 @@ def bench_budgeted(args, name, steps, warmup):
         log("bench: %s %s: %.3f s, accuracy %.5f, held-out Nystrom residual %.6f, primal %.8g"
             % (name, mode, runs[mode]["value"], acc, runs[mode]["resid"], runs[mode]["primal"]))
     ds.unpin()
-    main_mode = "learned"
+    main_mode = "learned" if "learned" in runs else "plain"
     r = runs[main_mode]
     cpu = None
     if args.cpu_outer != 0:
         log("bench: cpu baseline (oracle SGMA sample and IRWLSpar row sample)")
+        p = runs.get("plain", r)
         cpu = cpu_baseline_budgeted(wl, tr, col[:rp[n]], val[:rp[n]], y[:n], props,
-                                    runs["plain"]["sgma_stats"]["sgma_steps"],
-                                    runs["plain"]["par_stats"]["irwlspar_iters"])
+                                    p["sgma_stats"]["sgma_steps"], p["par_stats"]["irwlspar_iters"])
     block = {
         "workload": name, "mode": main_mode, "s": r["value"], "steps": steps, "warmup": warmup,
         "dtype": "f64+f32-mlp+bf16-mlp",
@@ -495,12 +497,12 @@ def bench_budgeted(args, name, steps, warmup):
                      "irwlspar_iters": v["par_stats"]["irwlspar_iters"]} for k, v in runs.items()},
         "roofline": roofline(r, steps, name, main_mode),
         "cpu_baseline": cpu,
-        "plain_s": runs["plain"]["value"],
-        "speedup_vs_plain": runs["plain"]["value"] / r["value"],
-        "accuracy_delta": r["acc"] - runs["plain"]["acc"],
     }
+    if "plain" in runs:
+        block.update(plain_s=runs["plain"]["value"], speedup_vs_plain=runs["plain"]["value"] / r["value"],
+                     accuracy_delta=r["acc"] - runs["plain"]["acc"])
     block.update(kernel_tables(r, steps))
-    if "learned_exact_gains" in runs:
+    if "learned_exact_gains" in runs and "plain" in runs:
         block["exact_gains_speedup_vs_plain"] = runs["plain"]["value"] / runs["learned_exact_gains"]["value"]
         block["exact_gains_accuracy_delta"] = runs["learned_exact_gains"]["acc"] - runs["plain"]["acc"]
     return block
@@ -627,8 +629,9 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS),
                     help="all (default): the headline four_gaussians_n200000_d256 plus the budgeted and batched "
                          "workloads; or one workload alone, as the headline")
-    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"],
-                    help="dense workloads: plain FP64, mixed precision, or both (value = the mixed trainer)")
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "learned", "both"],
+                    help="dense workloads: plain FP64, mixed precision, or both (value = the mixed trainer); "
+                         "budgeted workload: plain or learned proposal alone, or every mode (both)")
     ap.add_argument("--proposal-rows", type=int, default=None,
                     help="budgeted, learned mode: rows on which the shortlist's gains are computed "
                          "(0 = all rows; default: the workload's own setting)")

@@ -653,12 +653,9 @@ void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const doubl
         TimerScope ts(ncand <= 16 ? IRWLS_T_GAIN16 : IRWLS_T_SGMA, 2.0 * n * (double)s * ncand,
                       8.0 * (double)n * s + 8.0 * (double)n * ncand);
         if (ncand <= 16)
-            hipLaunchKernelGGL(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz,
-                               KSM, ldm, part);
+            ts.launch(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
         else
-            hipLaunchKernelGGL(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, stream(), n, s, ncand, KSC, Z, ldz, KSM,
-                               ldm, part);
-        IRWLS_LAUNCH_CHECK();
+            ts.launch(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
     }
     hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, rt, ncand, qout);
     IRWLS_LAUNCH_CHECK();
@@ -718,9 +715,8 @@ void irwls_dev_wgram(int B, const double *Phi, const int32_t *act, const double 
     {
         TimerScope ts(IRWLS_T_SYRKD, (double)nact * B * (double)(B + 1), 8.0 * (double)nact * B);
         if (nact > 0) {
-            hipLaunchKernelGGL(wgram_kernel, dim3(ntiles * nsplit), dim3(256), 0, stream(), B, Phi, act, sw, nact,
-                               chunk, ntiles, nsplit % 8 == 0 ? 1 : 0, part);
-            IRWLS_LAUNCH_CHECK();
+            ts.launch(wgram_kernel, dim3(ntiles * nsplit), dim3(256), 0, B, Phi, act, sw, nact, chunk, ntiles,
+                      nsplit % 8 == 0 ? 1 : 0, part);
         } else {
             irwls_dev_memset0(part, BB * sizeof(double));
         }
@@ -751,8 +747,7 @@ void irwls_dev_wvec(int B, const double *Phi, const int32_t *act, const double *
 void irwls_dev_resid_rows(int n, int B, const double *Phi, const double *beta, const double *y, double *e) {
     if (n <= 0) return;
     TimerScope ts(IRWLS_T_MATVEC, 2.0 * n * (double)B, 8.0 * (double)n * B + 16.0 * n + 8.0 * B);
-    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), n, B, Phi, beta, y, e);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, n, B, Phi, beta, y, e);
 }
 
 int irwls_dev_irls_weights(int n, const double *e, const double *y, double C, double M, double *D, int32_t *act,

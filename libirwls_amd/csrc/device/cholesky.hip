@@ -872,8 +872,7 @@ T *factor(T *A, int n, int nrows, int lda) {
     {
         const int kb = (n < NB) ? n : NB;
         TimerScope ts(IRWLS_T_POTF2, kb * (double)kb * kb / 3.0, 8.0 * (double)kb * kb);
-        hipLaunchKernelGGL(diag_first_kernel<T>, dim3(1), dim3(256), 0, stream(), A, lda, kb, info, Dinv);
-        IRWLS_LAUNCH_CHECK();
+        ts.launch(diag_first_kernel<T>, dim3(1), dim3(256), 0, A, lda, kb, info, Dinv);
     }
     // Steps with a large trailing matrix: the panel solve (trsm) and the
     // trailing update (syrk, which factors the next diagonal tile) as two
@@ -900,9 +899,8 @@ T *factor(T *A, int n, int nrows, int lda) {
         const int prow = nrows - k0 - kb;
         if (kbk < kf && prow > 0) {
             TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, (double)sizeof(T) * 2.0 * prow * kb);
-            hipLaunchKernelGGL(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, stream(), A, n, nrows, lda,
-                               k0, kb, info, Dinv + (size_t)kbk * 2 * TILE);
-            IRWLS_LAUNCH_CHECK();
+            ts.launch(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+                      Dinv + (size_t)kbk * 2 * TILE);
         }
         const int rest = n - k0 - kb;
         if (rest <= 0) break;   // kernels return at once after a failure
@@ -912,12 +910,11 @@ T *factor(T *A, int n, int nrows, int lda) {
         TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
                       (double)sizeof(T) * (double)rest * rest);
         if (kbk < kf)
-            hipLaunchKernelGGL(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb,
-                               info, Dinv + (size_t)(kbk + 1) * 2 * TILE);
+            ts.launch(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+                      Dinv + (size_t)(kbk + 1) * 2 * TILE);
         else
-            hipLaunchKernelGGL(step_fused_kernel<T>, dim3(tc, tr), dim3(256), 0, stream(), A, n, nrows, lda, k0, kb,
-                               info, Dinv + (size_t)kbk * 2 * TILE, Dinv + (size_t)(kbk + 1) * 2 * TILE);
-        IRWLS_LAUNCH_CHECK();
+            ts.launch(step_fused_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+                      Dinv + (size_t)kbk * 2 * TILE, Dinv + (size_t)(kbk + 1) * 2 * TILE);
     }
     {
         const int k0 = kf * NB;
@@ -930,9 +927,8 @@ T *factor(T *A, int n, int nrows, int lda) {
                 fl += 2.0 * (double)(nrows - q0 - qb) * qb * qb;
             }
             TimerScope ts(IRWLS_T_TRSM, fl, (double)sizeof(T) * 2.0 * (double)(nrows - k0) * (n - k0));
-            hipLaunchKernelGGL(panel_solve_kernel<T>, dim3((prow + NB - 1) / NB, nblk - kf), dim3(256), 0, stream(), A,
-                               n, nrows, lda, info, Dinv, kf);
-            IRWLS_LAUNCH_CHECK();
+            ts.launch(panel_solve_kernel<T>, dim3((prow + NB - 1) / NB, nblk - kf), dim3(256), 0, A, n, nrows, lda,
+                      info, Dinv, kf);
         }
     }
     return Dinv;
@@ -1287,7 +1283,7 @@ int irwls_dev_potrf(double *A, int n, int lda) {
 void irwls_dev_potrs(const double *L, int n, int lda, double *b) {
     if (n <= 0) return;
     double *tmp = (double *)scratch(9, (size_t)n * sizeof(double));
-    TimerScope ts(IRWLS_T_POTRS, 2.0 * (double)n * n, 8.0 * (double)n * n);
+    TimerScope ts(IRWLS_T_POTRS, 2.0 * (double)n * n, 8.0 * (double)n * n, false, true);
     sweep_forward(L, n, lda, b, tmp);     // z in tmp, b is scratch
     sweep_backward(L, n, lda, tmp, b);    // x in b
 }
@@ -1305,7 +1301,7 @@ int irwls_dev_posv_aug(double *A, int n, int lda, const double *rhs, double *x) 
     hipLaunchKernelGGL(gather_row, dim3((n + 255) / 256), dim3(256), 0, stream(), A, lda, n, n, z);
     IRWLS_LAUNCH_CHECK();
     {
-        TimerScope ts(IRWLS_T_POTRS, 1.0 * (double)n * n, 4.0 * (double)n * n);
+        TimerScope ts(IRWLS_T_POTRS, 1.0 * (double)n * n, 4.0 * (double)n * n, false, true);
         sweep_backward_inv(A, n, lda, Dinv, z, x);
     }
     int info = 0;
@@ -1389,7 +1385,7 @@ int irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *A, cons
         hipGraphExec_t ex = graph_for(pl, best);
         double fl = 0.0;
         for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
-        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true);   // batches of different round slots may overlap
+        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true, true);   // batches of different round slots may overlap
         IRWLS_HIP(hipGraphLaunch(ex, stream()));
     }
     int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <stdint.h>
@@ -37,17 +38,37 @@ __device__ __forceinline__ double rdl(double v, int lane) {
 // The library stream: every kernel and copy of the library is issued here.
 hipStream_t stream();
 
-// Timing brackets around launches of one kind (no-ops unless enabled; a
-// negative kind is never timed).
+// Event pair `which` (0 start, 1 stop) of timer slot idx.
+hipEvent_t timer_event(int idx, int which);
+
+// Timing of launches of one kind (no-ops unless enabled; a negative kind is
+// never timed).  A scope's kernels are launched through launch(): the first
+// kernel's dispatch carries the start event and every kernel's the stop
+// event (hipExtLaunchKernel), so the time is the kernels' own, from the
+// first one's start to the last one's end, as a kernel trace reports it.
+// With markers = true the scope instead records an event before and after
+// whatever it encloses (copies, graph launches, calls of other functions).
 struct TimerScope {
     int kind;
     double flops, bytes;
-    bool on;
-    int idx;   // event pair reserved for this launch
+    bool on, markers;
+    int idx;   // event pair reserved for this scope
+    int nk;    // kernels launched through launch()
     // any_thread: also on worker threads (for a launch that only one thread
     // at a time can make, such as a batched solve)
-    TimerScope(int kind, double flops, double bytes, bool any_thread = false);
+    TimerScope(int kind, double flops, double bytes, bool any_thread = false, bool markers = false);
     ~TimerScope();
+    template <typename F, typename... A>
+    void launch(F kernel, dim3 grid, dim3 block, unsigned shm, A... args) {
+        if (on && !markers) {
+            hipExtLaunchKernelGGL(kernel, grid, block, shm, stream(), nk == 0 ? timer_event(idx, 0) : nullptr,
+                                  timer_event(idx, 1), 0u, args...);
+            nk++;
+        } else {
+            hipLaunchKernelGGL(kernel, grid, block, shm, stream(), args...);
+        }
+        IRWLS_LAUNCH_CHECK();
+    }
 };
 
 // Scratch device buffer owned by the context, grown on demand.

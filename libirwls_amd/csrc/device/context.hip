@@ -52,6 +52,7 @@ static bool resolve_pool_locked() {
         if (!g_evdone[i]) return false;
     IRWLS_HIP(hipDeviceSynchronize());
     for (int i = 0; i < g_evn; i++) {
+        if (g_evkind[i] < 0) continue;
         float ms = 0.f;
         IRWLS_HIP(hipEventElapsedTime(&ms, g_evs[i][0], g_evs[i][1]));
         g_ms[g_evkind[i]] += ms;
@@ -65,10 +66,12 @@ static void resolve_pool() {
     resolve_pool_locked();
 }
 
-TimerScope::TimerScope(int k, double f, double b, bool any_thread)
+hipEvent_t timer_event(int idx, int which) { return g_evs[idx][which]; }
+
+TimerScope::TimerScope(int k, double f, double b, bool any_thread, bool mk)
     : kind(k), flops(f), bytes(b),
-      on(k >= 0 && g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)),
-      idx(-1) {
+      on(k >= 0 && g_timing && (any_thread || !t_stream) && ((g_timing_mask >> k) & 1u)), markers(mk),
+      idx(-1), nk(0) {
     if (!on) return;
     std::lock_guard<std::mutex> lk(g_tmu);
     if (g_evn == kPool && !resolve_pool_locked()) {
@@ -77,15 +80,23 @@ TimerScope::TimerScope(int k, double f, double b, bool any_thread)
     }
     idx = g_evn++;
     g_evdone[idx] = false;
-    IRWLS_HIP(hipEventRecord(g_evs[idx][0], stream()));
+    if (markers) IRWLS_HIP(hipEventRecord(g_evs[idx][0], stream()));
 }
 
 TimerScope::~TimerScope() {
     if (!on) return;
     std::lock_guard<std::mutex> lk(g_tmu);
-    IRWLS_HIP(hipEventRecord(g_evs[idx][1], stream()));
-    g_evkind[idx] = kind;
+    const bool counted = markers || nk > 0;
+    if (markers) {
+        IRWLS_HIP(hipEventRecord(g_evs[idx][1], stream()));
+    } else if (nk == 0) {
+        // no kernel ran in the scope: an empty pair, not counted
+        IRWLS_HIP(hipEventRecord(g_evs[idx][0], stream()));
+        IRWLS_HIP(hipEventRecord(g_evs[idx][1], stream()));
+    }
+    g_evkind[idx] = counted ? kind : -1;
     g_evdone[idx] = true;
+    if (!counted) return;
     g_launch[kind] += 1;
     g_flops[kind] += flops;
     g_bytes[kind] += bytes;

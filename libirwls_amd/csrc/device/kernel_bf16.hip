@@ -468,14 +468,10 @@ void irwls_dev_kernel_matvec_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
     const double flops = 2.0 * r->n * (double)c->n * r->ld;
     const double bytes = 4.0 * ((double)r->n * r->ld + (double)c->n * r->ld) + 8.0 * 2.0 * r->n;
     TimerScope ts(IRWLS_T_BF16, flops, bytes);
-    hipLaunchKernelGGL(kmv_bf16_kernel, dim3(rt, nsplit), dim3(512), 0, stream(), side_of(r), side_of(c), r->ld,
-                       kernel, exp2_scale(gamma), w, tps, nsplit, part, a, b, cst, rs, out);
-    IRWLS_LAUNCH_CHECK();
-    if (nsplit > 1) {
-        hipLaunchKernelGGL(kmv_finish_bf16, dim3((r->n + 255) / 256), dim3(256), 0, stream(), part, r->n, nsplit, a,
-                           b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
-    }
+    ts.launch(kmv_bf16_kernel, dim3(rt, nsplit), dim3(512), 0, side_of(r), side_of(c), r->ld, kernel,
+              exp2_scale(gamma), w, tps, nsplit, part, a, b, cst, rs, out);
+    if (nsplit > 1)
+        ts.launch(kmv_finish_bf16, dim3((r->n + 255) / 256), dim3(256), 0, part, r->n, nsplit, a, b, cst, rs, out);
 }
 
 void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf16 *c, int kernel, double gamma,
@@ -488,9 +484,8 @@ void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
     const double flops = 2.0 * r->n * (double)c->n * r->ld;
     const double bytes = 4.0 * ((double)r->n * r->ld + (double)c->n * r->ld) + 8.0 * (double)r->n * c->n;
     TimerScope ts(IRWLS_T_BF16, flops, bytes);
-    hipLaunchKernelGGL(kmat_bf16_kernel, dim3((r->n + BT - 1) / BT, (c->n + BT - 1) / BT), dim3(256), 0, stream(),
-                       side_of(r), side_of(c), r->ld, kernel, exp2_scale(gamma), K, ldk);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(kmat_bf16_kernel, dim3((r->n + BT - 1) / BT, (c->n + BT - 1) / BT), dim3(256), 0, side_of(r),
+              side_of(c), r->ld, kernel, exp2_scale(gamma), K, ldk);
 }
 
 }  // extern "C"

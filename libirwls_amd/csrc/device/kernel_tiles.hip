@@ -779,18 +779,14 @@ void kmv_dense_cols_run(const irwls_rows *r, const irwls_rows *c, int d, int ker
         const double nz = r->X ? (double)r->n * d : r->nnz_per_row * r->n;
         TimerScope ts(timed ? IRWLS_T_KMV : -1, 2.0 * ((double)csd.n * d + nz),
                       8.0 * ((double)csd.n * d + nz + 2.0 * r->n));
-        hipLaunchKernelGGL(primal_part_dense, dim3((d + 255) / 256, ns), dim3(256), 0, stream(), csd, d, w, per,
-                           part);
-        IRWLS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(primal_sum, dim3((d + 255) / 256), dim3(256), 0, stream(), part, ns, d, u);
-        IRWLS_LAUNCH_CHECK();
+        ts.launch(primal_part_dense, dim3((d + 255) / 256, ns), dim3(256), 0, csd, d, w, per, part);
+        ts.launch(primal_sum, dim3((d + 255) / 256), dim3(256), 0, part, ns, d, u);
         if (r->X)
-            hipLaunchKernelGGL(primal_apply_dense, dim3((r->n + 3) / 4), dim3(256), 0, stream(), dense_side(r), d,
-                               u, a, b, cst, rs, out);
+            ts.launch(primal_apply_dense, dim3((r->n + 3) / 4), dim3(256), 0, dense_side(r), d, u, a, b, cst, rs,
+                      out);
         else
-            hipLaunchKernelGGL(primal_apply_csr, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
-                               r->val, r->idx, r->n, u, a, b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
+            ts.launch(primal_apply_csr, dim3((r->n + 3) / 4), dim3(256), 0, r->rowptr, r->col, r->val, r->idx, r->n,
+                      u, a, b, cst, rs, out);
         return;
     }
     const DenseSide rsd = dense_side(r);
@@ -799,14 +795,10 @@ void kmv_dense_cols_run(const irwls_rows *r, const irwls_rows *c, int d, int ker
     const double flops = 2.0 * rsd.n * (double)csd.n * d;
     const double bytes = 8.0 * ((double)rsd.n * d + (double)csd.n * d + 2.0 * rsd.n);
     TimerScope ts(timed ? IRWLS_T_KMV : -1, flops, bytes);
-    hipLaunchKernelGGL(kmv_dense_kernel, dim3(rt, nsplit), dim3(256), 0, stream(), rsd, csd, d, kernel, gamma, w,
-                       tps, nsplit, part, a, b, cst, rs, out);
-    IRWLS_LAUNCH_CHECK();
-    if (nsplit > 1) {
-        hipLaunchKernelGGL(kmv_finish_kernel, dim3((rsd.n + 255) / 256), dim3(256), 0, stream(), part, rsd.n,
-                           nsplit, a, b, cst, rs, out);
-        IRWLS_LAUNCH_CHECK();
-    }
+    ts.launch(kmv_dense_kernel, dim3(rt, nsplit), dim3(256), 0, rsd, csd, d, kernel, gamma, w, tps, nsplit, part, a,
+              b, cst, rs, out);
+    if (nsplit > 1)
+        ts.launch(kmv_finish_kernel, dim3((rsd.n + 255) / 256), dim3(256), 0, part, rsd.n, nsplit, a, b, cst, rs, out);
 }
 
 }  // namespace irwls
@@ -853,10 +845,9 @@ void irwls_dev_kernel_matvec(const irwls_rows *r, const irwls_rows *c, int d, in
             // algorithmic: the rows' nonzeros (value + column), their row pointers, the column table, the output
             const double nz = r->nnz_per_row * r->n;
             TimerScope ts(IRWLS_T_CSR, 2.0 * nz * cnk, 12.0 * nz + 8.0 * cnk * (double)d + 24.0 * r->n);
-            hipLaunchKernelGGL(kmv_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col,
-                               r->val, r->nrm, r->idx, r->n, XcT, cnk, d, cm, cn, cnk, kernel, gamma, w + j0,
-                               j0 == 0 ? a : 1.0, b, j0 == 0 ? cst : 0.0, rs, out);
-            IRWLS_LAUNCH_CHECK();
+            ts.launch(kmv_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, r->rowptr, r->col, r->val, r->nrm, r->idx,
+                      r->n, XcT, cnk, d, cm, cn, cnk, kernel, gamma, w + j0, j0 == 0 ? a : 1.0, b,
+                      j0 == 0 ? cst : 0.0, rs, out);
         }
     }
 }
@@ -870,9 +861,8 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         DenseSide csd = dense_cols(c, d, 6, 7);
         const double bytes = 8.0 * ((double)r->n * d + (double)c->n * d + (double)r->n * c->n);
         TimerScope ts(IRWLS_T_KMAT, flops, bytes);
-        hipLaunchKernelGGL(kmat_dense_kernel, dim3((r->n + TB - 1) / TB, (c->n + TB - 1) / TB), dim3(256), 0,
-                           stream(), rsd, csd, d, kernel, gamma, K, ldk);
-        IRWLS_LAUNCH_CHECK();
+        ts.launch(kmat_dense_kernel, dim3((r->n + TB - 1) / TB, (c->n + TB - 1) / TB), dim3(256), 0, rsd, csd, d,
+                  kernel, gamma, K, ldk);
     } else if (c->n > csr_col_chunk(d)) {
         // high-dimensional CSR: the block in column chunks
         const int chunk = csr_col_chunk(d);
@@ -888,23 +878,19 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         const double nz = r->nnz_per_row * r->n;
         TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
         if (r->idx)
-            hipLaunchKernelGGL(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(), r->rowptr,
-                               r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K,
-                               ldk);
+            ts.launch(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col, r->val,
+                      r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
         else
-            hipLaunchKernelGGL(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, stream(),
-                               r->rowptr, r->col, r->val, r->nrm, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma,
-                               K, ldk);
-        IRWLS_LAUNCH_CHECK();
+            ts.launch(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col,
+                      r->val, r->nrm, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
     } else {
         double *XcT, *cn;
         unsigned long long *cm;
         cols_T(c, d, &XcT, &cn, &cm);
         const double nz = r->nnz_per_row * r->n;
         TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        hipLaunchKernelGGL(kmat_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, stream(), r->rowptr, r->col, r->val,
-                           r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
-        IRWLS_LAUNCH_CHECK();
+        ts.launch(kmat_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, r->rowptr, r->col, r->val, r->nrm, r->idx,
+                  r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
     }
 }
 

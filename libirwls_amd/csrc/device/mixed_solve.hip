@@ -300,7 +300,7 @@ int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, 
     const size_t vb = (size_t)2 * n * sizeof(double);
     const int gcol = n;                     // work-groups (and partial pairs) of the column products
     const int gvec = (n + 255) / 256;       // work-groups of the elementwise kernels
-    TimerScope ts(IRWLS_T_PCG, 0.0, 0.0);
+    TimerScope ts(IRWLS_T_PCG, 0.0, 0.0, false, true);
     // partials of |B|^2 sit after the residual partials in part_b (2 (n + 1)
     // entries >= 4 gvec) and travel with every residual read
     hipLaunchKernelGGL(sumsq2_parts, dim3(gvec), dim3(256), 0, stream(), n, B, g_pcg.part_b + 2 * gvec);

@@ -465,11 +465,8 @@ void irwls_dev_mlp_grad(const float *P, const float *X, const float *T, int nb, 
     const int nblk = (nb + SB - 1) / SB;
     mlp_reserve(nblk);
     TimerScope ts(IRWLS_T_MLP, 6.0 * nb * (double)(NIN * NH + NH * NH + NH), 4.0 * NP * nblk);
-    hipLaunchKernelGGL(mlp_fwd_bwd, dim3(nblk), dim3(256), 0, stream(), P, X, T, nb, g_mlp.gpart, g_mlp.lpart);
-    IRWLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(mlp_reduce, dim3((NP + 255) / 256), dim3(256), 0, stream(), g_mlp.gpart, g_mlp.lpart, nblk, nb,
-                       grad, loss);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(mlp_fwd_bwd, dim3(nblk), dim3(256), 0, P, X, T, nb, g_mlp.gpart, g_mlp.lpart);
+    ts.launch(mlp_reduce, dim3((NP + 255) / 256), dim3(256), 0, g_mlp.gpart, g_mlp.lpart, nblk, nb, grad, loss);
 }
 
 void irwls_dev_mlp_adam(float *P, const float *grad, float *m, float *v, int step, float lr) {
@@ -494,8 +491,7 @@ void irwls_dev_mlp_pack_bf16(const float *P, uint16_t *Q) {
 void irwls_dev_mlp_infer_bf16(const uint16_t *Q, const float *P, const float *X, int nb, float *Y) {
     if (nb <= 0) return;
     TimerScope ts(IRWLS_T_MLP, 2.0 * nb * (double)(NIN * NH + NH * NH + NH), 2.0 * PK_N + 4.0 * nb * NIN);
-    hipLaunchKernelGGL(mlp_infer_bf16, dim3((nb + SB - 1) / SB), dim3(256), 0, stream(), Q, P, X, nb, Y);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(mlp_infer_bf16, dim3((nb + SB - 1) / SB), dim3(256), 0, Q, P, X, nb, Y);
 }
 
 void irwls_dev_sgma_features(int P, int s, int B, const double *KNC, int ldk, const double *eta, const double *gsub,

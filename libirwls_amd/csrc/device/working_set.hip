@@ -104,14 +104,11 @@ void irwls_dev_gemv(const double *A, int m, int n, int lda, int trans, const dou
         const int nsl = 4 * groups;
         const int per = (n + nsl - 1) / nsl;
         double *part = (double *)scratch(0, (size_t)nsl * m * sizeof(double));
-        hipLaunchKernelGGL(gemv_n_kernel, dim3((m + 63) / 64, groups), dim3(256), 0, stream(), A, m, n, lda, x, per,
-                           part);
-        IRWLS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(gemv_finish, dim3((m + 255) / 256), dim3(256), 0, stream(), part, m, nsl, y);
+        ts.launch(gemv_n_kernel, dim3((m + 63) / 64, groups), dim3(256), 0, A, m, n, lda, x, per, part);
+        ts.launch(gemv_finish, dim3((m + 255) / 256), dim3(256), 0, part, m, nsl, y);
     } else {
-        hipLaunchKernelGGL(gemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, stream(), A, m, n, lda, x, y);
+        ts.launch(gemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, A, m, n, lda, x, y);
     }
-    IRWLS_LAUNCH_CHECK();
 }
 
 
@@ -119,18 +116,15 @@ void irwls_dev_build_kkt(const double *Kw, int ldk, const int32_t *S1, int nS1, 
                          double *H, int ldh) {
     const int N = nS1 + 1;
     TimerScope ts(IRWLS_T_KKT, 0.0, 8.0 * (double)N * N * 2.0);
-    hipLaunchKernelGGL(build_kkt_kernel, dim3((N + 255) / 256, N), dim3(256), 0, stream(), Kw, ldk, S1, nS1, ys, as,
-                       H, ldh);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(build_kkt_kernel, dim3((N + 255) / 256, N), dim3(256), 0, Kw, ldk, S1, nS1, ys, as, H, ldh);
 }
 
 void irwls_dev_gemv_gather(const double *Kw, int ldk, const int32_t *rows, int nr, const int32_t *cols, int nc,
                            const double *w, double a, double b, double cst, const double *rs, double *out) {
     if (nr <= 0) return;
     TimerScope ts(IRWLS_T_GEMV, 2.0 * nr * (double)nc, 8.0 * nr * (double)nc);
-    hipLaunchKernelGGL(gemv_gather_kernel, dim3((nr + 3) / 4), dim3(256), 0, stream(), Kw, ldk, rows, nr, cols, nc,
-                       w, a, b, cst, rs, out);
-    IRWLS_LAUNCH_CHECK();
+    ts.launch(gemv_gather_kernel, dim3((nr + 3) / 4), dim3(256), 0, Kw, ldk, rows, nr, cols, nc, w, a, b, cst, rs,
+              out);
 }
 
 }  // extern "C"
