@@ -274,6 +274,37 @@ def roofline(r, steps, workload=None, mode=None):
     return roof
 
 
+def predict_figure(lib, L, td, m, pp, n_sv, reps=3):
+    """softTest over the test set with the test rows pinned in HBM: time per
+    call, and the rates of its kernel-block x vector product (HIP-event
+    timed): algorithmic bytes (test rows, support vectors, output) per
+    second against the HBM peak and flops per second against the FP64
+    MFMA peak."""
+    td.pin(with_means=False)
+    lib.soft_test(td, m, pp)
+    lib.timing(True)
+    L.irwls_dev_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.soft_test(td, m, pp)
+    L.irwls_dev_sync()
+    dt = (time.perf_counter() - t0) / reps
+    tm = lib.timers()
+    lib.timing(False)
+    td.unpin()
+    kind = max(("kmv", "csr"), key=lambda k: tm[k]["ms"])
+    t = tm[kind]
+    if not t["launches"]:
+        return None
+    sec = t["ms"] * 1e-3
+    gbs = t["bytes"] / sec / 1e9
+    tfl = t["flops"] / sec / 1e12
+    return {"rows": td.ds.l, "n_sv": n_sv, "s_per_call": dt, "rows_per_s": td.ds.l / dt, "kernel": kind,
+            "launches_per_call": t["launches"] / reps, "kernel_ms_per_call": t["ms"] / reps,
+            "algorithmic_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+            "tflops": tfl, "fp64_mfma_frac": tfl / PEAK_FP64_TFLOPS}
+
+
 def cpu_baseline_full(wl, X, y, gpu_outer, max_outer):
     """Oracle trainFULL on the host cores: the whole run (max_outer < 0) or
     its first max_outer outer iterations, extrapolated to the GPU plain
@@ -335,6 +366,8 @@ def bench_full(args, name, steps, warmup):
                           nsv=int(np.count_nonzero(beta[:-1])))
     main_mode = "mixed" if "mixed" in runs else "plain"
     r = runs[main_mode]
+    pred = predict_figure(lib, L, td, lib.full_model(props, ds, r["beta"]), pp, r["nsv"])
+    log("bench: %s predict: %s" % (name, pred))
     # drop-in: the same call on an unpinned dataset (AoS conversion and upload inside the timed call)
     ds.unpin()
     lib.set_options(precision=1 if main_mode == "mixed" else 0)
@@ -366,6 +399,7 @@ def bench_full(args, name, steps, warmup):
         "test_accuracy": r["acc"],
         "runs": {k: summary(v) for k, v in runs.items()},
         "drop_in_s": float(np.mean(drop)),
+        "predict": pred,
         "roofline": roofline(r, steps, name, main_mode),
         "cpu_baseline": cpu,
     }
@@ -469,7 +503,7 @@ def bench_budgeted(args, name, steps, warmup):
         acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
         ftr = lib.soft_test(ds, m, pp)
         runs[mode] = dict(value=float(np.mean(times)), sgma_s=state["sgma_s"], acc=acc, tm=tm, tw=tw, dom=dom,
-                          sgma_stats=state["sst"], par_stats=state["pst"], steps=nst,
+                          sgma_stats=state["sst"], par_stats=state["pst"], steps=nst, model=m,
                           resid=objective.nystrom_residual(ds, cent, hold, wl["gamma"]),
                           primal=objective.primal_objective(ds, cent, beta, ftr, wl["C"], wl["gamma"]))
         log("bench: %s %s: %.3f s, accuracy %.5f, held-out Nystrom residual %.6f, primal %.8g"
@@ -477,6 +511,8 @@ def bench_budgeted(args, name, steps, warmup):
     ds.unpin()
     main_mode = "learned" if "learned" in runs else "plain"
     r = runs[main_mode]
+    pred = predict_figure(lib, L, td, r["model"], pp, wl["B"])
+    log("bench: %s predict: %s" % (name, pred))
     cpu = None
     if args.cpu_outer != 0:
         log("bench: cpu baseline (oracle SGMA sample and IRWLSpar row sample)")
@@ -495,6 +531,7 @@ def bench_budgeted(args, name, steps, warmup):
                      "sgma_steps": v["sgma_stats"]["sgma_steps"], "learned_steps": v["sgma_stats"]["learned_steps"],
                      "mlp_loss0": v["sgma_stats"]["mlp_loss0"], "mlp_loss": v["sgma_stats"]["mlp_loss"],
                      "irwlspar_iters": v["par_stats"]["irwlspar_iters"]} for k, v in runs.items()},
+        "predict": pred,
         "roofline": roofline(r, steps, name, main_mode),
         "cpu_baseline": cpu,
     }

@@ -289,6 +289,21 @@ irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int 
     return upload(ds, with_means, base, min_cols);
 }
 
+int irwls_devdata_pinned_range(const svm_dataset *ds, int *base, int *maxindex) {
+    int found = 0;
+    pthread_mutex_lock(&g_dd_mu);
+    for (int i = 0; i < MAX_PINNED && !found; i++) {
+        const irwls_devdata *p = g_pinned[i];
+        if (p && p->key == ds->features && p->key_x == ds->x && p->l == ds->l && p->dense == (ds->sparse == 0)) {
+            *base = p->base;
+            *maxindex = p->base + p->d - 1;
+            found = 1;
+        }
+    }
+    pthread_mutex_unlock(&g_dd_mu);
+    return found;
+}
+
 void irwls_devdata_release(irwls_devdata *dd) {
     if (!dd || dd->pinned) return;
     destroy(dd);

@@ -60,6 +60,11 @@ int irwls_dataset_maxindex(const svm_dataset *ds, int rows);
    class-mean rows l and l+1 as well; base as above (-1 = derive it);
    min_cols widens the column space (prediction aligns data and model). */
 irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols);
+/* Base and largest feature index of a pinned dataset's device copy (its
+   column range, which may extend past the largest index present: those
+   columns are zero); 0 when the dataset is not pinned.  Spares a caller the
+   scan of every row that irwls_dataset_base and irwls_dataset_maxindex make. */
+int irwls_devdata_pinned_range(const svm_dataset *ds, int *base, int *maxindex);
 void irwls_devdata_release(irwls_devdata *dd);
 /* Row view of the whole device dataset (idx may select rows). */
 irwls_rows irwls_devdata_rows(const irwls_devdata *dd, const int32_t *d_idx, int n);

@@ -22,6 +22,14 @@ static svm_dataset sv_view(const model *m) {
     return sv;
 }
 
+/* Feature range of the rows to predict: that of their pinned device copy,
+   or a scan of the rows. */
+static void test_range(const svm_dataset *ds, int *base, int *maxindex) {
+    if (irwls_devdata_pinned_range(ds, base, maxindex)) return;
+    *base = irwls_dataset_base(ds, ds->l);
+    *maxindex = irwls_dataset_maxindex(ds, ds->l);
+}
+
 static double *decision_values(svm_dataset dataset, model m) {
     const int n = dataset.l;
     double *out = (double *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
@@ -32,9 +40,11 @@ static double *decision_values(svm_dataset dataset, model m) {
     }
     irwls_dev_init();
     svm_dataset sv = sv_view(&m);
-    const int bt = irwls_dataset_base(&dataset, n), bm = irwls_dataset_base(&sv, m.nSVs);
+    int bt, mt;
+    test_range(&dataset, &bt, &mt);
+    const int bm = irwls_dataset_base(&sv, m.nSVs);
     const int base = (bt < bm) ? bt : bm;
-    const int mt = irwls_dataset_maxindex(&dataset, n), mm = irwls_dataset_maxindex(&sv, m.nSVs);
+    const int mm = irwls_dataset_maxindex(&sv, m.nSVs);
     const int cols = ((mt > mm) ? mt : mm) + 1 - base;
     irwls_devdata *dt = irwls_devdata_acquire(&dataset, 0, base, cols);
     irwls_devdata *dm = irwls_devdata_acquire(&sv, 0, base, cols);
@@ -70,7 +80,8 @@ int softTestMany(svm_dataset dataset, const model *models, int nmodels, predictP
     const int n = dataset.l;
     if (n <= 0 || nmodels <= 0) return 0;
     irwls_dev_init();
-    const int bt = irwls_dataset_base(&dataset, n), mt = irwls_dataset_maxindex(&dataset, n);
+    int bt, mt;
+    test_range(&dataset, &bt, &mt);
     double *d_out = (double *)irwls_dev_alloc((size_t)n * nmodels * sizeof(double));
     /* kept until the single read-back, so no release waits on a queued launch */
     irwls_devdata **dt = (irwls_devdata **)irwls_xcalloc((size_t)nmodels, sizeof(irwls_devdata *));

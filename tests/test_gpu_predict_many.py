@@ -48,3 +48,42 @@ def test_bundle_equals_single_model_predictions_and_ovr():
     print("one-vs-rest accuracy %.4f" % acc)
     assert acc > 0.85
     assert np.array_equal(lib.one_vs_rest(td, models[:4], pp, classes=[10, 11, 12, 13]), cls + 10)
+
+
+@pytest.mark.parametrize("kind", ["dense", "csr"])
+def test_pinned_test_rows_predict_as_unpinned(kind):
+    """softTest and softTestMany with the test rows pinned in HBM (their
+    feature range then comes from the device copy, not a scan of the rows)
+    equal the unpinned calls bit for bit, also for a model whose support
+    vectors reach a higher feature index than the test rows."""
+    from libirwls_amd import synth
+    rng = np.random.default_rng(11)
+    if kind == "dense":
+        X, c = _mixture(1500, 12, 5)
+        y = np.where(c == 0, 1.0, -1.0)
+        ds = lib.Dataset.dense(X, y)
+        Xt, _ = _mixture(400, 9, 6)   # fewer features than the model's rows
+        td = lib.Dataset.dense(Xt, np.zeros(len(Xt)))
+    else:
+        rp, col, val = synth.sparse_uniform(1500, 60, 0.1, rng)
+        y = np.where(rng.random(1500) < 0.5, 1.0, -1.0)
+        ds = lib.Dataset.csr(rp, col, val, y)
+        rpt, colt, valt = synth.sparse_uniform(400, 40, 0.1, rng)
+        td = lib.Dataset.csr(rpt, colt, valt, np.zeros(400))
+    props = abi.make_properties(gamma=0.2, C_=10.0, W=200)
+    m = lib.full_model(props, ds, lib.train_full(ds, props, seed=0))
+    pb = abi.make_properties(gamma=0.2, C_=1.0, size=40, algorithm=0)
+    cent = lib.random_centroids(ds, pb, seed=2)
+    mb = lib.budgeted_model(pb, ds, cent, lib.irwls_par(ds, cent, pb))
+    pp = abi.make_predict_properties(labels=0)
+    want = [lib.soft_test(td, mm, pp) for mm in (m, mb)]
+    want_many = lib.soft_test_many(td, [m, mb], pp)
+    td.pin(with_means=False)
+    try:
+        got = [lib.soft_test(td, mm, pp) for mm in (m, mb)]
+        got_many = lib.soft_test_many(td, [m, mb], pp)
+    finally:
+        td.unpin()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+    assert np.array_equal(got_many, want_many)
