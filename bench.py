@@ -159,13 +159,14 @@ def dominant(timers):
 def measured_traffic(workload, mode, kind):
     """HBM bytes per launch of a kernel kind, from the committed summary of
     the two counter runs (FETCH_SIZE, WRITE_SIZE) for this workload and mode,
-    calibrated against the copy probe; None when there is no such summary."""
+    calibrated against the copy probe (its line `traffic_bytes_per_launch
+    <kind>: <bytes>`); None when there is no such figure."""
     import re
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "traffic_%s_%s_%s.md" % (kind, mode, workload))
+                        "traffic_%s_%s.md" % (mode, workload))
     try:
         with open(path) as f:
-            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+            m = re.search(r"^traffic_bytes_per_launch %s:\s*([0-9.eE+]+)" % re.escape(kind), f.read(), re.M)
         return float(m.group(1)) if m else None
     except OSError:
         return None
