@@ -229,25 +229,28 @@ __global__ __launch_bounds__(256, 2) void kmat_dense_kernel(DenseSide rs, DenseS
     }
 }
 
-// Materialised block for at most 16 columns (a shortlist of candidates):
-// one thread per CSR row with every column's dot product in registers, so a
-// narrow block costs one pass over the rows' nonzeros instead of one wave per
-// row with most lanes idle.  A nonzero whose feature no column has (mask 0)
-// adds exact zeros and is skipped, as in csr_row_dot.
+// Materialised block for at most 16 columns (a shortlist of candidates) on
+// selected rows: a group of G lanes per row (G = the column count rounded up
+// to a power of two), lane c of the group column c.  Each lane walks the
+// row's nonzeros in order and adds v x_c where the column has the feature,
+// the same fused multiply-adds in the same order as one thread per row with
+// every column in registers; the group's lanes load the same nonzero (one
+// transaction) and adjacent column-table entries.  A nonzero whose feature
+// no column has (mask 0) adds exact zeros and is skipped, as in csr_row_dot.
 constexpr int NARROW = 16;
 
 __global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
     const double *__restrict__ rnrm, const int32_t *__restrict__ ridx, int nrows,
     const double *__restrict__ XcT, int ldc, int d, const unsigned long long *__restrict__ cmask,
-    const double *__restrict__ cnrm, int ncols, int kernel, double gamma, double *__restrict__ K, int ldk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *__restrict__ cnrm, int ncols, int glog, int kernel, double gamma, double *__restrict__ K,
+    int ldk) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> glog, c = t & ((1 << glog) - 1);
     if (i >= nrows) return;
     const int src = ridx ? ridx[i] : i;
     const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
-    double acc[NARROW];
-#pragma unroll
-    for (int c = 0; c < NARROW; c++) acc[c] = 0.0;
+    double acc = 0.0;
     // four nonzeros per step: their columns, values and masks load together
     for (int64_t p = p0; p < p1; p += 4) {
         int f[4];
@@ -261,19 +264,17 @@ __global__ __launch_bounds__(256) void kmat_csr_narrow_kernel(
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) m[u] = (unsigned)f[u] < (unsigned)d ? (unsigned)cmask[f[u]] : 0u;
+        double x[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (m[u] == 0u) continue;
-            const double *xr = XcT + (size_t)f[u] * ldc;
+        for (int u = 0; u < 4; u++) x[u] = ((m[u] >> c) & 1u) ? XcT[(size_t)f[u] * ldc + c] : 0.0;
 #pragma unroll
-            for (int c = 0; c < NARROW; c++)
-                if ((m[u] >> c) & 1u) acc[c] = fma(v[u], xr[c], acc[c]);
-        }
+        for (int u = 0; u < 4; u++)
+            if ((m[u] >> c) & 1u) acc = fma(v[u], x[u], acc);
     }
-    const double nr = rnrm ? rnrm[src] : 0.0;
-#pragma unroll
-    for (int c = 0; c < NARROW; c++)
-        if (c < ncols) K[(size_t)i * ldk + c] = kfun(kernel, gamma, acc[c], nr, cnrm ? cnrm[c] : 0.0);
+    if (c < ncols) {
+        const double nr = rnrm ? rnrm[src] : 0.0;
+        K[(size_t)i * ldk + c] = kfun(kernel, gamma, acc, nr, cnrm ? cnrm[c] : 0.0);
+    }
 }
 
 // The same product for a block of 256 consecutive rows (no row selection):
@@ -463,7 +464,9 @@ __global__ __launch_bounds__(256) void kmat_csr_kernel(
     const int src = ridx ? ridx[i] : i;
     const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
     const double nr = rnrm ? rnrm[src] : 0.0;
-    for (int c0 = 0; c0 < ncols; c0 += 64) {
+    // grid.y strides over the 64-column chunks (one chunk per wave when the
+    // grid has a y extent per chunk)
+    for (int c0 = blockIdx.y * 64; c0 < ncols; c0 += gridDim.y * 64) {
         const int j = c0 + lane;
         const bool jok = j < ncols;
         const double dot = csr_row_dot(col, val, p0, p1, XcT, ldc, d, cmask + (size_t)(c0 >> 6) * d, j);
@@ -877,9 +880,13 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         cols_T(c, d, &XcT, &cn, &cm);
         const double nz = r->nnz_per_row * r->n;
         TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        if (r->idx)
-            ts.launch(kmat_csr_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col, r->val,
-                      r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
+        if (r->idx) {
+            int glog = 0;
+            while ((1 << glog) < c->n) glog++;
+            const long threads = (long)r->n << glog;
+            ts.launch(kmat_csr_narrow_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, r->rowptr,
+                      r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, glog, kernel, gamma, K, ldk);
+        }
         else
             ts.launch(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col,
                       r->val, r->nrm, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
@@ -889,8 +896,12 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
         cols_T(c, d, &XcT, &cn, &cm);
         const double nz = r->nnz_per_row * r->n;
         TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        ts.launch(kmat_csr_kernel, dim3((r->n + 3) / 4), dim3(256), 0, r->rowptr, r->col, r->val, r->nrm, r->idx,
-                  r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
+        // one wave per (row, 64-column chunk) while the rows alone give few waves
+        const int chunks = (c->n + 63) / 64;
+        const long waves = (long)r->n * chunks;
+        const int gy = r->n >= 8192 ? 1 : (waves <= 65536 ? chunks : (int)((65536 + r->n - 1) / r->n));
+        ts.launch(kmat_csr_kernel, dim3((r->n + 3) / 4, gy > 0 ? gy : 1), dim3(256), 0, r->rowptr, r->col, r->val,
+                  r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
     }
 }
 
