@@ -118,6 +118,26 @@ int  irwls_dev_kernel_matvec_noalloc(const irwls_rows *r, const irwls_rows *c, i
 void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d,
                              int kernel, double gamma, double *d_K, int ldk);
 
+/* A column side of CSR rows that only grows by appending rows (SGMA's
+   centroids): its transposed column block, norms and feature masks live
+   between kernel-block calls and are extended by the appended rows only,
+   instead of being rebuilt for every call.  No counterpart in the reference
+   (which evaluates kernelFunction pairwise); for the budgeted trainer.
+   create returns NULL when cap x d doubles exceed 1 GiB (callers then use
+   irwls_dev_kernel_matrix). */
+typedef struct irwls_coltab irwls_coltab;
+irwls_coltab *irwls_dev_coltab_create(int d, int cap);
+void irwls_dev_coltab_destroy(irwls_coltab *t);
+/* Columns held. */
+int irwls_dev_coltab_size(const irwls_coltab *t);
+/* Appends rows size(t) .. c->n - 1 of the CSR side c (c->idx selecting);
+   its first size(t) rows must be the ones already held. */
+void irwls_dev_coltab_extend(irwls_coltab *t, const irwls_rows *c);
+/* irwls_dev_kernel_matrix(r, c) for the CSR rows r and the held columns:
+   the same values bit for bit. */
+void irwls_dev_kernel_matrix_coltab(const irwls_rows *r, const irwls_coltab *t, int kernel, double gamma,
+                                    double *d_K, int ldk);
+
 /* ---- full-training working-set helpers ---------------------------------- */
 /* H (column-major N x N, leading dimension ldh, N = nS1+1):
    H[i][j] = Kw[s_i][s_j] y_i y_j + (i==j)/a_i, border y, corner 0.

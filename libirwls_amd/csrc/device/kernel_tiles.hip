@@ -572,6 +572,26 @@ __global__ __launch_bounds__(256) void cols_T_csr_chunk(const int64_t *__restric
     for (int f = t; f < d; f += 256) mask[(size_t)blockIdx.x * d + f] = m[f];
 }
 
+// Appends columns j0 .. j0 + gridDim.x - 1 (CSR rows idx[j]) to a
+// transposed block of leading dimension ldc: entries, mask bits (ORed into
+// the chunk's existing masks) and norms.  One 64-thread work-group per column.
+__global__ void cols_append_csr(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                const double *__restrict__ val, const int32_t *__restrict__ idx,
+                                const double *__restrict__ nrm, int j0, int d, int ldc, double *__restrict__ XcT,
+                                double *__restrict__ cn, unsigned long long *__restrict__ mask) {
+    const int j = j0 + blockIdx.x;
+    const int src = idx ? idx[j] : j;
+    const int64_t p0 = rowptr[src], p1 = rowptr[src + 1];
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int f = col[p];
+        if ((unsigned)f < (unsigned)d) {
+            XcT[(size_t)f * ldc + j] = val[p];
+            atomicOr(&mask[(size_t)(j >> 6) * d + f], 1ull << (j & 63));
+        }
+    }
+    if (threadIdx.x == 0) cn[j] = nrm ? nrm[src] : 0.0;
+}
+
 __global__ void row_norms_kernel(const double *__restrict__ X, int ldx, int n, int d, double *__restrict__ nrm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -730,6 +750,33 @@ static int csr_col_chunk(int d) {
     return c > (size_t)INT32_MAX ? INT32_MAX : (int)c;
 }
 
+// CSR rows x a transposed column block (XcT with leading dimension ldc,
+// norms, feature masks) of ncols columns: the materialised kernel block.
+static void csr_block(const irwls_rows *r, const double *XcT, int ldc, const double *cn,
+                      const unsigned long long *cm, int ncols, int d, int kernel, double gamma, double *K, int ldk) {
+    const double nz = r->nnz_per_row * r->n;
+    TimerScope ts(IRWLS_T_CSR, 2.0 * nz * ncols, 12.0 * nz + 8.0 * ncols * (double)d + 8.0 * r->n * (2.0 + ncols));
+    if (ncols <= NARROW) {
+        if (r->idx) {
+            int glog = 0;
+            while ((1 << glog) < ncols) glog++;
+            const long threads = (long)r->n << glog;
+            ts.launch(kmat_csr_narrow_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, r->rowptr,
+                      r->col, r->val, r->nrm, r->idx, r->n, XcT, ldc, d, cm, cn, ncols, glog, kernel, gamma, K, ldk);
+        } else {
+            ts.launch(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col,
+                      r->val, r->nrm, r->n, XcT, ldc, d, cm, cn, ncols, kernel, gamma, K, ldk);
+        }
+        return;
+    }
+    // one wave per (row, 64-column chunk) while the rows alone give few waves
+    const int chunks = (ncols + 63) / 64;
+    const long waves = (long)r->n * chunks;
+    const int gy = r->n >= 8192 ? 1 : (waves <= 65536 ? chunks : (int)((65536 + r->n - 1) / r->n));
+    ts.launch(kmat_csr_kernel, dim3((r->n + 3) / 4, gy > 0 ? gy : 1), dim3(256), 0, r->rowptr, r->col, r->val, r->nrm,
+              r->idx, r->n, XcT, ldc, d, cm, cn, ncols, kernel, gamma, K, ldk);
+}
+
 // Columns [j0, j0 + cn) of a column side, as a side of its own.
 static irwls_rows cols_slice(const irwls_rows *c, int j0, int cn) {
     irwls_rows s = *c;
@@ -874,35 +921,66 @@ void irwls_dev_kernel_matrix(const irwls_rows *r, const irwls_rows *c, int d, in
             const irwls_rows cs = cols_slice(c, j0, cnk);
             irwls_dev_kernel_matrix(r, &cs, d, kernel, gamma, K + j0, ldk);
         }
-    } else if (c->n <= NARROW) {
-        double *XcT, *cn;
-        unsigned long long *cm;
-        cols_T(c, d, &XcT, &cn, &cm);
-        const double nz = r->nnz_per_row * r->n;
-        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        if (r->idx) {
-            int glog = 0;
-            while ((1 << glog) < c->n) glog++;
-            const long threads = (long)r->n << glog;
-            ts.launch(kmat_csr_narrow_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, r->rowptr,
-                      r->col, r->val, r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, glog, kernel, gamma, K, ldk);
-        }
-        else
-            ts.launch(kmat_csr_block_narrow_kernel, dim3((r->n + 255) / 256), dim3(256), 0, r->rowptr, r->col,
-                      r->val, r->nrm, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
     } else {
         double *XcT, *cn;
         unsigned long long *cm;
         cols_T(c, d, &XcT, &cn, &cm);
-        const double nz = r->nnz_per_row * r->n;
-        TimerScope ts(IRWLS_T_CSR, 2.0 * nz * c->n, 12.0 * nz + 8.0 * c->n * (double)d + 8.0 * r->n * (2.0 + c->n));
-        // one wave per (row, 64-column chunk) while the rows alone give few waves
-        const int chunks = (c->n + 63) / 64;
-        const long waves = (long)r->n * chunks;
-        const int gy = r->n >= 8192 ? 1 : (waves <= 65536 ? chunks : (int)((65536 + r->n - 1) / r->n));
-        ts.launch(kmat_csr_kernel, dim3((r->n + 3) / 4, gy > 0 ? gy : 1), dim3(256), 0, r->rowptr, r->col, r->val,
-                  r->nrm, r->idx, r->n, XcT, c->n, d, cm, cn, c->n, kernel, gamma, K, ldk);
+        csr_block(r, XcT, c->n, cn, cm, c->n, d, kernel, gamma, K, ldk);
     }
+}
+
+// ---- growing CSR column sides ----------------------------------------------
+struct irwls_coltab {
+    double *XcT;                // d x cap, entry (f, j) at f * cap + j (set where the mask bit is)
+    double *cn;                 // cap norms
+    unsigned long long *mask;   // (cap / 64 chunks) x d feature masks
+    int d, cap, n;
+};
+
+irwls_coltab *irwls_dev_coltab_create(int d, int cap) {
+    if (d <= 0 || cap <= 0 || (size_t)cap * d * sizeof(double) > kCsrColBudget) return nullptr;
+    irwls_coltab *t = (irwls_coltab *)calloc(1, sizeof(irwls_coltab));
+    if (!t) return nullptr;
+    t->d = d;
+    t->cap = cap;
+    t->n = 0;
+    const size_t nchunk = (size_t)(cap + 63) / 64;
+    t->XcT = (double *)irwls_dev_alloc((size_t)cap * d * sizeof(double));
+    t->cn = (double *)irwls_dev_alloc((size_t)cap * sizeof(double));
+    t->mask = (unsigned long long *)irwls_dev_alloc(nchunk * d * sizeof(unsigned long long));   // zero-filled
+    return t;
+}
+
+void irwls_dev_coltab_destroy(irwls_coltab *t) {
+    if (!t) return;
+    irwls_dev_free(t->XcT);
+    irwls_dev_free(t->cn);
+    irwls_dev_free(t->mask);
+    free(t);
+}
+
+int irwls_dev_coltab_size(const irwls_coltab *t) { return t ? t->n : 0; }
+
+void irwls_dev_coltab_extend(irwls_coltab *t, const irwls_rows *c) {
+    if (c->X || c->n > t->cap) {
+        fprintf(stderr, "libirwls: irwls_dev_coltab_extend needs CSR rows, at most %d (got %d)\n", t->cap, c->n);
+        exit(2);
+    }
+    if (c->n <= t->n) return;
+    hipLaunchKernelGGL(cols_append_csr, dim3(c->n - t->n), dim3(64), 0, stream(), c->rowptr, c->col, c->val, c->idx,
+                       c->nrm, t->n, t->d, t->cap, t->XcT, t->cn, t->mask);
+    IRWLS_LAUNCH_CHECK();
+    t->n = c->n;
+}
+
+void irwls_dev_kernel_matrix_coltab(const irwls_rows *r, const irwls_coltab *t, int kernel, double gamma, double *K,
+                                    int ldk) {
+    if (r->n <= 0 || t->n <= 0) return;
+    if (r->X) {
+        fprintf(stderr, "libirwls: irwls_dev_kernel_matrix_coltab needs CSR rows\n");
+        exit(2);
+    }
+    csr_block(r, t->XcT, t->cap, t->cn, t->mask, t->n, t->d, kernel, gamma, K, ldk);
 }
 
 }  // extern "C"

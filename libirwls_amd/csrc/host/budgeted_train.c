@@ -53,15 +53,32 @@ static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b
    (n x nc row-major), KNC(c, e) = k(cand_c, centroid_e) (ld B),
    Z = L^-T L^-1 KNC^T (ld B), eta_c = 1 - KNC_c . Z_c and
    q_c = |KSM(:, c) - KSC Z_c|^2 (src/budgeted-train.c:176-203). */
+/* K(rows, centroids 0..s-1) with leading dimension B: from the kept column
+   table of the centroids (CSR data), else built for the call. */
+static void kmat_centroids(irwls_devdata *dd, properties props, const irwls_rows *rows, const int32_t *d_cent,
+                           int s, irwls_coltab *ctab, double *d_K, int B) {
+    if (ctab) {
+        if (irwls_dev_coltab_size(ctab) != s) {
+            fprintf(stderr, "libirwls: SGMA centroid table holds %d columns at step %d\n",
+                    irwls_dev_coltab_size(ctab), s);
+            exit(2);
+        }
+        irwls_dev_kernel_matrix_coltab(rows, ctab, props.kernelType, props.Kgamma, d_K, B);
+        return;
+    }
+    irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
+    irwls_dev_kernel_matrix(rows, &ce, dd->d, props.kernelType, props.Kgamma, d_K, B);
+}
+
 static void score_exact(irwls_devdata *dd, properties props, const int32_t *d_rows, int n, int B, int s, int nc,
-                        const int32_t *d_c, const int32_t *d_cent, const double *d_Li, const double *d_KSC,
-                        double *d_KSM, double *d_KNC, double *d_W, double *d_Z, double *d_eta, double *d_q) {
+                        const int32_t *d_c, const int32_t *d_cent, irwls_coltab *ctab, const double *d_Li,
+                        const double *d_KSC, double *d_KSM, double *d_KNC, double *d_W, double *d_Z, double *d_eta,
+                        double *d_q) {
     /* rows: all n rows (d_rows NULL) or the n rows listed; KSC has leading dimension n */
     irwls_rows all = irwls_devdata_rows(dd, d_rows, n);
     irwls_rows cr = irwls_devdata_rows(dd, d_c, nc);
-    irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
     irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, nc);
-    irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
+    kmat_centroids(dd, props, &cr, d_cent, s, ctab, d_KNC, B);
     irwls_dev_gemm(0, 0, s, nc, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
     irwls_dev_gemm(1, 0, s, nc, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
     irwls_dev_sgma_eta(s, nc, d_KNC, B, d_Z, B, d_eta);
@@ -167,6 +184,8 @@ int *SGMA(svm_dataset dataset, properties props) {
         d_KSCe = (double *)irwls_dev_alloc((size_t)me * Bz * sizeof(double));
         d_KSMe = (double *)irwls_dev_alloc((size_t)me * (kmax > 0 ? kmax : 1) * sizeof(double));
     }
+    /* the centroids' transposed column block, kept and grown by one column per step (CSR data) */
+    irwls_coltab *ctab = (!dd->dense && B > 0) ? irwls_dev_coltab_create(dd->d, B) : NULL;
     int full_col = 1;   /* the step's centroid has its full KSC column */
     const double *knc = NULL;       /* device row of k(new centroid, previous centroids) */
     int npos = 0;
@@ -184,11 +203,11 @@ int *SGMA(svm_dataset dataset, properties props) {
                 }
                 irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
                 if (me)
-                    score_exact(dd, props, d_sube, me, B, s, ncand, d_cidx, d_cent, d_Li, d_KSCe, d_KSMe, d_KNC, d_W,
-                                d_Z, d_eta, d_q);
+                    score_exact(dd, props, d_sube, me, B, s, ncand, d_cidx, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
+                                d_KNC, d_W, d_Z, d_eta, d_q);
                 else
-                    score_exact(dd, props, NULL, n, B, s, ncand, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W,
-                                d_Z, d_eta, d_q);
+                    score_exact(dd, props, NULL, n, B, s, ncand, d_cidx, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
+                                d_W, d_Z, d_eta, d_q);
                 irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
                 const int best = best_gain(ncand, q, eta, &value);
                 cent[s] = cidx[best];
@@ -216,8 +235,7 @@ int *SGMA(svm_dataset dataset, properties props) {
                 }
                 irwls_dev_h2d(d_cidx, cidx, (size_t)pool * sizeof(int32_t));
                 irwls_rows cr = irwls_devdata_rows(dd, d_cidx, pool);
-                irwls_rows ce = irwls_devdata_rows(dd, d_cent, s);
-                irwls_dev_kernel_matrix(&cr, &ce, dd->d, props.kernelType, props.Kgamma, d_KNC, B);
+                kmat_centroids(dd, props, &cr, d_cent, s, ctab, d_KNC, B);
                 irwls_dev_gemm(0, 0, s, pool, s, 1.0, d_Li, B, d_KNC, B, 0.0, d_W, B);
                 irwls_dev_gemm(1, 0, s, pool, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
                 irwls_dev_sgma_eta(s, pool, d_KNC, B, d_Z, B, d_eta);
@@ -243,11 +261,11 @@ int *SGMA(svm_dataset dataset, properties props) {
                 if (keep > 0) {
                     irwls_dev_h2d(d_cidx, cidx2, (size_t)keep * sizeof(int32_t));
                     if (me)
-                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx, d_cent, d_Li, d_KSCe, d_KSMe, d_KNC,
-                                    d_W, d_Z, d_eta, d_q);
+                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
+                                    d_KNC, d_W, d_Z, d_eta, d_q);
                     else
-                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx, d_cent, d_Li, d_KSC, d_KSM, d_KNC, d_W,
-                                    d_Z, d_eta, d_q);
+                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
+                                    d_W, d_Z, d_eta, d_q);
                     irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
                     const int best = best_gain(keep, q, eta, &value);
                     cent[s] = cidx2[best];
@@ -286,6 +304,10 @@ int *SGMA(svm_dataset dataset, properties props) {
         if (cent[s] == n || (cent[s] < n && dataset.y[cent[s]] > 0)) npos++;
         int32_t cs = cent[s];
         irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
+        if (ctab) {
+            irwls_rows ce = irwls_devdata_rows(dd, d_cent, s + 1);
+            irwls_dev_coltab_extend(ctab, &ce);
+        }
         if (s <= 1) {
             irwls_rows cc = irwls_devdata_rows(dd, d_cent + s, 1);
             if (me) {
@@ -333,6 +355,7 @@ int *SGMA(svm_dataset dataset, properties props) {
             irwls_dev_scatter_row(d_Li, B, s, s + 1, d_v2);
         }
     }
+    irwls_dev_coltab_destroy(ctab);
     irwls_dev_free(d_KSC);
     irwls_dev_free(d_L);
     irwls_dev_free(d_Li);

@@ -188,3 +188,43 @@ def test_csr_kernel_high_dimensional_column_chunks(with_idx):
     wd, od, rsd = Dev(w), Dev(out0), Dev(rs)
     L.irwls_dev_kernel_matvec(C.byref(rows), C.byref(cols), d, 1, g, wd.p, 0.5, -2.0, 0.125, rsd.p, od.p)
     np.testing.assert_allclose(od.get(), 0.5 * out0 + rs * (-2.0 * (Kw @ w)) + 0.125, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_growing_column_table_equals_kernel_matrix(kind):
+    """The kept centroid table, grown a few columns at a time across the
+    narrow (at most 16) and wide (several 64-column chunks) kernels, gives
+    the same kernel blocks bit for bit as the block built for each call, on
+    all rows and on a row selection."""
+    rng = np.random.default_rng(77 + kind)
+    n, d, g = 900, 300, 0.25
+    rp, col, val = synth.sparse_uniform(n, d, 0.05, rng)
+    xn = np.array([np.sum(val[rp[i]:rp[i + 1]] ** 2) for i in range(n)])
+    L = lib.load()
+    vp = C.c_void_p
+    L.irwls_dev_coltab_create.restype = vp
+    L.irwls_dev_coltab_create.argtypes = [C.c_int, C.c_int]
+    L.irwls_dev_coltab_destroy.argtypes = [vp]
+    L.irwls_dev_coltab_size.argtypes = [vp]
+    L.irwls_dev_coltab_extend.argtypes = [vp, C.POINTER(lib.irwls_rows)]
+    L.irwls_dev_kernel_matrix_coltab.argtypes = [C.POINTER(lib.irwls_rows), vp, C.c_int, C.c_double, vp, C.c_int]
+    cap = 160
+    cidx = rng.choice(n, 150, replace=False).astype(np.int32)
+    rows, k1 = _csr_rows(rp, col, val, xn)
+    ridx = rng.choice(n, 211, replace=False).astype(np.int32)
+    rsel, k2 = _csr_rows(rp, col, val, xn, ridx)
+    cols, k3 = _csr_rows(rp, col, val, xn, cidx)
+    t = L.irwls_dev_coltab_create(d, cap)
+    assert t
+    try:
+        for m in [1, 5, 16, 17, 64, 65, 129, 150]:
+            cols.n = m
+            L.irwls_dev_coltab_extend(t, C.byref(cols))
+            assert L.irwls_dev_coltab_size(t) == m
+            for r, nr in ((rows, n), (rsel, len(ridx))):
+                want, got = Dev(np.zeros((nr, cap))), Dev(np.zeros((nr, cap)))
+                L.irwls_dev_kernel_matrix(C.byref(r), C.byref(cols), d, kind, g, want.p, cap)
+                L.irwls_dev_kernel_matrix_coltab(C.byref(r), t, kind, g, got.p, cap)
+                assert np.array_equal(got.get()[:, :m], want.get()[:, :m]), (m, nr)
+    finally:
+        L.irwls_dev_coltab_destroy(t)
