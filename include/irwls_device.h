@@ -287,6 +287,13 @@ void irwls_dev_sgma_eta(int s, int ncand, const double *d_KNC, int ldk, const do
 void irwls_dev_gather_col(const double *d_src, int ld, int col, int n, double *d_dst);
 /* M[c*ld + row] = v[c], c < count */
 void irwls_dev_scatter_row(double *d_M, int ld, int row, int count, const double *d_v);
+/* SGMA's factor growth (src/budgeted-train.c:247-279) on the device: with
+   L2 = L^-1 k (s entries), IL2 = L2^T L^-1 (s entries) and kcc = k(c, c) in
+   device memory, L3 = sqrt(kcc + 1e-5 - sum_i L2_i^2) (index order, no fused
+   multiply-add), IL3 = 1/L3; writes row s of the column-major L and L^-1
+   (leading dimension ld): (L2, L3) and (-IL3 IL2, IL3).  No host round trip. */
+void irwls_dev_sgma_lrow(int s, int ld, const double *d_kcc, const double *d_L2, const double *d_IL2, double *d_L,
+                         double *d_Li);
 /* K1 (ld ldk) = KC + sum_q (sw_q Phi[a_q])^T (sw_q Phi[a_q]) over the nact
    active rows a (NULL: rows 0..nact-1) of the row-major n x B matrix Phi;
    sw NULL means weight 1. */

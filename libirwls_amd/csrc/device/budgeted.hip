@@ -157,34 +157,36 @@ __device__ __forceinline__ void gain_load_a(double a[4][2], const double *__rest
     }
 }
 
+template <int TC>
 __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
                                                         const double *__restrict__ Z, int ldz,
                                                         const double *__restrict__ KSM, int ldm,
                                                         double *__restrict__ part) {
-    __shared__ double Zs[2][GK][T];
-    __shared__ double red[4][T];
+    __shared__ double Zs[2][GK][TC];
+    __shared__ double red[4][TC];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, kq = lane >> 4;
     const int r0 = blockIdx.x * GR + wave * 32;
-    const int c0 = blockIdx.y * T;
+    const int c0 = blockIdx.y * TC;
     const int ra0 = r0 + li;
     const bool ok0 = ra0 < n, ok1 = ra0 + 16 < n;
-    // Z staging: thread t carries candidate c0 + t/4, k = 4 (t % 4) .. +3 of a chunk
-    const int zc = threadIdx.x >> 2, zk = (threadIdx.x & 3) * 4;
+    // Z staging: thread t carries candidate c0 + t / (GK / ZK), k = ZK (t % (GK / ZK)) .. +ZK-1 of a chunk
+    constexpr int ZK = GK * TC / 256, NB = TC / 16;
+    const int zc = threadIdx.x / (GK / ZK), zk = (threadIdx.x % (GK / ZK)) * ZK;
     const bool zok = c0 + zc < ncand;
     const double *zp = Z + (size_t)(zok ? c0 + zc : 0) * ldz;
-    double z[4];
+    double z[ZK];
 #pragma unroll
-    for (int q = 0; q < 4; q++) z[q] = (zok && zk + q < s) ? zp[zk + q] : 0.0;
+    for (int q = 0; q < ZK; q++) z[q] = (zok && zk + q < s) ? zp[zk + q] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) Zs[0][zk + q][zc] = z[q];
+    for (int q = 0; q < ZK; q++) Zs[0][zk + q][zc] = z[q];
     double a[4][2], an[4][2];
     gain_load_a(a, KSC, n, s, 0, ra0, ok0, ok1);
-    v4d acc[2][4];
+    v4d acc[2][NB];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++)
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
     __syncthreads();
     int buf = 0;
     for (int kc = 0; kc < s; kc += GK) {
@@ -192,25 +194,25 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
         if (more) {
             gain_load_a(an, KSC, n, s, kc + GK, ra0, ok0, ok1);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < ZK; q++) {
                 const int k = kc + GK + zk + q;
                 z[q] = (zok && k < s) ? zp[k] : 0.0;
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            double bv[4];
+            double bv[NB];
 #pragma unroll
-            for (int nb = 0; nb < 4; nb++) bv[nb] = Zs[buf][4 * u + kq][nb * 16 + li];
+            for (int nb = 0; nb < NB; nb++) bv[nb] = Zs[buf][4 * u + kq][nb * 16 + li];
 #pragma unroll
             for (int mb = 0; mb < 2; mb++)
 #pragma unroll
-                for (int nb = 0; nb < 4; nb++)
+                for (int nb = 0; nb < NB; nb++)
                     acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][mb], bv[nb], acc[mb][nb], 0, 0, 0);
         }
         if (more) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) Zs[buf ^ 1][zk + q][zc] = z[q];
+            for (int q = 0; q < ZK; q++) Zs[buf ^ 1][zk + q][zc] = z[q];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 a[u][0] = an[u][0];
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
     }
     // accumulator register r of lane l: (row l/16 + 4r, candidate l%16) of its block
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
+    for (int nb = 0; nb < NB; nb++) {
         const int c = c0 + nb * 16 + li;
         double q = 0.0;
 #pragma unroll
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
         if (lane < 16) red[wave][nb * 16 + lane] = q;
     }
     __syncthreads();
-    if (threadIdx.x < T) {
+    if (threadIdx.x < TC) {
         const int c = c0 + threadIdx.x;
         if (c < ncand)
             part[(size_t)blockIdx.x * ncand + c] =
@@ -367,6 +369,31 @@ __global__ void gather_col_kernel(const double *__restrict__ src, int ld, int co
 __global__ void scatter_row_kernel(double *__restrict__ M, int ld, int row, int count, const double *__restrict__ v) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < count) M[(size_t)c * ld + row] = v[c];
+}
+
+// Row s of L and L^-1 of K_C + jitter (src/budgeted-train.c:247-279) from
+// L2 = L^-1 k and IL2 = L2^T L^-1: one thread forms
+// L3 = sqrt(kcc + 1e-5 - sum_i L2_i^2) with the sum taken in index order and
+// every product and difference rounded on its own (no fused multiply-add),
+// and IL3 = 1 / L3; then L(s, 0..s) = (L2, L3), L^-1(s, 0..s) = (-IL3 IL2, IL3).
+__global__ __launch_bounds__(256) void sgma_lrow_kernel(int s, int ld, const double *__restrict__ kcc,
+                                                        const double *__restrict__ L2,
+                                                        const double *__restrict__ IL2, double *__restrict__ L,
+                                                        double *__restrict__ Li) {
+    __shared__ double sh[2];
+    if (threadIdx.x == 0) {
+        double L3 = kcc[0] + 0.00001;
+        for (int i = 0; i < s; i++) L3 = __dsub_rn(L3, __dmul_rn(L2[i], L2[i]));
+        L3 = sqrt(L3);
+        sh[0] = L3;
+        sh[1] = 1.0 / L3;
+    }
+    __syncthreads();
+    const double L3 = sh[0], IL3 = sh[1];
+    for (int c = threadIdx.x; c <= s; c += blockDim.x) {
+        L[(size_t)c * ld + s] = (c < s) ? L2[c] : L3;
+        Li[(size_t)c * ld + s] = (c < s) ? __dmul_rn(-IL3, IL2[c]) : IL3;
+    }
 }
 
 // ----------------------------------------------------------------- wgram ---
@@ -654,8 +681,11 @@ void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const doubl
                       8.0 * (double)n * s + 8.0 * (double)n * ncand);
         if (ncand <= 16)
             ts.launch(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
-        else
-            ts.launch(sgma_gain_kernel, dim3(rt, ct), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
+        else if ((long)rt * ct >= 2 * device_cus())
+            ts.launch(sgma_gain_kernel<T>, dim3(rt, ct), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
+        else   // few row tiles: 32-candidate tiles, twice the work-groups, the same sums in the same order
+            ts.launch(sgma_gain_kernel<32>, dim3(rt, (ncand + 31) / 32), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM,
+                      ldm, part);
     }
     hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, rt, ncand, qout);
     IRWLS_LAUNCH_CHECK();
@@ -689,6 +719,13 @@ void irwls_dev_gather_col(const double *src, int ld, int col, int n, double *dst
 void irwls_dev_scatter_row(double *M, int ld, int row, int count, const double *v) {
     if (count <= 0) return;
     hipLaunchKernelGGL(scatter_row_kernel, dim3((count + 255) / 256), dim3(256), 0, stream(), M, ld, row, count, v);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_lrow(int s, int ld, const double *kcc, const double *L2, const double *IL2, double *L,
+                         double *Li) {
+    if (s < 0) return;
+    hipLaunchKernelGGL(sgma_lrow_kernel, dim3(1), dim3(256), 0, stream(), s, ld, kcc, L2, IL2, L, Li);
     IRWLS_LAUNCH_CHECK();
 }
 

@@ -37,13 +37,18 @@ int *randomCentroids(svm_dataset dataset, properties props) {
     return c;
 }
 
-/* k(x_a, x_b) of two device rows by a 1x1 kernel block */
-static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b, properties props, double *d_out) {
+/* k(x_a, x_b) of two device rows by a 1x1 kernel block, left in d_out */
+static void kernel_pair_dev(const irwls_devdata *dd, int32_t *d_pair, int a, int b, properties props, double *d_out) {
     int32_t h[2] = {a, b};
     irwls_dev_h2d(d_pair, h, sizeof(h));
     irwls_rows r = irwls_devdata_rows(dd, d_pair, 1);
     irwls_rows c = irwls_devdata_rows(dd, d_pair + 1, 1);
     irwls_dev_kernel_matrix(&r, &c, dd->d, props.kernelType, props.Kgamma, d_out, 1);
+}
+
+/* the same, read back */
+static double kernel_pair(const irwls_devdata *dd, int32_t *d_pair, int a, int b, properties props, double *d_out) {
+    kernel_pair_dev(dd, d_pair, a, b, props, d_out);
     double v = 0.0;
     irwls_dev_d2h(&v, d_out, sizeof(double));
     return v;
@@ -136,10 +141,6 @@ int *SGMA(svm_dataset dataset, properties props) {
     double *eta = eq, *q = eq + cmax;
     float *score = (float *)irwls_xmalloc((size_t)cmax * sizeof(float));
     char *taken = (char *)irwls_xmalloc((size_t)cmax);
-    /* L2 and IL2 likewise */
-    double *lv = (double *)irwls_xcalloc(2 * ((size_t)(B > 0 ? B : 1) + 1), sizeof(double));
-    double *L2 = lv, *IL2 = lv + (B > 0 ? B : 1) + 1;
-    double *row = (double *)irwls_xcalloc((size_t)(B > 0 ? B : 1) + 1, sizeof(double));
     const size_t Bz = (size_t)(B > 0 ? B : 1);
     const int kmax = ncand > keep ? ncand : keep;   /* exact-scored candidates per step */
     /* learned proposal with proposal_rows = m < n: every gain of the run (the
@@ -161,7 +162,7 @@ int *SGMA(svm_dataset dataset, properties props) {
     double *d_tmp = (double *)irwls_dev_alloc(64 * sizeof(double));
     int32_t *d_cidx = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
     int32_t *d_cent = (int32_t *)irwls_dev_alloc(Bz * sizeof(int32_t));
-    int32_t *d_pair = (int32_t *)irwls_dev_alloc(2 * sizeof(int32_t));
+    int32_t *d_pair = (int32_t *)irwls_dev_alloc(4 * sizeof(int32_t));
     irwls_rows all = irwls_devdata_rows(dd, NULL, n);
     irwls_proposal *lp = learned ? irwls_proposal_begin(n, B, cmax, warm * ncand) : NULL;
     int32_t *d_sube = NULL;
@@ -325,9 +326,10 @@ int *SGMA(svm_dataset dataset, properties props) {
             irwls_proposal_add_centroid_rows(lp, dd, props, s, d_cent + s);
         }
         full_col = 1;
-        /* grow L and L^-1 of K_C + jitter by one row (src/budgeted-train.c:247-279) */
-        const double kcc = kernel_pair(dd, d_pair, cent[s], cent[s], props, d_tmp);
+        /* grow L and L^-1 of K_C + jitter by one row (src/budgeted-train.c:247-279),
+           on the device: L3 and the new rows never come back to the host */
         if (s == 0) {
+            const double kcc = kernel_pair(dd, d_pair, cent[s], cent[s], props, d_tmp);
             const double l0 = sqrt(kcc + 0.000001);
             const double il0 = 1.0 / l0;
             irwls_dev_h2d(d_L, &l0, sizeof(double));
@@ -338,21 +340,10 @@ int *SGMA(svm_dataset dataset, properties props) {
                 irwls_dev_h2d(d_tmp, &k01, sizeof(double));
                 knc = d_tmp;
             }
+            kernel_pair_dev(dd, d_pair + 2, cent[s], cent[s], props, d_tmp + 8);   /* kcc */
             irwls_dev_gemv(d_Li, s, s, B, 0, knc, d_v1);        /* L2 = L^-1 k */
             irwls_dev_gemv(d_Li, s, s, B, 1, d_v1, d_v2);       /* IL2 = L2^T L^-1 */
-            irwls_dev_d2h(lv, d_lv, (Bz + 1 + (size_t)s) * sizeof(double));
-            double L3 = kcc + 0.00001;
-            for (int i = 0; i < s; i++) L3 = L3 - (L2[i] * L2[i]);
-            L3 = sqrt(L3);
-            const double IL3 = 1.0 / L3;
-            memcpy(row, L2, (size_t)s * sizeof(double));
-            row[s] = L3;
-            irwls_dev_h2d(d_v1, row, (size_t)(s + 1) * sizeof(double));
-            irwls_dev_scatter_row(d_L, B, s, s + 1, d_v1);
-            for (int i = 0; i < s; i++) row[i] = -IL3 * IL2[i];
-            row[s] = IL3;
-            irwls_dev_h2d(d_v2, row, (size_t)(s + 1) * sizeof(double));
-            irwls_dev_scatter_row(d_Li, B, s, s + 1, d_v2);
+            irwls_dev_sgma_lrow(s, B, d_tmp + 8, d_v1, d_v2, d_L, d_Li);
         }
     }
     irwls_dev_coltab_destroy(ctab);
@@ -381,8 +372,6 @@ int *SGMA(svm_dataset dataset, properties props) {
     free(score);
     free(taken);
     free(eq);
-    free(lv);
-    free(row);
     g_bstats.t_total = irwls_now() - t_start;
     irwls_stats_set(&g_bstats);
     return cent;

@@ -2,6 +2,8 @@
 prediction) against the oracle: identical SGMA index sequences with the
 reference's 64 candidates, IRWLSpar weights within 1e-6 relative, identical
 predicted labels."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -168,3 +170,34 @@ def test_irwlspar_m_decay_path(seed, decays):
     rel = np.linalg.norm(got - want) / np.linalg.norm(want)
     print("relative weight error %.3e" % rel)
     assert rel < 1e-6
+
+
+@pytest.mark.parametrize("s", [1, 2, 37, 300])
+def test_sgma_factor_row_on_device(s):
+    """irwls_dev_sgma_lrow: row s of L and L^-1 equal, bit for bit, the
+    reference's host arithmetic (src/budgeted-train.c:247-279) restated in
+    Python floats: L3 = sqrt(kcc + 1e-5 - sum L2_i^2) summed in index order,
+    IL3 = 1/L3, L^-1 row = -IL3 * IL2."""
+    import math
+    from gpu_util import Dev
+    rng = np.random.default_rng(500 + s)
+    B = s + 5
+    L2 = rng.standard_normal(s) * (0.5 / np.sqrt(s))   # sum of squares about 0.25 < kcc
+    IL2 = rng.standard_normal(s)
+    kcc = 1.0 + float(rng.random()) * 1e-3
+    Ld, Lid = Dev(np.full((B, B), 7.0)), Dev(np.full((B, B), 7.0))
+    kd, l2d, il2d = Dev(np.array([kcc])), Dev(L2), Dev(IL2)
+    Lb = lib.load()
+    vp = C.c_void_p
+    Lb.irwls_dev_sgma_lrow.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    Lb.irwls_dev_sgma_lrow(s, B, kd.p, l2d.p, il2d.p, Ld.p, Lid.p)
+    L3 = kcc + 0.00001
+    for i in range(s):
+        L3 = L3 - float(L2[i]) * float(L2[i])
+    L3 = math.sqrt(L3)
+    IL3 = 1.0 / L3
+    Lg = Ld.get().reshape(B, B)       # column-major B x B: element (r, c) at [c, r]
+    Lig = Lid.get().reshape(B, B)
+    assert np.array_equal(Lg[:s, s], L2) and Lg[s, s] == L3
+    assert np.array_equal(Lig[:s, s], np.array([-IL3 * float(v) for v in IL2])) and Lig[s, s] == IL3
+    assert np.all(Lg[s + 1:, s] == 7.0) and np.all(Lg[:, :s] == 7.0) and np.all(Lg[:, s + 1:] == 7.0)
