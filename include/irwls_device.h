@@ -181,10 +181,12 @@ void irwls_dev_mlp_forward_f32(const float *d_P, const float *d_X, int nb, float
 /* BF16 inference: pack the weight matrices once (irwls_mlp_npacked() uint16), then score. */
 void irwls_dev_mlp_pack_bf16(const float *d_P, uint16_t *d_Q);
 void irwls_dev_mlp_infer_bf16(const uint16_t *d_Q, const float *d_P, const float *d_X, int nb, float *d_Y);
-/* 32 features per candidate of a pool of P <= 1024 into F [P][32]. */
+/* 32 features per candidate of a pool of P <= 1024 into F [P][32].  The
+   positive fraction of the basis is pos_frac, or (float)*d_npos / (float)s
+   when d_npos (a device count of positive centroids) is given. */
 void irwls_dev_sgma_features(int P, int s, int B, const double *d_KNC, int ldk, const double *d_eta,
                              const double *d_gsub, const double *d_gsub2, const double *d_ycand,
-                             const double *d_qvcand, float pos_frac, float *d_F);
+                             const double *d_qvcand, float pos_frac, const int32_t *d_npos, float *d_F);
 /* dst[j] = src[idx[j]], j < m */
 void irwls_dev_gather_idx(const double *d_src, const int32_t *d_idx, int m, double *d_dst);
 
@@ -294,6 +296,21 @@ void irwls_dev_scatter_row(double *d_M, int ld, int row, int count, const double
    (leading dimension ld): (L2, L3) and (-IL3 IL2, IL3).  No host round trip. */
 void irwls_dev_sgma_lrow(int s, int ld, const double *d_kcc, const double *d_L2, const double *d_IL2, double *d_L,
                          double *d_Li);
+/* SGMA's choice of the step (src/budgeted-train.c:208-216) on the device:
+   the first maximum of g_i = (eta_i > 0 ? (1/eta_i) q_i : 0) over nc
+   candidates; st[0] = its position, *d_value = g, cent[s] = cand[position],
+   and st[1] counts the chosen rows of label > 0 (row n, the positive class
+   mean, counts; labels y on the device). */
+void irwls_dev_sgma_pick(int nc, const double *d_q, const double *d_eta, const int32_t *d_cand, int s, int n,
+                         const double *d_y, int32_t *d_cent, int32_t *d_st, double *d_value);
+/* dst[i] = src[i * ld + *d_col], i < n (the column a device index names). */
+void irwls_dev_gather_col_at(const double *d_src, int ld, const int32_t *d_col, int n, double *d_dst);
+/* dst[j] = src[*d_row * ld + j], j < count. */
+void irwls_dev_copy_row_at(const double *d_src, int ld, const int32_t *d_row, int count, double *d_dst);
+/* The k highest of P <= 1024 scores, ties to the earlier index, one at a
+   time as a serial scan with strict > picks them (a NaN in the first
+   untaken place is picked, NaNs elsewhere never): out[r] = cidx[pick r]. */
+void irwls_dev_sgma_topk(int P, const float *d_score, int k, const int32_t *d_cidx, int32_t *d_out);
 /* K1 (ld ldk) = KC + sum_q (sw_q Phi[a_q])^T (sw_q Phi[a_q]) over the nact
    active rows a (NULL: rows 0..nact-1) of the row-major n x B matrix Phi;
    sw NULL means weight 1. */

@@ -396,6 +396,82 @@ __global__ __launch_bounds__(256) void sgma_lrow_kernel(int s, int ld, const dou
     }
 }
 
+__global__ void sgma_pick_kernel(int nc, const double *__restrict__ q, const double *__restrict__ eta,
+                                 const int32_t *__restrict__ cand, int s, int n, const double *__restrict__ y,
+                                 int32_t *__restrict__ cent, int32_t *__restrict__ st, double *__restrict__ value) {
+    if (threadIdx.x != 0) return;
+    double bestv = 0.0;
+    int best = 0;
+    for (int i = 0; i < nc; i++) {
+        const double g = (eta[i] > 0.0) ? __dmul_rn(1.0 / eta[i], q[i]) : 0.0;
+        if (i == 0 || g > bestv) {
+            bestv = g;
+            best = i;
+        }
+    }
+    st[0] = best;
+    value[0] = bestv;
+    const int c = cand[best];
+    cent[s] = c;
+    if (c == n || (c < n && y[c] > 0.0)) st[1] += 1;
+}
+
+__global__ void gather_col_at_kernel(const double *__restrict__ src, int ld, const int32_t *__restrict__ col, int n,
+                                     double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(size_t)i * ld + col[0]];
+}
+
+__global__ void copy_row_at_kernel(const double *__restrict__ src, int ld, const int32_t *__restrict__ row, int count,
+                                   double *__restrict__ dst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < count) dst[j] = src[(size_t)row[0] * ld + j];
+}
+
+// One wave: lane l holds entries l + 64 u (u < 16, P <= 1024); round r takes
+// the first untaken entry if its score is NaN, else the largest score among
+// the untaken non-NaN ones with the smallest index, which is what a serial
+// scan `if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i` picks.
+__global__ __launch_bounds__(64) void sgma_topk_kernel(int P, const float *__restrict__ score, int k,
+                                                       const int32_t *__restrict__ cidx, int32_t *__restrict__ out) {
+    const int lane = threadIdx.x;
+    const int none = 1 << 30;
+    unsigned taken = 0u;
+    for (int r = 0; r < k; r++) {
+        int first = none, bi = none;
+        float bv = 0.0f;
+        for (int u = 0; u < 16; u++) {
+            const int i = lane + 64 * u;
+            if (i >= P) break;
+            if ((taken >> u) & 1u) continue;
+            if (first == none) first = i;
+            const float v = score[i];
+            if (v == v && (bi == none || v > bv)) {
+                bv = v;
+                bi = i;
+            }
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            const int f2 = __shfl_xor(first, o);
+            const float v2 = __shfl_xor(bv, o);
+            const int i2 = __shfl_xor(bi, o);
+            first = f2 < first ? f2 : first;
+            if (i2 != none && (bi == none || v2 > bv || (v2 == bv && i2 < bi))) {
+                bv = v2;
+                bi = i2;
+            }
+        }
+        int pick = bi;
+        if (first != none) {
+            const float f = score[first];
+            if (f != f || pick == none) pick = first;
+        }
+        if (pick == none) pick = 0;   // k > P: not reached (the caller keeps k <= P)
+        if (lane == (pick & 63)) taken |= 1u << (pick >> 6);
+        if (lane == 0) out[r] = cidx[pick];
+    }
+}
+
 // ----------------------------------------------------------------- wgram ---
 // Partial Gram matrices over row chunks: part[split][c*B + r] (lower tiles,
 // mirrored) = sum_{q in chunk} w_q Phi[a_q][r] Phi[a_q][c] with
@@ -726,6 +802,36 @@ void irwls_dev_sgma_lrow(int s, int ld, const double *kcc, const double *L2, con
                          double *Li) {
     if (s < 0) return;
     hipLaunchKernelGGL(sgma_lrow_kernel, dim3(1), dim3(256), 0, stream(), s, ld, kcc, L2, IL2, L, Li);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_pick(int nc, const double *q, const double *eta, const int32_t *cand, int s, int n,
+                         const double *y, int32_t *cent, int32_t *st, double *value) {
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(sgma_pick_kernel, dim3(1), dim3(64), 0, stream(), nc, q, eta, cand, s, n, y, cent, st, value);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_gather_col_at(const double *src, int ld, const int32_t *col, int n, double *dst) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather_col_at_kernel, dim3((n + 255) / 256), dim3(256), 0, stream(), src, ld, col, n, dst);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_copy_row_at(const double *src, int ld, const int32_t *row, int count, double *dst) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(copy_row_at_kernel, dim3((count + 255) / 256), dim3(256), 0, stream(), src, ld, row, count,
+                       dst);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_topk(int P, const float *score, int k, const int32_t *cidx, int32_t *out) {
+    if (P <= 0 || k <= 0) return;
+    if (P > 1024 || k > P) {
+        fprintf(stderr, "libirwls: irwls_dev_sgma_topk needs k <= P <= 1024 (P=%d, k=%d)\n", P, k);
+        exit(2);
+    }
+    hipLaunchKernelGGL(sgma_topk_kernel, dim3(1), dim3(64), 0, stream(), P, score, k, cidx, out);
     IRWLS_LAUNCH_CHECK();
 }
 

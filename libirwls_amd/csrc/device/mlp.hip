@@ -347,6 +347,7 @@ __global__ __launch_bounds__(256) void sgma_features_kernel(int P, int s, int B,
                                                             const double *__restrict__ gsub2,
                                                             const double *__restrict__ ycand,
                                                             const double *__restrict__ qvcand, float pos_frac,
+                                                            const int32_t *__restrict__ npos,
                                                             const float *__restrict__ kst, float *__restrict__ F) {
     __shared__ float est[FMAXP], leta[FMAXP], lg[FMAXP];
     __shared__ float red[6][256];
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(256) void sgma_features_kernel(int P, int s, int B,
         f[24] = expf(fmaxf(est[c] - emax, -20.f));
         f[25] = est[c] - emax;
         f[26] = (emax > emin) ? (est[c] - emin) / (emax - emin) : 0.f;
-        f[27] = pos_frac;
+        f[27] = npos ? (float)npos[0] / (float)s : pos_frac;
         f[28] = ksum * is * y;
         f[29] = (float)fmin(fmax(eta[c], 0.0), 1.0);
         f[30] = 1.0f;
@@ -495,7 +496,8 @@ void irwls_dev_mlp_infer_bf16(const uint16_t *Q, const float *P, const float *X,
 }
 
 void irwls_dev_sgma_features(int P, int s, int B, const double *KNC, int ldk, const double *eta, const double *gsub,
-                             const double *gsub2, const double *ycand, const double *qvcand, float pos_frac, float *F) {
+                             const double *gsub2, const double *ycand, const double *qvcand, float pos_frac,
+                             const int32_t *npos, float *F) {
     if (P <= 0) return;
     if (P > FMAXP) {
         fprintf(stderr, "libirwls: irwls_dev_sgma_features: pool of %d candidates (at most %d)\n", P, FMAXP);
@@ -505,7 +507,7 @@ void irwls_dev_sgma_features(int P, int s, int B, const double *KNC, int ldk, co
     hipLaunchKernelGGL(knc_stats_kernel, dim3((P + 3) / 4), dim3(256), 0, stream(), P, s, KNC, ldk, kst);
     IRWLS_LAUNCH_CHECK();
     hipLaunchKernelGGL(sgma_features_kernel, dim3(1), dim3(256), 0, stream(), P, s, B, KNC, ldk, eta, gsub, gsub2,
-                       ycand, qvcand, pos_frac, kst, F);
+                       ycand, qvcand, pos_frac, npos, kst, F);
     IRWLS_LAUNCH_CHECK();
 }
 

@@ -191,8 +191,20 @@ int *SGMA(svm_dataset dataset, properties props) {
     const double *knc = NULL;       /* device row of k(new centroid, previous centroids) */
     int npos = 0;
 
+    /* Without progress printing (and unless the network alone picks), each
+       step's choice stays on the device: the argmax, the chosen column and
+       row, the positive count and the shortlist are device kernels, so the
+       host only draws candidates and queues launches, and reads the
+       centroids back once at the end.  The printing path reads each choice
+       back as it is made.  Both give the same centroids. */
+    const int devpick = props.verbose != 1 && !(learned && keep == 0);
+    int32_t *d_st = (int32_t *)irwls_dev_alloc(4 * sizeof(int32_t));        /* [0] chosen position, [1] positives */
+    int32_t *d_cidx2 = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
+    double *d_knc = (double *)irwls_dev_alloc(Bz * sizeof(double));
+    const double *d_y = dd->y;   /* labels of the device rows (the class-mean rows included) */
+
     for (int s = 0; s < B; s++) {
-        double value;
+        double value = 0.0;
         if (s > 1) {
             g_bstats.sgma_steps++;
             if (!irwls_proposal_trained(lp)) {
@@ -209,21 +221,34 @@ int *SGMA(svm_dataset dataset, properties props) {
                 else
                     score_exact(dd, props, NULL, n, B, s, ncand, d_cidx, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
                                 d_W, d_Z, d_eta, d_q);
-                irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
-                const int best = best_gain(ncand, q, eta, &value);
-                cent[s] = cidx[best];
-                /* the chosen candidate's column is the new KSC column */
-                if (me) {
-                    irwls_dev_gather_col(d_KSMe, ncand, best, me, d_KSCe + (size_t)s * me);
-                    full_col = 0;
-                } else {
-                    irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
-                }
-                knc = d_KNC + (size_t)best * B;
-                if (lp) {
-                    /* warm-up: log the candidates' features with their exact gains */
+                if (lp)   /* warm-up: the candidates' features, logged below with their exact gains */
                     irwls_proposal_features(lp, dd, &dataset, props, s, ncand, d_cidx, cidx, d_KNC, d_Z, d_eta,
-                                            (float)npos / (float)s);
+                                            (float)npos / (float)s, devpick ? d_st + 1 : NULL);
+                if (devpick) {
+                    irwls_dev_sgma_pick(ncand, d_q, d_eta, d_cidx, s, n, d_y, d_cent, d_st, d_tmp + 16);
+                    if (me) {
+                        irwls_dev_gather_col_at(d_KSMe, ncand, d_st, me, d_KSCe + (size_t)s * me);
+                        full_col = 0;
+                    } else {
+                        irwls_dev_gather_col_at(d_KSM, ncand, d_st, n, d_KSC + (size_t)s * n);
+                    }
+                    irwls_dev_copy_row_at(d_KNC, B, d_st, s, d_knc);
+                    knc = d_knc;
+                    if (lp) irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
+                } else {
+                    irwls_dev_d2h(eq, d_eq, ((size_t)cmax + ncand) * sizeof(double));
+                    const int best = best_gain(ncand, q, eta, &value);
+                    cent[s] = cidx[best];
+                    /* the chosen candidate's column is the new KSC column */
+                    if (me) {
+                        irwls_dev_gather_col(d_KSMe, ncand, best, me, d_KSCe + (size_t)s * me);
+                        full_col = 0;
+                    } else {
+                        irwls_dev_gather_col(d_KSM, ncand, best, n, d_KSC + (size_t)s * n);
+                    }
+                    knc = d_KNC + (size_t)best * B;
+                }
+                if (lp) {
                     irwls_proposal_log(lp, ncand, q, eta);
                     if (s - 1 == warm) (void)irwls_proposal_train(lp, 300, 3e-3f, &g_bstats);
                 }
@@ -241,57 +266,78 @@ int *SGMA(svm_dataset dataset, properties props) {
                 irwls_dev_gemm(1, 0, s, pool, s, 1.0, d_Li, B, d_W, B, 0.0, d_Z, B);
                 irwls_dev_sgma_eta(s, pool, d_KNC, B, d_Z, B, d_eta);
                 irwls_proposal_features(lp, dd, &dataset, props, s, pool, d_cidx, cidx, d_KNC, d_Z, d_eta,
-                                        (float)npos / (float)s);
-                irwls_proposal_score(lp, pool, score);
-                /* shortlist: the highest scores, ties to the earlier draw */
-                const int k = keep > 0 ? keep : 1;
-                memset(taken, 0, (size_t)pool);
-                for (int r = 0; r < k; r++) {
-                    int bi = -1;
-                    for (int i = 0; i < pool; i++)
-                        if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i;
-                    taken[bi] = 1;
-                    cidx2[r] = cidx[bi];
-                    if (r == 0 && keep == 0) {
-                        /* the network's choice without exact scoring */
-                        cent[s] = cidx[bi];
-                        knc = d_KNC + (size_t)bi * B;
-                        value = score[bi];
-                    }
-                }
-                if (keep > 0) {
-                    irwls_dev_h2d(d_cidx, cidx2, (size_t)keep * sizeof(int32_t));
+                                        (float)npos / (float)s, devpick ? d_st + 1 : NULL);
+                if (devpick) {
+                    /* shortlist: the highest scores, ties to the earlier draw; then their exact gains */
+                    irwls_dev_sgma_topk(pool, irwls_proposal_score_dev(lp, pool), keep, d_cidx, d_cidx2);
                     if (me)
-                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
+                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
                                     d_KNC, d_W, d_Z, d_eta, d_q);
                     else
-                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
+                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
                                     d_W, d_Z, d_eta, d_q);
-                    irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
-                    const int best = best_gain(keep, q, eta, &value);
-                    cent[s] = cidx2[best];
+                    irwls_dev_sgma_pick(keep, d_q, d_eta, d_cidx2, s, n, d_y, d_cent, d_st, d_tmp + 16);
                     if (me) {
-                        irwls_dev_gather_col(d_KSMe, keep, best, me, d_KSCe + (size_t)s * me);
+                        irwls_dev_gather_col_at(d_KSMe, keep, d_st, me, d_KSCe + (size_t)s * me);
                         full_col = 0;
                     } else {
-                        irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
+                        irwls_dev_gather_col_at(d_KSM, keep, d_st, n, d_KSC + (size_t)s * n);
                     }
-                    knc = d_KNC + (size_t)best * B;
+                    irwls_dev_copy_row_at(d_KNC, B, d_st, s, d_knc);
+                    knc = d_knc;
+                    g_bstats.learned_steps++;
                 } else {
-                    const int32_t c1 = cent[s];
-                    irwls_dev_h2d(d_pair, &c1, sizeof(int32_t));
-                    irwls_rows cc = irwls_devdata_rows(dd, d_pair, 1);
-                    if (me) {
-                        irwls_rows re = irwls_devdata_rows(dd, d_sube, me);
-                        irwls_dev_kernel_matrix(&re, &cc, dd->d, props.kernelType, props.Kgamma,
-                                                d_KSCe + (size_t)s * me, 1);
-                        full_col = 0;
-                    } else {
-                        irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma,
-                                                d_KSC + (size_t)s * n, 1);
+                    irwls_proposal_score(lp, pool, score);
+                    /* shortlist: the highest scores, ties to the earlier draw */
+                    const int k = keep > 0 ? keep : 1;
+                    memset(taken, 0, (size_t)pool);
+                    for (int r = 0; r < k; r++) {
+                        int bi = -1;
+                        for (int i = 0; i < pool; i++)
+                            if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i;
+                        taken[bi] = 1;
+                        cidx2[r] = cidx[bi];
+                        if (r == 0 && keep == 0) {
+                            /* the network's choice without exact scoring */
+                            cent[s] = cidx[bi];
+                            knc = d_KNC + (size_t)bi * B;
+                            value = score[bi];
+                        }
                     }
+                    if (keep > 0) {
+                        irwls_dev_h2d(d_cidx2, cidx2, (size_t)keep * sizeof(int32_t));
+                        if (me)
+                            score_exact(dd, props, d_sube, me, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSCe,
+                                        d_KSMe, d_KNC, d_W, d_Z, d_eta, d_q);
+                        else
+                            score_exact(dd, props, NULL, n, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSC, d_KSM,
+                                        d_KNC, d_W, d_Z, d_eta, d_q);
+                        irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
+                        const int best = best_gain(keep, q, eta, &value);
+                        cent[s] = cidx2[best];
+                        if (me) {
+                            irwls_dev_gather_col(d_KSMe, keep, best, me, d_KSCe + (size_t)s * me);
+                            full_col = 0;
+                        } else {
+                            irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
+                        }
+                        knc = d_KNC + (size_t)best * B;
+                    } else {
+                        const int32_t c1 = cent[s];
+                        irwls_dev_h2d(d_pair, &c1, sizeof(int32_t));
+                        irwls_rows cc = irwls_devdata_rows(dd, d_pair, 1);
+                        if (me) {
+                            irwls_rows re = irwls_devdata_rows(dd, d_sube, me);
+                            irwls_dev_kernel_matrix(&re, &cc, dd->d, props.kernelType, props.Kgamma,
+                                                    d_KSCe + (size_t)s * me, 1);
+                            full_col = 0;
+                        } else {
+                            irwls_dev_kernel_matrix(&all, &cc, dd->d, props.kernelType, props.Kgamma,
+                                                    d_KSC + (size_t)s * n, 1);
+                        }
+                    }
+                    g_bstats.learned_steps++;
                 }
-                g_bstats.learned_steps++;
             }
         } else {
             cent[s] = (s == 0) ? n : n + 1;
@@ -302,9 +348,12 @@ int *SGMA(svm_dataset dataset, properties props) {
             if (s == 1) printf("Best Error Descent %f, Average of negative data is centroid %d\n", value, s);
             if (s > 1) printf("Best Error Descent %f, Data with index %d is centroid %d\n", value, cent[s], s);
         }
-        if (cent[s] == n || (cent[s] < n && dataset.y[cent[s]] > 0)) npos++;
-        int32_t cs = cent[s];
-        irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
+        if (s <= 1 || !devpick) {
+            if (cent[s] == n || (cent[s] < n && dataset.y[cent[s]] > 0)) npos++;
+            int32_t cs = cent[s];
+            irwls_dev_h2d(d_cent + s, &cs, sizeof(int32_t));
+            if (s == 1 && devpick) irwls_dev_h2d(d_st + 1, &npos, sizeof(int32_t));
+        }
         if (ctab) {
             irwls_rows ce = irwls_devdata_rows(dd, d_cent, s + 1);
             irwls_dev_coltab_extend(ctab, &ce);
@@ -340,12 +389,18 @@ int *SGMA(svm_dataset dataset, properties props) {
                 irwls_dev_h2d(d_tmp, &k01, sizeof(double));
                 knc = d_tmp;
             }
-            kernel_pair_dev(dd, d_pair + 2, cent[s], cent[s], props, d_tmp + 8);   /* kcc */
+            /* kcc = k(c_s, c_s) from the device index of the new centroid */
+            irwls_rows cs1 = irwls_devdata_rows(dd, d_cent + s, 1);
+            irwls_dev_kernel_matrix(&cs1, &cs1, dd->d, props.kernelType, props.Kgamma, d_tmp + 8, 1);
             irwls_dev_gemv(d_Li, s, s, B, 0, knc, d_v1);        /* L2 = L^-1 k */
             irwls_dev_gemv(d_Li, s, s, B, 1, d_v1, d_v2);       /* IL2 = L2^T L^-1 */
             irwls_dev_sgma_lrow(s, B, d_tmp + 8, d_v1, d_v2, d_L, d_Li);
         }
     }
+    if (devpick && B > 2) irwls_dev_d2h(cent + 2, d_cent + 2, (size_t)(B - 2) * sizeof(int32_t));
+    irwls_dev_free(d_st);
+    irwls_dev_free(d_cidx2);
+    irwls_dev_free(d_knc);
     irwls_dev_coltab_destroy(ctab);
     irwls_dev_free(d_KSC);
     irwls_dev_free(d_L);

@@ -129,10 +129,11 @@ void irwls_proposal_add_centroid_rows(irwls_proposal *lp, irwls_devdata *dd, pro
                                       const int32_t *d_c);
 void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_dataset *ds, properties props,
                              int s, int P, const int32_t *d_c, const int32_t *cidx, const double *d_KNC,
-                             const double *d_Z, const double *d_eta, float pos_frac);
+                             const double *d_Z, const double *d_eta, float pos_frac, const int32_t *d_npos);
 void irwls_proposal_log(irwls_proposal *lp, int P, const double *q, const double *eta);
 int irwls_proposal_train(irwls_proposal *lp, int steps, float lr, irwls_gpu_stats *st);
 void irwls_proposal_score(irwls_proposal *lp, int P, float *score);
+const float *irwls_proposal_score_dev(irwls_proposal *lp, int P);
 
 /* Batched training (batch.c).  irwls_rand is libc rand(), or the calling
    worker's private glibc-compatible stream in a batched run.

@@ -127,7 +127,7 @@ void irwls_proposal_add_centroid_rows(irwls_proposal *lp, irwls_devdata *dd, pro
    at step s, given their KNC rows (ld B), Z (ld B) and eta (device). */
 void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_dataset *ds, properties props,
                              int s, int P, const int32_t *d_c, const int32_t *cidx, const double *d_KNC,
-                             const double *d_Z, const double *d_eta, float pos_frac) {
+                             const double *d_Z, const double *d_eta, float pos_frac, const int32_t *d_npos) {
     irwls_rows sub = irwls_devdata_rows(dd, lp->d_sub, lp->m);
     irwls_rows cr = irwls_devdata_rows(dd, d_c, P);
     irwls_dev_kernel_matrix(&sub, &cr, dd->d, props.kernelType, props.Kgamma, lp->d_KSMs, P);
@@ -139,7 +139,7 @@ void irwls_proposal_features(irwls_proposal *lp, irwls_devdata *dd, const svm_da
     irwls_dev_h2d(lp->d_y, lp->hy, (size_t)P * sizeof(double));
     irwls_dev_h2d(lp->d_qv, lp->hqv, (size_t)P * sizeof(double));
     irwls_dev_sgma_features(P, s, lp->B, d_KNC, lp->B, d_eta, lp->d_gs, lp->d_gs2, lp->d_y, lp->d_qv, pos_frac,
-                            lp->d_F);
+                            d_npos, lp->d_F);
 }
 
 /* Log the pool's features with targets log(gain) - mean over the pool. */
@@ -235,8 +235,13 @@ int irwls_proposal_train(irwls_proposal *lp, int steps, float lr, irwls_gpu_stat
     return 1;
 }
 
-/* Scores of the pool (host, P floats) from the last features. */
-void irwls_proposal_score(irwls_proposal *lp, int P, float *score) {
+/* Scores of the pool (device, P floats) from the last features. */
+const float *irwls_proposal_score_dev(irwls_proposal *lp, int P) {
     irwls_dev_mlp_infer_bf16(lp->d_Q, lp->d_P, lp->d_F, P, lp->d_score);
-    irwls_dev_d2h(score, lp->d_score, (size_t)P * sizeof(float));
+    return lp->d_score;
+}
+
+/* The same, read back (host, P floats). */
+void irwls_proposal_score(irwls_proposal *lp, int P, float *score) {
+    irwls_dev_d2h(score, irwls_proposal_score_dev(lp, P), (size_t)P * sizeof(float));
 }

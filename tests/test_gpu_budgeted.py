@@ -201,3 +201,72 @@ def test_sgma_factor_row_on_device(s):
     assert np.array_equal(Lg[:s, s], L2) and Lg[s, s] == L3
     assert np.array_equal(Lig[:s, s], np.array([-IL3 * float(v) for v in IL2])) and Lig[s, s] == IL3
     assert np.all(Lg[s + 1:, s] == 7.0) and np.all(Lg[:, :s] == 7.0) and np.all(Lg[:, s + 1:] == 7.0)
+
+
+def test_sgma_topk_on_device():
+    """irwls_dev_sgma_topk against the serial shortlist scan (strict >, the
+    first untaken entry as the start): ties go to the earlier index, a NaN
+    is taken only where it is the first untaken entry."""
+    from gpu_util import Dev
+    Lb = lib.load()
+    vp = C.c_void_p
+    Lb.irwls_dev_sgma_topk.argtypes = [C.c_int, vp, C.c_int, vp, vp]
+    rng = np.random.default_rng(9)
+
+    def serial(sc, k):
+        taken = np.zeros(len(sc), bool)
+        out = []
+        for _ in range(k):
+            bi = -1
+            for i in range(len(sc)):
+                if not taken[i] and (bi < 0 or sc[i] > sc[bi]):
+                    bi = i
+            taken[bi] = True
+            out.append(bi)
+        return out
+
+    cases = [rng.standard_normal(236).astype(np.float32),
+             np.round(rng.standard_normal(300) * 2).astype(np.float32),          # many ties
+             np.full(64, 1.5, np.float32), rng.standard_normal(1024).astype(np.float32),
+             rng.standard_normal(17).astype(np.float32)]
+    nan1 = rng.standard_normal(100).astype(np.float32)
+    nan1[[0, 7, 50]] = np.nan
+    nan2 = rng.standard_normal(100).astype(np.float32)
+    nan2[[3, 4]] = np.nan
+    cases += [nan1, nan2, np.array([np.nan, np.nan, 1.0, np.nan], np.float32)]
+    for sc in cases:
+        P = len(sc)
+        cidx = rng.permutation(100000)[:P].astype(np.int32)
+        for k in sorted({1, 8, 16, P}):
+            if k > P:
+                continue
+            out = Dev(np.zeros(k, np.int32))
+            sd, cd = Dev(sc), Dev(cidx)
+            Lb.irwls_dev_sgma_topk(P, sd.p, k, cd.p, out.p)
+            assert list(out.get()) == [int(cidx[i]) for i in serial(sc, k)], (P, k)
+
+
+@pytest.mark.parametrize("learned", [0, 1])
+def test_sgma_device_choice_equals_host_choice(learned, capfd):
+    """The asynchronous SGMA step (choice, shortlist and positive count on
+    the device) and the progress-printing step (each choice read back and
+    made on the host) give the same centroids, plain and learned, on CSR and
+    dense data."""
+    rng = np.random.default_rng(31)
+    rp, col, val = synth.sparse_uniform(3000, 120, 0.08, rng)
+    ys = np.where(rng.random(3000) < 0.5, 1.0, -1.0)
+    X, yd = synth.two_gaussians(2500, 10, seed=8)
+    try:
+        if learned:
+            lib.set_options(sgma_candidates=24, proposal=1, proposal_keep=6, proposal_pool=96, proposal_warmup=20)
+        for ds in (lib.Dataset.csr(rp, col, val, ys), lib.Dataset.dense(X, yd)):
+            got = []
+            for verbose in (0, 1):
+                props = abi.make_properties(gamma=0.3, C_=1.0, size=60, verbose=verbose)
+                got.append(lib.sgma(ds, props, seed=4))
+            assert np.array_equal(got[0], got[1])
+            assert len(set(got[0])) == 60
+    finally:
+        lib.set_options(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_warmup=0)
+    C.CDLL(None).fflush(None)
+    assert "Best Error Descent" in capfd.readouterr().out
