@@ -618,7 +618,9 @@ __global__ void wgram_finish(int B, int nsplit, const double *__restrict__ part,
 }
 
 // K2[r] = sum_q (sw_q Phi[a_q][r]) (sw_q y_q) over the active rows; one thread
-// per centroid, rows in order (deterministic).
+// per centroid, rows in order (deterministic).  Eight rows' loads are issued
+// before their eight terms are added, in row order, so each wave keeps eight
+// row reads in flight.
 __global__ void wvec_kernel(int B, const double *__restrict__ Phi, const int32_t *__restrict__ act,
                             const double *__restrict__ sw, const double *__restrict__ yv, double ymul, int nact,
                             int chunk, double *__restrict__ part) {
@@ -629,7 +631,20 @@ __global__ void wvec_kernel(int B, const double *__restrict__ Phi, const int32_t
     int q1 = q0 + chunk;
     if (q1 > nact) q1 = nact;
     double s = 0.0;
-    for (int q = q0; q < q1; q++) {
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+        double x[8], w[8], yy[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int src = act ? act[q + u] : q + u;
+            x[u] = Phi[(size_t)src * B + r];
+            w[u] = sw ? sw[q + u] : 1.0;
+            yy[u] = yv[src];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += (w[u] * x[u]) * (w[u] * (ymul * yy[u]));
+    }
+    for (; q < q1; q++) {
         const int src = act ? act[q] : q;
         const double w = sw ? sw[q] : 1.0;
         s += (w * Phi[(size_t)src * B + r]) * (w * (ymul * yv[src]));
