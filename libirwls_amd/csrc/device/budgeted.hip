@@ -376,17 +376,37 @@ __global__ void scatter_row_kernel(double *__restrict__ M, int ld, int row, int 
 // L3 = sqrt(kcc + 1e-5 - sum_i L2_i^2) with the sum taken in index order and
 // every product and difference rounded on its own (no fused multiply-add),
 // and IL3 = 1 / L3; then L(s, 0..s) = (L2, L3), L^-1(s, 0..s) = (-IL3 IL2, IL3).
+// The squares are formed by all threads into LDS (LROW_MAX at a time); the
+// one thread's chain is then only the differences, read sixteen at a time.
+constexpr int LROW_MAX = 4096;
 __global__ __launch_bounds__(256) void sgma_lrow_kernel(int s, int ld, const double *__restrict__ kcc,
                                                         const double *__restrict__ L2,
                                                         const double *__restrict__ IL2, double *__restrict__ L,
                                                         double *__restrict__ Li) {
+    __shared__ double sq[LROW_MAX];
     __shared__ double sh[2];
+    double acc = kcc[0] + 0.00001;   // carried by thread 0
+    for (int i0 = 0; i0 < s; i0 += LROW_MAX) {
+        const int m = (s - i0 < LROW_MAX) ? s - i0 : LROW_MAX;
+        __syncthreads();   // the previous segment has been read
+        for (int i = threadIdx.x; i < m; i += blockDim.x) sq[i] = __dmul_rn(L2[i0 + i], L2[i0 + i]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int i = 0;
+            for (; i + 16 <= m; i += 16) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) v[u] = sq[i + u];
+#pragma unroll
+                for (int u = 0; u < 16; u++) acc = __dsub_rn(acc, v[u]);
+            }
+            for (; i < m; i++) acc = __dsub_rn(acc, sq[i]);
+        }
+    }
     if (threadIdx.x == 0) {
-        double L3 = kcc[0] + 0.00001;
-        for (int i = 0; i < s; i++) L3 = __dsub_rn(L3, __dmul_rn(L2[i], L2[i]));
-        L3 = sqrt(L3);
-        sh[0] = L3;
-        sh[1] = 1.0 / L3;
+        const double r = sqrt(acc);
+        sh[0] = r;
+        sh[1] = 1.0 / r;
     }
     __syncthreads();
     const double L3 = sh[0], IL3 = sh[1];
@@ -436,16 +456,19 @@ __global__ __launch_bounds__(64) void sgma_topk_kernel(int P, const float *__res
                                                        const int32_t *__restrict__ cidx, int32_t *__restrict__ out) {
     const int lane = threadIdx.x;
     const int none = 1 << 30;
+    float sc[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) sc[u] = (lane + 64 * u < P) ? score[lane + 64 * u] : 0.0f;
     unsigned taken = 0u;
     for (int r = 0; r < k; r++) {
         int first = none, bi = none;
         float bv = 0.0f;
+#pragma unroll
         for (int u = 0; u < 16; u++) {
             const int i = lane + 64 * u;
-            if (i >= P) break;
-            if ((taken >> u) & 1u) continue;
+            if (i >= P || ((taken >> u) & 1u)) continue;
             if (first == none) first = i;
-            const float v = score[i];
+            const float v = sc[u];
             if (v == v && (bi == none || v > bv)) {
                 bv = v;
                 bi = i;
@@ -463,8 +486,14 @@ __global__ __launch_bounds__(64) void sgma_topk_kernel(int P, const float *__res
         }
         int pick = bi;
         if (first != none) {
-            const float f = score[first];
-            if (f != f || pick == none) pick = first;
+            const float f = __shfl(sc[0], first & 63);   // sc[first >> 6] of lane first & 63
+            float fv = f;
+#pragma unroll
+            for (int u = 1; u < 16; u++) {
+                const float g = __shfl(sc[u], first & 63);
+                if ((first >> 6) == u) fv = g;
+            }
+            if (fv != fv || pick == none) pick = first;
         }
         if (pick == none) pick = 0;   // k > P: not reached (the caller keeps k <= P)
         if (lane == (pick & 63)) taken |= 1u << (pick >> 6);

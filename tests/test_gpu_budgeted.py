@@ -172,7 +172,7 @@ def test_irwlspar_m_decay_path(seed, decays):
     assert rel < 1e-6
 
 
-@pytest.mark.parametrize("s", [1, 2, 37, 300])
+@pytest.mark.parametrize("s", [1, 2, 37, 300, 4100])
 def test_sgma_factor_row_on_device(s):
     """irwls_dev_sgma_lrow: row s of L and L^-1 equal, bit for bit, the
     reference's host arithmetic (src/budgeted-train.c:247-279) restated in
