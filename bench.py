@@ -543,7 +543,50 @@ def bench_budgeted(args, name, steps, warmup):
     if "learned_exact_gains" in runs and "plain" in runs:
         block["exact_gains_speedup_vs_plain"] = runs["plain"]["value"] / runs["learned_exact_gains"]["value"]
         block["exact_gains_accuracy_delta"] = runs["learned_exact_gains"]["acc"] - runs["plain"]["acc"]
+    if args.accuracy_seeds > 0:
+        block["accuracy_over_seeds"] = budgeted_seed_sweep(wl, args.accuracy_seeds, modes)
     return block
+
+
+def budgeted_seed_sweep(wl, nseeds, modes):
+    """Test accuracy and primal objective of each budgeted mode on nseeds
+    further data sets of the workload's generator (seeds seed+1 ...), one
+    untimed run per mode and seed."""
+    from libirwls_amd import abi, lib, objective, synth
+    pp = abi.make_predict_properties(labels=0)
+    props = abi.make_properties(gamma=wl["gamma"], C_=wl["C"], size=wl["B"])
+    reset = dict(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
+    out = {m: [] for m in modes}
+    n = wl["n"]
+    for k in range(1, nseeds + 1):
+        seed = wl["seed"] + k
+        log("bench: accuracy sweep, data seed %d" % seed)
+        rp, col, val, y = synth.sparse_rbf_labels(n + wl["n_test"], wl["d"], wl["density"], gamma=wl["gamma"],
+                                                  seed=seed)
+        ds = lib.Dataset.csr(rp[:n + 1], col[:rp[n]], val[:rp[n]], y[:n])
+        te = rp[n:] - rp[n]
+        td = lib.Dataset.csr(te, col[rp[n]:], val[rp[n]:], y[n:])
+        ds.pin(with_means=True)
+        for mode, opts in modes.items():
+            lib.set_options(**opts)
+            cent = lib.sgma(ds, props, seed=1)
+            beta = lib.irwls_par(ds, cent, props)
+            lib.set_options(**reset)
+            m = lib.budgeted_model(props, ds, cent, beta)
+            acc = float(np.mean(lib.hard_test(td, m, pp) == y[n:]))
+            primal = objective.primal_objective(ds, cent, beta, lib.soft_test(ds, m, pp), wl["C"], wl["gamma"])
+            out[mode].append({"seed": seed, "test_accuracy": acc, "primal_objective": primal})
+            log("bench: seed %d %s: accuracy %.5f, primal %.8g" % (seed, mode, acc, primal))
+        ds.unpin()
+    summary = {"seeds": [wl["seed"] + k for k in range(1, nseeds + 1)], "runs": out}
+    if "plain" in out:
+        for mode in out:
+            if mode == "plain":
+                continue
+            d = [a["test_accuracy"] - b["test_accuracy"] for a, b in zip(out[mode], out["plain"])]
+            summary[mode + "_accuracy_delta_mean"] = float(np.mean(d))
+            summary[mode + "_accuracy_delta_max_abs"] = float(np.max(np.abs(d)))
+    return summary
 
 
 def cpu_baseline_batched(probs):
@@ -677,6 +720,9 @@ def main():
                     help="only time the kernel-matrix products (FP64 and BF16 x3 MFMA) at the mixed workload's shape")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the traffic-counter calibration copy (for rocprofv3 --pmc)")
+    ap.add_argument("--accuracy-seeds", type=int, default=0,
+                    help="budgeted workload: also report accuracy and objective of each mode on this many further "
+                         "data seeds (untimed)")
     ap.add_argument("--cpu-outer", type=int, default=None,
                     help="CPU baselines: 0 skips them; dense workloads: outer iterations in the sample "
                          "(-1 = the whole run; default: the workload's own setting)")
