@@ -29,6 +29,13 @@ void  irwls_dev_shutdown(void);
 void *irwls_dev_alloc(size_t bytes);        /* zero-filled */
 void  irwls_dev_free(void *d_ptr);          /* waits for the stream, keeps the block for reuse */
 void  irwls_dev_h2d(void *d_dst, const void *src, size_t bytes);   /* synchronous */
+/* Host-to-device copy of `total` bytes that the caller produces chunk by
+   chunk: fill(arg, buf, off, len) writes bytes [off, off + len) of the
+   source into the pinned buffer buf (len <= chunk); each chunk's transfer
+   runs while the next is filled (two pinned buffers).  Returns when every
+   chunk has been copied. */
+void  irwls_dev_h2d_chunked(void *d_dst, size_t total, size_t chunk,
+                            void (*fill)(void *arg, void *buf, size_t off, size_t len), void *arg);
 void  irwls_dev_d2h(void *dst, const void *d_src, size_t bytes);   /* synchronous */
 void  irwls_dev_d2d(void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
 void  irwls_dev_memset0(void *d_dst, size_t bytes);                /* fill kernel */

@@ -413,6 +413,35 @@ void irwls_dev_h2d(void *dst, const void *src, size_t bytes) {
     g_stage_head += (bytes + 255) & ~(size_t)255;
 }
 
+// Chunked uploads take their pinned buffers from the pool in one size.
+static const size_t kChunkPool = (size_t)32 << 20;
+
+void irwls_dev_h2d_chunked(void *dst, size_t total, size_t chunk,
+                           void (*fill)(void *arg, void *buf, size_t off, size_t len), void *arg) {
+    if (total == 0) return;
+    if (chunk == 0 || chunk > total) chunk = total;
+    const size_t cap = chunk <= kChunkPool ? kChunkPool : chunk;
+    void *buf[2] = {pinned_get(cap), pinned_get(cap)};
+    hipEvent_t ev[2];
+    for (int b = 0; b < 2; b++) IRWLS_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+    bool busy[2] = {false, false};
+    size_t k = 0;
+    for (size_t off = 0; off < total; off += chunk, k++) {
+        const int b = (int)(k & 1);
+        const size_t len = (total - off < chunk) ? total - off : chunk;
+        if (busy[b]) IRWLS_HIP(hipEventSynchronize(ev[b]));
+        fill(arg, buf[b], off, len);
+        IRWLS_HIP(hipMemcpyAsync((char *)dst + off, buf[b], len, hipMemcpyHostToDevice, stream()));
+        IRWLS_HIP(hipEventRecord(ev[b], stream()));
+        busy[b] = true;
+    }
+    for (int b = 0; b < 2; b++) {
+        if (busy[b]) IRWLS_HIP(hipEventSynchronize(ev[b]));
+        IRWLS_HIP(hipEventDestroy(ev[b]));
+        pinned_put(buf[b], cap);
+    }
+}
+
 void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
     if (t_bounce && bytes <= kBounce) {

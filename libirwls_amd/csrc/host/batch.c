@@ -67,6 +67,8 @@ static __thread batch_sched *t_sched;
 static __thread int t_slot;
 static __thread struct random_data *t_rng;
 
+int irwls_batch_worker(void) { return t_sched != NULL; }
+
 int irwls_rand(void) {
     if (t_rng) {
         int32_t r = 0;

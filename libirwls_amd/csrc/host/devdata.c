@@ -148,6 +148,127 @@ static void upload_fail(int row, int index) {
     exit(2);
 }
 
+/* ---- upload: the AoS rows walked on several host threads, written into
+   pinned chunks whose transfers overlap the filling of the next chunk ---- */
+
+typedef struct {
+    const svm_dataset *ds;
+    int below1[16], maxidx[16];
+} scan_job;
+
+/* irwls_dataset_base and irwls_dataset_maxindex in one pass */
+static void scan_rows(void *arg, int t, long b, long e) {
+    scan_job *j = (scan_job *)arg;
+    int below = 0, m = 0;
+    for (long i = b; i < e; i++)
+        for (const svm_sample *s = j->ds->x[i]; s->index != -1; s++) {
+            if (s->index < 1) below = 1;
+            if (s->index > m) m = s->index;
+        }
+    j->below1[t] = below;
+    j->maxidx[t] = m;
+}
+
+typedef struct {
+    const svm_dataset *ds;
+    int base, d;
+    size_t row_bytes;
+    double *buf;      /* the chunk being filled */
+    long r0;          /* its first row */
+} dense_job;
+
+static void fill_dense_rows(void *arg, int t, long b, long e) {
+    (void)t;
+    dense_job *j = (dense_job *)arg;
+    for (long i = b; i < e; i++) {
+        double *row = j->buf + (size_t)(i - j->r0) * j->d;
+        memset(row, 0, j->row_bytes);
+        for (const svm_sample *s = j->ds->x[i]; s->index != -1; s++) {
+            const int c = s->index - j->base;
+            if (c < 0) upload_fail((int)i, s->index);
+            if (c < j->d) row[c] = s->value;
+        }
+    }
+}
+
+static void fill_dense_rows_rel(void *arg, int t, long b, long e) {
+    dense_job *j = (dense_job *)arg;
+    fill_dense_rows(arg, t, j->r0 + b, j->r0 + e);
+}
+
+static void fill_dense(void *arg, void *buf, size_t off, size_t len) {
+    dense_job *j = (dense_job *)arg;
+    j->buf = (double *)buf;
+    j->r0 = (long)(off / j->row_bytes);
+    const long nr = (long)(len / j->row_bytes);
+    irwls_parallel_ranges(nr, irwls_host_threads(nr * (long)j->d, 1L << 17), fill_dense_rows_rel, j);
+}
+
+typedef struct {
+    const svm_dataset *ds;
+    int base;
+    const int64_t *rp;
+    int rows;
+    int64_t *len;      /* per row: entry count (count pass) */
+    void *buf;         /* the chunk being filled */
+    int64_t k0, k1;    /* its entries */
+    int what;          /* 0: columns (int32), 1: values (double) */
+} csr_job;
+
+static void count_rows(void *arg, int t, long b, long e) {
+    (void)t;
+    csr_job *j = (csr_job *)arg;
+    for (long i = b; i < e; i++) {
+        int64_t c = 0;
+        for (const svm_sample *s = j->ds->x[i]; s->index != -1; s++) {
+            if (s->index < j->base) upload_fail((int)i, s->index);
+            c++;
+        }
+        j->len[i] = c;
+    }
+}
+
+/* the row holding entry k: the last row whose start is <= k */
+static int row_of(const int64_t *rp, int rows, int64_t k) {
+    int lo = 0, hi = rows - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rp[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+static void fill_csr_part(void *arg, int t, long b, long e) {
+    (void)t;
+    csr_job *j = (csr_job *)arg;
+    const int64_t k0 = j->k0 + b, k1 = j->k0 + e;
+    if (k0 >= k1) return;
+    int i = row_of(j->rp, j->rows, k0);
+    int64_t k = k0;
+    while (k < k1) {
+        const svm_sample *s = j->ds->x[i] + (k - j->rp[i]);
+        const int64_t stop = j->rp[i + 1] < k1 ? j->rp[i + 1] : k1;
+        for (; k < stop; k++, s++) {
+            if (j->what == 0) ((int32_t *)j->buf)[k - j->k0] = s->index - j->base;
+            else ((double *)j->buf)[k - j->k0] = s->value;
+        }
+        i++;
+    }
+}
+
+static void fill_csr(void *arg, void *buf, size_t off, size_t len) {
+    csr_job *j = (csr_job *)arg;
+    const size_t es = j->what == 0 ? sizeof(int32_t) : sizeof(double);
+    j->buf = buf;
+    j->k0 = (int64_t)(off / es);
+    j->k1 = j->k0 + (int64_t)(len / es);
+    const long ne = (long)(j->k1 - j->k0);
+    irwls_parallel_ranges(ne, irwls_host_threads(ne, 1L << 18), fill_csr_part, j);
+}
+
+static const size_t kUploadChunk = (size_t)32 << 20;
+
 static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, int min_cols) {
     irwls_devdata *dd = (irwls_devdata *)irwls_xcalloc(1, sizeof(irwls_devdata));
     dd->key = ds->features;
@@ -158,9 +279,19 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
     dd->rows = ds->l + (with_means ? 2 : 0);
     dd->dense = (ds->sparse == 0);
     const int rows = dd->rows;
-    if (base < 0) base = irwls_dataset_base(ds, rows);
+    scan_job sj;
+    memset(&sj, 0, sizeof(sj));
+    sj.ds = ds;
+    const int nts = irwls_host_threads(rows, 4096);
+    irwls_parallel_ranges(rows, nts, scan_rows, &sj);
+    int below1 = 0, maxidx = 0;
+    for (int t = 0; t < nts; t++) {
+        below1 |= sj.below1[t];
+        if (sj.maxidx[t] > maxidx) maxidx = sj.maxidx[t];
+    }
+    if (base < 0) base = below1 ? 0 : 1;
     dd->base = base;
-    int d = irwls_dataset_maxindex(ds, rows) + 1 - base;
+    int d = maxidx + 1 - base;
     if (d < min_cols) d = min_cols;
     if (d < 1) d = 1;
     dd->d = d;
@@ -173,45 +304,40 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
     dd->qv = (double *)irwls_dev_alloc(rr * sizeof(double));
     if (ds->quadratic_value) irwls_dev_h2d(dd->qv, ds->quadratic_value, (size_t)rows * sizeof(double));
     if (dd->dense) {
-        double *hx = (double *)irwls_xcalloc((size_t)rows * d, sizeof(double));
-        for (int i = 0; i < rows; i++)
-            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
-                const int c = s->index - base;
-                if (c < 0) upload_fail(i, s->index);
-                if (c < d) hx[(size_t)i * d + c] = s->value;
-            }
         dd->X = (double *)irwls_dev_alloc((size_t)rows * d * sizeof(double));
-        irwls_dev_h2d(dd->X, hx, (size_t)rows * d * sizeof(double));
-        free(hx);
+        dense_job j;
+        memset(&j, 0, sizeof(j));
+        j.ds = ds;
+        j.base = base;
+        j.d = d;
+        j.row_bytes = (size_t)d * sizeof(double);
+        size_t per = kUploadChunk / j.row_bytes;
+        if (per < 1) per = 1;
+        irwls_dev_h2d_chunked(dd->X, (size_t)rows * j.row_bytes, per * j.row_bytes, fill_dense, &j);
         dd->nrm = (double *)irwls_dev_alloc(rr * sizeof(double));
         irwls_dev_row_norms(dd->X, d, rows, d, dd->nrm);
     } else {
-        int64_t nnz = 0;
-        for (int i = 0; i < rows; i++)
-            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) nnz++;
         int64_t *rp = (int64_t *)irwls_xmalloc((size_t)(rows + 1) * sizeof(int64_t));
-        int32_t *cl = (int32_t *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
-        double *vl = (double *)irwls_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
-        int64_t k = 0;
+        csr_job j;
+        memset(&j, 0, sizeof(j));
+        j.ds = ds;
+        j.base = base;
+        j.rows = rows;
+        j.len = rp + 1;
+        irwls_parallel_ranges(rows, irwls_host_threads(rows, 4096), count_rows, &j);
         rp[0] = 0;
-        for (int i = 0; i < rows; i++) {
-            for (const svm_sample *s = ds->x[i]; s->index != -1; s++) {
-                if (s->index < base) upload_fail(i, s->index);
-                cl[k] = s->index - base;
-                vl[k] = s->value;
-                k++;
-            }
-            rp[i + 1] = k;
-        }
+        for (int i = 0; i < rows; i++) rp[i + 1] += rp[i];
+        const int64_t nnz = rp[rows];
+        j.rp = rp;
         dd->rowptr = (int64_t *)irwls_dev_alloc((size_t)(rows + 1) * sizeof(int64_t));
         dd->col = (int32_t *)irwls_dev_alloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
         dd->val = (double *)irwls_dev_alloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
         irwls_dev_h2d(dd->rowptr, rp, (size_t)(rows + 1) * sizeof(int64_t));
-        irwls_dev_h2d(dd->col, cl, (size_t)nnz * sizeof(int32_t));
-        irwls_dev_h2d(dd->val, vl, (size_t)nnz * sizeof(double));
+        j.what = 0;
+        irwls_dev_h2d_chunked(dd->col, (size_t)nnz * sizeof(int32_t), kUploadChunk, fill_csr, &j);
+        j.what = 1;
+        irwls_dev_h2d_chunked(dd->val, (size_t)nnz * sizeof(double), kUploadChunk, fill_csr, &j);
         free(rp);
-        free(cl);
-        free(vl);
         dd->nnz_per_row = rows > 0 ? (double)nnz / rows : 0.0;
         dd->nrm = dd->qv;
     }
@@ -219,7 +345,6 @@ static irwls_devdata *upload(const svm_dataset *ds, int with_means, int base, in
     dd->fp_mean = fp_means(ds, rows);
     return dd;
 }
-
 
 static void destroy(irwls_devdata *dd) {
     if (dd->Xh) irwls_dev_free(dd->Xh);

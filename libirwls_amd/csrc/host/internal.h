@@ -59,6 +59,15 @@ int irwls_dataset_maxindex(const svm_dataset *ds, int rows);
 /* Upload (or find the pinned copy of) a dataset.  with_means asks for the two
    class-mean rows l and l+1 as well; base as above (-1 = derive it);
    min_cols widens the column space (prediction aligns data and model). */
+/* Host parallel loops (par.c): fn(arg, t, begin, end) over nt contiguous
+   ranges of [0, n), the caller's thread taking t = 0; irwls_host_threads
+   gives min(16, cores, work / grain), at least 1. */
+typedef void (*irwls_range_fn)(void *arg, int t, long begin, long end);
+void irwls_parallel_ranges(long n, int nt, irwls_range_fn fn, void *arg);
+int irwls_host_threads(long work, long grain);
+/* 1 on a worker thread of trainFULLBatched */
+int irwls_batch_worker(void);
+
 irwls_devdata *irwls_devdata_acquire(const svm_dataset *ds, int with_means, int base, int min_cols);
 /* Base and largest feature index of a pinned dataset's device copy (its
    column range, which may extend past the largest index present: those

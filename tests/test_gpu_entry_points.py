@@ -158,3 +158,50 @@ def test_compiled_c_caller(tmp_path, kind):
     labels = np.loadtxt(lf)
     pp = abi.make_predict_properties(labels=0)
     assert np.array_equal(labels, lib.hard_test(lib.Dataset.read(te), m, pp))
+
+
+@pytest.mark.parametrize("kind", ["csr", "dense"])
+def test_upload_across_chunks(kind):
+    """The device upload writes the rows through 32 MiB pinned chunks on
+    several host threads: a test set of several chunks (CSR with empty rows;
+    dense) predicts as the same rows uploaded a piece of one chunk at a time
+    (CSR bit for bit: its row products do not depend on the row count;
+    dense to 1e-13, whose column splits follow the row count)."""
+    rng = np.random.default_rng(5)
+    d = 300 if kind == "csr" else 40
+    if kind == "csr":
+        rp, col, val = synth.sparse_uniform(3000, d, 0.03, rng)
+        ds = lib.Dataset.csr(rp, col, val, np.where(rng.random(3000) < 0.5, 1.0, -1.0))
+        n = 1_500_000
+        lens0 = rng.integers(0, 13, 20000)
+        lens0[::97] = 0                                 # empty rows
+        cols0 = np.concatenate([np.sort(rng.choice(d, k, replace=False)) for k in lens0]).astype(np.int32)
+        lens = np.tile(lens0, n // 20000)
+        trp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        tcol = np.tile(cols0, n // 20000)
+        tval = rng.random(len(tcol))
+        assert trp[-1] == len(tcol) and trp[-1] * 8 > 2 * (32 << 20)
+
+        def piece(a, b):
+            return lib.Dataset.csr(trp[a:b + 1] - trp[a], tcol[trp[a]:trp[b]], tval[trp[a]:trp[b]], np.zeros(b - a))
+        big = piece(0, n)
+    else:
+        X, y = synth.two_gaussians(3000, d, seed=12)
+        ds = lib.Dataset.dense(X, y)
+        n = 220_000
+        Xt = rng.standard_normal((n, d)).astype(np.float64)
+        assert n * d * 8 > 32 << 20
+
+        def piece(a, b):
+            return lib.Dataset.dense(Xt[a:b], np.zeros(b - a))
+        big = piece(0, n)
+    props = abi.make_properties(gamma=0.2, C_=10.0, W=300)
+    m = lib.full_model(props, ds, lib.train_full(ds, props, seed=0))
+    pp = abi.make_predict_properties(labels=0)
+    got = lib.soft_test(big, m, pp)
+    step = 100_000 if kind == "csr" else 20_000
+    want = np.concatenate([lib.soft_test(piece(a, min(a + step, n)), m, pp) for a in range(0, n, step)])
+    if kind == "csr":
+        assert np.array_equal(got, want)
+    else:
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
