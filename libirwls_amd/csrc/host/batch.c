@@ -69,6 +69,51 @@ static __thread struct random_data *t_rng;
 
 int irwls_batch_worker(void) { return t_sched != NULL; }
 
+/* A block of libc rand() draws without its lock (about 25 ns a call): libc
+   is switched to a scratch state while the caller's state is stepped with
+   random_r, the reentrant form of the same generator (rand() is random()),
+   and the state is handed back with its rear position advanced, so the
+   draws and every later rand() are exactly those of calling rand() as many
+   times.  The rear position travels in the state's header word, as
+   initstate/setstate store it.  In a batched run the worker's private
+   stream is used directly. */
+typedef struct {
+    struct random_data rd;
+    char own[128], scratch[128];
+    char *caller;
+    int active;
+} rand_block;
+_Static_assert(sizeof(rand_block) <= sizeof(irwls_rand_block), "irwls_rand_block too small");
+
+void irwls_rand_begin(irwls_rand_block *ob) {
+    rand_block *b = (rand_block *)ob;
+    memset(b, 0, sizeof(*b));
+    if (t_rng) return;
+    initstate_r(1u, b->own, sizeof(b->own), &b->rd);              /* a valid buffer for setstate_r to leave */
+    b->caller = initstate(1u, b->scratch, sizeof(b->scratch));    /* libc on scratch; caller's position saved */
+    setstate_r(b->caller, &b->rd);                                /* step the caller's state here */
+    b->active = 1;
+}
+
+int irwls_rand_draw(irwls_rand_block *ob) {
+    rand_block *b = (rand_block *)ob;
+    int32_t r = 0;
+    if (b->active) random_r(&b->rd, &r);
+    else if (t_rng) random_r(t_rng, &r);
+    else r = rand();
+    return r;
+}
+
+void irwls_rand_end(irwls_rand_block *ob) {
+    rand_block *b = (rand_block *)ob;
+    if (!b->active) return;
+    /* the header word: rear position and generator type, as setstate reads it */
+    int32_t *st = b->rd.state;
+    if (b->rd.rand_type != 0) st[-1] = 5 * (int32_t)(b->rd.rptr - st) + b->rd.rand_type;   /* 5 = MAX_TYPES */
+    setstate(b->caller);
+    b->active = 0;
+}
+
 int irwls_rand(void) {
     if (t_rng) {
         int32_t r = 0;

@@ -25,12 +25,15 @@ static irwls_gpu_stats g_bstats;
 int *randomCentroids(svm_dataset dataset, properties props) {
     int *perm = (int *)irwls_xmalloc((size_t)(dataset.l > 0 ? dataset.l : 1) * sizeof(int));
     for (int i = 0; i < dataset.l; ++i) perm[i] = i;
+    irwls_rand_block rb;
+    irwls_rand_begin(&rb);
     for (int i = dataset.l - 1; i >= 0; --i) {
-        int j = rand() % (i + 1);
+        int j = irwls_rand_draw(&rb) % (i + 1);
         int t = perm[i];
         perm[i] = perm[j];
         perm[j] = t;
     }
+    irwls_rand_end(&rb);
     int *c = (int *)irwls_xmalloc((size_t)(props.size > 0 ? props.size : 1) * sizeof(int));
     for (int i = 0; i < props.size; i++) c[i] = perm[i];
     free(perm);
@@ -209,11 +212,14 @@ int *SGMA(svm_dataset dataset, properties props) {
             g_bstats.sgma_steps++;
             if (!irwls_proposal_trained(lp)) {
                 /* candidate draws in the reference's serial order */
+                irwls_rand_block rb;
+                irwls_rand_begin(&rb);
                 for (int i = 0; i < ncand; i++) {
-                    int ix = rand() % n;
-                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
+                    int ix = irwls_rand_draw(&rb) % n;
+                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = irwls_rand_draw(&rb) % n;
                     cidx[i] = ix;
                 }
+                irwls_rand_end(&rb);
                 irwls_dev_h2d(d_cidx, cidx, (size_t)ncand * sizeof(int32_t));
                 if (me)
                     score_exact(dd, props, d_sube, me, B, s, ncand, d_cidx, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
@@ -254,11 +260,14 @@ int *SGMA(svm_dataset dataset, properties props) {
                 }
             } else {
                 /* learned proposal: score a larger pool, then exact gains of the shortlist */
+                irwls_rand_block rb;
+                irwls_rand_begin(&rb);
                 for (int i = 0; i < pool; i++) {
-                    int ix = rand() % n;
-                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = rand() % n;
+                    int ix = irwls_rand_draw(&rb) % n;
+                    while (dataset.y[ix] != ((i % 2) * 2.0 - 1)) ix = irwls_rand_draw(&rb) % n;
                     cidx[i] = ix;
                 }
+                irwls_rand_end(&rb);
                 irwls_dev_h2d(d_cidx, cidx, (size_t)pool * sizeof(int32_t));
                 irwls_rows cr = irwls_devdata_rows(dd, d_cidx, pool);
                 kmat_centroids(dd, props, &cr, d_cent, s, ctab, d_KNC, B);

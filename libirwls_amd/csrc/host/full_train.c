@@ -29,12 +29,15 @@ void irwls_stats_set(const irwls_gpu_stats *s) { g_stats = *s; }
 int *rpermute(int n) {
     int *p = (int *)irwls_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
     for (int k = 0; k < n; k++) p[k] = k;
+    irwls_rand_block rb;
+    irwls_rand_begin(&rb);
     for (int k = n - 1; k > 0; k--) {
-        int j = irwls_rand() % (k + 1);
+        int j = irwls_rand_draw(&rb) % (k + 1);
         int t = p[j];
         p[j] = p[k];
         p[k] = t;
     }
+    irwls_rand_end(&rb);
     return p;
 }
 

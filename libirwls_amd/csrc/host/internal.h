@@ -150,6 +150,16 @@ const float *irwls_proposal_score_dev(irwls_proposal *lp, int P);
    augmented matrix A, lda >= n+1, right-hand side and solution on the
    device) and returns 1 when done, or returns 0 outside a batched run. */
 int irwls_rand(void);
+/* A run of libc rand() draws without libc's per-call lock: the same values
+   and the same libc state afterwards as calling rand() (irwls_rand() on a
+   batched worker) once per draw.  No other thread may call rand() between
+   begin and end. */
+typedef struct {
+    _Alignas(16) char opaque[448];   /* the generator state and two state buffers (batch.c) */
+} irwls_rand_block;
+void irwls_rand_begin(irwls_rand_block *b);
+int irwls_rand_draw(irwls_rand_block *b);
+void irwls_rand_end(irwls_rand_block *b);
 int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
 
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
