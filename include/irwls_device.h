@@ -225,6 +225,9 @@ void irwls_dev_kernel_matrix_bf16(const irwls_rows_bf16 *r, const irwls_rows_bf1
    thread's stream. */
 void irwls_dev_thread_begin(void);
 void irwls_dev_thread_end(void);
+/* Creates worker streams on the calling thread until the pool holds n (the
+   batched trainer calls it before starting its workers). */
+void irwls_dev_reserve_streams(int n);
 /* Seconds the calling thread has spent waiting for its stream (read-backs
    and synchronisations) since its last reset. */
 double irwls_dev_wait_seconds(int reset);

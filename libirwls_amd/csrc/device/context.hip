@@ -255,7 +255,13 @@ static void *pinned_get(size_t bytes) {
 static void pinned_put(void *q, size_t bytes) {
     if (!q) return;
     std::lock_guard<std::mutex> g(g_pmu);
-    g_pinned[bytes].push_back(q);
+    auto &v = g_pinned[bytes];
+    for (void *o : v)
+        if (o == q) {
+            fprintf(stderr, "libirwls: pinned buffer %p returned to its pool twice\n", q);
+            exit(2);
+        }
+    v.push_back(q);
 }
 
 }  // namespace irwls
@@ -570,6 +576,19 @@ double irwls_dev_wait_seconds(int reset) {
     return w;
 }
 
+// Worker streams are created by the thread that starts the workers
+// (irwls_dev_reserve_streams) and pooled, so no stream is ever created by a
+// worker thread that ends while other threads still use the stream.
+void irwls_dev_reserve_streams(int n) {
+    irwls_dev_init();
+    std::lock_guard<std::mutex> g(g_pmu);
+    while ((int)g_streams.size() < n) {
+        hipStream_t s = nullptr;
+        IRWLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        g_streams.push_back(s);
+    }
+}
+
 void irwls_dev_thread_begin(void) {
     irwls_dev_init();
     if (t_stream) return;
@@ -599,6 +618,11 @@ void irwls_dev_thread_end(void) {
     IRWLS_HIP(hipEventDestroy(t_sync_event));
     t_sync_event = nullptr;
     std::lock_guard<std::mutex> g(g_pmu);
+    for (hipStream_t s : g_streams)
+        if (s == t_stream) {
+            fprintf(stderr, "libirwls: worker stream returned to its pool twice\n");
+            exit(2);
+        }
     g_streams.push_back(t_stream);
     t_stream = nullptr;
 }

@@ -240,6 +240,11 @@ static void *worker(void *arg) {
     pthread_mutex_lock(&s->mu);
     s->active--;
     pthread_cond_broadcast(&s->cv);   /* the waiting ones may now be all that is left */
+    /* The thread stays until every problem has finished: nothing a worker's
+       HIP calls created or registered (on its stack, in its thread-local
+       storage, or in a profiler's per-thread state) goes away while other
+       workers still launch. */
+    while (s->active > 0) pthread_cond_wait(&s->cv, &s->mu);
     pthread_mutex_unlock(&s->mu);
     return NULL;
 }
@@ -265,6 +270,8 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
     s.lda = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
     s.rhs = (const double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
     s.x = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
+    /* every worker's stream comes from the pool, created here on the calling thread */
+    irwls_dev_reserve_streams(nprob);
     batch_job *jobs = (batch_job *)irwls_xcalloc((size_t)nprob, sizeof(batch_job));
     pthread_t *th = (pthread_t *)irwls_xcalloc((size_t)nprob, sizeof(pthread_t));
     for (int p = 0; p < nprob; p++) {
