@@ -37,6 +37,8 @@ void  irwls_dev_h2d(void *d_dst, const void *src, size_t bytes);   /* synchronou
 void  irwls_dev_h2d_chunked(void *d_dst, size_t total, size_t chunk,
                             void (*fill)(void *arg, void *buf, size_t off, size_t len), void *arg);
 void  irwls_dev_d2h(void *dst, const void *d_src, size_t bytes);   /* synchronous */
+/* Two device-to-host copies with one wait for the stream (synchronous). */
+void  irwls_dev_d2h2(void *dst1, const void *d_src1, size_t bytes1, void *dst2, const void *d_src2, size_t bytes2);
 void  irwls_dev_d2d(void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
 void  irwls_dev_memset0(void *d_dst, size_t bytes);                /* fill kernel */
 /* Plain copy kernel (16-byte aligned pointers, bytes a multiple of 16): its
@@ -157,6 +159,24 @@ void irwls_dev_gemv_gather(const double *d_Kw, int ldk, const int32_t *d_rows, i
                            const int32_t *d_cols, int nc, const double *d_w,
                            double a, double b, double cst, const double *d_rs,
                            double *d_out);
+
+/* KKT right-hand side (src/full-train.c:160-168): rhs[i] = 1 - G13[i] -
+   gin[S1[i]] for i < nS1 (G13 read as zero when use_g13 is 0), rhs[nS1] =
+   last. */
+void irwls_dev_kkt_rhs(const int32_t *d_S1, int nS1, const double *d_G13, int use_g13, const double *d_gin,
+                       double last, double *d_rhs);
+/* Weights after a KKT solve and the operands of the inner error update
+   (src/full-train.c:205-249): bnew = 0, bnew[S1[i]] = x[i] y[S1[i]],
+   bnew[S3[o]] = C y[S3[o]], bnew[l] = x[nS1]; the columns j < l with
+   bnew[j] != beta[j] in ascending order, w = bnew[j] - beta[j], their count
+   (*d_nc) and -(bnew[l] - beta[l]) (*d_cst); then beta = bnew. */
+void irwls_dev_inner_update(const double *d_x, const int32_t *d_S1, int nS1, const int32_t *d_S3, int nS3,
+                            const double *d_y, double C, int l, double *d_beta, double *d_bnew, int32_t *d_cols,
+                            double *d_w, int32_t *d_nc, double *d_cst);
+/* irwls_dev_gemv_gather over rows 0..nr-1 with the column count and the
+   constant read from device memory (as irwls_dev_inner_update leaves them). */
+void irwls_dev_gemv_gather_dn(const double *d_Kw, int ldk, int nr, const int32_t *d_cols, const double *d_w,
+                              const int32_t *d_nc, const double *d_cst, double a, double b, double *d_out);
 
 /* y = A x (trans 0) or A^T x (trans 1), A column-major m x n. */
 void irwls_dev_gemv(const double *d_A, int m, int n, int lda, int trans, const double *d_x, double *d_y);

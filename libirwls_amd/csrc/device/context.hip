@@ -461,6 +461,32 @@ void irwls_dev_d2h(void *dst, const void *src, size_t bytes) {
     stage_reset();
 }
 
+// Two device-to-host copies with one wait for the stream: through the
+// thread's pinned bounce buffer when both fit, else straight into dst.
+static thread_local char *t_pair = nullptr;
+
+void irwls_dev_d2h2(void *dst1, const void *src1, size_t b1, void *dst2, const void *src2, size_t b2) {
+    if (b1 + b2 == 0) return;
+    char *bb = t_bounce;
+    if (!bb) {
+        if (!t_pair) t_pair = (char *)pinned_get(kBounce);
+        bb = t_pair;
+    }
+    const size_t o2 = (b1 + 255) & ~(size_t)255;
+    if (o2 + b2 <= kBounce) {
+        if (b1) IRWLS_HIP(hipMemcpyAsync(bb, src1, b1, hipMemcpyDeviceToHost, stream()));
+        if (b2) IRWLS_HIP(hipMemcpyAsync(bb + o2, src2, b2, hipMemcpyDeviceToHost, stream()));
+        wait_stream();
+        if (b1) memcpy(dst1, bb, b1);
+        if (b2) memcpy(dst2, bb + o2, b2);
+    } else {
+        if (b1) IRWLS_HIP(hipMemcpyAsync(dst1, src1, b1, hipMemcpyDeviceToHost, stream()));
+        if (b2) IRWLS_HIP(hipMemcpyAsync(dst2, src2, b2, hipMemcpyDeviceToHost, stream()));
+        wait_stream();
+    }
+    stage_reset();
+}
+
 }  // extern "C"
 
 namespace irwls {

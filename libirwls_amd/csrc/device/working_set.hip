@@ -53,6 +53,88 @@ __global__ __launch_bounds__(256) void gemv_gather_kernel(
     }
 }
 
+// gemv_gather_kernel with the column count and the constant read from device
+// memory (written by inner_update_kernel): out[i] = a out[i] + b (K w)_i + cst.
+__global__ __launch_bounds__(256) void gemv_gather_dn_kernel(const double *__restrict__ Kw, int ldk, int nr,
+                                                             const int32_t *__restrict__ cols,
+                                                             const double *__restrict__ w,
+                                                             const int32_t *__restrict__ nc_p,
+                                                             const double *__restrict__ cst_p, double a, double b,
+                                                             double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nr) return;
+    const int nc = *nc_p;
+    const double *krow = Kw + (size_t)i * ldk;
+    double acc = 0.0;
+    for (int j = lane; j < nc; j += 64) acc += krow[cols[j]] * w[j];
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) out[i] = (a == 0.0 ? 0.0 : a * out[i]) + (b * acc) + *cst_p;
+}
+
+// Right-hand side of the KKT system (src/full-train.c:160-168):
+// rhs_i = 1 - G13_i - GIN[S1_i] (i < nS1; G13 read as zero when use_g13 is 0),
+// rhs_nS1 = last (-G13_nS1 - GIN_l, formed by the host).
+__global__ void kkt_rhs_kernel(const int32_t *__restrict__ S1, int nS1, const double *__restrict__ G13, int use_g13,
+                               const double *__restrict__ gin, double last, double *__restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nS1) rhs[i] = 1.0 - (use_g13 ? G13[i] : 0.0) - gin[S1[i]];
+    else if (i == nS1) rhs[i] = last;
+}
+
+// The weights after a KKT solve and the error-update operands
+// (src/full-train.c:205-249), one work-group: bnew = 0, bnew[S1_i] = x_i y[S1_i],
+// bnew[S3_o] = C y[S3_o], bnew_l = x_nS1; then the columns j < l whose
+// weight changed, in ascending order, with w = bnew_j - beta_j, their
+// count, the constant -(bnew_l - beta_l), and beta = bnew.
+__global__ __launch_bounds__(1024) void inner_update_kernel(const double *__restrict__ x,
+                                                            const int32_t *__restrict__ S1, int nS1,
+                                                            const int32_t *__restrict__ S3, int nS3,
+                                                            const double *__restrict__ y, double C, int l,
+                                                            double *__restrict__ beta, double *__restrict__ bnew,
+                                                            int32_t *__restrict__ cols, double *__restrict__ w,
+                                                            int32_t *__restrict__ nc_out, double *__restrict__ cst_out) {
+    __shared__ int part[1024 / 64];
+    __shared__ int base_s;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
+    for (int j = t; j <= l; j += blockDim.x) bnew[j] = 0.0;
+    __syncthreads();
+    for (int i = t; i < nS1; i += blockDim.x) bnew[S1[i]] = x[i] * y[S1[i]];
+    for (int o = t; o < nS3; o += blockDim.x) bnew[S3[o]] = C * y[S3[o]];
+    if (t == 0) {
+        bnew[l] = x[nS1];
+        base_s = 0;
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < l; j0 += blockDim.x) {
+        const int j = j0 + t;
+        const bool ch = (j < l) && (bnew[j] != beta[j]);
+        const unsigned long long m = __ballot(ch);
+        const int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) part[wv] = __popcll(m);
+        __syncthreads();
+        int off = base_s;
+        for (int k = 0; k < wv; k++) off += part[k];
+        if (ch) {
+            cols[off + below] = j;
+            w[off + below] = bnew[j] - beta[j];
+        }
+        __syncthreads();
+        if (t == 0) {
+            int s = 0;
+            for (int k = 0; k < nw; k++) s += part[k];
+            base_s += s;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        *nc_out = base_s;
+        *cst_out = -(bnew[l] - beta[l]);
+    }
+    __syncthreads();
+    for (int j = t; j <= l; j += blockDim.x) beta[j] = bnew[j];
+}
+
 // y = A x (trans 0) or A^T x (trans 1) for a column-major m x n matrix A.
 // trans 0: lanes over 64 rows; the columns are cut into nsl slices, one per
 // wave (four per work-group along y), each summed in column order into
@@ -117,6 +199,28 @@ void irwls_dev_build_kkt(const double *Kw, int ldk, const int32_t *S1, int nS1, 
     const int N = nS1 + 1;
     TimerScope ts(IRWLS_T_KKT, 0.0, 8.0 * (double)N * N * 2.0);
     ts.launch(build_kkt_kernel, dim3((N + 255) / 256, N), dim3(256), 0, Kw, ldk, S1, nS1, ys, as, H, ldh);
+}
+
+void irwls_dev_kkt_rhs(const int32_t *S1, int nS1, const double *G13, int use_g13, const double *gin, double last,
+                       double *rhs) {
+    hipLaunchKernelGGL(kkt_rhs_kernel, dim3((nS1 + 1 + 255) / 256), dim3(256), 0, stream(), S1, nS1, G13, use_g13,
+                       gin, last, rhs);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_inner_update(const double *x, const int32_t *S1, int nS1, const int32_t *S3, int nS3, const double *y,
+                            double C, int l, double *beta, double *bnew, int32_t *cols, double *w, int32_t *nc,
+                            double *cst) {
+    hipLaunchKernelGGL(inner_update_kernel, dim3(1), dim3(1024), 0, stream(), x, S1, nS1, S3, nS3, y, C, l, beta,
+                       bnew, cols, w, nc, cst);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_gemv_gather_dn(const double *Kw, int ldk, int nr, const int32_t *cols, const double *w,
+                              const int32_t *nc, const double *cst, double a, double b, double *out) {
+    if (nr <= 0) return;
+    TimerScope ts(IRWLS_T_GEMV, 0.0, 0.0);
+    ts.launch(gemv_gather_dn_kernel, dim3((nr + 3) / 4), dim3(256), 0, Kw, ldk, nr, cols, w, nc, cst, a, b, out);
 }
 
 void irwls_dev_gemv_gather(const double *Kw, int ldk, const int32_t *rows, int nr, const int32_t *cols, int nc,

@@ -563,6 +563,12 @@ void irwls_ws_reserve(irwls_ws *ws, int cap) {
     ws->H = (double *)irwls_dev_alloc((c + 2) * (c + 1) * sizeof(double));
     ws->esub = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
     ws->G13 = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->x = (double *)irwls_dev_alloc((c + 2) * sizeof(double));
+    ws->yl = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->gin = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->beta = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->bnew = (double *)irwls_dev_alloc((c + 1) * sizeof(double));
+    ws->nc_cst = (double *)irwls_dev_alloc(2 * sizeof(double));
     const size_t ni = (c + 2) / 2, nd = c + 1;   /* c + 1 int32 or double entries, in doubles */
     ws->span_len[0] = ni + 3 * nd;
     ws->span_len[1] = ni + nd;
@@ -616,6 +622,12 @@ void irwls_ws_free(irwls_ws *ws) {
     irwls_dev_free(ws->H);
     irwls_dev_free(ws->esub);
     irwls_dev_free(ws->G13);
+    irwls_dev_free(ws->x);
+    irwls_dev_free(ws->yl);
+    irwls_dev_free(ws->gin);
+    irwls_dev_free(ws->beta);
+    irwls_dev_free(ws->bnew);
+    irwls_dev_free(ws->nc_cst);
     irwls_dev_free(ws->pk);
     free(ws->hpk);
     memset(ws, 0, sizeof(*ws));

@@ -41,28 +41,30 @@ int *rpermute(int n) {
     return p;
 }
 
-/* Dense solve of the KKT system on the device.  FP64: H (N x N, leading
-   dimension N+1) is factored with the right-hand side riding in row N, then
-   one backward sweep.  Mixed precision: the closed form of the same result
-   from two SPD solves (mixed_kkt.c), FP64 when that declines.  The solution
-   is copied to x (host). */
-static void kkt_solve(irwls_ws *ws, int N, const double *et, double *x, const int32_t *S1, const double *ys,
-                      long gen) {
+/* Dense solve of the KKT system on the device, right-hand side in ws->rhs
+   (formed on the device).  FP64: H (N x N, leading dimension N+1) is
+   factored with the right-hand side riding in row N, then one backward
+   sweep; the solution stays in ws->x and 0 is returned.  Mixed precision:
+   the closed form of the same result from two SPD solves (mixed_kkt.c),
+   computed on the host into x and also copied to ws->x, and 1 is returned;
+   FP64 when that declines.  `last` is the right-hand side's last entry. */
+static int kkt_solve(irwls_ws *ws, int N, double last, double *x, const int32_t *S1, const double *ys, long gen) {
     double t0 = irwls_now();
     int done = 0;
     if (irwls_opts()->precision == 1 || irwls_opts()->precision == 2) {
-        done = irwls_kkt_solve_mixed(ws->H, N - 1, S1, gen, ys, et, x, &g_stats);
+        done = irwls_kkt_solve_mixed(ws->H, N - 1, S1, gen, ys, ws->rhs, ws->ys, last, x, &g_stats);
         if (!done && N - 1 >= 128) g_stats.fp64_fallbacks++;
+        if (done) irwls_dev_h2d(ws->x, x, (size_t)N * sizeof(double));
     }
     if (!done) {
-        /* the right-hand side went up with the KKT span; failure code ignored, as the reference; in a batched run the solve
+        /* failure code ignored, as the reference; in a batched run the solve
            joins the other problems' pending solves */
-        if (!irwls_batch_solve(ws->H, N, N + 1, ws->rhs, ws->w)) (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->w);
-        irwls_dev_d2h(x, ws->w, (size_t)N * sizeof(double));
+        if (!irwls_batch_solve(ws->H, N, N + 1, ws->rhs, ws->x)) (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->x);
         g_stats.factorizations++;
     }
     g_stats.solves++;
     g_stats.t_solve += irwls_now() - t0;
+    return done;
 }
 
 double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_widx, int l, const double *y,
@@ -77,7 +79,6 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
     int32_t *S1 = (int32_t *)irwls_xcalloc(l + 1, sizeof(int32_t));
     int32_t *S3 = (int32_t *)irwls_xcalloc(l + 1, sizeof(int32_t));
     double *et = (double *)irwls_xcalloc(l + 1, sizeof(double));
-    double *G13 = (double *)irwls_xcalloc(l + 1, sizeof(double));
     double *hbuf = (double *)irwls_xcalloc(l + 1, sizeof(double));
     double *hbuf2 = (double *)irwls_xcalloc(l + 1, sizeof(double));
     int32_t *cols = (int32_t *)irwls_xcalloc(l + 1, sizeof(int32_t));
@@ -89,6 +90,11 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
     /* the working-set kernel block, evaluated once */
     if (l > 0) irwls_kmat(dd, d_widx, l, d_widx, l, props.kernelType, props.Kgamma, ws->Kw, l);
     irwls_dev_h2d(ws->esub, e, (size_t)l * sizeof(double));
+    /* the labels, GIN and the weights on the device, where the right-hand
+       sides, the weights after each solve and the error updates are formed */
+    irwls_dev_h2d(ws->yl, y, (size_t)l * sizeof(double));
+    irwls_dev_h2d(ws->gin, GIN, (size_t)(l + 1) * sizeof(double));
+    irwls_dev_h2d(ws->beta, beta, (size_t)(l + 1) * sizeof(double));
 
     /* initial weights and groups (src/full-train.c:131-155) */
     for (int i = 0; i < l; i++) {
@@ -97,29 +103,46 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
         else if (beta[i] == y[i] * C) { S3[nS3++] = i; grp[i] = 3; }
         else { S1[nS1++] = i; grp[i] = 1; }
     }
+    /* G13 (device, entries 0..nS1-1) and its last entry g (host): zero until
+       the first regrouping computes them */
+    int use_g13 = 0;
+    double g13last = 0.0;
+    memcpy(ws->hS3, S3, (size_t)nS3 * sizeof(int32_t));
+    irwls_ws_upload(ws, 2);   /* S3 of the current groups */
+    int32_t *d_nc = (int32_t *)ws->nc_cst;
+    double *d_cst = ws->nc_cst + 1;
 
     /* loop rule of src/full-train.c:158 (C precedence) */
     while ((iter < 5) || ((minb < 0.0 || maxb > C) && iter < 1000 && since < 5 && dW / nW > 1e-6)) {
         iter++;
         g_stats.inner_iters++;
         const int N = nS1 + 1;
-        /* KKT system: H from the kernel block, rhs on the host */
+        /* KKT system: H from the kernel block, the right-hand side
+           1 - G13 - GIN[S1] and -G13_n - GIN_l on the device */
         for (int i = 0; i < nS1; i++) {
             hbuf[i] = y[S1[i]];
             hbuf2[i] = a[S1[i]];
-            et[i] = 1.0 - G13[i] - GIN[S1[i]];
         }
-        et[nS1] = -G13[nS1] - GIN[l];
+        const double last = -g13last - GIN[l];
         memcpy(ws->hS1, S1, (size_t)N * sizeof(int32_t));
         memcpy(ws->hys, hbuf, (size_t)N * sizeof(double));
         memcpy(ws->has, hbuf2, (size_t)N * sizeof(double));
-        memcpy(ws->hrhs, et, (size_t)N * sizeof(double));
         irwls_ws_upload(ws, 0);
+        irwls_dev_kkt_rhs(ws->S1, nS1, ws->G13, use_g13, ws->gin, last, ws->rhs);
         irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
-        memset(bAux, 0, (size_t)N * sizeof(double));
-        kkt_solve(ws, N, et, bAux, S1, hbuf, gen);
+        const int on_host = kkt_solve(ws, N, last, bAux, S1, hbuf, gen);
 
-        /* weights, bounds and change (src/full-train.c:205-230) */
+        /* weights after the solve and the error update over the changed
+           weights (src/full-train.c:205-249), on the device; one read-back
+           of the solution and the errors */
+        irwls_dev_inner_update(ws->x, ws->S1, nS1, ws->S3, nS3, ws->yl, C, l, ws->beta, ws->bnew, ws->cols, ws->w,
+                               d_nc, d_cst);
+        irwls_dev_gemv_gather_dn(ws->Kw, l, l, ws->cols, ws->w, d_nc, d_cst, 1.0, -1.0, ws->esub);
+        if (on_host) irwls_dev_d2h(e, ws->esub, (size_t)l * sizeof(double));
+        else irwls_dev_d2h2(bAux, ws->x, (size_t)N * sizeof(double), e, ws->esub, (size_t)l * sizeof(double));
+
+        /* weights, bounds and change (src/full-train.c:205-230), the same
+           values as the device's */
         dW = 0.0;
         nW = 0.0;
         maxb = 0.0;
@@ -137,21 +160,6 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
             dW += t * t;
             nW += beta[i] * beta[i];
         }
-        /* error update over the changed weights (src/full-train.c:236-249) */
-        int nc = 0;
-        for (int j = 0; j < l; j++) {
-            if (bNew[j] != beta[j]) {
-                cols[nc] = j;
-                hbuf[nc] = bNew[j] - beta[j];
-                nc++;
-            }
-        }
-        memcpy(ws->hcols, cols, (size_t)nc * sizeof(int32_t));
-        memcpy(ws->hw, hbuf, (size_t)nc * sizeof(double));
-        irwls_ws_upload(ws, 1);
-        irwls_dev_gemv_gather(ws->Kw, l, NULL, l, ws->cols, nc, ws->w, 1.0, -1.0, -(bNew[l] - beta[l]), NULL,
-                              ws->esub);
-        irwls_dev_d2h(e, ws->esub, (size_t)l * sizeof(double));
         if (dW / nW < bestDW) {
             bestDW = dW / nW;
             since = 0;
@@ -182,8 +190,11 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
                 nS3++;
             }
         }
-        /* G13 = K(S1,S3) (C y) (src/full-train.c:313-344) */
-        memset(G13, 0, (size_t)(nS1 + 1) * sizeof(double));
+        /* G13 = K(S1,S3) (C y) (src/full-train.c:313-344), left on the device
+           for the next right-hand side; the new S3 goes up with it */
+        use_g13 = 0;
+        g13last = 0.0;
+        memcpy(ws->hS3, S3, (size_t)nS3 * sizeof(int32_t));
         if (nS3 > 0) {
             double g = 0.0;
             for (int o = 0; o < nS3; o++) {
@@ -191,17 +202,20 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
                 g += et[o] * y[S3[o]];
             }
             for (int i = 0; i < nS1; i++) hbuf2[i] = y[S1[i]];
+            g13last = g;
             if (nS1 > 0) {
                 memcpy(ws->hS1g, S1, (size_t)nS1 * sizeof(int32_t));
-                memcpy(ws->hS3, S3, (size_t)nS3 * sizeof(int32_t));
                 memcpy(ws->hwg, hbuf, (size_t)nS3 * sizeof(double));
                 memcpy(ws->hrs, hbuf2, (size_t)nS1 * sizeof(double));
                 irwls_ws_upload(ws, 2);
                 irwls_dev_gemv_gather(ws->Kw, l, ws->S1g, nS1, ws->S3, nS3, ws->wg, 0.0, 1.0, 0.0, ws->rs,
                                       ws->G13);
-                irwls_dev_d2h(G13, ws->G13, (size_t)nS1 * sizeof(double));
+                use_g13 = 1;
+            } else {
+                irwls_ws_upload(ws, 2);
             }
-            G13[nS1] = g;
+        } else {
+            irwls_ws_upload(ws, 2);
         }
     }
     free(a);
@@ -211,7 +225,6 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
     free(S1);
     free(S3);
     free(et);
-    free(G13);
     free(hbuf);
     free(hbuf2);
     free(cols);

@@ -108,6 +108,11 @@ typedef struct irwls_ws {
     double *ys, *as, *rhs, *w, *wg, *rs;
     int32_t *hS1, *hcols, *hS1g, *hS3;
     double *hys, *has, *hrhs, *hw, *hwg, *hrs;
+    /* device state of the inner loop: the solve's solution (x), the labels
+       (yl), GIN, the weights (beta) and the weights after a solve (bnew) of
+       the working set; nc_cst holds the changed-column count (int32 at
+       word 0) and the error update's constant (word 1) */
+    double *x, *yl, *gin, *beta, *bnew, *nc_cst;
 } irwls_ws;
 
 void irwls_ws_reserve(irwls_ws *ws, int cap);
@@ -125,7 +130,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
    block (one per subIRWLS call): factors of an older one are never reused. */
 long irwls_mixed_new_generation(void);
 int irwls_kkt_solve_mixed(const double *d_H, int nS1, const int32_t *S1, long gen, const double *ys,
-                          const double *et, double *x, irwls_gpu_stats *st);
+                          const double *d_rhs, const double *d_ys, double last, double *x, irwls_gpu_stats *st);
 void irwls_mixed_release(void);
 
 /* Learned SGMA proposal (proposal.c). */

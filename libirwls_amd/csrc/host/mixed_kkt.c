@@ -135,10 +135,12 @@ static int map_to_factor(const int32_t *S1, int n) {
 }
 
 /* Solves with H (device, N = nS1+1, leading dimension N+1, unfactored) for
-   the right-hand side et; ys = y over S1 (host).  Writes the N entries of x
-   and returns 1, or returns 0 when the caller should use the FP64 path. */
+   the right-hand side whose first nS1 entries are d_rhs (device) and whose
+   last entry is `last`; ys = y over S1 (host, and d_ys on the device).
+   Writes the N entries of x and returns 1, or returns 0 when the caller
+   should use the FP64 path. */
 int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen, const double *ys,
-                          const double *et, double *x, irwls_gpu_stats *st) {
+                          const double *d_rhs, const double *d_ys, double last, double *x, irwls_gpu_stats *st) {
     const int n = nS1;
     const int ldh = nS1 + 2;
     if (n < 128) return 0;
@@ -158,9 +160,8 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
         if (!refactor(H, n, ldh, S1, gen, st)) return 0;
     }
     /* right-hand sides: r1 and the border y */
-    memcpy(g_mc.hX, et, (size_t)n * sizeof(double));
-    memcpy(g_mc.hX + n, ys, (size_t)n * sizeof(double));
-    irwls_dev_h2d(g_mc.B, g_mc.hX, (size_t)2 * n * sizeof(double));
+    irwls_dev_d2d(g_mc.B, d_rhs, (size_t)n * sizeof(double));
+    irwls_dev_d2d(g_mc.B + n, d_ys, (size_t)n * sizeof(double));
     double rel[2];
     int it;
     const int nf = g_mc.n;
@@ -192,7 +193,7 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
         yu += ys[i] * u[i];
     }
     const double s = 0.0 - yz;   /* Schur value of the last pivot (H_nn = 0) */
-    const double b = (et[n] - yu) / (s * s);
+    const double b = (last - yu) / (s * s);
     for (int i = 0; i < n; i++) x[i] = u[i] - z[i] * b;
     x[n] = b;
     return 1;
