@@ -479,7 +479,10 @@ def bench_budgeted(args, name, steps, warmup):
              "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"],
                              proposal_rows=prow)}
     if prow:
-        modes["learned_exact_gains"] = dict(modes["learned"], proposal_rows=0)
+        # the network shortlists 16 of plain SGMA's own candidates (the same
+        # rand() draws), whose exact gains on all rows decide
+        modes["learned_exact_gains"] = dict(modes["learned"], proposal_pool=wl["cand"], proposal_keep=16,
+                                            proposal_rows=0)
     if args.mode in ("plain", "learned"):
         modes = {args.mode: modes[args.mode]}
     reset = dict(sgma_candidates=64, proposal=0, proposal_keep=8, proposal_pool=0, proposal_rows=0)
@@ -503,7 +506,7 @@ def bench_budgeted(args, name, steps, warmup):
         m = lib.budgeted_model(props, ds, cent, beta)
         acc = float(np.mean(lib.hard_test(td, m, pp) == yt))
         ftr = lib.soft_test(ds, m, pp)
-        runs[mode] = dict(value=float(np.mean(times)), sgma_s=state["sgma_s"], acc=acc, tm=tm, tw=tw, dom=dom,
+        runs[mode] = dict(value=float(np.mean(times)), sgma_s=state["sgma_s"], acc=acc, tm=tm, tw=tw, dom=dom, cent=cent,
                           sgma_stats=state["sst"], par_stats=state["pst"], steps=nst, model=m,
                           resid=objective.nystrom_residual(ds, cent, hold, wl["gamma"]),
                           primal=objective.primal_objective(ds, cent, beta, ftr, wl["C"], wl["gamma"]))
@@ -543,6 +546,11 @@ def bench_budgeted(args, name, steps, warmup):
     if "learned_exact_gains" in runs and "plain" in runs:
         block["exact_gains_speedup_vs_plain"] = runs["plain"]["value"] / runs["learned_exact_gains"]["value"]
         block["exact_gains_accuracy_delta"] = runs["learned_exact_gains"]["acc"] - runs["plain"]["acc"]
+        block["exact_gains_config"] = {"pool": "plain SGMA's %d candidates" % wl["cand"], "keep": 16,
+                                       "gains_on_rows": n,
+                                       "centroids_shared_with_plain": float(len(
+                                           set(np.asarray(runs["learned_exact_gains"]["cent"]).tolist())
+                                           & set(np.asarray(runs["plain"]["cent"]).tolist())) / wl["B"])}
     if args.accuracy_seeds > 0:
         block["accuracy_over_seeds"] = budgeted_seed_sweep(wl, args.accuracy_seeds, modes)
     return block
