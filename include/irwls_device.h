@@ -69,6 +69,7 @@ enum irwls_timer_kind {
 };
 void irwls_dev_timing(int enable);          /* enable/disable and reset */
 void irwls_dev_timing_mask(unsigned mask);  /* bit k set: kind k is timed (default: all) */
+void irwls_dev_timing_stride(int stride);   /* time every stride-th launch of a kind (default 1: all) */
 /* Accumulated launches, milliseconds, algorithmic flops and bytes. */
 void irwls_dev_timer_get(int kind, long *launches, double *ms, double *flops, double *bytes);
 

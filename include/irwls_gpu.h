@@ -48,6 +48,11 @@ void irwls_gpu_unpin_dataset(svm_dataset dataset);
 void irwls_gpu_timing(int enable);
 /* Restrict timing to the kinds whose bit is set (bit k = timer kind k; default all). */
 void irwls_gpu_timing_mask(unsigned mask);
+/* Time only every stride-th launch of each timed kind (1, the default:
+   every launch).  The events add a few microseconds to each timed launch;
+   a stride keeps that out of the measured run while the sampled launches
+   still give the kind's average launch duration. */
+void irwls_gpu_timing_stride(int stride);
 void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double *bytes);
 
 /* Counters of the last training call. */

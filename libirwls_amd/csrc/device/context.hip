@@ -30,6 +30,11 @@ bool dev_initialised() { return g_init; }
 // for the totals, so timing does not serialise the stream.
 static bool g_timing = false;
 static unsigned g_timing_mask = ~0u;   // kinds that are timed while timing is on
+// Every g_stride-th launch scope of a timed kind carries events (1: all).
+// The others run unobserved, and their work is not counted either, so a
+// kind's time, launches, flops and bytes describe the same sampled launches.
+static int g_stride = 1;
+static long g_seen[IRWLS_T_COUNT];
 static long g_launch[IRWLS_T_COUNT];
 static double g_ms[IRWLS_T_COUNT], g_flops[IRWLS_T_COUNT], g_bytes[IRWLS_T_COUNT];
 static const int kPool = 4096;
@@ -74,6 +79,10 @@ TimerScope::TimerScope(int k, double f, double b, bool any_thread, bool mk)
       idx(-1), nk(0) {
     if (!on) return;
     std::lock_guard<std::mutex> lk(g_tmu);
+    if (g_stride > 1 && (g_seen[kind]++ % g_stride) != 0) {
+        on = false;
+        return;
+    }
     if (g_evn == kPool && !resolve_pool_locked()) {
         on = false;   // pool full of pairs still in flight: this launch goes untimed
         return;
@@ -568,10 +577,13 @@ void irwls_dev_trim(void) {
 
 void irwls_dev_timing_mask(unsigned mask) { g_timing_mask = mask; }
 
+void irwls_dev_timing_stride(int stride) { g_stride = stride > 1 ? stride : 1; }
+
 void irwls_dev_timing(int enable) {
     irwls_dev_init();
     resolve_pool();
     g_timing = enable != 0;
+    memset(g_seen, 0, sizeof(g_seen));
     memset(g_launch, 0, sizeof(g_launch));
     memset(g_ms, 0, sizeof(g_ms));
     memset(g_flops, 0, sizeof(g_flops));

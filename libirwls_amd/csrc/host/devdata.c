@@ -81,6 +81,7 @@ void irwls_gpu_set_options(const irwls_gpu_options *o) {
 
 void irwls_gpu_timing(int enable) { irwls_dev_timing(enable); }
 void irwls_gpu_timing_mask(unsigned mask) { irwls_dev_timing_mask(mask); }
+void irwls_gpu_timing_stride(int stride) { irwls_dev_timing_stride(stride); }
 
 void irwls_gpu_timer(int kind, long *launches, double *ms, double *flops, double *bytes) {
     irwls_dev_timer_get(kind, launches, ms, flops, bytes);

@@ -350,11 +350,13 @@ def set_options(**kw):
     return o
 
 
-def timing(enable=True, kinds=None):
-    """Per-kernel HIP-event timers; `kinds` restricts them to those timer names."""
+def timing(enable=True, kinds=None, stride=1):
+    """Per-kernel HIP-event timers; `kinds` restricts them to those timer
+    names, `stride` times only every stride-th launch of each kind."""
     L = load()
     mask = 0xFFFFFFFF if kinds is None else sum(1 << TIMER_KINDS.index(k) for k in kinds)
     L.irwls_gpu_timing_mask(mask)
+    L.irwls_gpu_timing_stride(int(stride))
     L.irwls_gpu_timing(1 if enable else 0)
 
 
