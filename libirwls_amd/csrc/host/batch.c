@@ -36,6 +36,7 @@
 #include "full-train.h"
 
 #define BATCH_SLOTS 4     /* batches in flight at once (round slots of the device solver) */
+#define BATCH_SMALL_N 512 /* systems up to this size are batched apart from the larger ones */
 
 typedef struct {
     pthread_mutex_t mu;
@@ -130,10 +131,23 @@ static int free_slot(const batch_sched *s) {
     return -1;
 }
 
-/* Runs the queued solves on round slot `slot`.  Called with the lock held;
-   releases it during the device work and returns with it held. */
-static void run_batch(batch_sched *s, int slot) {
-    const int k = s->npend;
+/* Size class of a system: a batch runs as long as its largest system, so
+   small systems are batched among themselves. */
+static int size_class_of(int n) { return n > BATCH_SMALL_N ? 1 : 0; }
+
+/* Queued problems of class c. */
+static int npend_class(const batch_sched *s, int c) {
+    int k = 0;
+    for (int i = 0; i < s->nprob; i++)
+        if (s->pending[i] == 1 && size_class_of(s->n[i]) == c) k++;
+    return k;
+}
+
+/* Runs the queued solves of size class c on round slot `slot`.  Called with
+   the lock held; releases it during the device work and returns with it
+   held. */
+static void run_batch(batch_sched *s, int slot, int c) {
+    const int k = npend_class(s, c);
     int *ids = (int *)irwls_xmalloc((size_t)k * sizeof(int));
     double **A = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
     int *n = (int *)irwls_xmalloc((size_t)k * sizeof(int));
@@ -142,7 +156,7 @@ static void run_batch(batch_sched *s, int slot) {
     double **x = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
     int m = 0;
     for (int i = 0; i < s->nprob && m < k; i++) {
-        if (s->pending[i] != 1) continue;
+        if (s->pending[i] != 1 || size_class_of(s->n[i]) != c) continue;
         ids[m] = i;
         A[m] = s->A[i];
         n[m] = s->n[i];
@@ -152,7 +166,7 @@ static void run_batch(batch_sched *s, int slot) {
         s->pending[i] = 2;
         m++;
     }
-    s->npend = 0;
+    s->npend -= m;
     s->inflight += m;
     s->busy |= 1u << slot;
     pthread_mutex_unlock(&s->mu);
@@ -172,11 +186,11 @@ static void run_batch(batch_sched *s, int slot) {
     pthread_cond_broadcast(&s->cv);
 }
 
-/* Earliest queue time of the queued problems (s->npend > 0). */
-static double first_queued(const batch_sched *s) {
+/* Earliest queue time of the queued problems of class c. */
+static double first_queued(const batch_sched *s, int c) {
     double t = 1e300;
     for (int i = 0; i < s->nprob; i++)
-        if (s->pending[i] == 1 && s->t_queue[i] < t) t = s->t_queue[i];
+        if (s->pending[i] == 1 && size_class_of(s->n[i]) == c && s->t_queue[i] < t) t = s->t_queue[i];
     return t;
 }
 
@@ -198,9 +212,10 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
         /* a thread whose own problem is queued starts the next batch, on a free round slot */
         const int slot = (s->pending[me] == 1) ? free_slot(s) : -1;
         if (slot >= 0) {
-            const double due = first_queued(s) + s->wait_s;
+            const int c = size_class_of(s->n[me]);
+            const double due = first_queued(s, c) + s->wait_s;
             if (s->npend == s->active - s->inflight || irwls_now() >= due) {
-                run_batch(s, slot);
+                run_batch(s, slot, c);
                 continue;
             }
             struct timespec ts;
