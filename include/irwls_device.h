@@ -170,14 +170,15 @@ void irwls_dev_kkt_rhs(const int32_t *d_S1, int nS1, const double *d_G13, int us
    (src/full-train.c:205-249): bnew = 0, bnew[S1[i]] = x[i] y[S1[i]],
    bnew[S3[o]] = C y[S3[o]], bnew[l] = x[nS1]; the columns j < l with
    bnew[j] != beta[j] in ascending order, w = bnew[j] - beta[j], their count
-   (*d_nc) and -(bnew[l] - beta[l]) (*d_cst); then beta = bnew. */
+   (int32 at word 0 of d_nc_cst) and -(bnew[l] - beta[l]) (word 1); then
+   beta = bnew. */
 void irwls_dev_inner_update(const double *d_x, const int32_t *d_S1, int nS1, const int32_t *d_S3, int nS3,
                             const double *d_y, double C, int l, double *d_beta, double *d_bnew, int32_t *d_cols,
-                            double *d_w, int32_t *d_nc, double *d_cst);
+                            double *d_w, double *d_nc_cst);
 /* irwls_dev_gemv_gather over rows 0..nr-1 with the column count and the
    constant read from device memory (as irwls_dev_inner_update leaves them). */
 void irwls_dev_gemv_gather_dn(const double *d_Kw, int ldk, int nr, const int32_t *d_cols, const double *d_w,
-                              const int32_t *d_nc, const double *d_cst, double a, double b, double *d_out);
+                              const double *d_nc_cst, double a, double b, double *d_out);
 
 /* y = A x (trans 0) or A^T x (trans 1), A column-major m x n. */
 void irwls_dev_gemv(const double *d_A, int m, int n, int lda, int trans, const double *d_x, double *d_y);
@@ -276,6 +277,39 @@ int  irwls_dev_posv_aug_batched(int nprob, double *const *d_A, const int *n, con
 int  irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *d_A, const int *n, const int *lda,
                                      const double *const *d_rhs, double *const *d_x, int *info);
 void irwls_dev_batched_autotune(int enable);
+
+/* One inner IRWLS iteration's device work on a working set
+   (src/full-train.c:158-249), all device pointers: the right-hand side
+   rhs[i] = 1 - G13[i] - gin[S1[i]] (G13 read as zero unless use_g13),
+   rhs[N-1] = last; the KKT matrix H (N = nS1 + 1, leading dimension
+   ldh >= N+1) from the kernel block Kw (l x l, leading dimension ldk), ys
+   and as (y and a over S1); its solve (LAPACK failure semantics) into x;
+   then the weights after it and the error update e -= Kw(:, changed) dw +
+   const, as irwls_dev_inner_update and irwls_dev_gemv_gather_dn do them
+   (beta, bnew: l+1 entries; cols, w: l; nc_cst: 2 words). */
+typedef struct irwls_inner_work {
+    double *H;
+    int N, ldh;
+    const double *Kw;
+    int ldk, l;
+    const int32_t *S1;
+    const double *ys, *as, *G13;
+    int use_g13;
+    const double *gin;
+    double last;
+    double *rhs, *x;
+    const int32_t *S3;
+    int nS3;
+    const double *yl;
+    double C;
+    double *beta, *bnew;
+    int32_t *cols;
+    double *w, *nc_cst, *e;
+} irwls_inner_work;
+/* The inner iterations of nprob problems as one batched round on round
+   slot `round` (as irwls_dev_posv_aug_batched_slot): every problem's
+   arithmetic is that of the single-problem kernels. */
+int  irwls_dev_inner_batched_slot(int round, int nprob, const irwls_inner_work *work);
 /* Rows per work-group (in 64-row tiles) chosen for batches whose largest
    size rounds up to the same multiple of 256 as nmax, 0 if none yet; ms3
    (may be NULL) receives the three timings.  nprob is not used. */

@@ -24,6 +24,7 @@
 // No kernel here waits on another work-group: every dependency is a launch
 // boundary on the library stream.
 #include "common.h"
+#include "working_set.h"
 
 #include <mutex>
 
@@ -1018,6 +1019,15 @@ struct BatchProb {
     int *info;
     const double *rhs;
     double *z, *x;
+    // the inner IRWLS iteration around the solve (irwls_inner_work; Kw null:
+    // the solve alone)
+    const double *Kw;
+    int ldk, l, use_g13, nS3;
+    double last, C;
+    const int32_t *S1, *S3;
+    const double *ys, *as, *G13, *gin, *yl;
+    double *rhs_w, *beta, *bnew, *w, *nc_cst, *e;
+    int32_t *cols;
 };
 
 __device__ __forceinline__ int nblk_of(int n) { return (n + NB - 1) / NB; }
@@ -1076,6 +1086,34 @@ __global__ __launch_bounds__(256) void b_bwd(const BatchProb *__restrict__ tab, 
     bwd_inv_body(p.A, p.n, p.lda, w, p.Dinv + (size_t)(2 * w + 1) * TILE, p.z, p.x, cpw, blockIdx.x);
 }
 
+// The inner iteration's work before and after the solve, for the problems
+// that carry it (Kw set), with the single-problem kernels' device bodies.
+__global__ void b_kkt_rhs(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    if (!p.Kw || p.n <= 0) return;
+    kkt_rhs_at(p.S1, p.n - 1, p.G13, p.use_g13, p.gin, p.last, p.rhs_w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void b_build_kkt(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (!p.Kw || i >= p.n || j >= p.n) return;
+    build_kkt_at(p.Kw, p.ldk, p.S1, p.n - 1, p.ys, p.as, p.A, p.lda, i, j);
+}
+
+__global__ __launch_bounds__(1024) void b_inner_update(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    if (!p.Kw || p.n <= 0) return;
+    inner_update_wg(p.x, p.S1, p.n - 1, p.S3, p.nS3, p.yl, p.C, p.l, p.beta, p.bnew, p.cols, p.w, p.nc_cst);
+}
+
+__global__ __launch_bounds__(256) void b_gemv_dn(const BatchProb *__restrict__ tab) {
+    const BatchProb p = tab[blockIdx.z];
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (!p.Kw || i >= p.l) return;
+    gemv_dn_row(p.Kw, p.ldk, p.cols, p.w, p.nc_cst, 1.0, -1.0, p.e, i);
+}
+
 // SPD test systems for the autotuner: A = I * (n+1) + 1/(1+|i-j|), rhs = 1.
 __global__ void b_fill_test(const BatchProb *__restrict__ tab) {
     const BatchProb p = tab[blockIdx.z];
@@ -1089,10 +1127,17 @@ __global__ void b_fill_test(const BatchProb *__restrict__ tab) {
 
 constexpr int B_CPW = 8;
 
-// The launch sequence of one batched solve, sized for problems up to ncap.
-void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStream_t st) {
+// The launch sequence of one batched solve, sized for problems up to ncap;
+// with lcap > 0 (working sets up to lcap rows) also the inner iteration's
+// right-hand side and KKT assembly before it, and the weights after it and
+// the error update, of every problem that carries them.
+void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int lcap, int R, hipStream_t st) {
     const int nblk = (ncap + NB - 1) / NB;
     const int vb = (ncap + 1 + 255) / 256;
+    if (lcap > 0) {
+        hipLaunchKernelGGL(b_kkt_rhs, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
+        hipLaunchKernelGGL(b_build_kkt, dim3(vb, ncap, nprob), dim3(256), 0, st, d_tab);
+    }
     hipLaunchKernelGGL(b_prepare, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
     hipLaunchKernelGGL(b_diag_first, dim3(1, 1, nprob), dim3(256), 0, st, d_tab);
     for (int k = 0; k < nblk; k++) {
@@ -1118,6 +1163,10 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStr
         const int g = cols > 0 ? (cols + 4 * B_CPW - 1) / (4 * B_CPW) : 1;
         hipLaunchKernelGGL(b_bwd, dim3(g, 1, nprob), dim3(256), 0, st, d_tab, w, B_CPW);
     }
+    if (lcap > 0) {
+        hipLaunchKernelGGL(b_inner_update, dim3(1, 1, nprob), dim3(1024), 0, st, d_tab);
+        hipLaunchKernelGGL(b_gemv_dn, dim3((lcap + 3) / 4, 1, nprob), dim3(256), 0, st, d_tab);
+    }
 }
 
 // Per batch shape class: the captured graph of each tile shape and the
@@ -1127,7 +1176,7 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int R, hipStr
 // graphs of one class serve every batch in it.  The tile shape is tuned once
 // per capacity, on the first batch that has it.
 struct BatchPlan {
-    int nslot = 0, ncap = 0;
+    int nslot = 0, ncap = 0, lcap = 0;
     int round = -1;           // the round slot whose calls use this plan
     BatchProb *d_tab = nullptr;
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
@@ -1153,9 +1202,11 @@ std::mutex g_bmu;
 int slots_for(int nprob) { return (nprob + 15) / 16 * 16; }
 
 // Called with g_bmu held.
-BatchPlan *plan_for(int nslot, int ncap, int round) {
+BatchPlan *plan_for(int nslot, int ncap, int lcap, int round) {
     for (int i = 0; i < g_nplans; i++)
-        if (g_plans[i].nslot == nslot && g_plans[i].ncap == ncap && g_plans[i].round == round) return &g_plans[i];
+        if (g_plans[i].nslot == nslot && g_plans[i].ncap == ncap && g_plans[i].lcap == lcap &&
+            g_plans[i].round == round)
+            return &g_plans[i];
     BatchPlan *pl = nullptr;
     if (g_nplans < kMaxPlans) {
         pl = &g_plans[g_nplans++];
@@ -1173,6 +1224,7 @@ BatchPlan *plan_for(int nslot, int ncap, int round) {
     }
     pl->nslot = nslot;
     pl->ncap = ncap;
+    pl->lcap = lcap;
     pl->round = round;
     pl->d_tab = (BatchProb *)irwls_dev_alloc((size_t)nslot * sizeof(BatchProb));
     return pl;
@@ -1201,7 +1253,7 @@ hipGraphExec_t graph_for(BatchPlan *pl, int e) {
     hipGraph_t g;
     capture_lock();
     IRWLS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-    batched_sequence(pl->d_tab, pl->nslot, pl->ncap, kShapes[e], stream());
+    batched_sequence(pl->d_tab, pl->nslot, pl->ncap, pl->lcap, kShapes[e], stream());
     IRWLS_HIP(hipStreamEndCapture(stream(), &g));
     capture_unlock();
     IRWLS_HIP(hipGraphInstantiate(&pl->exec[e], g, nullptr, nullptr, 0));
@@ -1216,6 +1268,7 @@ void autotune(BatchPlan *pl, BatchTune *tu, const BatchProb *host_tab) {
     BatchProb *t = (BatchProb *)malloc((size_t)np * sizeof(BatchProb));
     for (int i = 0; i < np; i++) {
         t[i] = host_tab[i];
+        t[i].Kw = nullptr;   // test systems: the solve alone
         const int nb = nblk_of_host(t[i].n);
         t[i].A = (double *)irwls_dev_alloc((size_t)t[i].lda * (t[i].n > 0 ? t[i].n : 1) * sizeof(double));
         t[i].Dinv = (double *)irwls_dev_alloc((size_t)(nb + 1) * 2 * TILE * sizeof(double));
@@ -1254,6 +1307,86 @@ void autotune(BatchPlan *pl, BatchTune *tu, const BatchProb *host_tab) {
         irwls_dev_free(t[i].z);
         irwls_dev_free(t[i].x);
     }
+    free(t);
+}
+
+// One batched round on round slot `round`: problems in[0..nprob) (A, n,
+// lda, rhs, x and, with lcap > 0, the inner iteration's operands) go into
+// the slot's device table, and the captured sequence of the table's class
+// runs on the calling thread's stream; info[p] gets problem p's potrf info
+// (info may be null).
+void batched_round(int round, int nprob, const BatchProb *in, int lcap, int *info) {
+    if (round < 0 || round >= kRounds) {
+        fprintf(stderr, "libirwls: batched solve: round slot %d outside [0, %d)\n", round, kRounds);
+        exit(2);
+    }
+    int nmax = 0;
+    for (int p = 0; p < nprob; p++) {
+        if (in[p].lda < in[p].n + 1) {
+            fprintf(stderr, "libirwls: irwls_dev_posv_aug_batched: problem %d needs lda >= n+1 (lda=%d, n=%d)\n", p,
+                    in[p].lda, in[p].n);
+            exit(2);
+        }
+        if (in[p].n > nmax) nmax = in[p].n;
+    }
+    const int ncap = (nmax + 255) / 256 * 256;
+    const int nslot = slots_for(nprob);
+    BatchPlan *pl;
+    BatchTune *tu;
+    {
+        std::lock_guard<std::mutex> lk(g_bmu);
+        pl = plan_for(nslot, ncap, lcap, round);
+        tu = tune_for(ncap);
+    }
+    // per-problem info words (together, read back with one copy), tile
+    // inverses and forward-sweep vectors; one pool per round slot
+    static double *pools[kRounds] = {};
+    static size_t pool_szs[kRounds] = {};
+    double *&pool = pools[round];
+    size_t &pool_sz = pool_szs[round];
+    size_t need = (size_t)nslot * 2 + 8;   // 4 info ints per slot
+    for (int p = 0; p < nprob; p++) need += (size_t)(nblk_of_host(in[p].n) + 1) * 2 * TILE + (size_t)in[p].n + 8;
+    if (pool_sz < need) {
+        if (pool) irwls_dev_free(pool);
+        pool = (double *)irwls_dev_alloc(need * sizeof(double));
+        pool_sz = need;
+    }
+    int *d_info = (int *)pool;   // slot p: d_info[4 p]
+    BatchProb *t = (BatchProb *)calloc((size_t)nslot, sizeof(BatchProb));
+    size_t off = (size_t)nslot * 2 + 8;
+    for (int p = 0; p < nslot; p++) {
+        if (p < nprob) t[p] = in[p];
+        t[p].info = d_info + 4 * p;
+        if (p >= nprob) continue;   // empty slot: n = 0, skipped by every kernel
+        t[p].Dinv = pool + off;
+        off += (size_t)(nblk_of_host(in[p].n) + 1) * 2 * TILE;
+        t[p].z = pool + off;
+        off += (size_t)in[p].n + 8;
+    }
+    int best;
+    {
+        std::lock_guard<std::mutex> lk(g_bmu);   // one tuning per capacity, whichever slot gets there first
+        if (tu->best < 0) {
+            if (g_autotune) autotune(pl, tu, t);
+            else tu->best = 0;
+        }
+        best = tu->best;
+    }
+    irwls_dev_h2d(pl->d_tab, t, (size_t)nslot * sizeof(BatchProb));
+    {
+        // capture (first use of this class on this slot) before the timer's
+        // first event is recorded on this stream
+        hipGraphExec_t ex = graph_for(pl, best);
+        double fl = 0.0;
+        for (int p = 0; p < nprob; p++) fl += (double)in[p].n * in[p].n * in[p].n / 3.0;
+        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true, true);   // batches of different round slots may overlap
+        IRWLS_HIP(hipGraphLaunch(ex, stream()));
+    }
+    int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));
+    irwls_dev_d2h(hinfo, d_info, (size_t)nslot * 4 * sizeof(int));
+    if (info)
+        for (int p = 0; p < nprob; p++) info[p] = hinfo[4 * p];
+    free(hinfo);
     free(t);
 }
 
@@ -1317,83 +1450,61 @@ int irwls_dev_posv_aug_batched(int nprob, double *const *A, const int *n, const 
 int irwls_dev_posv_aug_batched_slot(int round, int nprob, double *const *A, const int *n, const int *lda,
                                     const double *const *rhs, double *const *x, int *info) {
     if (nprob <= 0) return 0;
-    if (round < 0 || round >= kRounds) {
-        fprintf(stderr, "libirwls: irwls_dev_posv_aug_batched_slot: round slot %d outside [0, %d)\n", round,
-                kRounds);
-        exit(2);
-    }
-    int nmax = 0;
+    BatchProb *in = (BatchProb *)calloc((size_t)nprob, sizeof(BatchProb));
     for (int p = 0; p < nprob; p++) {
-        if (lda[p] < n[p] + 1) {
-            fprintf(stderr, "libirwls: irwls_dev_posv_aug_batched: problem %d needs lda >= n+1 (lda=%d, n=%d)\n", p,
-                    lda[p], n[p]);
+        in[p].A = A[p];
+        in[p].n = n[p];
+        in[p].lda = lda[p];
+        in[p].rhs = rhs[p];
+        in[p].x = x[p];
+    }
+    batched_round(round, nprob, in, 0, info);
+    free(in);
+    return 0;
+}
+
+int irwls_dev_inner_batched_slot(int round, int nprob, const irwls_inner_work *w) {
+    if (nprob <= 0) return 0;
+    BatchProb *in = (BatchProb *)calloc((size_t)nprob, sizeof(BatchProb));
+    int lmax = 0;
+    for (int p = 0; p < nprob; p++) {
+        const irwls_inner_work &q = w[p];
+        BatchProb &b = in[p];
+        b.A = q.H;
+        b.n = q.N;
+        b.lda = q.ldh;
+        b.rhs = q.rhs;
+        b.x = q.x;
+        b.Kw = q.Kw;
+        b.ldk = q.ldk;
+        b.l = q.l;
+        b.use_g13 = q.use_g13;
+        b.nS3 = q.nS3;
+        b.last = q.last;
+        b.C = q.C;
+        b.S1 = q.S1;
+        b.S3 = q.S3;
+        b.ys = q.ys;
+        b.as = q.as;
+        b.G13 = q.G13;
+        b.gin = q.gin;
+        b.yl = q.yl;
+        b.rhs_w = q.rhs;
+        b.beta = q.beta;
+        b.bnew = q.bnew;
+        b.w = q.w;
+        b.nc_cst = q.nc_cst;
+        b.e = q.e;
+        b.cols = q.cols;
+        if (q.l > lmax) lmax = q.l;
+        if (!q.Kw || q.l < q.N - 1) {
+            fprintf(stderr, "libirwls: irwls_dev_inner_batched_slot: problem %d needs its kernel block and l >= N-1\n",
+                    p);
             exit(2);
         }
-        if (n[p] > nmax) nmax = n[p];
     }
-    const int ncap = (nmax + 255) / 256 * 256;
-    const int nslot = slots_for(nprob);
-    BatchPlan *pl;
-    BatchTune *tu;
-    {
-        std::lock_guard<std::mutex> lk(g_bmu);
-        pl = plan_for(nslot, ncap, round);
-        tu = tune_for(ncap);
-    }
-    // per-problem info words (together, read back with one copy), tile
-    // inverses and forward-sweep vectors; one pool per round slot
-    static double *pools[kRounds] = {};
-    static size_t pool_szs[kRounds] = {};
-    double *&pool = pools[round];
-    size_t &pool_sz = pool_szs[round];
-    size_t need = (size_t)nslot * 2 + 8;   // 4 info ints per slot
-    for (int p = 0; p < nprob; p++) need += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE + (size_t)n[p] + 8;
-    if (pool_sz < need) {
-        if (pool) irwls_dev_free(pool);
-        pool = (double *)irwls_dev_alloc(need * sizeof(double));
-        pool_sz = need;
-    }
-    int *d_info = (int *)pool;   // slot p: d_info[4 p]
-    BatchProb *t = (BatchProb *)calloc((size_t)nslot, sizeof(BatchProb));
-    size_t off = (size_t)nslot * 2 + 8;
-    for (int p = 0; p < nslot; p++) {
-        t[p].info = d_info + 4 * p;
-        if (p >= nprob) continue;   // empty slot: n = 0, skipped by every kernel
-        t[p].A = A[p];
-        t[p].n = n[p];
-        t[p].lda = lda[p];
-        t[p].Dinv = pool + off;
-        off += (size_t)(nblk_of_host(n[p]) + 1) * 2 * TILE;
-        t[p].z = pool + off;
-        off += (size_t)n[p] + 8;
-        t[p].rhs = rhs[p];
-        t[p].x = x[p];
-    }
-    int best;
-    {
-        std::lock_guard<std::mutex> lk(g_bmu);   // one tuning per capacity, whichever slot gets there first
-        if (tu->best < 0) {
-            if (g_autotune) autotune(pl, tu, t);
-            else tu->best = 0;
-        }
-        best = tu->best;
-    }
-    irwls_dev_h2d(pl->d_tab, t, (size_t)nslot * sizeof(BatchProb));
-    {
-        // capture (first use of this class on this slot) before the timer's
-        // first event is recorded on this stream
-        hipGraphExec_t ex = graph_for(pl, best);
-        double fl = 0.0;
-        for (int p = 0; p < nprob; p++) fl += (double)n[p] * n[p] * n[p] / 3.0;
-        TimerScope ts(IRWLS_T_SYRK, fl, 0.0, true, true);   // batches of different round slots may overlap
-        IRWLS_HIP(hipGraphLaunch(ex, stream()));
-    }
-    int *hinfo = (int *)malloc((size_t)nslot * 4 * sizeof(int));
-    irwls_dev_d2h(hinfo, d_info, (size_t)nslot * 4 * sizeof(int));
-    if (info)
-        for (int p = 0; p < nprob; p++) info[p] = hinfo[4 * p];
-    free(hinfo);
-    free(t);
+    batched_round(round, nprob, in, (lmax + 255) / 256 * 256, nullptr);
+    free(in);
     return 0;
 }
 

@@ -7,6 +7,7 @@
 // Reference loops replaced: src/full-train.c:169-186 (H), :236-249 (inner e),
 // :313-344 (G13).
 #include "common.h"
+#include "working_set.h"
 
 using namespace irwls;
 
@@ -20,18 +21,7 @@ __global__ void build_kkt_kernel(const double *__restrict__ Kw, int ldk, const i
     const int i = blockIdx.x * blockDim.x + threadIdx.x;   // row (contiguous)
     const int j = blockIdx.y;                               // column
     if (i >= N) return;
-    double v;
-    if (i < nS1 && j < nS1) {
-        v = Kw[(size_t)S1[i] * ldk + S1[j]] * ys[i] * ys[j];
-        if (i == j) v += 1.0 / as[i];
-    } else if (i < nS1) {
-        v = ys[i];          // last column: y border
-    } else if (j < nS1) {
-        v = ys[j];          // last row: y border
-    } else {
-        v = 0.0;
-    }
-    H[(size_t)j * ldh + i] = v;
+    build_kkt_at(Kw, ldk, S1, nS1, ys, as, H, ldh, i, j);
 }
 
 // One wave per output row; lanes stride the gathered columns.
@@ -58,81 +48,29 @@ __global__ __launch_bounds__(256) void gemv_gather_kernel(
 __global__ __launch_bounds__(256) void gemv_gather_dn_kernel(const double *__restrict__ Kw, int ldk, int nr,
                                                              const int32_t *__restrict__ cols,
                                                              const double *__restrict__ w,
-                                                             const int32_t *__restrict__ nc_p,
-                                                             const double *__restrict__ cst_p, double a, double b,
+                                                             const double *__restrict__ nc_cst, double a, double b,
                                                              double *__restrict__ out) {
-    const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= nr) return;
-    const int nc = *nc_p;
-    const double *krow = Kw + (size_t)i * ldk;
-    double acc = 0.0;
-    for (int j = lane; j < nc; j += 64) acc += krow[cols[j]] * w[j];
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) out[i] = (a == 0.0 ? 0.0 : a * out[i]) + (b * acc) + *cst_p;
+    gemv_dn_row(Kw, ldk, cols, w, nc_cst, a, b, out, i);
 }
 
-// Right-hand side of the KKT system (src/full-train.c:160-168):
-// rhs_i = 1 - G13_i - GIN[S1_i] (i < nS1; G13 read as zero when use_g13 is 0),
-// rhs_nS1 = last (-G13_nS1 - GIN_l, formed by the host).
+// Right-hand side of the KKT system (src/full-train.c:160-168).
 __global__ void kkt_rhs_kernel(const int32_t *__restrict__ S1, int nS1, const double *__restrict__ G13, int use_g13,
                                const double *__restrict__ gin, double last, double *__restrict__ rhs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nS1) rhs[i] = 1.0 - (use_g13 ? G13[i] : 0.0) - gin[S1[i]];
-    else if (i == nS1) rhs[i] = last;
+    kkt_rhs_at(S1, nS1, G13, use_g13, gin, last, rhs, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // The weights after a KKT solve and the error-update operands
-// (src/full-train.c:205-249), one work-group: bnew = 0, bnew[S1_i] = x_i y[S1_i],
-// bnew[S3_o] = C y[S3_o], bnew_l = x_nS1; then the columns j < l whose
-// weight changed, in ascending order, with w = bnew_j - beta_j, their
-// count, the constant -(bnew_l - beta_l), and beta = bnew.
+// (src/full-train.c:205-249), one work-group (inner_update_wg).
 __global__ __launch_bounds__(1024) void inner_update_kernel(const double *__restrict__ x,
                                                             const int32_t *__restrict__ S1, int nS1,
                                                             const int32_t *__restrict__ S3, int nS3,
                                                             const double *__restrict__ y, double C, int l,
                                                             double *__restrict__ beta, double *__restrict__ bnew,
                                                             int32_t *__restrict__ cols, double *__restrict__ w,
-                                                            int32_t *__restrict__ nc_out, double *__restrict__ cst_out) {
-    __shared__ int part[1024 / 64];
-    __shared__ int base_s;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
-    for (int j = t; j <= l; j += blockDim.x) bnew[j] = 0.0;
-    __syncthreads();
-    for (int i = t; i < nS1; i += blockDim.x) bnew[S1[i]] = x[i] * y[S1[i]];
-    for (int o = t; o < nS3; o += blockDim.x) bnew[S3[o]] = C * y[S3[o]];
-    if (t == 0) {
-        bnew[l] = x[nS1];
-        base_s = 0;
-    }
-    __syncthreads();
-    for (int j0 = 0; j0 < l; j0 += blockDim.x) {
-        const int j = j0 + t;
-        const bool ch = (j < l) && (bnew[j] != beta[j]);
-        const unsigned long long m = __ballot(ch);
-        const int below = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) part[wv] = __popcll(m);
-        __syncthreads();
-        int off = base_s;
-        for (int k = 0; k < wv; k++) off += part[k];
-        if (ch) {
-            cols[off + below] = j;
-            w[off + below] = bnew[j] - beta[j];
-        }
-        __syncthreads();
-        if (t == 0) {
-            int s = 0;
-            for (int k = 0; k < nw; k++) s += part[k];
-            base_s += s;
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        *nc_out = base_s;
-        *cst_out = -(bnew[l] - beta[l]);
-    }
-    __syncthreads();
-    for (int j = t; j <= l; j += blockDim.x) beta[j] = bnew[j];
+                                                            double *__restrict__ nc_cst) {
+    inner_update_wg(x, S1, nS1, S3, nS3, y, C, l, beta, bnew, cols, w, nc_cst);
 }
 
 // y = A x (trans 0) or A^T x (trans 1) for a column-major m x n matrix A.
@@ -209,18 +147,17 @@ void irwls_dev_kkt_rhs(const int32_t *S1, int nS1, const double *G13, int use_g1
 }
 
 void irwls_dev_inner_update(const double *x, const int32_t *S1, int nS1, const int32_t *S3, int nS3, const double *y,
-                            double C, int l, double *beta, double *bnew, int32_t *cols, double *w, int32_t *nc,
-                            double *cst) {
+                            double C, int l, double *beta, double *bnew, int32_t *cols, double *w, double *nc_cst) {
     hipLaunchKernelGGL(inner_update_kernel, dim3(1), dim3(1024), 0, stream(), x, S1, nS1, S3, nS3, y, C, l, beta,
-                       bnew, cols, w, nc, cst);
+                       bnew, cols, w, nc_cst);
     IRWLS_LAUNCH_CHECK();
 }
 
 void irwls_dev_gemv_gather_dn(const double *Kw, int ldk, int nr, const int32_t *cols, const double *w,
-                              const int32_t *nc, const double *cst, double a, double b, double *out) {
+                              const double *nc_cst, double a, double b, double *out) {
     if (nr <= 0) return;
     TimerScope ts(IRWLS_T_GEMV, 0.0, 0.0);
-    ts.launch(gemv_gather_dn_kernel, dim3((nr + 3) / 4), dim3(256), 0, Kw, ldk, nr, cols, w, nc, cst, a, b, out);
+    ts.launch(gemv_gather_dn_kernel, dim3((nr + 3) / 4), dim3(256), 0, Kw, ldk, nr, cols, w, nc_cst, a, b, out);
 }
 
 void irwls_dev_gemv_gather(const double *Kw, int ldk, const int32_t *rows, int nr, const int32_t *cols, int nc,

@@ -1,19 +1,24 @@
 /*
  * Batched full training: many independent trainFULL problems (one-vs-rest
- * tasks, C/gamma grid points) in one call, their dense solves batched.
+ * tasks, C/gamma grid points) in one call, their inner IRWLS iterations
+ * batched.
  *
  * Each problem runs the unchanged trainFULL control flow on a host thread of
  * its own, with a device stream, scratch and staging buffers of its own
- * (irwls_dev_thread_begin), so the problems' kernel-block products, KKT
- * assembly and vector updates overlap on the device.  A problem that
- * reaches its KKT solve (src/full-train.c:192-199) finishes its stream's
- * work, queues the solve and waits; when every problem that is not already
- * in a running batch is waiting, or when the earliest waiting one has waited
- * a set time (IRWLS_BATCH_WAIT_US, default 1000 microseconds), one batched
- * factorization and solve runs for the waiting ones
- * (irwls_dev_posv_aug_batched_slot: one replay of a captured launch
- * sequence, tile shape chosen by its autotuner), and they continue.  Up to
- * BATCH_SLOTS batches run at once, each on the stream of the waiting problem
+ * (irwls_dev_thread_begin), so the problems' kernel-block products overlap
+ * on the device.  The device work of each inner IRWLS iteration
+ * (src/full-train.c:158-249: right-hand side, KKT assembly, the solve, the
+ * weights after it and the error update) runs in batched rounds: a problem
+ * that reaches it finishes its stream's work, queues the iteration and
+ * waits; when every problem that is not already in a running round is
+ * waiting, or when the earliest waiting one has waited a set time
+ * (IRWLS_BATCH_WAIT_US, default 1000 microseconds), one round runs the
+ * waiting ones' iterations together (irwls_dev_inner_batched_slot: one
+ * replay of a captured launch sequence that reads each problem's operands
+ * from a device table, tile shape chosen by its autotuner), and they
+ * continue.  Systems up to 512 rows are batched apart from larger ones.  Up
+ * to
+ * BATCH_SLOTS rounds run at once, each on the stream of the waiting problem
  * whose thread started it, so a problem that queues while a batch runs need
  * not wait for that batch.  The time limit keeps a problem busy with a longer
  * step (an outer iteration's kernel products) from holding back the others.
@@ -47,10 +52,7 @@ typedef struct {
     double wait_s;        /* longest wait before a partial batch runs */
     int *pending;         /* 0 idle, 1 queued, 2 in a running batch */
     double *t_queue;      /* when each queued problem queued */
-    double **A;
-    int *n, *lda;
-    const double **rhs;
-    double **x;
+    irwls_inner_work *work;   /* each queued problem's inner iteration */
 } batch_sched;
 
 typedef struct {
@@ -139,7 +141,7 @@ static int size_class_of(int n) { return n > BATCH_SMALL_N ? 1 : 0; }
 static int npend_class(const batch_sched *s, int c) {
     int k = 0;
     for (int i = 0; i < s->nprob; i++)
-        if (s->pending[i] == 1 && size_class_of(s->n[i]) == c) k++;
+        if (s->pending[i] == 1 && size_class_of(s->work[i].N) == c) k++;
     return k;
 }
 
@@ -149,20 +151,12 @@ static int npend_class(const batch_sched *s, int c) {
 static void run_batch(batch_sched *s, int slot, int c) {
     const int k = npend_class(s, c);
     int *ids = (int *)irwls_xmalloc((size_t)k * sizeof(int));
-    double **A = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
-    int *n = (int *)irwls_xmalloc((size_t)k * sizeof(int));
-    int *lda = (int *)irwls_xmalloc((size_t)k * sizeof(int));
-    const double **rhs = (const double **)irwls_xmalloc((size_t)k * sizeof(double *));
-    double **x = (double **)irwls_xmalloc((size_t)k * sizeof(double *));
+    irwls_inner_work *w = (irwls_inner_work *)irwls_xmalloc((size_t)k * sizeof(irwls_inner_work));
     int m = 0;
     for (int i = 0; i < s->nprob && m < k; i++) {
-        if (s->pending[i] != 1 || size_class_of(s->n[i]) != c) continue;
+        if (s->pending[i] != 1 || size_class_of(s->work[i].N) != c) continue;
         ids[m] = i;
-        A[m] = s->A[i];
-        n[m] = s->n[i];
-        lda[m] = s->lda[i];
-        rhs[m] = s->rhs[i];
-        x[m] = s->x[i];
+        w[m] = s->work[i];
         s->pending[i] = 2;
         m++;
     }
@@ -170,7 +164,7 @@ static void run_batch(batch_sched *s, int slot, int c) {
     s->inflight += m;
     s->busy |= 1u << slot;
     pthread_mutex_unlock(&s->mu);
-    irwls_dev_posv_aug_batched_slot(slot, m, A, n, lda, rhs, x, NULL);
+    irwls_dev_inner_batched_slot(slot, m, w);
     irwls_dev_sync();
     pthread_mutex_lock(&s->mu);
     for (int j = 0; j < m; j++) s->pending[ids[j]] = 0;
@@ -178,11 +172,7 @@ static void run_batch(batch_sched *s, int slot, int c) {
     s->busy &= ~(1u << slot);
     s->rounds++;
     free(ids);
-    free(A);
-    free(n);
-    free(lda);
-    free(rhs);
-    free(x);
+    free(w);
     pthread_cond_broadcast(&s->cv);
 }
 
@@ -190,21 +180,17 @@ static void run_batch(batch_sched *s, int slot, int c) {
 static double first_queued(const batch_sched *s, int c) {
     double t = 1e300;
     for (int i = 0; i < s->nprob; i++)
-        if (s->pending[i] == 1 && size_class_of(s->n[i]) == c && s->t_queue[i] < t) t = s->t_queue[i];
+        if (s->pending[i] == 1 && size_class_of(s->work[i].N) == c && s->t_queue[i] < t) t = s->t_queue[i];
     return t;
 }
 
-int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x) {
+int irwls_batch_inner(const irwls_inner_work *w) {
     batch_sched *s = t_sched;
     if (!s) return 0;
     const int me = t_slot;
-    irwls_dev_sync();   /* the matrix and right-hand side are complete on this thread's stream */
+    irwls_dev_sync();   /* the iteration's inputs are complete on this thread's stream */
     pthread_mutex_lock(&s->mu);
-    s->A[me] = d_A;
-    s->n[me] = n;
-    s->lda[me] = lda;
-    s->rhs[me] = d_rhs;
-    s->x[me] = d_x;
+    s->work[me] = *w;
     s->pending[me] = 1;
     s->t_queue[me] = irwls_now();
     s->npend++;
@@ -212,7 +198,7 @@ int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *
         /* a thread whose own problem is queued starts the next batch, on a free round slot */
         const int slot = (s->pending[me] == 1) ? free_slot(s) : -1;
         if (slot >= 0) {
-            const int c = size_class_of(s->n[me]);
+            const int c = size_class_of(s->work[me].N);
             const double due = first_queued(s, c) + s->wait_s;
             if (s->npend == s->active - s->inflight || irwls_now() >= due) {
                 run_batch(s, slot, c);
@@ -280,11 +266,7 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
         const char *w = getenv("IRWLS_BATCH_WAIT_US");
         s.wait_s = (w && *w) ? atof(w) * 1e-6 : 1e-3;
     }
-    s.A = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
-    s.n = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
-    s.lda = (int *)irwls_xcalloc((size_t)nprob, sizeof(int));
-    s.rhs = (const double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
-    s.x = (double **)irwls_xcalloc((size_t)nprob, sizeof(double *));
+    s.work = (irwls_inner_work *)irwls_xcalloc((size_t)nprob, sizeof(irwls_inner_work));
     /* every worker's stream comes from the pool, created here on the calling thread */
     irwls_dev_reserve_streams(nprob);
     batch_job *jobs = (batch_job *)irwls_xcalloc((size_t)nprob, sizeof(batch_job));
@@ -324,11 +306,7 @@ int trainFULLBatched(const svm_dataset *datasets, const properties *props, int n
     pthread_mutex_destroy(&s.mu);
     free(s.pending);
     free(s.t_queue);
-    free(s.A);
-    free(s.n);
-    free(s.lda);
-    free(s.rhs);
-    free(s.x);
+    free(s.work);
     free(jobs);
     free(th);
     return 0;

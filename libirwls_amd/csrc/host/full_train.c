@@ -57,9 +57,8 @@ static int kkt_solve(irwls_ws *ws, int N, double last, double *x, const int32_t 
         if (done) irwls_dev_h2d(ws->x, x, (size_t)N * sizeof(double));
     }
     if (!done) {
-        /* failure code ignored, as the reference; in a batched run the solve
-           joins the other problems' pending solves */
-        if (!irwls_batch_solve(ws->H, N, N + 1, ws->rhs, ws->x)) (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->x);
+        /* failure code ignored, as the reference */
+        (void)irwls_dev_posv_aug(ws->H, N, N + 1, ws->rhs, ws->x);
         g_stats.factorizations++;
     }
     g_stats.solves++;
@@ -109,8 +108,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
     double g13last = 0.0;
     memcpy(ws->hS3, S3, (size_t)nS3 * sizeof(int32_t));
     irwls_ws_upload(ws, 2);   /* S3 of the current groups */
-    int32_t *d_nc = (int32_t *)ws->nc_cst;
-    double *d_cst = ws->nc_cst + 1;
+    const int fp64_solve = !(irwls_opts()->precision == 1 || irwls_opts()->precision == 2);
 
     /* loop rule of src/full-train.c:158 (C precedence) */
     while ((iter < 5) || ((minb < 0.0 || maxb > C) && iter < 1000 && since < 5 && dW / nW > 1e-6)) {
@@ -128,16 +126,29 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
         memcpy(ws->hys, hbuf, (size_t)N * sizeof(double));
         memcpy(ws->has, hbuf2, (size_t)N * sizeof(double));
         irwls_ws_upload(ws, 0);
-        irwls_dev_kkt_rhs(ws->S1, nS1, ws->G13, use_g13, ws->gin, last, ws->rhs);
-        irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
-        const int on_host = kkt_solve(ws, N, last, bAux, S1, hbuf, gen);
-
-        /* weights after the solve and the error update over the changed
-           weights (src/full-train.c:205-249), on the device; one read-back
-           of the solution and the errors */
-        irwls_dev_inner_update(ws->x, ws->S1, nS1, ws->S3, nS3, ws->yl, C, l, ws->beta, ws->bnew, ws->cols, ws->w,
-                               d_nc, d_cst);
-        irwls_dev_gemv_gather_dn(ws->Kw, l, l, ws->cols, ws->w, d_nc, d_cst, 1.0, -1.0, ws->esub);
+        int on_host = 0;
+        if (fp64_solve && irwls_batch_worker()) {
+            /* a batched run: the whole iteration's device work joins the
+               other problems' pending iterations in one batched round */
+            const double t0 = irwls_now();
+            irwls_inner_work wk = {ws->H, N, N + 1, ws->Kw, l, l, ws->S1, ws->ys, ws->as, ws->G13, use_g13, ws->gin,
+                                   last, ws->rhs, ws->x, ws->S3, nS3, ws->yl, C, ws->beta, ws->bnew, ws->cols, ws->w,
+                                   ws->nc_cst, ws->esub};
+            (void)irwls_batch_inner(&wk);
+            g_stats.factorizations++;
+            g_stats.solves++;
+            g_stats.t_solve += irwls_now() - t0;
+        } else {
+            irwls_dev_kkt_rhs(ws->S1, nS1, ws->G13, use_g13, ws->gin, last, ws->rhs);
+            irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
+            on_host = kkt_solve(ws, N, last, bAux, S1, hbuf, gen);
+            /* weights after the solve and the error update over the changed
+               weights (src/full-train.c:205-249), on the device */
+            irwls_dev_inner_update(ws->x, ws->S1, nS1, ws->S3, nS3, ws->yl, C, l, ws->beta, ws->bnew, ws->cols,
+                                   ws->w, ws->nc_cst);
+            irwls_dev_gemv_gather_dn(ws->Kw, l, l, ws->cols, ws->w, ws->nc_cst, 1.0, -1.0, ws->esub);
+        }
+        /* one read-back of the solution and the errors */
         if (on_host) irwls_dev_d2h(e, ws->esub, (size_t)l * sizeof(double));
         else irwls_dev_d2h2(bAux, ws->x, (size_t)N * sizeof(double), e, ws->esub, (size_t)l * sizeof(double));
 

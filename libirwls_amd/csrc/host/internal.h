@@ -151,9 +151,9 @@ const float *irwls_proposal_score_dev(irwls_proposal *lp, int P);
 
 /* Batched training (batch.c).  irwls_rand is libc rand(), or the calling
    worker's private glibc-compatible stream in a batched run.
-   irwls_batch_solve joins a batched run's pending KKT solves (device
-   augmented matrix A, lda >= n+1, right-hand side and solution on the
-   device) and returns 1 when done, or returns 0 outside a batched run. */
+   irwls_batch_inner joins a batched run's pending inner iterations (all
+   operands on the device, irwls_inner_work) and returns 1 when done, or
+   returns 0 outside a batched run. */
 int irwls_rand(void);
 /* A run of libc rand() draws without libc's per-call lock: the same values
    and the same libc state afterwards as calling rand() (irwls_rand() on a
@@ -165,7 +165,7 @@ typedef struct {
 void irwls_rand_begin(irwls_rand_block *b);
 int irwls_rand_draw(irwls_rand_block *b);
 void irwls_rand_end(irwls_rand_block *b);
-int irwls_batch_solve(double *d_A, int n, int lda, const double *d_rhs, double *d_x);
+int irwls_batch_inner(const irwls_inner_work *w);
 
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
 void irwls_stats_set(const irwls_gpu_stats *s);
