@@ -126,19 +126,23 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
         memcpy(ws->hys, hbuf, (size_t)N * sizeof(double));
         memcpy(ws->has, hbuf2, (size_t)N * sizeof(double));
         irwls_ws_upload(ws, 0);
-        int on_host = 0;
+        int on_host = 0, batched = 0;
         if (fp64_solve && irwls_batch_worker()) {
             /* a batched run: the whole iteration's device work joins the
-               other problems' pending iterations in one batched round */
+               other problems' pending iterations in one batched round (or
+               runs below, when no other problem would join it) */
             const double t0 = irwls_now();
             irwls_inner_work wk = {ws->H, N, N + 1, ws->Kw, l, l, ws->S1, ws->ys, ws->as, ws->G13, use_g13, ws->gin,
                                    last, ws->rhs, ws->x, ws->S3, nS3, ws->yl, C, ws->beta, ws->bnew, ws->cols, ws->w,
                                    ws->nc_cst, ws->esub};
-            (void)irwls_batch_inner(&wk);
-            g_stats.factorizations++;
-            g_stats.solves++;
-            g_stats.t_solve += irwls_now() - t0;
-        } else {
+            batched = irwls_batch_inner(&wk);
+            if (batched) {
+                g_stats.factorizations++;
+                g_stats.solves++;
+                g_stats.t_solve += irwls_now() - t0;
+            }
+        }
+        if (!batched) {
             irwls_dev_kkt_rhs(ws->S1, nS1, ws->G13, use_g13, ws->gin, last, ws->rhs);
             irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
             on_host = kkt_solve(ws, N, last, bAux, S1, hbuf, gen);
@@ -229,6 +233,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
             irwls_ws_upload(ws, 2);
         }
     }
+    if (fp64_solve) irwls_batch_inner_end();
     free(a);
     free(grp);
     free(bAux);

@@ -166,6 +166,9 @@ void irwls_rand_begin(irwls_rand_block *b);
 int irwls_rand_draw(irwls_rand_block *b);
 void irwls_rand_end(irwls_rand_block *b);
 int irwls_batch_inner(const irwls_inner_work *w);
+/* Marks the end of the calling worker's inner loop (no-op outside a batched
+   run): batches stop waiting for it. */
+void irwls_batch_inner_end(void);
 
 /* Counters reported by irwls_gpu_last_stats (the last training call). */
 void irwls_stats_set(const irwls_gpu_stats *s);

@@ -71,16 +71,23 @@ def test_posv_aug_batched_equals_single_and_replays():
 
 
 @pytest.mark.parametrize("precision", [0, 3])
-@pytest.mark.parametrize("wait_us", [None, "50"])
-def test_train_full_batched_equals_standalone(wait_us, precision, monkeypatch):
+@pytest.mark.parametrize("wait_us,alone", [(None, None), ("50", None), (None, "0")])
+def test_train_full_batched_equals_standalone(wait_us, alone, precision, monkeypatch, tmp_path):
     """Six problems of uneven size (two datasets, three C/gamma points):
     every batched result equals its standalone trainFULL run, in FP64 and
     with BF16 hi/lo kernel products (precision 3, whose BF16 copies of a
     shared dataset the worker threads build once between them).  With a
     50 us trigger, batches start as soon as a problem has waited that long,
-    so several run at once on different round slots."""
+    so several run at once on different round slots.  By default an
+    iteration that no other problem would join runs the single-problem
+    kernels; with IRWLS_BATCH_ALONE=0 every iteration goes through a round.
+    The round trace accounts for every batched iteration."""
     if wait_us is not None:
         monkeypatch.setenv("IRWLS_BATCH_WAIT_US", wait_us)
+    if alone is not None:
+        monkeypatch.setenv("IRWLS_BATCH_ALONE", alone)
+    trace = tmp_path / "rounds.txt"
+    monkeypatch.setenv("IRWLS_BATCH_TRACE", str(trace))
     tasks = []
     for seed, (n, d) in enumerate([(900, 8), (1600, 16)]):
         X, y = synth.two_gaussians(n, d, seed=900 + seed)
@@ -103,3 +110,15 @@ def test_train_full_batched_equals_standalone(wait_us, precision, monkeypatch):
     assert st["batch_rounds"] > 0
     for i, (g, w) in enumerate(zip(got, want)):
         assert np.array_equal(g, w), (i, np.abs(g - w).max())
+    lines = trace.read_text().splitlines()
+    head = [ln for ln in lines if ln.startswith("# call:")]
+    rounds = [ln.split() for ln in lines if ln and not ln.startswith("#")]
+    probs = [ln for ln in lines if ln.startswith("# problem")]
+    assert len(head) == 1 and len(rounds) == st["batch_rounds"] and len(probs) == len(tasks)
+    n_alone = int(head[0].split(", ")[2].split()[0])
+    if alone == "0":
+        assert n_alone == 0
+    # every iteration ran either in a round or alone
+    assert sum(int(r[2]) for r in rounds) + n_alone == st["inner_iters"]
+    for r in rounds:
+        assert float(r[0]) <= float(r[1]) and int(r[2]) >= 1 and int(r[3]) in (0, 1)
