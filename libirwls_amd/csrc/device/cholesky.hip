@@ -702,14 +702,16 @@ __global__ __launch_bounds__(256) void step_fused_kernel(T *__restrict__ A, int 
 template <typename T>
 __global__ __launch_bounds__(256) void panel_solve_kernel(T *__restrict__ A, int n, int nrows, int lda,
                                                           const int *__restrict__ info, const T *__restrict__ Dinv,
-                                                          int kfirst) {
+                                                          int kfirst, int ident) {
     const int kbk = kfirst + blockIdx.y;
     const int k0 = kbk * NB;
     const int kb = (n - k0 < NB) ? n - k0 : NB;
-    if (k0 + kb + (int)blockIdx.x * NB >= nrows) return;
+    // identity rows below n: rows from n + k0 + kb on are zero in this panel (factor())
+    const int rows = (ident && n + k0 + kb < nrows) ? n + k0 + kb : nrows;
+    if (k0 + kb + (int)blockIdx.x * NB >= rows) return;
     // the step that solved block k saw only failures in blocks <= k
     const int inf = info[0];
-    trsm_body_inf<T>(A, n, nrows, lda, k0, kb, (inf > k0 + kb) ? 0 : inf, Dinv + (size_t)kbk * 2 * TILE, blockIdx.x);
+    trsm_body_inf<T>(A, n, rows, lda, k0, kb, (inf > k0 + kb) ? 0 : inf, Dinv + (size_t)kbk * 2 * TILE, blockIdx.x);
 }
 
 // -------------------------------------------------------- triangular sweeps ---
@@ -864,8 +866,14 @@ T *dinv_buf(int nblk) {
     return DinvPool<T>::buf;
 }
 
+// ident: rows n..nrows-1 start as the identity (the mixed-precision factor,
+// whose bottom block becomes L^-T).  Its row n+j is zero in the columns of
+// every block before the one holding column j, and a step only changes rows
+// whose panel entries are nonzero, so step k (panel columns k0..k0+kb-1)
+// works on rows below n + k0 + kb only: the rows past it would be rewritten
+// with their own values.
 template <typename T>
-T *factor(T *A, int n, int nrows, int lda) {
+T *factor(T *A, int n, int nrows, int lda, bool ident = false) {
     int *info = info_buf();
     irwls_dev_memset0(info, 4 * sizeof(int));
     const int nblk = (n + NB - 1) / NB;
@@ -885,10 +893,12 @@ T *factor(T *A, int n, int nrows, int lda) {
     // forms give the same bits.
     const long fused_tiles = (long)device_cus() * (sizeof(T) == sizeof(double) ? 3 : 2);
     int kf = nblk - 1;   // first fused step
+    // rows a step with panel columns k0..k0+kb-1 reads and writes
+    auto rows_of = [&](int k0, int kb) { return (ident && n + k0 + kb < nrows) ? n + k0 + kb : nrows; };
     for (int kbk = 0; kbk + 1 < nblk; kbk++) {
         const int k0 = kbk * NB;
         const int rest = n - k0 - NB;
-        const long tc = (rest + NB - 1) / NB, tr = (nrows - k0 - NB + NB - 1) / NB;
+        const long tc = (rest + NB - 1) / NB, tr = (rows_of(k0, NB) - k0 - NB + NB - 1) / NB;
         if (tc * tr - tc * (tc - 1) / 2 <= fused_tiles) {
             kf = kbk;
             break;
@@ -897,39 +907,44 @@ T *factor(T *A, int n, int nrows, int lda) {
     for (int kbk = 0; kbk < nblk; kbk++) {
         const int k0 = kbk * NB;
         const int kb = (n - k0 < NB) ? (n - k0) : NB;
-        const int prow = nrows - k0 - kb;
+        const int rows = rows_of(k0, kb);
+        const int prow = rows - k0 - kb;
         if (kbk < kf && prow > 0) {
             TimerScope ts(IRWLS_T_TRSM, 2.0 * prow * (double)kb * kb, (double)sizeof(T) * 2.0 * prow * kb);
-            ts.launch(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+            ts.launch(trsm_kernel<T>, dim3((prow + NB - 1) / NB), dim3(256), 0, A, n, rows, lda, k0, kb, info,
                       Dinv + (size_t)kbk * 2 * TILE);
         }
         const int rest = n - k0 - kb;
         if (rest <= 0) break;   // kernels return at once after a failure
         const int tc = (rest + NB - 1) / NB;
-        const int tr = (nrows - k0 - kb + NB - 1) / NB;
-        // algorithmic: the rank-64 update of the trailing lower triangle (and the rows below n)
-        TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (nrows - n) * rest * kb,
+        const int tr = (rows - k0 - kb + NB - 1) / NB;
+        // algorithmic: the rank-64 update of the trailing lower triangle (and the rows below n it changes)
+        TimerScope ts(IRWLS_T_SYRK, (double)rest * rest * kb + 2.0 * (rows - n) * rest * kb,
                       (double)sizeof(T) * (double)rest * rest);
         if (kbk < kf)
-            ts.launch(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+            ts.launch(syrk_next_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, rows, lda, k0, kb, info,
                       Dinv + (size_t)(kbk + 1) * 2 * TILE);
         else
-            ts.launch(step_fused_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, nrows, lda, k0, kb, info,
+            ts.launch(step_fused_kernel<T>, dim3(tc, tr), dim3(256), 0, A, n, rows, lda, k0, kb, info,
                       Dinv + (size_t)kbk * 2 * TILE, Dinv + (size_t)(kbk + 1) * 2 * TILE);
     }
     {
         const int k0 = kf * NB;
         const int kb = (n - k0 < NB) ? n - k0 : NB;
-        const int prow = nrows - k0 - kb;   // rows below block kf's diagonal tile: the most of the fused steps
+        // rows below a fused step's diagonal tile: the most of them in block nblk-1 (identity
+        // rows) or block kf (otherwise)
+        int prow = 0;
+        double fl = 0.0;
+        for (int kbk = kf; kbk < nblk; kbk++) {
+            const int q0 = kbk * NB, qb = (n - q0 < NB) ? (n - q0) : NB;
+            const int pr = rows_of(q0, qb) - q0 - qb;
+            if (pr > prow) prow = pr;
+            fl += 2.0 * (double)pr * qb * qb;
+        }
         if (prow > 0) {
-            double fl = 0.0;
-            for (int kbk = kf; kbk < nblk; kbk++) {
-                const int q0 = kbk * NB, qb = (n - q0 < NB) ? (n - q0) : NB;
-                fl += 2.0 * (double)(nrows - q0 - qb) * qb * qb;
-            }
-            TimerScope ts(IRWLS_T_TRSM, fl, (double)sizeof(T) * 2.0 * (double)(nrows - k0) * (n - k0));
+            TimerScope ts(IRWLS_T_TRSM, fl, (double)sizeof(T) * 2.0 * (double)(rows_of(k0, kb) - k0) * (n - k0));
             ts.launch(panel_solve_kernel<T>, dim3((prow + NB - 1) / NB, nblk - kf), dim3(256), 0, A, n, nrows, lda,
-                      info, Dinv, kf);
+                      info, Dinv, kf, ident ? 1 : 0);
         }
     }
     return Dinv;
@@ -1531,7 +1546,7 @@ int irwls_dev_potrf_inv_f32(const double *A, int n, int lda, float *F, int ldf, 
     }
     hipLaunchKernelGGL(to_f32_with_identity, dim3((2 * n + 255) / 256, n), dim3(256), 0, stream(), A, n, lda, F, ldf);
     IRWLS_LAUNCH_CHECK();
-    factor<float>(F, n, 2 * n, ldf);
+    factor<float>(F, n, 2 * n, ldf, true);
     const int nt = (n + 63) / 64;
     hipLaunchKernelGGL(transpose_z, dim3(nt, nt), dim3(256), 0, stream(), F, n, ldf, ZT, ldz);
     IRWLS_LAUNCH_CHECK();

@@ -32,8 +32,11 @@ def _kernel_spd(n, seed, dmin=1e-3):
     return rbf(X, X, 0.3) * np.outer(y, y) + np.diag(rng.uniform(dmin, 1.0, n)), rng
 
 
-@pytest.mark.parametrize("n", [64, 200, 513])
+@pytest.mark.parametrize("n", [64, 200, 513, 3000])
 def test_potrf_inv_f32(n):
+    """FP32 factor and L^-T from the identity rows; n = 3000 also takes the
+    two-launch steps (panel solve, then trailing update) before the fused ones,
+    with each step's rows bounded where the identity rows are still zero."""
     A, _ = _kernel_spd(n, 300 + n, dmin=0.05)
     L = _bind()
     Ad = Dev(np.asfortranarray(A).ravel(order="F"))
