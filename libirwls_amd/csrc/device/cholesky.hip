@@ -633,13 +633,11 @@ __device__ __forceinline__ void mfma_rows_lds(typename Mma<T>::acc_t (&acc)[2][2
 // as syrk_next_body does.  S holds that tile (diagonal work-group) or the
 // solved rows J (the others, as T).
 template <typename T>
-__global__ __launch_bounds__(256) void step_fused_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
-                                                         int *__restrict__ info, const T *__restrict__ Dinv,
-                                                         T *__restrict__ Dnext) {
-    __shared__ double S[NB][NB + 1];
-    __shared__ T PI[NB][NB + 1];
-    __shared__ int sfail;
-    const int tj = blockIdx.x, ti = blockIdx.y;
+__device__ __forceinline__ void step_fused_body(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                int *__restrict__ info, const T *__restrict__ Dinv,
+                                                T *__restrict__ Dnext, int tj, int ti, TileLds S, T (*PI)[NB + 1],
+                                                int *psfail) {
+    int &sfail = *psfail;
     const int t0 = k0 + kb;
     if (t0 + tj * NB >= n || t0 + ti * NB >= nrows) return;
     const int inf = info[0];
@@ -694,6 +692,16 @@ __global__ __launch_bounds__(256) void step_fused_kernel(T *__restrict__ A, int 
     const int fail = tile_factor<T>(S, kb2, &sfail, A, lda, t0, Dnext, Dnext + TILE);
     if (fail >= 0) tile_store<T>(S, A, lda, t0, kb2, fail, Dnext, Dnext + TILE);
     if (threadIdx.x == 0 && fail >= 0) info[0] = t0 + fail + 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void step_fused_kernel(T *__restrict__ A, int n, int nrows, int lda, int k0, int kb,
+                                                         int *__restrict__ info, const T *__restrict__ Dinv,
+                                                         T *__restrict__ Dnext) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ T PI[NB][NB + 1];
+    __shared__ int sfail;
+    step_fused_body<T>(A, n, nrows, lda, k0, kb, info, Dinv, Dnext, blockIdx.x, blockIdx.y, S, PI, &sfail);
 }
 
 // The panels of the fused steps at once, in place, after the last step:
@@ -1088,6 +1096,33 @@ __global__ __launch_bounds__(256) void b_syrk(const BatchProb *__restrict__ tab,
     }
 }
 
+// One step of the fused form (step_fused_body: panel rows solved on the fly)
+// for every problem that still has block column kbk + 1; the panels of the
+// fused steps are solved in place afterwards by b_panel_solve.
+__global__ __launch_bounds__(256) void b_step_fused(const BatchProb *__restrict__ tab, int kbk) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ double PI[NB][NB + 1];
+    __shared__ int sfail;
+    const BatchProb p = tab[blockIdx.z];
+    if (kbk + 1 >= nblk_of(p.n)) return;
+    step_fused_body<double>(p.A, p.n, p.n + 1, p.lda, kbk * NB, NB, p.info, p.Dinv + (size_t)kbk * 2 * TILE,
+                            p.Dinv + (size_t)(kbk + 1) * 2 * TILE, blockIdx.x, blockIdx.y, S, PI, &sfail);
+}
+
+// The panels of block columns kfirst.. (those the fused steps left
+// unsolved), as panel_solve_kernel does for one system.
+__global__ __launch_bounds__(256) void b_panel_solve(const BatchProb *__restrict__ tab, int kfirst) {
+    const BatchProb p = tab[blockIdx.z];
+    const int kbk = kfirst + blockIdx.y;
+    if (kbk >= nblk_of(p.n)) return;
+    const int k0 = kbk * NB;
+    const int kb = (p.n - k0 < NB) ? p.n - k0 : NB;
+    if (k0 + kb + (int)blockIdx.x * NB >= p.n + 1) return;
+    const int inf = p.info[0];
+    trsm_body_inf<double>(p.A, p.n, p.n + 1, p.lda, k0, kb, (inf > k0 + kb) ? 0 : inf,
+                          p.Dinv + (size_t)kbk * 2 * TILE, blockIdx.x);
+}
+
 __global__ void b_gather_z(const BatchProb *__restrict__ tab) {
     const BatchProb p = tab[blockIdx.z];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1155,7 +1190,27 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int lcap, int
     }
     hipLaunchKernelGGL(b_prepare, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
     hipLaunchKernelGGL(b_diag_first, dim3(1, 1, nprob), dim3(256), 0, st, d_tab);
+    // the first fused step: once every problem's trailing tiles of a step fit
+    // in about three work-groups per CU (sized by the largest system), as
+    // factor() decides for one system; both forms give the same bits
+    const long fused_tiles = (long)device_cus() * 3;
+    int kf = nblk - 1;
+    for (int k = 0; k + 1 < nblk; k++) {
+        const long tc = (ncap - k * NB - NB + NB - 1) / NB, tr = (ncap + 1 - k * NB - NB + NB - 1) / NB;
+        if ((tc * tr - tc * (tc - 1) / 2) * nprob <= fused_tiles) {
+            kf = k;
+            break;
+        }
+    }
     for (int k = 0; k < nblk; k++) {
+        if (k >= kf) {
+            if (k + 1 < nblk) {
+                const int tc = nblk - k - 1;
+                const int tr = (ncap + 1 - (k + 1) * NB + NB - 1) / NB;
+                hipLaunchKernelGGL(b_step_fused, dim3(tc, tr, nprob), dim3(256), 0, st, d_tab, k);
+            }
+            continue;
+        }
         const int rt = (ncap + 1 - k * NB - NB + NB - 1) / NB;   // panel row tiles below block k (rhs row included)
         if (rt > 0) {
             const int g = (rt + R - 1) / R;
@@ -1171,6 +1226,12 @@ void batched_sequence(const BatchProb *d_tab, int nprob, int ncap, int lcap, int
             else if (R == 2) hipLaunchKernelGGL(b_syrk<2>, dim3(tc, g, nprob), dim3(256), 0, st, d_tab, k);
             else hipLaunchKernelGGL(b_syrk<4>, dim3(tc, g, nprob), dim3(256), 0, st, d_tab, k);
         }
+    }
+    if (kf < nblk) {
+        const int prow = ncap + 1 - kf * NB - NB;   // rows below block kf's diagonal tile
+        if (prow > 0)
+            hipLaunchKernelGGL(b_panel_solve, dim3((prow + NB - 1) / NB, nblk - kf, nprob), dim3(256), 0, st, d_tab,
+                               kf);
     }
     hipLaunchKernelGGL(b_gather_z, dim3(vb, 1, nprob), dim3(256), 0, st, d_tab);
     for (int w = nblk - 1; w >= 0; w--) {
