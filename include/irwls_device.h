@@ -335,6 +335,28 @@ int  irwls_dev_pcg_spd2(const double *d_A, int n, int lda, const double *d_B, co
 int  irwls_dev_pcg_spd2_mapped(const double *d_A, int n, int lda, const double *d_B, const float *d_F, int ldf,
                                const float *d_ZT, int ldz, int nf, const int32_t *d_map, const int32_t *d_inv,
                                double tol, int maxit, double *d_X, double *relres);
+/* Rows removed from the factored set: with B = Z Z^T and R the factor
+   positions of rs <= 64 removed rows (device), irwls_dev_pcg_schur_setup
+   computes d_U = B(:, R) (nf x rs, column-major, leading dimension nf) and
+   factors S = B(R, R) as one FP64 tile, leaving L_S^-1 and its transpose in
+   d_Sinv (2 x 64 x 64).  With check set it waits for the device and returns
+   that factorization's info (0: usable); without, it returns 0 at once (a
+   solve with an unusable correction misses its residual target and reports
+   it, as any solve that does not converge).
+   irwls_dev_pcg_spd2_schur is irwls_dev_pcg_spd2_mapped whose kept rows are
+   preconditioned by B_KK - B(K, R) S^-1 B(R, K), the exact inverse of the
+   kept block of the factored matrix (rs = 0: the mapped form). */
+int  irwls_dev_pcg_schur_setup(const float *d_ZT, int ldz, int nf, const int32_t *d_R, int rs, double *d_U,
+                               double *d_Sinv, int check);
+int  irwls_dev_pcg_spd2_schur(const double *d_A, int n, int lda, const double *d_B, const float *d_F, int ldf,
+                              const float *d_ZT, int ldz, int nf, const int32_t *d_map, const int32_t *d_inv,
+                              const int32_t *d_R, int rs, const double *d_U, const double *d_Sinv, double tol,
+                              int maxit, double *d_X, double *relres);
+/* Factor one SPD tile (n <= 64, column-major at d_A with leading dimension
+   lda) in place and write L^-1 (column-major, leading dimension 64) and its
+   transpose to d_Dinv (2 x 64 x 64).  With check set, waits and returns the
+   potrf info; without, returns 0 at once. */
+int  irwls_dev_tile_potrf_inv(double *d_A, int lda, int n, double *d_Dinv, int check);
 
 /* ---- budgeted training (SGMA, parametric IRWLS) ------------------------- */
 /* C = alpha op(A) op(B) + beta C, all column-major (op = transpose when set). */

@@ -1598,6 +1598,21 @@ int irwls_dev_batched_choice(int nprob, int nmax, double *ms3) {
     return 0;
 }
 
+int irwls_dev_tile_potrf_inv(double *A, int lda, int n, double *Dinv, int check) {
+    if (n <= 0 || n > NB) {
+        fprintf(stderr, "libirwls: irwls_dev_tile_potrf_inv needs 1 <= n <= %d (n=%d)\n", NB, n);
+        exit(2);
+    }
+    int *info = info_buf();
+    irwls_dev_memset0(info, 4 * sizeof(int));
+    hipLaunchKernelGGL(diag_first_kernel<double>, dim3(1), dim3(256), 0, stream(), A, lda, n, info, Dinv);
+    IRWLS_LAUNCH_CHECK();
+    if (!check) return 0;
+    int h = 0;
+    irwls_dev_d2h(&h, info, sizeof(int));
+    return h;
+}
+
 int irwls_dev_potrf_inv_f32(const double *A, int n, int lda, float *F, int ldf, float *ZT, int ldz) {
     if (n <= 0) return 0;
     if (ldf < 2 * n || ldz < n) {
