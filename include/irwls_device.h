@@ -336,13 +336,13 @@ int  irwls_dev_pcg_spd2_mapped(const double *d_A, int n, int lda, const double *
                                const float *d_ZT, int ldz, int nf, const int32_t *d_map, const int32_t *d_inv,
                                double tol, int maxit, double *d_X, double *relres);
 /* Rows removed from the factored set: with B = Z Z^T and R the factor
-   positions of rs <= 64 removed rows (device), irwls_dev_pcg_schur_setup
+   positions of rs <= 256 removed rows (device), irwls_dev_pcg_schur_setup
    computes d_U = B(:, R) (nf x rs, column-major, leading dimension nf) and
-   factors S = B(R, R) as one FP64 tile, leaving L_S^-1 and its transpose in
-   d_Sinv (2 x 64 x 64).  With check set it waits for the device and returns
-   that factorization's info (0: usable); without, it returns 0 at once (a
-   solve with an unusable correction misses its residual target and reports
-   it, as any solve that does not converge).
+   d_Sinv = S^-1 (rs x rs, column-major) for S = B(R, R), factored in FP64.
+   With check set it waits for the device and returns that factorization's
+   info (0: usable); without, it returns 0 at once (a solve with an unusable
+   correction misses its residual target and reports it, as any solve that
+   does not converge).
    irwls_dev_pcg_spd2_schur is irwls_dev_pcg_spd2_mapped whose kept rows are
    preconditioned by B_KK - B(K, R) S^-1 B(R, K), the exact inverse of the
    kept block of the factored matrix (rs = 0: the mapped form). */
@@ -357,6 +357,10 @@ int  irwls_dev_pcg_spd2_schur(const double *d_A, int n, int lda, const double *d
    transpose to d_Dinv (2 x 64 x 64).  With check set, waits and returns the
    potrf info; without, returns 0 at once. */
 int  irwls_dev_tile_potrf_inv(double *d_A, int lda, int n, double *d_Dinv, int check);
+/* FP64 factor of [A; I] (2n x n, column-major, ldg >= 2n; the caller writes
+   the identity in rows n..2n-1): rows 0..n-1 receive L, rows n..2n-1
+   Z = L^-T (upper triangular).  check as above. */
+int  irwls_dev_potrf_ident(double *d_G, int n, int ldg, int check);
 
 /* ---- budgeted training (SGMA, parametric IRWLS) ------------------------- */
 /* C = alpha op(A) op(B) + beta C, all column-major (op = transpose when set). */

@@ -1613,6 +1613,19 @@ int irwls_dev_tile_potrf_inv(double *A, int lda, int n, double *Dinv, int check)
     return h;
 }
 
+int irwls_dev_potrf_ident(double *G, int n, int ldg, int check) {
+    if (n <= 0) return 0;
+    if (ldg < 2 * n) {
+        fprintf(stderr, "libirwls: irwls_dev_potrf_ident needs ldg >= 2n (n=%d, ldg=%d)\n", n, ldg);
+        exit(2);
+    }
+    factor<double>(G, n, 2 * n, ldg, true);
+    if (!check) return 0;
+    int info = 0;
+    irwls_dev_d2h(&info, info_buf(), sizeof(int));
+    return info;
+}
+
 int irwls_dev_potrf_inv_f32(const double *A, int n, int lda, float *F, int ldf, float *ZT, int ldz) {
     if (n <= 0) return 0;
     if (ldf < 2 * n || ldz < n) {

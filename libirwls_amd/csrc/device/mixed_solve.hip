@@ -189,44 +189,73 @@ __global__ void scatter_to_factor(int n, int nf, const int32_t *__restrict__ inv
 // factored matrix: B_KK - B_KR S^-1 B_RK with B = Z Z^T and S = B_RR, since
 // for SPD A the inverse of a principal block A_KK equals
 // (A^-1)_KK - (A^-1)_KR ((A^-1)_RR)^-1 (A^-1)_RK.  U = B(:, R) (nf x rs,
-// column-major) and the factor of S come from irwls_dev_pcg_schur_setup; per
-// application y = S^-1 zf(R) (schur_apply) and the gather subtracts U y.
+// column-major) and S^-1 (rs x rs, dense) come from
+// irwls_dev_pcg_schur_setup; per application y = S^-1 zf(R) (schur_apply)
+// and zf -= U y over the factor's index space (schur_correct), before the
+// kept rows are gathered.
+constexpr int SCHUR_MAXR = 256;
 __global__ __launch_bounds__(256) void schur_apply(const double *__restrict__ zf, int nf, const int32_t *__restrict__ R,
-                                                   int rs, const double *__restrict__ X, double *__restrict__ y) {
-    // X = L_S^-1, column-major with leading dimension 64 (lower triangular),
-    // staged in LDS (Xs[row][col]) so the two triangular products read no
-    // global memory in their chains
-    __shared__ double Xs[64][65];
-    __shared__ double w[2][64], tt[2][64];
-    const int t = threadIdx.x, c = t >> 6, a = t & 63;
-    for (int e = t; e < 64 * 64; e += 256) Xs[e & 63][e >> 6] = X[e];
-    if (t < 128) w[c][a] = (a < rs) ? zf[(size_t)c * nf + R[a]] : 0.0;
-    __syncthreads();
-    if (t < 128) {
-        double s = 0.0;
-        for (int b = 0; b <= a && b < rs; b++) s = fma(Xs[a][b], w[c][b], s);
-        tt[c][a] = (a < rs) ? s : 0.0;
+                                                   int rs, const double *__restrict__ Sinv, double *__restrict__ y) {
+    // one wave per row a of S^-1 (symmetric: its row a is its column a)
+    const int lane = threadIdx.x & 63, a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= rs) return;
+    const double *col = Sinv + (size_t)a * rs;
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = lane; b < rs; b += 64) {
+        const int f = R[b];
+        s0 = fma(col[b], zf[f], s0);
+        s1 = fma(col[b], zf[nf + f], s1);
     }
+    for (int o = 32; o >= 1; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        y[a] = s0;
+        y[SCHUR_MAXR + a] = s1;
+    }
+}
+
+// zf_c(f) -= sum_j U(f, j) y_c(j): work-group of 64 factor positions (one per
+// lane), its four waves summing quarters of j, added in wave order.
+__global__ __launch_bounds__(256) void schur_correct(double *__restrict__ zf, int nf, const double *__restrict__ U,
+                                                     int rs, const double *__restrict__ y) {
+    __shared__ double ws[2][4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int f = blockIdx.x * 64 + lane;
+    const int q = (rs + 3) / 4, j0 = wave * q, j1 = min(rs, j0 + q);
+    double s0 = 0.0, s1 = 0.0;
+    if (f < nf) {
+#pragma unroll 8
+        for (int j = j0; j < j1; j++) {
+            const double u = U[(size_t)j * nf + f];
+            s0 = fma(u, y[j], s0);
+            s1 = fma(u, y[SCHUR_MAXR + j], s1);
+        }
+    }
+    ws[0][wave][lane] = s0;
+    ws[1][wave][lane] = s1;
     __syncthreads();
-    if (t < 128) {
-        double s = 0.0;
-        for (int b = a; b < rs; b++) s = fma(Xs[b][a], tt[c][b], s);
-        y[c * 64 + a] = (a < rs) ? s : 0.0;
+    if (wave < 2 && f < nf) {
+        const int c = wave;
+        const double t = (ws[c][0][lane] + ws[c][1][lane]) + (ws[c][2][lane] + ws[c][3][lane]);
+        zf[(size_t)c * nf + f] -= t;
     }
 }
 
 // Partial sums of U(i, j) = sum_k Z(i, k) Z(R_j, k) (Z(i, k) = ZT[i*ldz + k],
-// zero for k < i): work-group (bx, s) takes rows 64 bx .. 64 bx + 63 and
-// k in [SCHUR_SK s, SCHUR_SK (s+1)) from k = 64 bx on (a split entirely
-// below that has no work and writes nothing); each thread a 4 x 4 block of
-// (row, column) pairs, operands staged in LDS as FP64 (the FP32 products are
-// exact there), partials to part[(s rs + j) nf + i].
+// zero for k < i): work-group (bx, s, cb) takes rows 64 bx .. 64 bx + 63,
+// columns 64 cb .. 64 cb + 63 and k in [SCHUR_SK s, SCHUR_SK (s+1)) from
+// k = 64 bx on (a split entirely below that has no work and writes nothing);
+// each thread a 4 x 4 block of (row, column) pairs, operands staged in LDS
+// as FP64 (the FP32 products are exact there), partials to
+// part[(s rs + j) nf + i].
 constexpr int SCHUR_SK = 128, SCHUR_KC = 32;
 __global__ __launch_bounds__(256) void schur_u_part(const float *__restrict__ ZT, int ldz, int nf,
                                                     const int32_t *__restrict__ R, int rs, double *__restrict__ part) {
     __shared__ double Am[SCHUR_KC][65];   // Am[kk][row] = Z(i0 + row, k0 + kk)
     __shared__ double Bm[SCHUR_KC][65];   // Bm[kk][j] = Z(R_j, k0 + kk)
-    const int i0 = blockIdx.x * 64, sp = blockIdx.y;
+    const int i0 = blockIdx.x * 64, sp = blockIdx.y, j0 = blockIdx.z * 64;
     const int kbeg = max(sp * SCHUR_SK, i0), kend = min(sp * SCHUR_SK + SCHUR_SK, nf);
     if (kbeg >= kend) return;
     const int t = threadIdx.x, tr = t & 15, tc = t >> 4;   // rows tr + 16 q, columns tc + 16 p
@@ -241,7 +270,7 @@ __global__ __launch_bounds__(256) void schur_u_part(const float *__restrict__ ZT
             const int kk = e & (SCHUR_KC - 1), row = e / SCHUR_KC, k = k0 + kk;
             const int i = i0 + row;
             Am[kk][row] = (i < nf && k < kend) ? (double)ZT[(size_t)i * ldz + k] : 0.0;
-            Bm[kk][row] = (row < rs && k < kend) ? (double)ZT[(size_t)R[row] * ldz + k] : 0.0;
+            Bm[kk][row] = (j0 + row < rs && k < kend) ? (double)ZT[(size_t)R[j0 + row] * ldz + k] : 0.0;
         }
         __syncthreads();
 #pragma unroll 8
@@ -263,7 +292,7 @@ __global__ __launch_bounds__(256) void schur_u_part(const float *__restrict__ ZT
         if (i >= nf) continue;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const int j = tc + 16 * p;
+            const int j = j0 + tc + 16 * p;
             if (j < rs) part[((size_t)sp * rs + j) * nf + i] = acc[q][p];
         }
     }
@@ -282,20 +311,19 @@ __global__ void schur_u_reduce(const double *__restrict__ part, int nsplit, int 
     U[e] = a;
 }
 
-__global__ void schur_s_tile(const double *__restrict__ U, int nf, const int32_t *__restrict__ R, int rs,
-                             double *__restrict__ S) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= 64 * 64) return;
-    const int a = e & 63, b = e >> 6;
-    S[b * 64 + a] = (a < rs && b < rs) ? U[(size_t)b * nf + R[a]] : (a == b ? 1.0 : 0.0);
+// S = U(R, :) (rs x rs) with the identity below it: G (2 rs x rs, column-major,
+// leading dimension ldg >= 2 rs), as the identity-row factor takes it.
+__global__ void schur_s_aug(const double *__restrict__ U, int nf, const int32_t *__restrict__ R, int rs,
+                            double *__restrict__ G, int ldg) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    if (a >= 2 * rs || b >= rs) return;
+    G[(size_t)b * ldg + a] = (a < rs) ? U[(size_t)b * nf + R[a]] : (a - rs == b ? 1.0 : 0.0);
 }
 
 __global__ __launch_bounds__(256) void gather_from_factor(int n, int nf, const int32_t *__restrict__ map,
                                                           const double *__restrict__ zf, const double *__restrict__ r,
                                                           const double *__restrict__ A, int lda,
-                                                          double *__restrict__ z, double *__restrict__ part,
-                                                          const double *__restrict__ U, int rs,
-                                                          const double *__restrict__ y) {
+                                                          double *__restrict__ z, double *__restrict__ part) {
     __shared__ double wsum[2][4];
     const int i = blockIdx.x * 256 + threadIdx.x;
     double p0 = 0.0, p1 = 0.0;
@@ -305,24 +333,6 @@ __global__ __launch_bounds__(256) void gather_from_factor(int n, int nf, const i
         if (f >= 0) {
             z0 = zf[f];
             z1 = zf[nf + f];
-            if (rs > 0) {
-                // eight column loads in flight per step, eight partial sums
-                double c0[8], c1[8];
-#pragma unroll
-                for (int q = 0; q < 8; q++) c0[q] = c1[q] = 0.0;
-                for (int j0 = 0; j0 < rs; j0 += 8) {
-                    double u[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) u[q] = (j0 + q < rs) ? U[(size_t)(j0 + q) * nf + f] : 0.0;
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        c0[q] = fma(u[q], y[j0 + q], c0[q]);
-                        c1[q] = fma(u[q], y[64 + j0 + q], c1[q]);
-                    }
-                }
-                z0 -= ((c0[0] + c0[1]) + (c0[2] + c0[3])) + ((c0[4] + c0[5]) + (c0[6] + c0[7]));
-                z1 -= ((c1[0] + c1[1]) + (c1[2] + c1[3])) + ((c1[4] + c1[5]) + (c1[6] + c1[7]));
-            }
         } else {
             const double dii = A[(size_t)i * lda + i];
             z0 = r[i] / dii;
@@ -354,8 +364,8 @@ struct PcgBufs {
     double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *w = nullptr;
     double *rf = nullptr, *zf = nullptr;
     double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr, *rz = nullptr;
-    double *y = nullptr;                       // [2][64]: S^-1 zf(R) of the current application
-    double *spart = nullptr, *stile = nullptr;  // split partials of U, the S tile
+    double *y = nullptr;                       // [2][SCHUR_MAXR]: S^-1 zf(R) of the current application
+    double *spart = nullptr, *sg = nullptr;    // split partials of U; S (tile, or with the identity below)
     size_t spart_n = 0;
 };
 thread_local PcgBufs g_pcg;
@@ -373,7 +383,7 @@ void pcg_reserve(int n) {
         *v = (double *)irwls_dev_alloc((size_t)2 * (n + 1) * sizeof(double));
     }
     if (!g_pcg.rz) g_pcg.rz = (double *)irwls_dev_alloc(4 * sizeof(double));
-    if (!g_pcg.y) g_pcg.y = (double *)irwls_dev_alloc(2 * 64 * sizeof(double));
+    if (!g_pcg.y) g_pcg.y = (double *)irwls_dev_alloc(2 * SCHUR_MAXR * sizeof(double));
     g_pcg.cap = n;
 }
 
@@ -387,9 +397,9 @@ struct Precond {
     int ldz;
     int nf;
     const int32_t *map, *inv;
-    const int32_t *R;       // removed rows' factor positions (rs of them, rs <= 64)
+    const int32_t *R;       // removed rows' factor positions (rs of them, rs <= SCHUR_MAXR)
     int rs;
-    const double *U, *X;    // B(:, R) and L_S^-1 (irwls_dev_pcg_schur_setup)
+    const double *U, *Sinv; // B(:, R) and S^-1 (irwls_dev_pcg_schur_setup)
 };
 
 int precond(const Precond &M, int n, const double *A, int lda, const double *r, double *w, double *z,
@@ -414,12 +424,16 @@ int precond(const Precond &M, int n, const double *A, int lda, const double *r, 
                        (const double *)nullptr, (double *)nullptr);
     IRWLS_LAUNCH_CHECK();
     if (M.rs > 0) {
-        hipLaunchKernelGGL(schur_apply, dim3(1), dim3(256), 0, stream(), g_pcg.zf, nf, M.R, M.rs, M.X, g_pcg.y);
+        hipLaunchKernelGGL(schur_apply, dim3((M.rs + 3) / 4), dim3(256), 0, stream(), g_pcg.zf, nf, M.R, M.rs, M.Sinv,
+                           g_pcg.y);
+        IRWLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(schur_correct, dim3((nf + 63) / 64), dim3(256), 0, stream(), g_pcg.zf, nf, M.U, M.rs,
+                           (const double *)g_pcg.y);
         IRWLS_LAUNCH_CHECK();
     }
     const int gv = (n + 255) / 256;
     hipLaunchKernelGGL(gather_from_factor, dim3(gv), dim3(256), 0, stream(), n, nf, M.map, g_pcg.zf, r, A, lda, z,
-                       part, M.U, M.rs, (const double *)g_pcg.y);
+                       part);
     IRWLS_LAUNCH_CHECK();
     return gv;
 }
@@ -430,8 +444,9 @@ extern "C" {
 
 int irwls_dev_pcg_schur_setup(const float *ZT, int ldz, int nf, const int32_t *R, int rs, double *U, double *Sinv,
                               int check) {
-    if (rs <= 0 || rs > 64 || nf <= 0) {
-        fprintf(stderr, "libirwls: irwls_dev_pcg_schur_setup needs 1 <= rs <= 64 (rs=%d, nf=%d)\n", rs, nf);
+    if (rs <= 0 || rs > SCHUR_MAXR || nf <= 0) {
+        fprintf(stderr, "libirwls: irwls_dev_pcg_schur_setup needs 1 <= rs <= %d (rs=%d, nf=%d)\n", SCHUR_MAXR, rs,
+                nf);
         exit(2);
     }
     const int nsplit = (nf + SCHUR_SK - 1) / SCHUR_SK;
@@ -441,17 +456,30 @@ int irwls_dev_pcg_schur_setup(const float *ZT, int ldz, int nf, const int32_t *R
         g_pcg.spart = (double *)irwls_dev_alloc(need * sizeof(double));
         g_pcg.spart_n = need;
     }
-    if (!g_pcg.stile) g_pcg.stile = (double *)irwls_dev_alloc(64 * 64 * sizeof(double));
-    hipLaunchKernelGGL(schur_u_part, dim3((nf + 63) / 64, nsplit), dim3(256), 0, stream(), ZT, ldz, nf, R, rs,
-                       g_pcg.spart);
+    if (!g_pcg.sg) g_pcg.sg = (double *)irwls_dev_alloc((size_t)2 * SCHUR_MAXR * SCHUR_MAXR * sizeof(double));
+    hipLaunchKernelGGL(schur_u_part, dim3((nf + 63) / 64, nsplit, (rs + 63) / 64), dim3(256), 0, stream(), ZT, ldz,
+                       nf, R, rs, g_pcg.spart);
     IRWLS_LAUNCH_CHECK();
     const size_t m = (size_t)nf * rs;
     hipLaunchKernelGGL(schur_u_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream(), g_pcg.spart, nsplit,
                        nf, rs, U);
     IRWLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(schur_s_tile, dim3(16), dim3(256), 0, stream(), U, nf, R, rs, g_pcg.stile);
+    // S^-1 = L_S^-T L_S^-1: from the tile inverse (rs <= 64, X = L_S^-1), or
+    // from the identity rows of [S; I] factored (Z_S = L_S^-T)
+    const int ldg = 2 * rs;
+    hipLaunchKernelGGL(schur_s_aug, dim3((2 * rs + 255) / 256, rs), dim3(256), 0, stream(), U, nf, R, rs, g_pcg.sg,
+                       ldg);
     IRWLS_LAUNCH_CHECK();
-    return irwls_dev_tile_potrf_inv(g_pcg.stile, 64, rs, Sinv, check);
+    int info;
+    if (rs <= 64) {
+        double *X = g_pcg.sg + (size_t)ldg * rs;   // 2 x 64 x 64 after G
+        info = irwls_dev_tile_potrf_inv(g_pcg.sg, ldg, rs, X, check);
+        irwls_dev_gemm(1, 0, rs, rs, rs, 1.0, X, 64, X, 64, 0.0, Sinv, rs);
+    } else {
+        info = irwls_dev_potrf_ident(g_pcg.sg, rs, ldg, check);
+        irwls_dev_gemm(0, 1, rs, rs, rs, 1.0, g_pcg.sg + rs, ldg, g_pcg.sg + rs, ldg, 0.0, Sinv, rs);
+    }
+    return info;
 }
 
 int irwls_dev_pcg_spd2_mapped(const double *A, int n, int lda, const double *B, const float *F, int ldf,
@@ -550,7 +578,7 @@ int irwls_dev_pcg_spd2(const double *A, int n, int lda, const double *B, const f
 
 void irwls_dev_release_pcg_buffers(void) {
     double **vs[] = {&g_pcg.r, &g_pcg.p, &g_pcg.q, &g_pcg.z, &g_pcg.w, &g_pcg.rf, &g_pcg.zf,
-                     &g_pcg.part_a, &g_pcg.part_b, &g_pcg.part_c, &g_pcg.rz, &g_pcg.y, &g_pcg.spart, &g_pcg.stile};
+                     &g_pcg.part_a, &g_pcg.part_b, &g_pcg.part_c, &g_pcg.rz, &g_pcg.y, &g_pcg.spart, &g_pcg.sg};
     for (double **v : vs) {
         if (*v) irwls_dev_free(*v);
         *v = nullptr;

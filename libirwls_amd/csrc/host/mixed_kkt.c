@@ -34,7 +34,7 @@
 
 #include "internal.h"
 
-#define SCHUR_MAX 64   /* removed rows the preconditioner corrects exactly (one tile) */
+#define SCHUR_MAX 64   /* removed rows the preconditioner corrects exactly */
 
 typedef struct {
     float *F;           /* 2 cap x cap: L over the identity rows' L^-T */
@@ -52,7 +52,7 @@ typedef struct {
     int removed, added; /* of the last map_to_factor */
     int32_t hR[SCHUR_MAX];   /* factor positions of the removed rows */
     int32_t *dR;        /* device copy */
-    double *dU, *dSinv; /* B(:, R) (cap x SCHUR_MAX) and the tile inverse of B(R, R) */
+    double *dU, *dSinv; /* B(:, R) (cap x SCHUR_MAX) and B(R, R)^-1 */
 } mixed_cache;
 
 /* per thread: the batched trainer runs one problem per host thread */
@@ -76,7 +76,7 @@ static void mc_reserve(int n) {
     free(g_mc.inv);
     g_mc.dU = (double *)irwls_dev_alloc((size_t)n * SCHUR_MAX * sizeof(double));
     if (!g_mc.dR) g_mc.dR = (int32_t *)irwls_dev_alloc(SCHUR_MAX * sizeof(int32_t));
-    if (!g_mc.dSinv) g_mc.dSinv = (double *)irwls_dev_alloc(2 * 64 * 64 * sizeof(double));
+    if (!g_mc.dSinv) g_mc.dSinv = (double *)irwls_dev_alloc((size_t)SCHUR_MAX * SCHUR_MAX * sizeof(double));
     g_mc.F = (float *)irwls_dev_alloc((size_t)2 * n * n * sizeof(float));
     g_mc.dmap = (int32_t *)irwls_dev_alloc((size_t)n * sizeof(int32_t));
     g_mc.dinv = (int32_t *)irwls_dev_alloc((size_t)n * sizeof(int32_t));
@@ -187,6 +187,9 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
             mode = 1;
         } else {
             changes = map_to_factor(S1, n);
+            /* larger removals come early in a working set, while the weights
+               (the diagonal) still change much: corrected, those solves ran
+               past their CG budget and refactored anyway */
             if (schur_enabled() && g_mc.removed > 0 && g_mc.removed <= SCHUR_MAX && g_mc.added <= small)
                 mode = 3;
             else if (changes <= small)
