@@ -286,6 +286,11 @@ double *trainFULL(svm_dataset dataset, properties props) {
     double *ysub = (double *)irwls_xcalloc(W + 1, sizeof(double));
     int32_t *cols = (int32_t *)irwls_xcalloc(n, sizeof(int32_t));
     double *wv = (double *)irwls_xcalloc(n, sizeof(double));
+    /* the nonzero weights of SIN in SIN order (cols/wv of the next GIN),
+       collected by the working-set rebuild */
+    int32_t *gcols = (int32_t *)irwls_xcalloc(n, sizeof(int32_t));
+    double *gw = (double *)irwls_xcalloc(n, sizeof(double));
+    int ng = 0;
     int32_t *d_SW = (int32_t *)irwls_dev_alloc((size_t)(W > 0 ? W : 1) * sizeof(int32_t));
     int32_t *d_cols = (int32_t *)irwls_dev_alloc((size_t)n * sizeof(int32_t));
     double *d_w = (double *)irwls_dev_alloc((size_t)n * sizeof(double));
@@ -309,26 +314,20 @@ double *trainFULL(svm_dataset dataset, properties props) {
     while (done == 0 && since < 300) {
         g_stats.outer_iters++;
         irwls_dev_h2d(d_SW, SW, (size_t)nSW * sizeof(int32_t));
-        /* GIN = y_SW K(SW, SIN_nz) beta (src/full-train.c:436-456) */
+        /* GIN = y_SW K(SW, SIN_nz) beta (src/full-train.c:436-456).  The
+           reference sums every beta over SIN in SIN order; the zeros leave
+           that sum as it is, so the nonzero ones in the same order (gcols,
+           gw: collected by the rebuild below) give the same value. */
         if (nSIn > 0) {
             memset(GIN, 0, (size_t)(nSW + 1) * sizeof(double));
-            int nc = 0;
             double gsum = 0.0;
-            for (int o = 0; o < nSIn; o++) {
-                const double b = bNew[SIN[o]];
-                if (b != 0.0) {
-                    cols[nc] = SIN[o];
-                    wv[nc] = b;
-                    nc++;
-                }
-                gsum += b;
-            }
-            if (nc > 0 && nSW > 0) {
+            for (int o = 0; o < ng; o++) gsum += gw[o];
+            if (ng > 0 && nSW > 0) {
                 for (int i = 0; i < nSW; i++) ysub[i] = y[SW[i]];
-                irwls_dev_h2d(d_cols, cols, (size_t)nc * sizeof(int32_t));
-                irwls_dev_h2d(d_w, wv, (size_t)nc * sizeof(double));
+                irwls_dev_h2d(d_cols, gcols, (size_t)ng * sizeof(int32_t));
+                irwls_dev_h2d(d_w, gw, (size_t)ng * sizeof(double));
                 irwls_dev_h2d(d_ysw, ysub, (size_t)nSW * sizeof(double));
-                irwls_kmv(dd, d_SW, nSW, d_cols, nc, props.kernelType, props.Kgamma, d_w, 0.0, 1.0, 0.0, d_ysw,
+                irwls_kmv(dd, d_SW, nSW, d_cols, ng, props.kernelType, props.Kgamma, d_w, 0.0, 1.0, 0.0, d_ysw,
                           d_GIN);
                 irwls_dev_d2h(GIN, d_GIN, (size_t)nSW * sizeof(double));
             }
@@ -363,26 +362,20 @@ double *trainFULL(svm_dataset dataset, properties props) {
                       NULL, d_e);
             irwls_dev_d2h(e, d_e, (size_t)n * sizeof(double));
         }
+        /* The change of the weights (src/full-train.c:519-533) and the
+           working-set rebuild (:540-643) in one pass over the samples: the
+           rebuild reads only bNew, e and y, and the two sums run over
+           i = 0..n in index order, as the reference's own loop forms them. */
         double dW = 0.0, nW = 0.0;
-        for (int i = 0; i < n + 1; i++) {
-            double t = beta[i] - bNew[i];
-            dW += t * t;
-            nW += beta[i] * beta[i];
-        }
-        if (dW / nW < props.Eta) done = 1;
-        memcpy(beta, bNew, (size_t)(n + 1) * sizeof(double));
-        if (dW / nW < bestN) {
-            bestN = dW / nW;
-            since = 0;
-        } else {
-            since++;
-        }
-        /* working-set rebuild (src/full-train.c:540-643) */
         int found[2][3] = {{0, 0, 0}, {0, 0, 0}};
         nSW = 0;
         nSIn = 0;
         nSC = 0;
+        ng = 0;
         for (int i = 0; i < n; i++) {
+            const double t = beta[i] - bNew[i];
+            dW += t * t;
+            nW += beta[i] * beta[i];
             const double by = bNew[i] * y[i];
             int state, viol;
             if (by == C) {
@@ -409,7 +402,24 @@ double *trainFULL(svm_dataset dataset, properties props) {
                 SC[nSC++] = i;
             } else {
                 SIN[nSIn++] = i;
+                if (bNew[i] != 0.0) {
+                    gcols[ng] = i;
+                    gw[ng++] = bNew[i];
+                }
             }
+        }
+        {
+            const double t = beta[n] - bNew[n];
+            dW += t * t;
+            nW += beta[n] * beta[n];
+        }
+        if (dW / nW < props.Eta) done = 1;
+        memcpy(beta, bNew, (size_t)(n + 1) * sizeof(double));
+        if (dW / nW < bestN) {
+            bestN = dW / nW;
+            since = 0;
+        } else {
+            since++;
         }
         if (props.verbose == 1) {
             printf(".");
@@ -421,8 +431,16 @@ double *trainFULL(svm_dataset dataset, properties props) {
             int *perm = rpermute(nSC);
             const int space = W - nSW;
             for (int i = 0; i < nSC; i++) {
-                if (i < space) SW[nSW++] = SC[perm[i]];
-                else SIN[nSIn++] = SC[perm[i]];
+                if (i < space) {
+                    SW[nSW++] = SC[perm[i]];
+                } else {
+                    const int j = SC[perm[i]];
+                    SIN[nSIn++] = j;
+                    if (bNew[j] != 0.0) {
+                        gcols[ng] = j;
+                        gw[ng++] = bNew[j];
+                    }
+                }
             }
             free(perm);
         }
@@ -446,6 +464,8 @@ double *trainFULL(svm_dataset dataset, properties props) {
     free(ysub);
     free(cols);
     free(wv);
+    free(gcols);
+    free(gw);
     if (props.verbose == 1) printf("\n");
     g_stats.t_total = irwls_now() - t_start;
     g_stats.t_wait = irwls_dev_wait_seconds(0) - wait0;
