@@ -335,6 +335,11 @@ int  irwls_dev_pcg_spd2(const double *d_A, int n, int lda, const double *d_B, co
 int  irwls_dev_pcg_spd2_mapped(const double *d_A, int n, int lda, const double *d_B, const float *d_F, int ldf,
                                const float *d_ZT, int ldz, int nf, const int32_t *d_map, const int32_t *d_inv,
                                double tol, int maxit, double *d_X, double *relres);
+/* The KKT system's failed-last-pivot result from its two SPD solves
+   (d_X = [u; z], u = A^-1 r1, z = A^-1 y; d_ys = y, n entries): s = -y^T z,
+   b = (last - y^T u) / s^2, d_x = u - z b (n entries) and d_x[n] = b, as
+   LAPACK's dpotrf/dpotrs leave it (src/ParallelAlgorithms.c:277-279, 441-445). */
+void irwls_dev_kkt_closed_form(int n, const double *d_X, const double *d_ys, double last, double *d_x);
 /* Rows removed from the factored set: with B = Z Z^T and R the factor
    positions of rs <= 256 removed rows (device), irwls_dev_pcg_schur_setup
    computes d_U = B(:, R) (nf x rs, column-major, leading dimension nf) and

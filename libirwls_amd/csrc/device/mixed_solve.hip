@@ -359,6 +359,41 @@ __global__ __launch_bounds__(256) void gather_from_factor(int n, int nf, const i
     }
 }
 
+// The failed-last-pivot result of the KKT system from its two SPD solves
+// u = A^-1 r1, z = A^-1 y (X = [u; z]): s = -y^T z, b = (last - y^T u) / s^2,
+// x = u - z b, x[n] = b.  One work-group; the two dot products are summed
+// in a fixed order (strided per thread, then a tree).
+__global__ __launch_bounds__(1024) void kkt_closed_form(int n, const double *__restrict__ X,
+                                                        const double *__restrict__ ys, double last,
+                                                        double *__restrict__ x) {
+    __shared__ double sz[1024], su[1024];
+    __shared__ double sb;
+    const int t = threadIdx.x;
+    double a = 0.0, c = 0.0;
+    for (int i = t; i < n; i += 1024) {
+        a = fma(ys[i], X[n + i], a);
+        c = fma(ys[i], X[i], c);
+    }
+    sz[t] = a;
+    su[t] = c;
+    __syncthreads();
+    for (int h = 512; h >= 1; h >>= 1) {
+        if (t < h) {
+            sz[t] += sz[t + h];
+            su[t] += su[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double sv = 0.0 - sz[0];   // Schur value of the last pivot (H_nn = 0)
+        sb = (last - su[0]) / (sv * sv);
+        x[n] = sb;
+    }
+    __syncthreads();
+    const double b = sb;
+    for (int i = t; i < n; i += 1024) x[i] = fma(-X[n + i], b, X[i]);
+}
+
 struct PcgBufs {
     int cap = 0;
     double *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *w = nullptr;
@@ -569,6 +604,12 @@ int irwls_dev_pcg_spd2_schur(const double *A, int n, int lda, const double *B, c
         }
     }
     return done ? it : -it;
+}
+
+void irwls_dev_kkt_closed_form(int n, const double *X, const double *ys, double last, double *x) {
+    if (n < 0) return;
+    hipLaunchKernelGGL(kkt_closed_form, dim3(1), dim3(1024), 0, stream(), n, X, ys, last, x);
+    IRWLS_LAUNCH_CHECK();
 }
 
 int irwls_dev_pcg_spd2(const double *A, int n, int lda, const double *B, const float *F, int ldf, const float *ZT,

@@ -42,19 +42,17 @@ int *rpermute(int n) {
 }
 
 /* Dense solve of the KKT system on the device, right-hand side in ws->rhs
-   (formed on the device).  FP64: H (N x N, leading dimension N+1) is
-   factored with the right-hand side riding in row N, then one backward
-   sweep; the solution stays in ws->x and 0 is returned.  Mixed precision:
-   the closed form of the same result from two SPD solves (mixed_kkt.c),
-   computed on the host into x and also copied to ws->x, and 1 is returned;
-   FP64 when that declines.  `last` is the right-hand side's last entry. */
-static int kkt_solve(irwls_ws *ws, int N, double last, double *x, const int32_t *S1, const double *ys, long gen) {
+   (formed on the device), solution into ws->x.  FP64: H (N x N, leading
+   dimension N+1) is factored with the right-hand side riding in row N, then
+   one backward sweep.  Mixed precision: the closed form of the same result
+   from two SPD solves (mixed_kkt.c); FP64 when that declines.  `last` is
+   the right-hand side's last entry. */
+static void kkt_solve(irwls_ws *ws, int N, double last, const int32_t *S1, long gen) {
     double t0 = irwls_now();
     int done = 0;
     if (irwls_opts()->precision == 1 || irwls_opts()->precision == 2) {
-        done = irwls_kkt_solve_mixed(ws->H, N - 1, S1, gen, ys, ws->rhs, ws->ys, last, x, &g_stats);
+        done = irwls_kkt_solve_mixed(ws->H, N - 1, S1, gen, ws->rhs, ws->ys, last, ws->x, &g_stats);
         if (!done && N - 1 >= 128) g_stats.fp64_fallbacks++;
-        if (done) irwls_dev_h2d(ws->x, x, (size_t)N * sizeof(double));
     }
     if (!done) {
         /* failure code ignored, as the reference */
@@ -63,7 +61,6 @@ static int kkt_solve(irwls_ws *ws, int N, double last, double *x, const int32_t 
     }
     g_stats.solves++;
     g_stats.t_solve += irwls_now() - t0;
-    return done;
 }
 
 double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_widx, int l, const double *y,
@@ -126,7 +123,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
         memcpy(ws->hys, hbuf, (size_t)N * sizeof(double));
         memcpy(ws->has, hbuf2, (size_t)N * sizeof(double));
         irwls_ws_upload(ws, 0);
-        int on_host = 0, batched = 0;
+        int batched = 0;
         if (fp64_solve && irwls_batch_worker()) {
             /* a batched run: the whole iteration's device work joins the
                other problems' pending iterations in one batched round (or
@@ -145,7 +142,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
         if (!batched) {
             irwls_dev_kkt_rhs(ws->S1, nS1, ws->G13, use_g13, ws->gin, last, ws->rhs);
             irwls_dev_build_kkt(ws->Kw, l, ws->S1, nS1, ws->ys, ws->as, ws->H, N + 1);
-            on_host = kkt_solve(ws, N, last, bAux, S1, hbuf, gen);
+            kkt_solve(ws, N, last, S1, gen);
             /* weights after the solve and the error update over the changed
                weights (src/full-train.c:205-249), on the device */
             irwls_dev_inner_update(ws->x, ws->S1, nS1, ws->S3, nS3, ws->yl, C, l, ws->beta, ws->bnew, ws->cols,
@@ -153,8 +150,7 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
             irwls_dev_gemv_gather_dn(ws->Kw, l, l, ws->cols, ws->w, ws->nc_cst, 1.0, -1.0, ws->esub);
         }
         /* one read-back of the solution and the errors */
-        if (on_host) irwls_dev_d2h(e, ws->esub, (size_t)l * sizeof(double));
-        else irwls_dev_d2h2(bAux, ws->x, (size_t)N * sizeof(double), e, ws->esub, (size_t)l * sizeof(double));
+        irwls_dev_d2h2(bAux, ws->x, (size_t)N * sizeof(double), e, ws->esub, (size_t)l * sizeof(double));
 
         /* weights, bounds and change (src/full-train.c:205-230), the same
            values as the device's */

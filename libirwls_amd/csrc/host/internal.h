@@ -129,8 +129,8 @@ double *irwls_sub_irwls_dev(irwls_ws *ws, irwls_devdata *dd, const int32_t *d_wi
 /* Mixed precision (mixed_kkt.c).  A new generation marks a new kernel
    block (one per subIRWLS call): factors of an older one are never reused. */
 long irwls_mixed_new_generation(void);
-int irwls_kkt_solve_mixed(const double *d_H, int nS1, const int32_t *S1, long gen, const double *ys,
-                          const double *d_rhs, const double *d_ys, double last, double *x, irwls_gpu_stats *st);
+int irwls_kkt_solve_mixed(const double *d_H, int nS1, const int32_t *S1, long gen, const double *d_rhs,
+                          const double *d_ys, double last, double *d_x, irwls_gpu_stats *st);
 void irwls_mixed_release(void);
 
 /* Learned SGMA proposal (proposal.c). */

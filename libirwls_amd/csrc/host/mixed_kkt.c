@@ -45,7 +45,6 @@ typedef struct {
     int valid, n;       /* factor of the system of size n is held */
     long gen;           /* subIRWLS call it belongs to */
     int32_t *S1;        /* its free set */
-    double *hX;
     int32_t *map, *inv; /* host copies */
     int32_t *mark;      /* working-set index -> factor position + 1 */
     int mark_cap;
@@ -71,7 +70,6 @@ static void mc_reserve(int n) {
     if (g_mc.dinv) irwls_dev_free(g_mc.dinv);
     if (g_mc.dU) irwls_dev_free(g_mc.dU);
     free(g_mc.S1);
-    free(g_mc.hX);
     free(g_mc.map);
     free(g_mc.inv);
     g_mc.dU = (double *)irwls_dev_alloc((size_t)n * SCHUR_MAX * sizeof(double));
@@ -86,7 +84,6 @@ static void mc_reserve(int n) {
     g_mc.B = (double *)irwls_dev_alloc((size_t)2 * n * sizeof(double));
     g_mc.X = (double *)irwls_dev_alloc((size_t)2 * n * sizeof(double));
     g_mc.S1 = (int32_t *)irwls_xmalloc((size_t)n * sizeof(int32_t));
-    g_mc.hX = (double *)irwls_xmalloc((size_t)2 * n * sizeof(double));
     g_mc.cap = n;
     g_mc.valid = 0;
 }
@@ -102,7 +99,6 @@ void irwls_mixed_release(void) {
     if (g_mc.dR) irwls_dev_free(g_mc.dR);
     if (g_mc.dSinv) irwls_dev_free(g_mc.dSinv);
     free(g_mc.S1);
-    free(g_mc.hX);
     free(g_mc.map);
     free(g_mc.inv);
     free(g_mc.mark);
@@ -170,11 +166,11 @@ static int map_to_factor(const int32_t *S1, int n) {
 
 /* Solves with H (device, N = nS1+1, leading dimension N+1, unfactored) for
    the right-hand side whose first nS1 entries are d_rhs (device) and whose
-   last entry is `last`; ys = y over S1 (host, and d_ys on the device).
-   Writes the N entries of x and returns 1, or returns 0 when the caller
+   last entry is `last`; d_ys = y over S1 (device).  Writes the N entries of
+   the solution to d_x (device) and returns 1, or returns 0 when the caller
    should use the FP64 path. */
-int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen, const double *ys,
-                          const double *d_rhs, const double *d_ys, double last, double *x, irwls_gpu_stats *st) {
+int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen, const double *d_rhs,
+                          const double *d_ys, double last, double *d_x, irwls_gpu_stats *st) {
     const int n = nS1;
     const int ldh = nS1 + 2;
     if (n < 128) return 0;
@@ -237,16 +233,7 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
         g_mc.valid = 0;
         return 0;
     }
-    irwls_dev_d2h(g_mc.hX, g_mc.X, (size_t)2 * n * sizeof(double));
-    const double *u = g_mc.hX, *z = g_mc.hX + n;
-    double yz = 0.0, yu = 0.0;
-    for (int i = 0; i < n; i++) {
-        yz += ys[i] * z[i];
-        yu += ys[i] * u[i];
-    }
-    const double s = 0.0 - yz;   /* Schur value of the last pivot (H_nn = 0) */
-    const double b = (last - yu) / (s * s);
-    for (int i = 0; i < n; i++) x[i] = u[i] - z[i] * b;
-    x[n] = b;
+    /* the closed form on the device, into the caller's solution vector */
+    irwls_dev_kkt_closed_form(n, g_mc.X, d_ys, last, d_x);
     return 1;
 }
