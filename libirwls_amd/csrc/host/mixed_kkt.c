@@ -26,6 +26,8 @@
  * factor).  Without the correction each removed row adds a rank-one
  * difference, about one more iteration each, so larger removals refactored.
  * More removed rows, many added ones, or CG missing its budget, refactor.
+ * Where the diagonal's change dominates (a corrected solve needing more
+ * than SCHUR_ITS iterations), the correction stops for that working set.
  */
 #include <math.h>
 #include <stdio.h>
@@ -35,6 +37,7 @@
 #include "internal.h"
 
 #define SCHUR_MAX 64   /* removed rows the preconditioner corrects exactly */
+#define SCHUR_ITS 10   /* a corrected solve needing more CG iterations stops the correction for its generation */
 
 typedef struct {
     float *F;           /* 2 cap x cap: L over the identity rows' L^-T */
@@ -49,6 +52,7 @@ typedef struct {
     int32_t *mark;      /* working-set index -> factor position + 1 */
     int mark_cap;
     int removed, added; /* of the last map_to_factor */
+    long schur_off;     /* generation whose corrected solves took too many iterations */
     int32_t hR[SCHUR_MAX];   /* factor positions of the removed rows */
     int32_t *dR;        /* device copy */
     double *dU, *dSinv; /* B(:, R) (cap x SCHUR_MAX) and B(R, R)^-1 */
@@ -186,7 +190,8 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
             /* larger removals come early in a working set, while the weights
                (the diagonal) still change much: corrected, those solves ran
                past their CG budget and refactored anyway */
-            if (schur_enabled() && g_mc.removed > 0 && g_mc.removed <= SCHUR_MAX && g_mc.added <= small)
+            if (schur_enabled() && g_mc.schur_off != gen && g_mc.removed > 0 && g_mc.removed <= SCHUR_MAX &&
+                g_mc.added <= small)
                 mode = 3;
             else if (changes <= small)
                 mode = 2;
@@ -222,6 +227,9 @@ int irwls_kkt_solve_mixed(const double *H, int nS1, const int32_t *S1, long gen,
         it = irwls_dev_pcg_spd2(H, n, ldh, g_mc.B, g_mc.F, 2 * n, g_mc.ZT, n, tol, mode ? 16 : 200, g_mc.X, rel);
     }
     st->pcg_iters += it < 0 ? -it : it;
+    /* where the diagonal's change dominates, the correction saves a
+       factorization but costs more CG iterations than it was worth */
+    if (mode == 3 && (it < 0 || it > SCHUR_ITS)) g_mc.schur_off = gen;
     if (it < 0 && mode != 0) {
         /* the earlier factor no longer preconditions well enough */
         mc_reserve(n);
