@@ -138,12 +138,22 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd(const float *__restrict__ P, 
 }
 
 // grad[p] = sum over blocks (fixed order); loss[0] = mean squared error.
+// Eight blocks' loads are issued before their eight additions, made in
+// block order, so each thread keeps eight reads in flight.
 __global__ void mlp_reduce(const float *__restrict__ gpart, const float *__restrict__ lpart, int nblk, int nb,
                            float *__restrict__ grad, float *__restrict__ loss) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < NP) {
         float a = 0.0f;
-        for (int b = 0; b < nblk; b++) a += gpart[(size_t)b * NP + p];
+        int b = 0;
+        for (; b + 8 <= nblk; b += 8) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = gpart[(size_t)(b + q) * NP + p];
+#pragma unroll
+            for (int q = 0; q < 8; q++) a += v[q];
+        }
+        for (; b < nblk; b++) a += gpart[(size_t)b * NP + p];
         grad[p] = a;
     }
     if (p == 0 && loss) {
