@@ -487,7 +487,8 @@ def bench_budgeted(args, name, steps, warmup):
     pp = abi.make_predict_properties(labels=0)
     prow = args.proposal_rows if args.proposal_rows is not None else wl.get("proposal_rows", 0)
     modes = {"plain": dict(sgma_candidates=wl["cand"], proposal=0),
-             "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8, proposal_pool=4 * wl["cand"],
+             "learned": dict(sgma_candidates=wl["cand"], proposal=1, proposal_keep=8,
+                             proposal_pool=args.proposal_pool if args.proposal_pool else wl["cand"],
                              proposal_rows=prow)}
     if prow:
         # the network shortlists 16 of plain SGMA's own candidates (the same
@@ -735,6 +736,9 @@ def main():
     ap.add_argument("--proposal-rows", type=int, default=None,
                     help="budgeted, learned mode: rows on which the shortlist's gains are computed "
                          "(0 = all rows; default: the workload's own setting)")
+    ap.add_argument("--proposal-pool", type=int, default=None,
+                    help="budgeted, learned mode: candidates the network scores per step "
+                         "(default: the workload's candidates, plain SGMA's own draws)")
     ap.add_argument("--kernel-matrix-probe", action="store_true",
                     help="only time the kernel-matrix products (FP64 and BF16 x3 MFMA) at the mixed workload's shape")
     ap.add_argument("--copy-probe", action="store_true",

@@ -22,7 +22,8 @@ typedef struct irwls_gpu_options {
     int proposal;         /* 0 = plain SGMA draws (default), 1 = learned shortlist */
     int proposal_keep;    /* learned proposal: candidates kept for exact scoring (0: the network's choice) */
     double refine_tol;    /* mixed precision: relative residual target (default 1e-10) */
-    int proposal_pool;    /* learned proposal: candidates drawn per step (0: 4 x sgma_candidates, at most 1024) */
+    int proposal_pool;    /* learned proposal: candidates drawn per step (0: sgma_candidates, the draws of plain
+                             SGMA, which the network shortlists; at most 1024) */
     int proposal_warmup;  /* learned proposal: plain SGMA steps that train the network (0: B/10 in [20, 100]) */
     int proposal_rows;    /* learned proposal: rows on which the shortlist's gains are computed (0: all n, the
                              exact gains; m < n: a fixed random subset of m rows, an estimate) */

@@ -122,7 +122,7 @@ int *SGMA(svm_dataset dataset, properties props) {
     const int ncand = opt->sgma_candidates;
     /* learned proposal: pool size, shortlist and warm-up */
     const int learned = (opt->proposal == 1) && B > 2 && n > 0;
-    int pool = opt->proposal_pool > 0 ? opt->proposal_pool : 4 * ncand;
+    int pool = opt->proposal_pool > 0 ? opt->proposal_pool : ncand;   /* default: plain SGMA's own draws */
     if (pool > 1024) pool = 1024;
     if (pool < ncand) pool = ncand;
     int keep = opt->proposal_keep;

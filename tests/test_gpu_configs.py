@@ -194,7 +194,7 @@ def test_sparse_rbf_n1000000_learned_proposal():
     pp = abi.make_predict_properties(labels=0)
     hold = lib.Dataset.csr(te[:2001], col[rp[n]:rp[n] + te[2000]], val[rp[n]:rp[n] + te[2000]], yt[:2000])
     modes = {"plain59": dict(sgma_candidates=59, proposal=0),
-             "learned": dict(sgma_candidates=59, proposal=1, proposal_keep=8, proposal_pool=236,
+             "learned": dict(sgma_candidates=59, proposal=1, proposal_keep=8, proposal_pool=59,
                              proposal_rows=65536),
              "random8": dict(sgma_candidates=8, proposal=0)}
     res = {}
