@@ -161,9 +161,12 @@ template <int TC>
 __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
                                                         const double *__restrict__ Z, int ldz,
                                                         const double *__restrict__ KSM, int ldm,
-                                                        double *__restrict__ part) {
+                                                        double *__restrict__ part, int kchunk,
+                                                        double *__restrict__ rpart) {
     __shared__ double Zs[2][GK][TC];
     __shared__ double red[4][TC];
+    // k range of this work-group: all of [0, s) unless split (rpart != NULL)
+    const int kb = blockIdx.z * kchunk, ke = min(s, kb + kchunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, kq = lane >> 4;
     const int r0 = blockIdx.x * GR + wave * 32;
@@ -177,11 +180,11 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
     const double *zp = Z + (size_t)(zok ? c0 + zc : 0) * ldz;
     double z[ZK];
 #pragma unroll
-    for (int q = 0; q < ZK; q++) z[q] = (zok && zk + q < s) ? zp[zk + q] : 0.0;
+    for (int q = 0; q < ZK; q++) z[q] = (zok && kb + zk + q < ke) ? zp[kb + zk + q] : 0.0;
 #pragma unroll
     for (int q = 0; q < ZK; q++) Zs[0][zk + q][zc] = z[q];
     double a[4][2], an[4][2];
-    gain_load_a(a, KSC, n, s, 0, ra0, ok0, ok1);
+    gain_load_a(a, KSC, n, ke, kb, ra0, ok0, ok1);
     v4d acc[2][NB];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++)
@@ -189,14 +192,14 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
         for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (v4d){0.0, 0.0, 0.0, 0.0};
     __syncthreads();
     int buf = 0;
-    for (int kc = 0; kc < s; kc += GK) {
-        const bool more = kc + GK < s;
+    for (int kc = kb; kc < ke; kc += GK) {
+        const bool more = kc + GK < ke;
         if (more) {
-            gain_load_a(an, KSC, n, s, kc + GK, ra0, ok0, ok1);
+            gain_load_a(an, KSC, n, ke, kc + GK, ra0, ok0, ok1);
 #pragma unroll
             for (int q = 0; q < ZK; q++) {
                 const int k = kc + GK + zk + q;
-                z[q] = (zok && k < s) ? zp[k] : 0.0;
+                z[q] = (zok && k < ke) ? zp[k] : 0.0;
             }
         }
 #pragma unroll
@@ -223,6 +226,21 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
         buf ^= 1;
     }
     // accumulator register r of lane l: (row l/16 + 4r, candidate l%16) of its block
+    if (rpart) {   // split k: this range's products, summed by gain_split_finish
+        double *rp = rpart + (size_t)blockIdx.z * n * ncand;
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            const int c = c0 + nb * 16 + li;
+#pragma unroll
+            for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = r0 + mb * 16 + kq + 4 * r;
+                    if (i < n && c < ncand) rp[(size_t)i * ncand + c] = acc[mb][nb][r];
+                }
+        }
+        return;
+    }
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
         const int c = c0 + nb * 16 + li;
@@ -262,9 +280,12 @@ __global__ __launch_bounds__(256) void sgma_gain_kernel(int n, int s, int ncand,
 __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int ncand, const double *__restrict__ KSC,
                                                                const double *__restrict__ Z, int ldz,
                                                                const double *__restrict__ KSM, int ldm,
-                                                               double *__restrict__ part) {
+                                                               double *__restrict__ part, int kchunk,
+                                                               double *__restrict__ rpart) {
     __shared__ double Zs[2][16][16];     // [k][candidate]
     __shared__ double red[4][16];
+    // k range of this work-group: all of [0, s) unless split (rpart != NULL)
+    const int kb = blockIdx.z * kchunk, ke = min(s, kb + kchunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = blockIdx.x * T + wave * 16;
     const int ra = row0 + (lane & 15);        // A operand row
@@ -277,26 +298,26 @@ __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int
     const bool zok = zc < ncand;
     const double *zp = Z + (size_t)(zok ? zc : 0) * ldz;
     double a[4], an[4];
-    double zv = (zok && zk < s) ? zp[zk] : 0.0;
+    double zv = (zok && kb + zk < ke) ? zp[kb + zk] : 0.0;
     Zs[0][zk][zc] = zv;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-        const int k = 4 * u + kq;
-        a[u] = (rok && k < s) ? ap[(size_t)k * n] : 0.0;
+        const int k = kb + 4 * u + kq;
+        a[u] = (rok && k < ke) ? ap[(size_t)k * n] : 0.0;
     }
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < s; k0 += 16) {
-        const bool more = k0 + 16 < s;
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        const bool more = k0 + 16 < ke;
         if (more) {
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int k = k0 + 16 + 4 * u + kq;
-                an[u] = (rok && k < s) ? ap[(size_t)k * n] : 0.0;
+                an[u] = (rok && k < ke) ? ap[(size_t)k * n] : 0.0;
             }
             const int k = k0 + 16 + zk;
-            zv = (zok && k < s) ? zp[k] : 0.0;
+            zv = (zok && k < ke) ? zp[k] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -310,6 +331,15 @@ __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int
         buf ^= 1;
     }
     // accumulator register r of lane l: (row (l/16) + 4r, candidate l%16)
+    if (rpart) {   // split k: this range's products, summed by gain_split_finish
+        double *rp = rpart + (size_t)blockIdx.z * n * ncand;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = row0 + (lane >> 4) + 4 * r;
+            if (i < n && cok) rp[(size_t)i * ncand + cb] = acc[r];
+        }
+        return;
+    }
     double q = 0.0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -326,6 +356,52 @@ __global__ __launch_bounds__(256) void sgma_gain_narrow_kernel(int n, int s, int
     if (threadIdx.x < 16 && threadIdx.x < ncand) {
         const int c = threadIdx.x;
         part[(size_t)blockIdx.x * ncand + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+    }
+}
+
+// Split-k gains, finish: part[b][c] = sum over the rb rows i of row block b
+// of (KSM[i][c] - sum_z rpart[z][i][c])^2, the k ranges added in order
+// z = 0, 1, ...  One wave per block: lane l takes candidate l % C of each
+// group of C (the power of two >= ncand, at most 64) and rows l / C + j (64 /
+// C) of the block; the row lanes of a candidate add by a fixed butterfly.
+// rb = gain_split_rows(ncand): 4 rows from 17 candidates on, 16 below, so
+// that a lane adds at most 4 rows.
+__device__ __host__ inline int gain_split_c(int ncand) {
+    int C = 1;
+    while (C < ncand && C < 64) C <<= 1;
+    return C;
+}
+__device__ __host__ inline int gain_split_rows(int ncand) { return gain_split_c(ncand) >= 32 ? 4 : 16; }
+
+__global__ __launch_bounds__(256) void gain_split_finish(int n, int ncand, int nz, const double *__restrict__ rpart,
+                                                         const double *__restrict__ KSM, int ldm,
+                                                         double *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int rb = gain_split_rows(ncand);
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r0 = b * rb;
+    if (r0 >= n) return;   // whole waves; the kernel has no barrier
+    const int C = gain_split_c(ncand);
+    const int cl = lane & (C - 1), rl = lane / C, RL = 64 / C;
+    const int r1 = min(n, r0 + rb);
+    const size_t zs = (size_t)n * ncand;
+    for (int c0 = 0; c0 < ncand; c0 += C) {
+        const int c = c0 + cl;
+        double q = 0.0;
+        if (c < ncand) {
+#pragma unroll 4
+            for (int i = r0 + rl; i < r1; i += RL) {
+                const double *rp = rpart + (size_t)i * ncand + c;
+                double v = rp[0];
+#pragma unroll
+                for (int z = 1; z < 8; z++)
+                    if (z < nz) v += rp[z * zs];
+                const double d = KSM[(size_t)i * ldm + c] - v;
+                q += d * d;
+            }
+        }
+        for (int o = C; o < 64; o <<= 1) q += __shfl_xor(q, o);
+        if (rl == 0 && c < ncand) part[(size_t)b * ncand + c] = q;
     }
 }
 
@@ -791,36 +867,83 @@ void irwls_dev_gemm(int transA, int transB, int m, int n, int k, double alpha, c
 // Rows per partial sum of the gain kernel chosen for ncand candidates.
 static int gain_tile_rows(int ncand) { return ncand <= 16 ? T : GR; }
 
-void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
-                         int ldm, double *qout) {
-    if (n <= 0 || ncand <= 0) return;
-    const int rt = (n + gain_tile_rows(ncand) - 1) / gain_tile_rows(ncand), ct = (ncand + T - 1) / T;
-    double *part = (double *)scratch(10, (size_t)rt * ncand * sizeof(double));
+// Launches the gain products and leaves per-row-group partial sums in
+// scratch slot 10 (ncand per group, in row order); returns the rows per group
+// (the row tile, or gain_split_rows with the k range split).
+static int sgma_gain_parts(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
+                           int ldm) {
+    const int tr = gain_tile_rows(ncand);
+    const int rt = (n + tr - 1) / tr, ct = (ncand + T - 1) / T;
+    const int rb = gain_split_rows(ncand);
+    const int nbs = (n + rb - 1) / rb;
+    double *part = (double *)scratch(10, (size_t)nbs * ncand * sizeof(double));
+    // Few row tiles (a row subset): each work-group's chain of k chunks,
+    // one dependent HBM load per chunk, bounds the kernel, so the k range
+    // is split over up to 8 work-groups of at least 128 k each (at most
+    // 8 work-groups per CU in all), and gain_split_finish adds the ranges'
+    // products in order before the squares.
+    const bool wide = ncand > 16 && (long)rt * ct >= 2 * device_cus();
+    const int cg = ncand <= 16 ? 1 : wide ? ct : (ncand + 31) / 32;
+    const long wgs = (long)rt * cg;
+    int nz = 1;
+    if (wgs < 8L * device_cus()) {
+        nz = (s + 127) / 128;
+        if (nz > 8) nz = 8;
+        const long cap = 8L * device_cus() / wgs;
+        if (nz > cap) nz = (int)cap;
+        if (nz < 1) nz = 1;
+    }
+    int kchunk = s;
+    double *rpart = nullptr;
+    if (nz > 1) {
+        kchunk = ((s + nz - 1) / nz + GK - 1) / GK * GK;
+        nz = (s + kchunk - 1) / kchunk;
+        rpart = (double *)scratch(8, (size_t)nz * n * ncand * sizeof(double));
+    }
     {
         TimerScope ts(ncand <= 16 ? IRWLS_T_GAIN16 : IRWLS_T_SGMA, 2.0 * n * (double)s * ncand,
                       8.0 * (double)n * s + 8.0 * (double)n * ncand);
         if (ncand <= 16)
-            ts.launch(sgma_gain_narrow_kernel, dim3(rt), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
-        else if ((long)rt * ct >= 2 * device_cus())
-            ts.launch(sgma_gain_kernel<T>, dim3(rt, ct), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part);
+            ts.launch(sgma_gain_narrow_kernel, dim3(rt, 1, nz), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm,
+                      part, kchunk, rpart);
+        else if (wide)
+            ts.launch(sgma_gain_kernel<T>, dim3(rt, ct, nz), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm, part,
+                      kchunk, rpart);
         else   // few row tiles: 32-candidate tiles, twice the work-groups, the same sums in the same order
-            ts.launch(sgma_gain_kernel<32>, dim3(rt, (ncand + 31) / 32), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM,
-                      ldm, part);
+            ts.launch(sgma_gain_kernel<32>, dim3(rt, cg, nz), dim3(256), 0, n, s, ncand, KSC, Z, ldz, KSM, ldm,
+                      part, kchunk, rpart);
     }
-    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, rt, ncand, qout);
+    if (!rpart) return tr;
+    hipLaunchKernelGGL(gain_split_finish, dim3((nbs + 3) / 4), dim3(256), 0, stream(), n, ncand, nz, rpart, KSM, ldm,
+                       part);
+    IRWLS_LAUNCH_CHECK();
+    return rb;
+}
+
+void irwls_dev_sgma_gain(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
+                         int ldm, double *qout) {
+    if (n <= 0 || ncand <= 0) return;
+    const int g = sgma_gain_parts(n, s, ncand, KSC, Z, ldz, KSM, ldm);
+    const double *part = (const double *)scratch(10, 0);
+    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, (n + g - 1) / g, ncand, qout);
     IRWLS_LAUNCH_CHECK();
 }
 
 void irwls_dev_sgma_gain_half(int n, int s, int ncand, const double *KSC, const double *Z, int ldz, const double *KSM,
                               int ldm, double *qout, double *qhalf) {
     if (n <= 0 || ncand <= 0) return;
-    irwls_dev_sgma_gain(n, s, ncand, KSC, Z, ldz, KSM, ldm, qout);
-    // the partials of the call above, one per row tile: the first half of the
-    // tiles gives the sum over their rows
-    const int rt = (n + gain_tile_rows(ncand) - 1) / gain_tile_rows(ncand);
-    const double *part = (const double *)scratch(10, (size_t)rt * ncand * sizeof(double));
+    const int g = sgma_gain_parts(n, s, ncand, KSC, Z, ldz, KSM, ldm);
+    const double *part = (const double *)scratch(10, 0);
+    const int np = (n + g - 1) / g;
+    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, np, ncand, qout);
+    IRWLS_LAUNCH_CHECK();
+    // the first half of the row tiles (at least one): the sum over their rows
+    const int tr = gain_tile_rows(ncand);
+    const int rt = (n + tr - 1) / tr;
     const int h = rt / 2 > 0 ? rt / 2 : 1;
-    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, h, ncand, qhalf);
+    const long hp = ((long)h * tr + g - 1) / g;
+    hipLaunchKernelGGL(col_sum_kernel, dim3(ncand), dim3(256), 0, stream(), part, (int)(hp < np ? hp : np), ncand,
+                       qhalf);
     IRWLS_LAUNCH_CHECK();
 }
 

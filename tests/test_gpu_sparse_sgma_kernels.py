@@ -90,12 +90,14 @@ def _bind_sgma(L):
 
 
 @pytest.mark.parametrize("ncand", [1, 5, 16, 17, 59, 64, 65, 130])
-@pytest.mark.parametrize("n,s", [(1037, 0), (1037, 1), (1000, 13), (1037, 100)])
+@pytest.mark.parametrize("n,s", [(1037, 0), (1037, 1), (1000, 13), (1037, 100), (1037, 300), (4096, 500)])
 def test_sgma_gain_against_numpy(ncand, n, s):
     """q[c] = sum_i (KSM[i, c] - (KSC Z)[i, c])^2 (KSC n x s column-major,
     Z s x ncand column-major, KSM row-major), and the same sums over the
     first half of the row tiles (64 rows per tile up to 16 candidates, 128
-    above)."""
+    above).  With few row tiles and s > 128 the k range is split over
+    work-groups (3 ranges at s = 300, 4 at s = 500) and the ranges' products
+    are added before the squares."""
     rng = np.random.default_rng(n + 31 * s + ncand)
     L = lib.load()
     _bind_sgma(L)
