@@ -21,32 +21,29 @@ constexpr int T = 64;
 constexpr int BK = 16;
 constexpr int LDP = BK + 1;
 
-// element (row, k) = M[(kc0 + k) * ld + r0 + row]   (k-outer layout)
-__device__ __forceinline__ void ld_kouter(double (*S)[LDP], const double *M, int ld, int r0, int rmax, int kc0,
-                                          int kmax) {
+// A 64 x BK chunk of an operand, element (row, k) = M[(kc0 + k) * ld + r0 +
+// row] (k-outer layout) or M[(r0 + row) * ld + kc0 + k] (ROUTER, row-outer),
+// zero past rmax rows or kmax: fetched into registers, then stored to LDS,
+// so that the next chunk's loads fly while the current one multiplies.
+template <bool ROUTER>
+__device__ __forceinline__ void fetch_chunk(double v[4], const double *M, int ld, int r0, int rmax, int kc0, int kmax) {
     const int t = threadIdx.x;
-    const int row = t & 63;
-    const int kq = (t >> 6) * 4;
+    const int row = ROUTER ? t >> 2 : t & 63;
+    const int kq = ROUTER ? (t & 3) * 4 : (t >> 6) * 4;
     const bool rok = (r0 + row) < rmax;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int k = kc0 + kq + q;
-        S[row][kq + q] = (rok && k < kmax) ? M[(size_t)k * ld + r0 + row] : 0.0;
+        v[q] = (rok && k < kmax) ? (ROUTER ? M[(size_t)(r0 + row) * ld + k] : M[(size_t)k * ld + r0 + row]) : 0.0;
     }
 }
-
-// element (row, k) = M[(r0 + row) * ld + kc0 + k]   (row-outer layout)
-__device__ __forceinline__ void ld_router(double (*S)[LDP], const double *M, int ld, int r0, int rmax, int kc0,
-                                          int kmax) {
+template <bool ROUTER>
+__device__ __forceinline__ void put_chunk(double (*S)[LDP], const double v[4]) {
     const int t = threadIdx.x;
-    const int row = t >> 2;
-    const int kq = (t & 3) * 4;
-    const bool rok = (r0 + row) < rmax;
+    const int row = ROUTER ? t >> 2 : t & 63;
+    const int kq = ROUTER ? (t & 3) * 4 : (t >> 6) * 4;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int k = kc0 + kq + q;
-        S[row][kq + q] = (rok && k < kmax) ? M[(size_t)(r0 + row) * ld + k] : 0.0;
-    }
+    for (int q = 0; q < 4; q++) S[row][kq + q] = v[q];
 }
 
 __device__ __forceinline__ void mfma_chunk(v4d acc[2][2], double (*As)[LDP], double (*Bs)[LDP]) {
@@ -87,23 +84,39 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int m, int n, int k, doub
                                                        int lda, const double *__restrict__ B, int ldb, double beta,
                                                        double *__restrict__ C, int ldc, int kper,
                                                        double *__restrict__ part) {
-    __shared__ double As[T][LDP];
-    __shared__ double Bs[T][LDP];
+    __shared__ double As[2][T][LDP];
+    __shared__ double Bs[2][T][LDP];
     const int r0 = blockIdx.x * T, c0 = blockIdx.y * T;
     const int kb = blockIdx.z * kper, ke = min(k, kb + kper);
     v4d acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    // op(A)(i, p): !TA -> A[i + p*lda] (k-outer); TA -> A[p + i*lda] (row-outer)
+    // op(B)(p, j) as (row j, k p): !TB -> B[p + j*ldb] (row-outer); TB -> B[j + p*ldb] (k-outer)
+    // The chunks are double-buffered in LDS: the next chunk is fetched into
+    // registers before the current one is multiplied, one barrier per chunk.
+    double va[4], vb[4];
+    if (kb < ke) {
+        fetch_chunk<TA>(va, A, lda, r0, m, kb, ke);
+        fetch_chunk<!TB>(vb, B, ldb, c0, n, kb, ke);
+        put_chunk<TA>(As[0], va);
+        put_chunk<!TB>(Bs[0], vb);
+    }
+    __syncthreads();
+    int buf = 0;
     for (int kc = kb; kc < ke; kc += BK) {
-        // op(A)(i, p): !TA -> A[i + p*lda] (k-outer); TA -> A[p + i*lda] (row-outer)
-        if (!TA) ld_kouter(As, A, lda, r0, m, kc, ke);
-        else ld_router(As, A, lda, r0, m, kc, ke);
-        // op(B)(p, j) as (row j, k p): !TB -> B[p + j*ldb] (row-outer); TB -> B[j + p*ldb] (k-outer)
-        if (!TB) ld_router(Bs, B, ldb, c0, n, kc, ke);
-        else ld_kouter(Bs, B, ldb, c0, n, kc, ke);
+        const bool more = kc + BK < ke;
+        if (more) {
+            fetch_chunk<TA>(va, A, lda, r0, m, kc + BK, ke);
+            fetch_chunk<!TB>(vb, B, ldb, c0, n, kc + BK, ke);
+        }
+        mfma_chunk(acc, As[buf], Bs[buf]);
+        if (more) {
+            put_chunk<TA>(As[buf ^ 1], va);
+            put_chunk<!TB>(Bs[buf ^ 1], vb);
+        }
         __syncthreads();
-        mfma_chunk(acc, As, Bs);
-        __syncthreads();
+        buf ^= 1;
     }
     for (int nb = 0; nb < 2; nb++) {
         const int j = c0 + tile_col(nb);
@@ -877,16 +890,19 @@ static int sgma_gain_parts(int n, int s, int ncand, const double *KSC, const dou
     const int rb = gain_split_rows(ncand);
     const int nbs = (n + rb - 1) / rb;
     double *part = (double *)scratch(10, (size_t)nbs * ncand * sizeof(double));
-    // Few row tiles (a row subset): each work-group's chain of k chunks,
-    // one dependent HBM load per chunk, bounds the kernel, so the k range
-    // is split over up to 8 work-groups of at least 128 k each (at most
-    // 8 work-groups per CU in all), and gain_split_finish adds the ranges'
-    // products in order before the squares.
+    // Few row tiles (at most 2 work-groups per CU, as on a row subsample):
+    // each work-group's chain of k chunks, one dependent HBM load per chunk,
+    // bounds the kernel, so the k range is split over up to 8 work-groups of
+    // at least 128 k each (at most 8 work-groups per CU in all), and
+    // gain_split_finish adds the ranges' products in order before the
+    // squares.  With more work-groups the chunks' loads already overlap: a
+    // split of the 65536-row shortlist gains (1024 work-groups) in two saved
+    // less than its finish cost.
     const bool wide = ncand > 16 && (long)rt * ct >= 2 * device_cus();
     const int cg = ncand <= 16 ? 1 : wide ? ct : (ncand + 31) / 32;
     const long wgs = (long)rt * cg;
     int nz = 1;
-    if (wgs < 8L * device_cus()) {
+    if (wgs <= 2L * device_cus()) {
         nz = (s + 127) / 128;
         if (nz > 8) nz = 8;
         const long cap = 8L * device_cus() / wgs;
