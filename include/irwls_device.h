@@ -407,6 +407,16 @@ void irwls_dev_copy_row_at(const double *d_src, int ld, const int32_t *d_row, in
    time as a serial scan with strict > picks them (a NaN in the first
    untaken place is picked, NaNs elsewhere never): out[r] = cidx[pick r]. */
 void irwls_dev_sgma_topk(int P, const float *d_score, int k, const int32_t *d_cidx, int32_t *d_out);
+/* The same, also writing the picks' positions: d_pos[r] = pick r. */
+void irwls_dev_sgma_topk_pos(int P, const float *d_score, int k, const int32_t *d_cidx, int32_t *d_out,
+                             int32_t *d_pos);
+/* Shortlist operands gathered from the pool's (the learned proposal scores
+   a subset of candidates whose KNC rows, Z columns and eta the pool step
+   already computed): KNC2 row r and Z2 column r (s <= ld entries, leading
+   dimension ld) and eta2[r] are those of pool position d_pos[r], r < k. */
+void irwls_dev_sgma_gather_shortlist(int k, const int32_t *d_pos, int s, int ld, const double *d_KNC,
+                                     const double *d_Z, const double *d_eta, double *d_KNC2, double *d_Z2,
+                                     double *d_eta2);
 /* K1 (ld ldk) = KC + sum_q (sw_q Phi[a_q])^T (sw_q Phi[a_q]) over the nact
    active rows a (NULL: rows 0..nact-1) of the row-major n x B matrix Phi;
    sw NULL means weight 1. */

@@ -542,7 +542,8 @@ __global__ void copy_row_at_kernel(const double *__restrict__ src, int ld, const
 // the untaken non-NaN ones with the smallest index, which is what a serial
 // scan `if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i` picks.
 __global__ __launch_bounds__(64) void sgma_topk_kernel(int P, const float *__restrict__ score, int k,
-                                                       const int32_t *__restrict__ cidx, int32_t *__restrict__ out) {
+                                                       const int32_t *__restrict__ cidx, int32_t *__restrict__ out,
+                                                       int32_t *__restrict__ pos) {
     const int lane = threadIdx.x;
     const int none = 1 << 30;
     float sc[16];
@@ -586,8 +587,29 @@ __global__ __launch_bounds__(64) void sgma_topk_kernel(int P, const float *__res
         }
         if (pick == none) pick = 0;   // k > P: not reached (the caller keeps k <= P)
         if (lane == (pick & 63)) taken |= 1u << (pick >> 6);
-        if (lane == 0) out[r] = cidx[pick];
+        if (lane == 0) {
+            out[r] = cidx[pick];
+            if (pos) pos[r] = pick;
+        }
     }
+}
+
+// Shortlist operands from the pool's: row r of KNC2 and column r of Z2 (s
+// entries each, leading dimension ld) and eta2[r] are those of pool position
+// pos[r].  One work-group per shortlisted candidate.
+__global__ __launch_bounds__(256) void gather_shortlist_kernel(int s, int ld, const int32_t *__restrict__ pos,
+                                                               const double *__restrict__ KNC,
+                                                               const double *__restrict__ Z,
+                                                               const double *__restrict__ eta,
+                                                               double *__restrict__ KNC2, double *__restrict__ Z2,
+                                                               double *__restrict__ eta2) {
+    const int r = blockIdx.x;
+    const int p = pos[r];
+    for (int e = threadIdx.x; e < s; e += 256) {
+        KNC2[(size_t)r * ld + e] = KNC[(size_t)p * ld + e];
+        Z2[(size_t)r * ld + e] = Z[(size_t)p * ld + e];
+    }
+    if (threadIdx.x == 0) eta2[r] = eta[p];
 }
 
 // ----------------------------------------------------------------- wgram ---
@@ -1014,7 +1036,29 @@ void irwls_dev_sgma_topk(int P, const float *score, int k, const int32_t *cidx, 
         fprintf(stderr, "libirwls: irwls_dev_sgma_topk needs k <= P <= 1024 (P=%d, k=%d)\n", P, k);
         exit(2);
     }
-    hipLaunchKernelGGL(sgma_topk_kernel, dim3(1), dim3(64), 0, stream(), P, score, k, cidx, out);
+    hipLaunchKernelGGL(sgma_topk_kernel, dim3(1), dim3(64), 0, stream(), P, score, k, cidx, out, nullptr);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_topk_pos(int P, const float *score, int k, const int32_t *cidx, int32_t *out, int32_t *pos) {
+    if (P <= 0 || k <= 0) return;
+    if (P > 1024 || k > P) {
+        fprintf(stderr, "libirwls: irwls_dev_sgma_topk_pos needs k <= P <= 1024 (P=%d, k=%d)\n", P, k);
+        exit(2);
+    }
+    hipLaunchKernelGGL(sgma_topk_kernel, dim3(1), dim3(64), 0, stream(), P, score, k, cidx, out, pos);
+    IRWLS_LAUNCH_CHECK();
+}
+
+void irwls_dev_sgma_gather_shortlist(int k, const int32_t *pos, int s, int ld, const double *KNC, const double *Z,
+                                     const double *eta, double *KNC2, double *Z2, double *eta2) {
+    if (k <= 0) return;
+    if (s > ld) {
+        fprintf(stderr, "libirwls: irwls_dev_sgma_gather_shortlist needs s <= ld (s=%d, ld=%d)\n", s, ld);
+        exit(2);
+    }
+    hipLaunchKernelGGL(gather_shortlist_kernel, dim3(k), dim3(256), 0, stream(), s, ld, pos, KNC, Z, eta, KNC2, Z2,
+                       eta2);
     IRWLS_LAUNCH_CHECK();
 }
 

@@ -93,6 +93,21 @@ static void score_exact(irwls_devdata *dd, properties props, const int32_t *d_ro
     irwls_dev_sgma_gain(n, s, nc, d_KSC, d_Z, B, d_KSM, nc, d_q);
 }
 
+/* Exact scoring of a learned shortlist: the nc candidates d_c sit at pool
+   positions d_pos, whose KNC rows, Z columns and eta the pool step already
+   computed (the same products per candidate), so only KSM and the gains are
+   new; the shortlist's KNC, Z and eta land in d_KNC2, d_Z2, d_eta2. */
+static void score_shortlist(irwls_devdata *dd, properties props, const int32_t *d_rows, int n, int B, int s, int nc,
+                            const int32_t *d_c, const int32_t *d_pos, const double *d_KSC, double *d_KSM,
+                            const double *d_KNC, const double *d_Z, const double *d_eta, double *d_KNC2,
+                            double *d_Z2, double *d_eta2, double *d_q) {
+    irwls_rows all = irwls_devdata_rows(dd, d_rows, n);
+    irwls_rows cr = irwls_devdata_rows(dd, d_c, nc);
+    irwls_dev_kernel_matrix(&all, &cr, dd->d, props.kernelType, props.Kgamma, d_KSM, nc);
+    irwls_dev_sgma_gather_shortlist(nc, d_pos, s, B, d_KNC, d_Z, d_eta, d_KNC2, d_Z2, d_eta2);
+    irwls_dev_sgma_gain(n, s, nc, d_KSC, d_Z2, B, d_KSM, nc, d_q);
+}
+
 static int cmp_i32(const void *a, const void *b) {
     const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
     return (x > y) - (x < y);
@@ -204,6 +219,12 @@ int *SGMA(svm_dataset dataset, properties props) {
     int32_t *d_st = (int32_t *)irwls_dev_alloc(4 * sizeof(int32_t));        /* [0] chosen position, [1] positives */
     int32_t *d_cidx2 = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
     double *d_knc = (double *)irwls_dev_alloc(Bz * sizeof(double));
+    /* the learned shortlist's pool positions and its KNC rows, Z columns and eta */
+    int32_t *pos = (int32_t *)irwls_xmalloc((size_t)cmax * sizeof(int32_t));
+    int32_t *d_pos = (int32_t *)irwls_dev_alloc((size_t)cmax * sizeof(int32_t));
+    double *d_KNC2 = (double *)irwls_dev_alloc((size_t)cmax * Bz * sizeof(double));
+    double *d_Z2 = (double *)irwls_dev_alloc(Bz * cmax * sizeof(double));
+    double *d_eta2 = (double *)irwls_dev_alloc((size_t)cmax * sizeof(double));
     const double *d_y = dd->y;   /* labels of the device rows (the class-mean rows included) */
 
     for (int s = 0; s < B; s++) {
@@ -278,21 +299,21 @@ int *SGMA(svm_dataset dataset, properties props) {
                                         (float)npos / (float)s, devpick ? d_st + 1 : NULL);
                 if (devpick) {
                     /* shortlist: the highest scores, ties to the earlier draw; then their exact gains */
-                    irwls_dev_sgma_topk(pool, irwls_proposal_score_dev(lp, pool), keep, d_cidx, d_cidx2);
+                    irwls_dev_sgma_topk_pos(pool, irwls_proposal_score_dev(lp, pool), keep, d_cidx, d_cidx2, d_pos);
                     if (me)
-                        score_exact(dd, props, d_sube, me, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSCe, d_KSMe,
-                                    d_KNC, d_W, d_Z, d_eta, d_q);
+                        score_shortlist(dd, props, d_sube, me, B, s, keep, d_cidx2, d_pos, d_KSCe, d_KSMe, d_KNC, d_Z,
+                                        d_eta, d_KNC2, d_Z2, d_eta2, d_q);
                     else
-                        score_exact(dd, props, NULL, n, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSC, d_KSM, d_KNC,
-                                    d_W, d_Z, d_eta, d_q);
-                    irwls_dev_sgma_pick(keep, d_q, d_eta, d_cidx2, s, n, d_y, d_cent, d_st, d_tmp + 16);
+                        score_shortlist(dd, props, NULL, n, B, s, keep, d_cidx2, d_pos, d_KSC, d_KSM, d_KNC, d_Z,
+                                        d_eta, d_KNC2, d_Z2, d_eta2, d_q);
+                    irwls_dev_sgma_pick(keep, d_q, d_eta2, d_cidx2, s, n, d_y, d_cent, d_st, d_tmp + 16);
                     if (me) {
                         irwls_dev_gather_col_at(d_KSMe, keep, d_st, me, d_KSCe + (size_t)s * me);
                         full_col = 0;
                     } else {
                         irwls_dev_gather_col_at(d_KSM, keep, d_st, n, d_KSC + (size_t)s * n);
                     }
-                    irwls_dev_copy_row_at(d_KNC, B, d_st, s, d_knc);
+                    irwls_dev_copy_row_at(d_KNC2, B, d_st, s, d_knc);
                     knc = d_knc;
                     g_bstats.learned_steps++;
                 } else {
@@ -306,6 +327,7 @@ int *SGMA(svm_dataset dataset, properties props) {
                             if (!taken[i] && (bi < 0 || score[i] > score[bi])) bi = i;
                         taken[bi] = 1;
                         cidx2[r] = cidx[bi];
+                        pos[r] = bi;
                         if (r == 0 && keep == 0) {
                             /* the network's choice without exact scoring */
                             cent[s] = cidx[bi];
@@ -315,13 +337,15 @@ int *SGMA(svm_dataset dataset, properties props) {
                     }
                     if (keep > 0) {
                         irwls_dev_h2d(d_cidx2, cidx2, (size_t)keep * sizeof(int32_t));
+                        irwls_dev_h2d(d_pos, pos, (size_t)keep * sizeof(int32_t));
                         if (me)
-                            score_exact(dd, props, d_sube, me, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSCe,
-                                        d_KSMe, d_KNC, d_W, d_Z, d_eta, d_q);
+                            score_shortlist(dd, props, d_sube, me, B, s, keep, d_cidx2, d_pos, d_KSCe, d_KSMe, d_KNC,
+                                            d_Z, d_eta, d_KNC2, d_Z2, d_eta2, d_q);
                         else
-                            score_exact(dd, props, NULL, n, B, s, keep, d_cidx2, d_cent, ctab, d_Li, d_KSC, d_KSM,
-                                        d_KNC, d_W, d_Z, d_eta, d_q);
-                        irwls_dev_d2h(eq, d_eq, ((size_t)cmax + keep) * sizeof(double));
+                            score_shortlist(dd, props, NULL, n, B, s, keep, d_cidx2, d_pos, d_KSC, d_KSM, d_KNC, d_Z,
+                                            d_eta, d_KNC2, d_Z2, d_eta2, d_q);
+                        irwls_dev_d2h(q, d_q, (size_t)keep * sizeof(double));
+                        irwls_dev_d2h(eta, d_eta2, (size_t)keep * sizeof(double));
                         const int best = best_gain(keep, q, eta, &value);
                         cent[s] = cidx2[best];
                         if (me) {
@@ -330,7 +354,7 @@ int *SGMA(svm_dataset dataset, properties props) {
                         } else {
                             irwls_dev_gather_col(d_KSM, keep, best, n, d_KSC + (size_t)s * n);
                         }
-                        knc = d_KNC + (size_t)best * B;
+                        knc = d_KNC2 + (size_t)best * B;
                     } else {
                         const int32_t c1 = cent[s];
                         irwls_dev_h2d(d_pair, &c1, sizeof(int32_t));
@@ -410,6 +434,11 @@ int *SGMA(svm_dataset dataset, properties props) {
     irwls_dev_free(d_st);
     irwls_dev_free(d_cidx2);
     irwls_dev_free(d_knc);
+    irwls_dev_free(d_pos);
+    irwls_dev_free(d_KNC2);
+    irwls_dev_free(d_Z2);
+    irwls_dev_free(d_eta2);
+    free(pos);
     irwls_dev_coltab_destroy(ctab);
     irwls_dev_free(d_KSC);
     irwls_dev_free(d_L);
