@@ -246,6 +246,41 @@ def test_sgma_topk_on_device():
             assert list(out.get()) == [int(cidx[i]) for i in serial(sc, k)], (P, k)
 
 
+def test_sgma_topk_positions_and_shortlist_gather():
+    """irwls_dev_sgma_topk_pos returns the same shortlist as
+    irwls_dev_sgma_topk and the picks' pool positions; the shortlist gather
+    copies the KNC rows, Z columns (s of ld entries) and eta of those
+    positions, in shortlist order, and leaves the entries past s alone."""
+    from gpu_util import Dev
+    Lb = lib.load()
+    vp = C.c_void_p
+    Lb.irwls_dev_sgma_topk.argtypes = [C.c_int, vp, C.c_int, vp, vp]
+    Lb.irwls_dev_sgma_topk_pos.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp]
+    Lb.irwls_dev_sgma_gather_shortlist.argtypes = [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+    rng = np.random.default_rng(21)
+    for P, k, s, ld in [(59, 8, 700, 1000), (236, 16, 999, 1000), (17, 17, 3, 5), (1024, 1, 300, 300)]:
+        sc = np.round(rng.standard_normal(P) * 3).astype(np.float32)    # ties
+        cidx = rng.permutation(10 ** 6)[:P].astype(np.int32)
+        sd, cd = Dev(sc), Dev(cidx)
+        out1, out2, pos = Dev(np.zeros(k, np.int32)), Dev(np.zeros(k, np.int32)), Dev(np.zeros(k, np.int32))
+        Lb.irwls_dev_sgma_topk(P, sd.p, k, cd.p, out1.p)
+        Lb.irwls_dev_sgma_topk_pos(P, sd.p, k, cd.p, out2.p, pos.p)
+        p_ = pos.get()
+        assert list(out2.get()) == list(out1.get())
+        assert list(cidx[p_]) == list(out1.get())
+        assert len(set(p_.tolist())) == k
+        KNC = rng.standard_normal((P, ld))
+        Z = rng.standard_normal((P, ld))
+        eta = rng.standard_normal(P)
+        KNC2, Z2, eta2 = Dev(np.full((k, ld), 7.0)), Dev(np.full((k, ld), 7.0)), Dev(np.zeros(k))
+        Kd, Zd, ed = Dev(KNC), Dev(Z), Dev(eta)
+        Lb.irwls_dev_sgma_gather_shortlist(k, pos.p, s, ld, Kd.p, Zd.p, ed.p, KNC2.p, Z2.p, eta2.p)
+        g1, g2 = KNC2.get().reshape(k, ld), Z2.get().reshape(k, ld)
+        assert np.array_equal(g1[:, :s], KNC[p_, :s]) and np.all(g1[:, s:] == 7.0)
+        assert np.array_equal(g2[:, :s], Z[p_, :s]) and np.all(g2[:, s:] == 7.0)
+        assert np.array_equal(eta2.get(), eta[p_])
+
+
 @pytest.mark.parametrize("learned", [0, 1])
 def test_sgma_device_choice_equals_host_choice(learned, capfd):
     """The asynchronous SGMA step (choice, shortlist and positive count on
